@@ -1,0 +1,94 @@
+/*
+ * linac_amd -- C ABI of the MI355X (gfx950) dense row-reduction library.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns an
+ * int status (LINAC_OK or a negative LINAC_ERR_*) and leaves a message for
+ * linac_last_error().  Matrices are row-major with a leading dimension `ld`
+ * (elements between consecutive rows, ld >= cols).
+ *
+ * Pointers are host memory unless LINAC_DEVICE_PTR is set in `flags`; with a
+ * host pointer the library copies to HBM, runs, and copies back in place.  With
+ * a device pointer the rows must start on 16-byte boundaries (ld * element size
+ * a multiple of 16 and a 16-byte aligned base).  `stream` may be NULL (the
+ * library's own stream for the current device).  All calls block until their
+ * results are complete.
+ *
+ * Reference interfaces these replace (linac 1.0.1):
+ *   linac_rref_*         cuda_row_reduce            linac/pycuda_row_reduce.py:18-151
+ *                        (the JIT+launch glue of    linac/pycuda_tools.py:17-49 and the
+ *                         kernels of                linac/row_reduce.cu:117-430)
+ *   linac_load_matrix_*  cuda_load_matrix           linac/pycuda_matrix_loader.py:28-124
+ *                        (kernel                    linac/matrix_loader.cu:88-112)
+ *   linac_available      CUDA_FOUND                 linac/_optional.py:11-23
+ */
+#ifndef LINAC_AMD_H
+#define LINAC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mapped to Python exceptions by the host package) ---- */
+#define LINAC_OK                   0
+#define LINAC_ERR_INVALID         -1  /* bad argument: ValueError                   */
+#define LINAC_ERR_NOMEM           -2  /* allocation or admission: MemoryError        */
+#define LINAC_ERR_DEVICE          -3  /* HIP runtime / no device: RuntimeError       */
+#define LINAC_ERR_NOT_INVERTIBLE  -4  /* head not invertible mod p: ZeroDivisionError */
+#define LINAC_ERR_FORKED          -5  /* used after fork() in a child: RuntimeError   */
+
+/* ---- flags ---- */
+#define LINAC_DEVICE_PTR      0x1u   /* matrix / operand pointers are device memory */
+#define LINAC_METHOD_MASK     0xF0u
+#define LINAC_METHOD_AUTO     0x00u  /* library picks (blocked when it applies)     */
+#define LINAC_METHOD_PLAIN    0x10u  /* unblocked: one fused step per column        */
+#define LINAC_METHOD_BLOCKED  0x20u  /* panel + exact mod-p MFMA trailing update    */
+#define LINAC_NO_FINALIZE     0x100u /* leave rows unpermuted (pivots_out says where)*/
+#define LINAC_PROFILE         0x200u /* time the call with HIP events (linac_profile_last) */
+
+/* ---- library ---- */
+const char* linac_last_error(void);
+int linac_version(void);
+/* Number of visible GPUs (0 when none); never fails on a machine without one. */
+int linac_device_count(int* count);
+/* 1 when a usable GPU is present, else 0: the meaning of CUDA_FOUND
+ * (linac/_optional.py:11-23). */
+int linac_available(void);
+/* Create the per-device context (stream, workspace).  Optional: every entry
+ * point initialises lazily on the current device. */
+int linac_init(int device);
+/* Synchronise and release cached device memory of the current device. */
+int linac_release_cache(void);
+/* 1 if n is prime (deterministic Miller-Rabin), else 0.  Host only. */
+int linac_is_prime(uint64_t n);
+/* HIP-event timings of this thread's last call made with LINAC_PROFILE, all
+ * recorded on the call's stream:
+ *   out[0] device ms of the elimination (first to last kernel),
+ *   out[1] sampled launches of the dominant kernel (plain: row update;
+ *          blocked: trailing mod-p GEMM),
+ *   out[2] their summed duration in ms,
+ *   out[3] their algorithmic HBM bytes, out[4] their algorithmic modular MACs,
+ *   out[5] the method that ran (1 plain, 2 blocked).
+ * Writes min(n, 6) values and returns how many were written. */
+int linac_profile_last(double* out, int n);
+
+/* ---- RREF over GF(p) ----
+ * Replaces cuda_row_reduce(matrix, field_characteristic=p) for p > 0
+ * (linac/pycuda_row_reduce.py:18-151; dtype rule :41-48).  Entries are reduced
+ * mod p first.  p must be prime (checked; composite -> LINAC_ERR_INVALID).
+ * linac_rref_u32: p < 2^32.  linac_rref_u64: p < 2^63.
+ * pivots_out (host, may be NULL) receives min(rows, cols) int64 slots: the
+ * pivot column of each non-zero row, in row order; *rank_out the rank.
+ * On return rows [0, rank) hold the RREF's non-zero rows and rows
+ * [rank, rows) are zero -- exactly the output of cuda_row_reduce. */
+int linac_rref_u32(uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p,
+                   uint32_t flags, void* stream, int64_t* pivots_out, int64_t* rank_out);
+int linac_rref_u64(uint64_t* a, int64_t rows, int64_t cols, int64_t ld, uint64_t p,
+                   uint32_t flags, void* stream, int64_t* pivots_out, int64_t* rank_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LINAC_AMD_H */

@@ -1,0 +1,115 @@
+"""ctypes binding of liblinac_amd.so, the C ABI declared in include/linac_amd.h.
+
+The shared library is built in-tree by ``linac_amd.build`` (hipcc, gfx950).
+There is no JIT and no CPU fallback: if the library is missing, every compute
+call raises ``RuntimeError`` naming the file it looked for.
+"""
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblinac_amd.so")
+
+OK = 0
+ERR_INVALID = -1
+ERR_NOMEM = -2
+ERR_DEVICE = -3
+ERR_NOT_INVERTIBLE = -4
+ERR_FORKED = -5
+
+DEVICE_PTR = 0x1
+METHOD_AUTO = 0x00
+METHOD_PLAIN = 0x10
+METHOD_BLOCKED = 0x20
+NO_FINALIZE = 0x100
+PROFILE = 0x200
+
+METHODS = {"auto": METHOD_AUTO, "plain": METHOD_PLAIN, "blocked": METHOD_BLOCKED}
+
+_lock = threading.Lock()
+_lib = None
+_load_error = None
+
+_i64 = ctypes.c_int64
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_vp = ctypes.c_void_p
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+# name -> (restype, argtypes); every entry point declared in include/linac_amd.h
+SIGNATURES = {
+    "linac_last_error": (ctypes.c_char_p, []),
+    "linac_version": (ctypes.c_int, []),
+    "linac_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "linac_available": (ctypes.c_int, []),
+    "linac_init": (ctypes.c_int, [ctypes.c_int]),
+    "linac_release_cache": (ctypes.c_int, []),
+    "linac_is_prime": (ctypes.c_int, [_u64]),
+    "linac_profile_last": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "linac_rref_u32": (ctypes.c_int, [_vp, _i64, _i64, _i64, _u32, _u32, _vp, _i64p, _i64p]),
+    "linac_rref_u64": (ctypes.c_int, [_vp, _i64, _i64, _i64, _u64, _u32, _vp, _i64p, _i64p]),
+}
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load():
+    """Load (once) and return the ctypes library; raise if it is not built."""
+    global _lib, _load_error
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(f"{LIB_PATH} is not built; run linac_amd.build.build() (hipcc, gfx950)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def last_error():
+    msg = load().linac_last_error()
+    return msg.decode(errors="replace") if msg else ""
+
+
+def check(rc):
+    """Map a C status to the exception the Python API promises."""
+    if rc == OK:
+        return
+    msg = last_error()
+    if rc == ERR_INVALID:
+        raise ValueError(msg)
+    if rc == ERR_NOMEM:
+        raise MemoryError(msg)
+    if rc == ERR_NOT_INVERTIBLE:
+        raise ZeroDivisionError(msg)
+    raise RuntimeError(msg or f"linac_amd error {rc}")
+
+
+def profile_last():
+    """Timings of this thread's last LINAC_PROFILE call (see linac_profile_last)."""
+    buf = (ctypes.c_double * 6)()
+    n = load().linac_profile_last(buf, 6)
+    keys = ["device_ms", "samples", "sampled_ms", "sampled_bytes", "sampled_macs", "method"]
+    return dict(zip(keys[:n], list(buf)[:n]))
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(load().linac_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def available():
+    try:
+        return bool(load().linac_available())
+    except (OSError, LibraryMissing):
+        return False

@@ -1,0 +1,30 @@
+// Declarations shared by the GF(p) elimination methods.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "field.hpp"
+#include "runtime.hpp"
+
+namespace linac {
+
+// 16-byte vector of field elements: one global_load/store_dwordx4 per access.
+template <class T>
+struct alignas(16) Vec16 {
+    T v[16 / sizeof(T)];
+};
+
+// Return value of a method that does not apply to a problem (distinct from
+// the negative error codes): the caller falls back to another method.
+constexpr int NOT_APPLICABLE = 1;
+
+// Blocked right-looking Gauss-Jordan (rref_blocked.hip).  Runs the whole
+// elimination on A (state in `ws`), leaving every pivot row normalised.
+// Returns LINAC_OK, NOT_APPLICABLE, or a negative error.
+template <class F>
+int blocked_dispatch(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
+                     const ElimWorkspace& ws, DeviceContext* ctx, bool forced, Profiler* prof);
+
+}  // namespace linac

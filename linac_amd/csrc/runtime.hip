@@ -1,0 +1,267 @@
+// Per-device context, error reporting, scratch memory and host utilities.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <unistd.h>
+
+#include <array>
+
+#include "runtime.hpp"
+
+namespace linac {
+
+static thread_local char g_error[2048] = "";
+static pid_t g_owner_pid = 0;  // process that first touched the HIP runtime
+static std::mutex g_ctx_mu;
+static std::array<DeviceContext*, 64> g_ctx{};
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+// A HIP runtime initialised in a parent cannot be used from a fork()ed child
+// (the driver state is not inherited).  Detect that before any HIP call and
+// fail cleanly, the analogue of CUDA's "initialization error" in a forked
+// child (tests/test_cuda_concurrency.py:18-27 of the reference).
+static int check_process() {
+    const pid_t me = getpid();
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (g_owner_pid == 0) {
+        g_owner_pid = me;
+        return LINAC_OK;
+    }
+    if (g_owner_pid != me)
+        return fail(LINAC_ERR_FORKED,
+                    "initialization error: the HIP runtime was initialised in process %d and this "
+                    "process (%d) is a fork of it; start workers with the 'spawn' method",
+                    (int)g_owner_pid, (int)me);
+    return LINAC_OK;
+}
+
+int DeviceContext::get(ScratchSlot slot, size_t bytes, void** out) {
+    Scratch& s = scratch[slot];
+    if (bytes == 0) bytes = 256;
+    if (s.bytes < bytes) {
+        if (s.ptr) {
+            LINAC_HIP(hipStreamSynchronize(stream));
+            LINAC_HIP(hipFree(s.ptr));
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        const size_t want = (size_t)round_up((int64_t)bytes, (int64_t)2 << 20);
+        hipError_t e = hipMalloc(&s.ptr, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            s.ptr = nullptr;
+            return fail(LINAC_ERR_NOMEM, "device allocation of %zu bytes failed: %s", want,
+                        hipGetErrorString(e));
+        }
+        s.bytes = want;
+    }
+    *out = s.ptr;
+    return LINAC_OK;
+}
+
+int DeviceContext::release() {
+    LINAC_HIP(hipStreamSynchronize(stream));
+    for (auto& s : scratch) {
+        if (s.ptr) LINAC_HIP(hipFree(s.ptr));
+        s.ptr = nullptr;
+        s.bytes = 0;
+    }
+    return LINAC_OK;
+}
+
+int current_context(DeviceContext** out) {
+    LINAC_TRY(check_process());
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(LINAC_ERR_DEVICE, "no usable GPU: %s", hipGetErrorString(e));
+    }
+    if (dev < 0 || dev >= (int)g_ctx.size()) return fail(LINAC_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (!g_ctx[dev]) {
+        auto* c = new DeviceContext();
+        c->device = dev;
+        hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (se != hipSuccess) {
+            delete c;
+            return fail(LINAC_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(se));
+        }
+        void* pin = nullptr;
+        se = hipHostMalloc(&pin, 4096, hipHostMallocDefault);
+        if (se != hipSuccess) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return fail(LINAC_ERR_DEVICE, "hipHostMalloc: %s", hipGetErrorString(se));
+        }
+        c->pinned = (int64_t*)pin;
+        g_ctx[dev] = c;
+    }
+    *out = g_ctx[dev];
+    return LINAC_OK;
+}
+
+int make_workspace(DeviceContext* ctx, int64_t N, int64_t M, ElimWorkspace* ws) {
+    const int64_t cap = std::max<int64_t>(1, std::min(N, M));
+    const size_t st_bytes = 256;
+    const size_t piv_bytes = (size_t)round_up(cap * 8, 256);
+    const size_t used_bytes = (size_t)round_up(N, 256);
+    void* base = nullptr;
+    LINAC_TRY(ctx->get(SLOT_STATE, st_bytes + 2 * piv_bytes + used_bytes, &base));
+    char* b = (char*)base;
+    ws->st = (ElimState*)b;
+    ws->piv_rows = (int64_t*)(b + st_bytes);
+    ws->piv_cols = (int64_t*)(b + st_bytes + piv_bytes);
+    ws->used = (uint8_t*)(b + st_bytes + 2 * piv_bytes);
+    ws->cap = cap;
+    return LINAC_OK;
+}
+
+static uint64_t mulmod_host(uint64_t a, uint64_t b, uint64_t m) {
+    return (uint64_t)(((unsigned __int128)a * b) % m);
+}
+
+static uint64_t powmod_host(uint64_t a, uint64_t e, uint64_t m) {
+    uint64_t r = 1 % m;
+    a %= m;
+    while (e) {
+        if (e & 1) r = mulmod_host(r, a, m);
+        a = mulmod_host(a, a, m);
+        e >>= 1;
+    }
+    return r;
+}
+
+// Deterministic Miller-Rabin: the first 12 prime bases decide every n < 3.3e24.
+bool is_prime_u64(uint64_t n) {
+    if (n < 2) return false;
+    static const uint64_t small[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    for (uint64_t q : small) {
+        if (n == q) return true;
+        if (n % q == 0) return false;
+    }
+    uint64_t d = n - 1;
+    int s = 0;
+    while ((d & 1) == 0) { d >>= 1; ++s; }
+    for (uint64_t a : small) {
+        uint64_t x = powmod_host(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool comp = true;
+        for (int i = 1; i < s; ++i) {
+            x = mulmod_host(x, x, n);
+            if (x == n - 1) { comp = false; break; }
+        }
+        if (comp) return false;
+    }
+    return true;
+}
+
+Profiler& thread_profiler() {
+    static thread_local Profiler prof;
+    return prof;
+}
+
+int Profiler::begin(hipStream_t s) {
+    for (auto& x : samples) {
+        (void)hipEventDestroy(x.a);
+        (void)hipEventDestroy(x.b);
+    }
+    samples.clear();
+    for (double& r : result) r = 0;
+    if (!t0) LINAC_HIP(hipEventCreate(&t0));
+    if (!t1) LINAC_HIP(hipEventCreate(&t1));
+    LINAC_HIP(hipEventRecord(t0, s));
+    return LINAC_OK;
+}
+
+int Profiler::sample_begin(hipStream_t s, int64_t tag, double bytes, double macs) {
+    Sample x{nullptr, nullptr, bytes, macs, tag};
+    LINAC_HIP(hipEventCreate(&x.a));
+    LINAC_HIP(hipEventCreate(&x.b));
+    LINAC_HIP(hipEventRecord(x.a, s));
+    samples.push_back(x);
+    return LINAC_OK;
+}
+
+int Profiler::sample_end(hipStream_t s) {
+    LINAC_HIP(hipEventRecord(samples.back().b, s));
+    return LINAC_OK;
+}
+
+int Profiler::finish(hipStream_t s) {
+    LINAC_HIP(hipEventRecord(t1, s));
+    LINAC_HIP(hipEventSynchronize(t1));
+    float ms = 0;
+    LINAC_HIP(hipEventElapsedTime(&ms, t0, t1));
+    result[0] = ms;
+    for (auto& x : samples) {
+        LINAC_HIP(hipEventElapsedTime(&ms, x.a, x.b));
+        result[1] += 1;
+        result[2] += ms;
+        result[3] += x.bytes;
+        result[4] += x.macs;
+    }
+    return LINAC_OK;
+}
+
+}  // namespace linac
+
+using namespace linac;
+
+extern "C" {
+
+const char* linac_last_error(void) { return g_error; }
+
+int linac_version(void) { return 10000; }
+
+int linac_device_count(int* count) {
+    if (!count) return fail(LINAC_ERR_INVALID, "count is NULL");
+    *count = 0;
+    LINAC_TRY(check_process());
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *count = n;
+    return LINAC_OK;
+}
+
+int linac_available(void) {
+    int n = 0;
+    if (linac_device_count(&n) != LINAC_OK) return 0;
+    return n > 0 ? 1 : 0;
+}
+
+int linac_init(int device) {
+    LINAC_TRY(check_process());
+    LINAC_HIP(hipSetDevice(device));
+    DeviceContext* ctx = nullptr;
+    return current_context(&ctx);
+}
+
+int linac_release_cache(void) {
+    DeviceContext* ctx = nullptr;
+    LINAC_TRY(current_context(&ctx));
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return ctx->release();
+}
+
+int linac_is_prime(uint64_t n) { return is_prime_u64(n) ? 1 : 0; }
+
+int linac_profile_last(double* out, int n) {
+    if (!out || n <= 0) return 0;
+    const int m = n < 6 ? n : 6;
+    for (int i = 0; i < m; ++i) out[i] = thread_profiler().result[i];
+    return m;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// Internal runtime of the linac_amd library: per-device context, error
+// reporting, cached device scratch, and the bookkeeping that every
+// elimination method shares.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "linac_amd.h"
+
+namespace linac {
+
+// Records a message for linac_last_error() and returns `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define LINAC_HIP(expr)                                                                    \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return ::linac::fail(LINAC_ERR_DEVICE, "%s: %s (%s:%d)", #expr,                \
+                                 hipGetErrorString(_e), __FILE__, __LINE__);               \
+    } while (0)
+
+#define LINAC_TRY(expr)            \
+    do {                           \
+        int _rc = (expr);          \
+        if (_rc != LINAC_OK) return _rc; \
+    } while (0)
+
+// Grow-only device buffers owned by a context, reused across calls so that a
+// steady stream of solves does not hit hipMalloc.
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+enum ScratchSlot {
+    SLOT_STATE = 0,   // ElimState + pivot lists + row flags
+    SLOT_MATRIX,      // device copy of a host matrix
+    SLOT_OUT,         // permutation target of the final gather
+    SLOT_PANEL_A,     // blocked method: panel / limb planes
+    SLOT_PANEL_B,
+    SLOT_PANEL_C,
+    SLOT_AUX,
+    SLOT_COUNT
+};
+
+struct DeviceContext {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    Scratch scratch[SLOT_COUNT];
+    int64_t* pinned = nullptr;  // small pinned host buffer for status reads
+    int get(ScratchSlot slot, size_t bytes, void** out);
+    int release();
+};
+
+// Context of the current device (created on first use).  Fails with
+// LINAC_ERR_FORKED in a child process forked after initialisation and with
+// LINAC_ERR_DEVICE when no GPU is usable.
+int current_context(DeviceContext** out);
+
+// Shared state of one elimination (lives in device memory).
+struct ElimState {
+    int64_t prow;          // pivot row of the current column, -1 if none
+    int64_t rank;          // pivots found so far
+    int64_t first_unused;  // every row below this index is a pivot row
+    uint64_t factor;       // field-specific multiplier derived from the head
+    int64_t aux[4];
+};
+
+struct ElimWorkspace {
+    ElimState* st;
+    int64_t* piv_rows;  // [min(N, M)] row of the k-th pivot
+    int64_t* piv_cols;  // [min(N, M)] column of the k-th pivot
+    uint8_t* used;      // [N] 1 once a row has been a pivot row
+    int64_t cap;        // min(N, M)
+};
+
+int make_workspace(DeviceContext* ctx, int64_t N, int64_t M, ElimWorkspace* ws);
+
+bool is_prime_u64(uint64_t n);
+
+// Event-based timing of one call (LINAC_PROFILE).  Samples are launches of
+// the method's dominant kernel, each tagged with its algorithmic bytes and
+// modular multiply-adds so that achieved rates need no model of the kernel.
+struct Profiler {
+    bool on = false;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    struct Sample {
+        hipEvent_t a, b;
+        double bytes, macs;
+        int64_t tag;  // column (plain) or panel start (blocked)
+    };
+    std::vector<Sample> samples;
+    double result[6] = {0, 0, 0, 0, 0, 0};  // see linac_profile_last
+    int begin(hipStream_t s);                        // resets and records t0
+    int sample_begin(hipStream_t s, int64_t tag, double bytes, double macs);
+    int sample_end(hipStream_t s);
+    int finish(hipStream_t s);                       // records t1, syncs, sums
+};
+Profiler& thread_profiler();
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+}  // namespace linac

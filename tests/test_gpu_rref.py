@@ -1,0 +1,142 @@
+"""GF(p) RREF on the GPU against the golden fixtures, the oracle and
+size-independent certificates.  Bit-exact throughout."""
+import hashlib
+import json
+import os
+
+import numpy
+import pytest
+
+from conftest import GOLDEN, load_small_cases
+from oracle import crref, modp
+
+pytestmark = pytest.mark.gpu
+
+METHODS = ["plain", "blocked"]
+
+
+def _rref(gpu, A, p, method, **kw):
+    try:
+        return gpu.cuda_row_reduce(A, p, method=method, **kw)
+    except ValueError as e:
+        if "does not apply" in str(e):
+            pytest.skip(f"{method} not applicable: {e}")
+        raise
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_reference_golden_vector(gpu, method):
+    """reference tests/test_row_reduce_random_cuda.py:52-63."""
+    p = 2 ** 31 - 19
+    numpy.random.seed(0)
+    A = numpy.random.randint(p, size=(4097, 4097), dtype=numpy.uint32)[:-1, :]
+    R = _rref(gpu, A, p, method)
+    known = numpy.load(os.path.join(GOLDEN, "known_result_4096_4097_2to32m19.npy"), allow_pickle=False)
+    assert R.dtype == numpy.uint32 and R.shape == (4096, 4097)
+    assert numpy.array_equal(R[:, -1], known)
+    assert numpy.array_equal(R[:, :-1], numpy.eye(4096, dtype=numpy.uint32))
+
+
+@pytest.mark.parametrize("case", range(len(load_small_cases())))
+def test_sympy_fixtures(gpu, case):
+    p, A, R, piv = load_small_cases()[case]
+    for method in ["plain", "auto"]:
+        out, pv, rank = gpu.rref_with_pivots(A, p, method=method)
+        assert out.dtype == (numpy.uint64 if p > 2 ** 32 - 1 else numpy.uint32)
+        assert numpy.array_equal(out.astype(numpy.uint64), R), method
+        assert pv == piv and rank == len(piv)
+
+
+@pytest.mark.parametrize("p, n", [(2 ** 31 - 1, 1024), (2 ** 31 - 1, 2048), (2 ** 31 - 1, 3072), (2 ** 31 + 11, 3072),
+                                  (2 ** 61 - 1, 2048)])
+@pytest.mark.parametrize("method", METHODS)
+def test_full_rank_gives_identity(gpu, p, n, method):
+    """reference tests/test_row_reduce_random_cuda.py:23-49 (seeded here)."""
+    A = numpy.random.RandomState(n).randint(0, p, size=(n, n), dtype=numpy.int64)
+    R = _rref(gpu, A, p, method)
+    assert numpy.array_equal(R, numpy.identity(n, dtype=R.dtype))
+
+
+@pytest.mark.parametrize("shape, rank, p, seed", [
+    ((300, 300), 250, 2 ** 31 - 1, 0), ((257, 513), 200, 2 ** 31 - 19, 1), ((600, 170), 170, 65537, 2),
+    ((129, 1000), 129, 2 ** 31 + 11, 3), ((400, 400), 17, 2 ** 61 - 1, 4), ((333, 444), 300, 2, 5),
+    ((200, 300), 150, 3, 6), ((1000, 1000), 700, 2 ** 31 - 1, 7), ((64, 4096), 64, 2 ** 32 - 5, 8),
+])
+@pytest.mark.parametrize("method", ["plain", "auto"])
+def test_matches_oracle_bit_exact(gpu, shape, rank, p, seed, method):
+    rows, cols = shape
+    if p < 2 ** 31:
+        A = modp.low_rank(rows, cols, rank, p, seed)
+    else:
+        rs = numpy.random.RandomState(seed)
+        L = rs.randint(0, p, size=(rows, rank), dtype=numpy.int64).astype(object)
+        U = rs.randint(0, p, size=(rank, cols), dtype=numpy.int64).astype(object)
+        A = numpy.array((L @ U) % p, dtype=numpy.uint64)
+    A[:, 3] = 0
+    Rg, pg, rk = gpu.rref_with_pivots(A, p, method=method)
+    Ro, po = crref.rref_modp(A, p)
+    assert pg == po and rk == len(po) == min(rank, rows)
+    assert numpy.array_equal(Rg.astype(numpy.uint64), Ro.astype(numpy.uint64))
+
+
+def test_c0_config_checksum(gpu):
+    cfg = json.load(open(os.path.join(GOLDEN, "c0_config.json")))
+    p = cfg["p"]
+    rs = numpy.random.RandomState(0)
+    L = rs.randint(p, size=(500, 400), dtype=numpy.uint32)
+    U = rs.randint(p, size=(400, 500), dtype=numpy.uint32)
+    A = modp.matmul_modp(L, U, p).astype(numpy.uint32)
+    for method in ["plain", "auto"]:
+        R, piv, rank = gpu.rref_with_pivots(A, p, method=method)
+        assert piv == cfg["pivots"] and rank == 400
+        assert hashlib.sha256(numpy.ascontiguousarray(R).tobytes()).hexdigest() == cfg["rref_sha256"]
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_planted_solution_large(gpu, method):
+    p = 2 ** 31 - 1
+    Ab, X = modp.planted(5000, p, 12, extra=3)
+    R, piv, rank = gpu.rref_with_pivots(Ab.astype(numpy.uint32), p, method=method) if method != "blocked" else \
+        (_rref(gpu, Ab.astype(numpy.uint32), p, method), None, None)
+    assert numpy.array_equal(R[:, 5000:].astype(numpy.int64), X)
+    assert numpy.array_equal(R[:, :5000], numpy.eye(5000, dtype=numpy.uint32))
+
+
+def test_u64_planted(gpu):
+    p = 2 ** 61 - 1
+    n = 700
+    rs = numpy.random.RandomState(9)
+    A = rs.randint(0, p, size=(n, n), dtype=numpy.int64).astype(object)
+    X = rs.randint(0, p, size=(n, 1), dtype=numpy.int64).astype(object)
+    Ab = numpy.concatenate([A, (A @ X) % p], axis=1)
+    R = gpu.cuda_row_reduce(Ab, p)
+    assert R.dtype == numpy.uint64
+    assert R[:, -1].tolist() == X[:, 0].tolist()
+
+
+def test_edge_cases(gpu):
+    p = 2 ** 31 - 1
+    assert gpu.cuda_row_reduce(numpy.zeros((0, 5), dtype=numpy.uint32), p).shape == (0, 5)
+    assert gpu.cuda_row_reduce(numpy.zeros((4, 0), dtype=numpy.uint32), p).shape == (4, 0)
+    assert gpu.cuda_row_reduce(numpy.array([[5]]), p).tolist() == [[1]]
+    assert gpu.cuda_row_reduce(numpy.array([[0]]), p).tolist() == [[0]]
+    Z = gpu.cuda_row_reduce(numpy.zeros((37, 53), dtype=numpy.uint32), p)
+    assert not Z.any()
+    # non-canonical and negative inputs are reduced mod p first
+    A = numpy.array([[p + 2, -1], [3, 2 * p + 4]], dtype=numpy.int64)
+    assert numpy.array_equal(gpu.cuda_row_reduce(A, p), crref.rref_modp(numpy.mod(A, p), p)[0])
+    # caller's array is not modified; dtype follows the reference
+    B = numpy.random.RandomState(1).randint(0, p, size=(50, 60))
+    B0 = B.copy()
+    R = gpu.cuda_row_reduce(B, p)
+    assert numpy.array_equal(B, B0) and R.dtype == numpy.uint32
+    assert gpu.cuda_row_reduce(B, p, _mod64=True).dtype == numpy.uint64
+    assert numpy.array_equal(gpu.cuda_row_reduce(B, p, _mod64=True), R.astype(numpy.uint64))
+
+
+def test_repeatable(gpu):
+    p = 2 ** 31 - 1
+    A = modp.low_rank(700, 900, 600, p, 21)
+    R1 = gpu.cuda_row_reduce(A, p)
+    R2 = gpu.cuda_row_reduce(A, p)
+    assert numpy.array_equal(R1, R2)
