@@ -1,0 +1,73 @@
+// Exact modular GEMM on the int8 matrix cores of gfx950.
+//
+//   C[m, n] <- (C[m, n] - sum_k L[m, k] * B[k, n]) mod p        (p odd, p <= 4278124287)
+//
+// Every operand entry x in [0, p) is first mapped to its centred representative
+// c in (-p/2, p/2] and written as four balanced base-256 digits
+//   c = d0 + 256 d1 + 65536 d2 + 2^24 d3,   d_i in [-128, 127],
+// which is possible for |c| <= 127 * (1 + 2^8 + 2^16 + 2^24) = 2139062143,
+// i.e. for every p <= 4278124287.  The operands are stored as four int8 "limb
+// planes" (plane l holds digit l of every entry, K contiguous).
+//
+// With D_{l1,l2} = sum_k L_l1[m, k] * B_l2[k, n] (one MFMA chain per limb pair),
+//   sum_k L[m,k] B[k,n] = sum_{s=0..6} 256^s S_s,   S_s = sum_{l1+l2=s} D_{l1,l2}.
+// Exactness of the int32 accumulators: |digit products| <= 2^14, so
+// |S_s| <= 4 * K * 2^14 <= 2^24 for K <= 256 (2^31 would allow K < 2^15).
+//
+// The seven S_s are folded modulo p in double precision, where every value is
+// an integer below 2^53 and therefore exact:
+//   P0 = S0 + 256 S1, P1 = S2 + 256 S3, P2 = S4 + 256 S5, P3 = S6  (|P| < 2^33)
+//   r = P3;  r = red(r * 2^16 + P2);  r = red(r * 2^16 + P1);  r = red(r * 2^16 + P0)
+// with red(x) = x - rint(x / p) * p (an FMA, exact because the result is a
+// small integer); |r| stays below ~1.5 p < 2^32, so r * 2^16 + P < 2^49.
+// A final floor-based reduction brings C - r into [0, p).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace linac {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Largest prime handled by the limb GEMM (centred values fit 4 balanced digits).
+constexpr uint64_t LIMB_P_MAX = 4278124287ull;
+
+__device__ __forceinline__ int32_t centred(uint32_t v, uint32_t p) {
+    return v > (p - 1) / 2 ? (int32_t)((int64_t)v - (int64_t)p) : (int32_t)v;
+}
+
+// Balanced base-256 digits of c (|c| <= 2139062143); d[3] is the top digit.
+__device__ __forceinline__ void balanced_digits(int32_t c, int8_t d[4]) {
+    int32_t x = c;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        const int8_t dl = (int8_t)(x & 0xFF);
+        d[l] = dl;
+        x = (x - dl) >> 8;
+    }
+    d[3] = (int8_t)x;
+}
+
+__device__ __forceinline__ double red_rint(double x, double p, double pinv) {
+    const double q = rint(x * pinv);
+    return fma(-q, p, x);
+}
+
+// (cval - sum_s 256^s S_s) mod p, or (- sum) mod p when cval is not used.
+__device__ __forceinline__ uint32_t fold_sub(uint32_t cval, const int32_t s[7], double p, double pinv) {
+    const double P0 = (double)s[0] + 256.0 * (double)s[1];
+    const double P1 = (double)s[2] + 256.0 * (double)s[3];
+    const double P2 = (double)s[4] + 256.0 * (double)s[5];
+    double r = (double)s[6];
+    r = red_rint(fma(r, 65536.0, P2), p, pinv);
+    r = red_rint(fma(r, 65536.0, P1), p, pinv);
+    r = red_rint(fma(r, 65536.0, P0), p, pinv);
+    double t = (double)cval - r;
+    const double q = floor(t * pinv);
+    t = fma(-q, p, t);
+    t = t < 0.0 ? t + p : t;
+    t = t >= p ? t - p : t;
+    return (uint32_t)t;
+}
+
+}  // namespace linac

