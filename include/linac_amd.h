@@ -87,6 +87,27 @@ int linac_rref_u32(uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t
 int linac_rref_u64(uint64_t* a, int64_t rows, int64_t cols, int64_t ld, uint64_t p,
                    uint32_t flags, void* stream, int64_t* pivots_out, int64_t* rank_out);
 
+/* ---- monomial-evaluation matrix loader ----
+ * Replaces cuda_load_matrix(bases, lindices, shape, field_characteristic)
+ * (linac/pycuda_matrix_loader.py:28-124; kernel linac/matrix_loader.cu:88-112):
+ *   out[r, c] = prod_{t < degree} bases[r, lindices[c, t]]   (mod p for u32/u64)
+ * bases: rows x basis_len (row-major), lindices: cols x degree uint32,
+ * out: rows x cols (row-major, contiguous).  Bases are reduced mod p first.
+ * 1 <= degree <= 32, 1 <= basis_len <= 4096; every index must be < basis_len
+ * (checked; LINAC_ERR_INVALID otherwise).  Systems sharing lindices are one
+ * call with their bases stacked row-wise.  c128 arrays are interleaved
+ * (re, im) doubles. */
+int linac_load_matrix_u32(uint32_t* out, const uint32_t* bases, const uint32_t* lindices, int64_t rows,
+                          int64_t cols, int64_t basis_len, int64_t degree, uint32_t p, uint32_t flags,
+                          void* stream);
+int linac_load_matrix_u64(uint64_t* out, const uint64_t* bases, const uint32_t* lindices, int64_t rows,
+                          int64_t cols, int64_t basis_len, int64_t degree, uint64_t p, uint32_t flags,
+                          void* stream);
+int linac_load_matrix_f64(double* out, const double* bases, const uint32_t* lindices, int64_t rows, int64_t cols,
+                          int64_t basis_len, int64_t degree, uint32_t flags, void* stream);
+int linac_load_matrix_c128(double* out, const double* bases, const uint32_t* lindices, int64_t rows, int64_t cols,
+                           int64_t basis_len, int64_t degree, uint32_t flags, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,3 +101,27 @@ def test_pivot_tools_match_reference_restatement():
     F[0, 1] = 1.0
     F[1, 3] = 1.0 + 1e-12
     assert linac_amd.pivot_columns_from_row_reduced_echelon_form(linac_amd.drop_bottom_zero_rows(F)) == [1, 3]
+
+
+def test_loader_validation_before_gpu_work():
+    bases = numpy.arange(12, dtype=numpy.uint32).reshape(3, 4)
+    with pytest.raises(ValueError, match="lindices"):
+        linac_amd.cuda_load_matrix(bases, numpy.array([[0, 4]], dtype=numpy.uint32), field_characteristic=7)
+    with pytest.raises(ValueError, match="degree"):
+        linac_amd.cuda_load_matrix(bases, numpy.zeros((2, 33), dtype=numpy.uint32), field_characteristic=7)
+    with pytest.raises(ValueError, match="not prime"):
+        linac_amd.cuda_load_matrix(bases, numpy.zeros((2, 2), dtype=numpy.uint32), field_characteristic=9)
+    with pytest.raises(AssertionError):
+        linac_amd.cuda_load_matrix(bases, numpy.zeros((2, 2), dtype=numpy.uint32), (3, 3), field_characteristic=7)
+    with pytest.raises(NotImplementedError):
+        linac_amd.load_matrices([], [], {'field': None}, use_cuda=False)
+
+
+def test_loader_dtype_rules_follow_reference():
+    """pycuda_matrix_loader.py:61-76 (note the 2**31-1 threshold)."""
+    from linac_amd.hip_matrix_loader import loader_dtype
+    assert loader_dtype(2 ** 31 - 1) == numpy.uint32
+    assert loader_dtype(2 ** 31 + 11) == numpy.uint64
+    assert loader_dtype(7, _mod64=True) == numpy.uint64
+    assert loader_dtype(0, numpy.ones((2, 2))) == numpy.float64
+    assert loader_dtype(0, numpy.ones((2, 2)) * 1j) == numpy.complex128
