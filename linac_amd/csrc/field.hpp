@@ -20,32 +20,6 @@
 
 namespace linac {
 
-// Modular inverse of a (0 < a < p, p odd, gcd = 1) by the binary extended
-// Euclidean algorithm: only shifts, adds and compares, so the chain stays
-// short on a single lane.  Invariants: x1*a == u, x2*a == v (mod p).
-template <class U, class W>
-__host__ __device__ inline U binary_inverse(U a, U p) {
-    U u = a, v = p, x1 = 1, x2 = 0;
-    while (u != 1 && v != 1) {
-        while ((u & 1) == 0) {
-            u >>= 1;
-            x1 = (x1 & 1) ? (U)(((W)x1 + p) >> 1) : (U)(x1 >> 1);
-        }
-        while ((v & 1) == 0) {
-            v >>= 1;
-            x2 = (x2 & 1) ? (U)(((W)x2 + p) >> 1) : (U)(x2 >> 1);
-        }
-        if (u >= v) {
-            u -= v;
-            x1 = x1 >= x2 ? x1 - x2 : (U)(x1 + (p - x2));
-        } else {
-            v -= u;
-            x2 = x2 >= x1 ? x2 - x1 : (U)(x2 + (p - x1));
-        }
-    }
-    return u == 1 ? x1 : x2;
-}
-
 struct FieldU32 {
     using T = uint32_t;
     uint32_t p;
@@ -63,14 +37,35 @@ struct FieldU32 {
         return a >= b ? a - b : a + (p - b);
     }
     __device__ __forceinline__ uint32_t to_mont(uint32_t a) const { return mont(a, r2); }
+    // h^{-1} = h^(p-2) (Fermat), square-and-multiply in the Montgomery domain:
+    // a fixed, branch-uniform sequence of about 2 log2(p) products.
+    __device__ __forceinline__ uint32_t inverse(uint32_t h) const {
+        const uint32_t e = p - 2u;
+        const uint32_t hm = to_mont(h);
+        uint32_t r = to_mont(1u);
+        for (int bit = 31 - __clz(e); bit >= 0; --bit) {
+            r = mont(r, r);
+            if ((e >> bit) & 1u) r = mont(r, hm);
+        }
+        return mont(r, 1u);
+    }
     // Normal-form multiplier X with mont(a, X) = a * h^{-1} * R (the
     // Montgomery form of the elimination factor a / h).
     __device__ __forceinline__ uint32_t factor_of_head(uint32_t h) const {
-        const uint32_t inv = binary_inverse<uint32_t, uint64_t>(h, p);
-        return to_mont(to_mont(inv));
+        return to_mont(to_mont(inverse(h)));
     }
-    __device__ __forceinline__ uint32_t inverse(uint32_t h) const {
-        return binary_inverse<uint32_t, uint64_t>(h, p);
+    // (h x - a y) R^{-1} mod p for canonical h, x, a, y: a fraction-free row
+    // update with a constant extra factor R^{-1}.  For p < 2^31 the two
+    // products share one reduction: h x + (p - a) y < 2 p^2 < p 2^32.
+    __device__ __forceinline__ uint32_t ff_update(uint32_t h, uint32_t x, uint32_t a, uint32_t y) const {
+        if (p < 0x80000000u) {
+            const uint64_t t = (uint64_t)h * x + (uint64_t)(p - a) * y;
+            const uint32_t t_lo = (uint32_t)t, t_hi = (uint32_t)(t >> 32);
+            const uint32_t m = t_lo * pneg;
+            const uint32_t u = t_hi + __umulhi(m, p) + (t_lo != 0u);
+            return u >= p ? u - p : u;
+        }
+        return sub(mont(h, x), mont(a, y));
     }
     // a * b mod p for two normal-form operands.
     __device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t b) const { return mont(to_mont(a), b); }
@@ -94,13 +89,17 @@ struct FieldU64 {
         return a >= b ? a - b : a + (p - b);
     }
     __device__ __forceinline__ uint64_t to_mont(uint64_t a) const { return mont(a, r2); }
-    __device__ __forceinline__ uint64_t factor_of_head(uint64_t h) const {
-        const uint64_t inv = binary_inverse<uint64_t, uint64_t>(h, p);  // p < 2^63: x + p cannot wrap
-        return to_mont(to_mont(inv));
-    }
     __device__ __forceinline__ uint64_t inverse(uint64_t h) const {
-        return binary_inverse<uint64_t, uint64_t>(h, p);  // p < 2^63: x + p cannot wrap
+        const uint64_t e = p - 2ull;
+        const uint64_t hm = to_mont(h);
+        uint64_t r = to_mont(1ull);
+        for (int bit = 63 - __clzll(e); bit >= 0; --bit) {
+            r = mont(r, r);
+            if ((e >> bit) & 1ull) r = mont(r, hm);
+        }
+        return mont(r, 1ull);
     }
+    __device__ __forceinline__ uint64_t factor_of_head(uint64_t h) const { return to_mont(to_mont(inverse(h))); }
     __device__ __forceinline__ uint64_t mul(uint64_t a, uint64_t b) const { return mont(to_mont(a), b); }
     __device__ __forceinline__ uint64_t reduce(uint64_t a) const { return a >= p ? a % p : a; }
 };

@@ -35,6 +35,8 @@
 // Pivot rows come out normalised.  Replaces the column loop of
 // linac/pycuda_row_reduce.py:94-131 for large GF(p) matrices.
 #include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <type_traits>
@@ -60,6 +62,7 @@ struct BlockState {
     int64_t R[CI];       // pivot rows of the last inner step (pivot order)
     int32_t Cp[CI];      // their columns relative to the inner panel start
     uint32_t apinv[CI * CI];  // Montgomery form of (X[R, c0 + Cp])^{-1}
+    unsigned long long tphase[8];  // accumulated wall-clock ticks per phase of k_inner_factor
 };
 
 // ---------------------------------------------------------------------------
@@ -91,7 +94,9 @@ __global__ __launch_bounds__(256) void k_epoch_init(int32_t* epoch, int64_t N) {
         epoch[r] = -1;
 }
 
-__global__ void k_block_state_init(BlockState* bs, int bcur) {
+__global__ void k_block_state_init(BlockState* bs, int bcur, int reset_timers) {
+    if (reset_timers)
+        for (int i = 0; i < 8; ++i) bs->tphase[i] = 0;
     bs->k = 0;
     bs->q0 = 0;
     bs->c0_next = 0;
@@ -99,30 +104,23 @@ __global__ void k_block_state_init(BlockState* bs, int bcur) {
     bs->bcur = bcur;
 }
 
-// flags[r] = row r is unused and has a non-zero entry in the next inner panel
-__global__ __launch_bounds__(256) void k_row_flags(const uint32_t* __restrict__ X, int64_t N,
-                                                   const BlockState* __restrict__ bs,
-                                                   const uint8_t* __restrict__ used, uint8_t* __restrict__ flags) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c0 = bs->c0_next;
-    if (r >= N || c0 >= bs->bcur) return;
-    const int w = min(BI, bs->bcur - c0);
-    uint8_t f = 0;
-    if (!used[r]) {
-        const uint32_t* row = X + r * XLD + c0;
-        uint32_t acc = 0;
-        for (int j = 0; j < w; ++j) acc |= row[j];
-        f = acc != 0u;
-    }
-    flags[r] = f;
-}
-
 // ---------------------------------------------------------------------------
 // inner factorisation: one workgroup
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_inner_factor(uint32_t* __restrict__ X, int64_t N,
-                                                      const uint8_t* __restrict__ flags, uint8_t* __restrict__ used,
+constexpr int IF_T = 1024;  // threads of the inner factorisation: 16 waves, two candidate rows each
+static_assert(IF_T / 64 * 2 == CI, "each wave owns two candidate rows");
+static_assert(BI + CI == 64, "one augmented column per lane");
+
+// One workgroup.  Candidate rows live in LDS as [panel columns | identity];
+// wave w owns rows 2w and 2w+1.  Per column every wave ballots the column
+// itself (all waves see the same data, so they agree on the pivot without a
+// barrier), then updates its own rows with lane j handling column j.  Within
+// a wave the reads of a row precede its writes in program order, so one
+// barrier per column suffices.  No rows are swapped: pivots are recorded by
+// their LDS row, and the identity columns index LDS rows directly.
+__global__ __launch_bounds__(IF_T) void k_inner_factor(uint32_t* __restrict__ X, int64_t N,
+                                                      uint8_t* __restrict__ used, int32_t* __restrict__ slot_of,
                                                       int32_t* __restrict__ epoch_of, int epoch,
                                                       int64_t colbase, FieldU32 f, ElimState* st,
                                                       int64_t* __restrict__ piv_rows, int64_t* __restrict__ piv_cols,
@@ -130,13 +128,12 @@ __global__ __launch_bounds__(256) void k_inner_factor(uint32_t* __restrict__ X, 
     constexpr int W2 = BI + CI;
     __shared__ uint32_t aug[CI][W2 + 1];
     __shared__ int64_t cand[CI + 1];
-    __shared__ int orig[CI];
-    __shared__ int pcol[CI];
-    __shared__ uint32_t am[CI];
-    __shared__ uint32_t invm[CI];
-    __shared__ int s_total, s_piv;
-    __shared__ int s_wcnt[4];
+    __shared__ int prow_of[CI];   // LDS row of the t-th pivot
+    __shared__ int pcol[CI];      // its column within the inner panel
+    __shared__ int s_total;
+    __shared__ int s_wcnt[IF_T / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const unsigned long long t0 = wall_clock64();
     const int c0 = bs->c0_next;
     const int bcur = bs->bcur;
     if (c0 >= bcur) {  // every inner panel of this outer panel is complete
@@ -148,9 +145,23 @@ __global__ __launch_bounds__(256) void k_inner_factor(uint32_t* __restrict__ X, 
     if (tid == 0) s_total = 0;
     __syncthreads();
     // 1. the first CI (+1 to know whether more exist) candidate rows, in row order
-    for (int64_t base = st->first_unused; base < N; base += 256) {
+    for (int64_t base = st->first_unused; base < N; base += IF_T) {
         const int64_t r = base + tid;
-        const bool fl = r < N && flags[r];
+        bool fl = false;
+        if (r < N && !used[r]) {  // unused row with a non-zero entry in the inner panel
+            const uint4* row = reinterpret_cast<const uint4*>(X + r * XLD + c0);  // 128-byte aligned
+            uint4 v[BI / 4];
+#pragma unroll
+            for (int q = 0; q < BI / 4; ++q) v[q] = row[q];
+            uint32_t acc = 0;
+#pragma unroll
+            for (int q = 0; q < BI / 4; ++q) {
+                const int j = 4 * q;
+                acc |= (j + 0 < w ? v[q].x : 0u) | (j + 1 < w ? v[q].y : 0u) | (j + 2 < w ? v[q].z : 0u) |
+                       (j + 3 < w ? v[q].w : 0u);
+            }
+            fl = acc != 0u;
+        }
         const unsigned long long m = __ballot(fl);
         if (lane == 0) s_wcnt[wid] = __popcll(m);
         __syncthreads();
@@ -159,10 +170,15 @@ __global__ __launch_bounds__(256) void k_inner_factor(uint32_t* __restrict__ X, 
         const int pos = before + __popcll(m & ((1ull << lane) - 1ull));
         if (fl && pos <= CI) cand[pos] = r;
         __syncthreads();
-        if (tid == 0) s_total += s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+        if (tid == 0) {
+            int sum = 0;
+            for (int q = 0; q < IF_T / 64; ++q) sum += s_wcnt[q];
+            s_total += sum;
+        }
         __syncthreads();
         if (s_total > CI) break;
     }
+    const unsigned long long t1 = wall_clock64();
     const int total = s_total;
     const int nC = total < CI ? total : CI;
     const int more = total > CI ? 1 : 0;
@@ -175,96 +191,121 @@ __global__ __launch_bounds__(256) void k_inner_factor(uint32_t* __restrict__ X, 
         return;
     }
 
-    // 2. [X[S, c0:c0+w] | I] in LDS
-    for (int x = tid; x < nC * W2; x += 256) {
-        const int i = x / W2, j = x - i * W2;
-        uint32_t v;
-        if (j < w) v = X[cand[i] * XLD + c0 + j];
-        else if (j < BI) v = 0u;
-        else v = (j - BI == i) ? 1u : 0u;
-        aug[i][j] = v;
+    // 2. [X[S, c0:c0+w] | I] in LDS; rows >= nC are zero and never pivot
+    {
+        const int i = tid >> 6, j = lane;  // two passes of 16 rows
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int ii = i + 16 * half;
+            uint32_t v = 0u;
+            if (ii < nC) {
+                if (j < w) v = X[cand[ii] * XLD + c0 + j];
+                else if (j >= BI) v = (j - BI == ii) ? 1u : 0u;
+            }
+            aug[ii][j] = v;
+        }
     }
-    if (tid < nC) orig[tid] = tid;
     __syncthreads();
 
-    // 3. fraction-free Gauss-Jordan: row_i <- h * row_i - a_i * row_t.
-    //    Rows stay scalar multiples of the exact Gauss-Jordan rows; pivot rows
-    //    are normalised once at the end, so no inverse sits on the chain.
+    const unsigned long long t2 = wall_clock64();
+    // 3. fraction-free Gauss-Jordan: row_i <- (h * row_i - a_i * row_pv) R^{-1}
+    //    (rows stay non-zero multiples of the exact Gauss-Jordan rows)
+    unsigned long long pmask = 0ull;  // LDS rows already used as pivots (same in every wave)
     int t = 0;
+    const int r0 = 2 * wid, r1 = 2 * wid + 1;
     for (int c = 0; c < w && t < nC; ++c) {
-        if (wid == 0) {
-            const bool nz = lane >= t && lane < nC && aug[lane < CI ? lane : 0][c] != 0u;
-            const unsigned long long m = __ballot(nz);
-            if (lane == 0) s_piv = m ? (__ffsll((long long)m) - 1) : -1;
-        }
-        __syncthreads();
-        const int pv = s_piv;
-        if (pv < 0) {
-            __syncthreads();
-            continue;
-        }
-        if (pv != t) {
-            for (int j = tid; j < W2; j += 256) {
-                const uint32_t a = aug[t][j];
-                aug[t][j] = aug[pv][j];
-                aug[pv][j] = a;
-            }
-            if (tid == 0) {
-                const int o = orig[t];
-                orig[t] = orig[pv];
-                orig[pv] = o;
+        const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != 0u;
+        const unsigned long long m = __ballot(nz);
+        if (m == 0ull) continue;  // no pivot in this column (every wave sees the same)
+        const int pv = __ffsll((long long)m) - 1;
+        const uint32_t h = aug[pv][c];
+        const uint32_t y = aug[pv][lane];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int r = q ? r1 : r0;
+            if (r < nC && r != pv) {
+                const uint32_t a = aug[r][c];
+                const uint32_t x = aug[r][lane];
+                aug[r][lane] = f.ff_update(h, x, a, y);
             }
         }
-        __syncthreads();
-        if (tid < nC) am[tid] = f.to_mont(aug[tid][c]);
-        __syncthreads();
-        const uint32_t hm = am[t];
-        for (int x = tid; x < nC * W2; x += 256) {
-            const int i = x / W2, j = x - i * W2;
-            if (i != t) aug[i][j] = f.sub(f.mont(hm, aug[i][j]), f.mont(am[i], aug[t][j]));
+        if (tid == 0) {
+            prow_of[t] = pv;
+            pcol[t] = c;
         }
-        if (tid == 0) pcol[t] = c;
+        pmask |= 1ull << pv;
         ++t;
         __syncthreads();
     }
     const int k = t;
-
-    // 4. normalise the k pivot rows (k independent inverses in parallel)
-    if (tid < k) invm[tid] = f.to_mont(f.inverse(aug[tid][pcol[tid]]));
     __syncthreads();
-    for (int x = tid; x < k * W2; x += 256) {
-        const int i = x / W2, j = x - i * W2;
-        aug[i][j] = f.mont(invm[i], aug[i][j]);
+    const unsigned long long t3 = wall_clock64();
+
+    // 4. normalise the pivot rows.  The k inverses run side by side in the
+    //    lanes of wave 0 (one Fermat chain in total instead of one per wave,
+    //    which would serialise on the SIMDs), then each wave scales its rows.
+    __shared__ uint32_t s_invm[CI];
+    __shared__ int s_tof[CI];
+    if (wid == 0 && lane < k) {
+        const int r = prow_of[lane];
+        s_invm[lane] = f.to_mont(f.inverse(aug[r][pcol[lane]]));
+        s_tof[r] = lane;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = q ? r1 : r0;
+        if ((pmask >> r) & 1ull) aug[r][lane] = f.mont(s_invm[s_tof[r]], aug[r][lane]);
     }
     __syncthreads();
 
+    const unsigned long long t4 = wall_clock64();
     // 5. outputs: Apiv^{-1} (Montgomery form), pivot rows/columns, bookkeeping
     const int64_t rank0 = st->rank;
     const int q0 = bs->q0;
-    for (int x = tid; x < k * k; x += 256) {
+    for (int x = tid; x < k * k; x += IF_T) {
         const int a = x / k, b = x - a * k;
-        bs->apinv[a * CI + b] = f.to_mont(aug[a][BI + orig[b]]);
+        bs->apinv[a * CI + b] = f.to_mont(aug[prow_of[a]][BI + prow_of[b]]);
     }
     if (tid < k) {
-        const int64_t r = cand[orig[tid]];
+        const int64_t r = cand[prow_of[tid]];
         bs->R[tid] = r;
         bs->Cp[tid] = pcol[tid];
         used[r] = 1;
         epoch_of[r] = epoch;
+        slot_of[r] = tid;
         piv_rows[rank0 + tid] = r;
         piv_cols[rank0 + tid] = colbase + c0 + pcol[tid];
         X[r * XLD + BO + q0 + tid] = 1u;  // Z[r, slot] = 1
     }
     __syncthreads();
+    // advance first_unused past the rows that are now pivots, 1024 rows per probe
+    __shared__ unsigned long long s_first;
+    if (tid == 0) s_first = ~0ull;
+    __syncthreads();
+    for (int64_t base = st->first_unused; base < N; base += IF_T) {
+        const int64_t r = base + tid;
+        const unsigned long long m = __ballot(r < N && !used[r]);
+        if (lane == 0 && m) atomicMin(&s_first, (unsigned long long)(base + wid * 64 + __ffsll((long long)m) - 1));
+        __syncthreads();
+        const bool found = s_first != ~0ull;
+        __syncthreads();
+        if (found) break;
+    }
     if (tid == 0) {
         st->rank = rank0 + k;
         bs->k = k;
         bs->q0 = q0 + k;
         bs->c0_cur = c0;
         bs->c0_next = (more && k < w) ? c0 : c0 + BI;
-        int64_t q = st->first_unused;
-        while (q < N && used[q]) ++q;
-        st->first_unused = q;
+        st->first_unused = s_first != ~0ull ? (int64_t)s_first : N;
+        const unsigned long long t5 = wall_clock64();
+        bs->tphase[0] += t1 - t0;
+        bs->tphase[1] += t2 - t1;
+        bs->tphase[2] += t3 - t2;
+        bs->tphase[3] += t4 - t3;
+        bs->tphase[4] += t5 - t4;
+        bs->tphase[5] += 1;
     }
 }
 
@@ -453,12 +494,399 @@ __global__ __launch_bounds__(256) void k_gemm_modp(uint32_t* __restrict__ C, int
 }
 
 // ---------------------------------------------------------------------------
+// the modular GEMM, second form: 512 threads, 128 x 64 tile per workgroup
+// (8 waves of 32 x 32), operands staged through LDS in 64-deep K stages with
+// double buffering, and a tile order that keeps each XCD's L2 on a small
+// working set: consecutive workgroup ids go to different XCDs, so the tile
+// list is first cut into 8 contiguous ranges (one per XCD), and each range is
+// walked in groups of 8 tile-rows so that a B tile is reused by 8
+// consecutive tiles while the 8 A tiles stay resident.
+// ---------------------------------------------------------------------------
+
+constexpr int GB_M = 128, GB_N = 64, GB_K = 64, GB_T = 512;
+constexpr int GB_ROW = GB_K + 16;  // LDS bytes per staged row (padded against bank conflicts)
+constexpr int GB_GROUP = 8;
+constexpr int NUM_XCD = 8;
+
+__device__ __forceinline__ bool gemm_tile_of(int64_t mt, int64_t nt, int64_t* tm, int64_t* tn) {
+    const int64_t total = mt * nt;
+    const int64_t per_xcd = (total + NUM_XCD - 1) / NUM_XCD;
+    const int64_t bid = blockIdx.x;
+    const int64_t t = (bid % NUM_XCD) * per_xcd + bid / NUM_XCD;
+    if (t >= total) return false;
+    const int64_t gsz = (int64_t)GB_GROUP * nt;
+    const int64_t g = t / gsz, tt = t - g * gsz;
+    const int64_t gm = min((int64_t)GB_GROUP, mt - g * GB_GROUP);
+    *tm = g * GB_GROUP + tt % gm;
+    *tn = tt / gm;
+    return true;
+}
+
+inline unsigned gemm2_grid(int64_t Mrows, int64_t Ncols) {
+    const int64_t total = ceil_div(Mrows, GB_M) * ceil_div(Ncols, GB_N);
+    return (unsigned)(ceil_div(total, NUM_XCD) * NUM_XCD);
+}
+
+template <bool CZERO>
+__global__ __launch_bounds__(GB_T) void k_gemm_modp2(uint32_t* __restrict__ C, int64_t ldc, int64_t Mrows,
+                                                     int64_t Ncols, const int8_t* __restrict__ L, int64_t planeL,
+                                                     const int8_t* __restrict__ B, int64_t planeB, int Kpad,
+                                                     double pd, double pinv, const int32_t* kptr,
+                                                     const int32_t* lo_ptr, const int32_t* hi_ptr, int hi_off) {
+    __shared__ __attribute__((aligned(16))) int8_t sA[2][4][GB_M * GB_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[2][4][GB_N * GB_ROW];
+    if (kptr && *kptr == 0) return;
+    int64_t tm, tn;
+    if (!gemm_tile_of(ceil_div(Mrows, GB_M), ceil_div(Ncols, GB_N), &tm, &tn)) return;
+    const int64_t m_base = tm * GB_M, n_base = tn * GB_N;
+    if (lo_ptr && n_base + GB_N <= *lo_ptr) return;
+    if (hi_ptr && n_base >= *hi_ptr + hi_off) return;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int r16 = lane & 15, g = lane >> 4;
+    const v4i zero = {0, 0, 0, 0};
+
+    // stage copy: A = 4 limbs x 128 rows x 4 sixteen-byte quarters (4 per thread),
+    //             B = 4 limbs x 64 cols x 4 quarters (2 per thread)
+    v4i ra[4], rb[2];
+    auto gload = [&](int kc) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = tid + q * GB_T;
+            const int l = c >> 9, rem = c & 511, row = rem >> 2, qu = rem & 3;
+            const int64_t grow = m_base + row;
+            ra[q] = grow < Mrows ? *reinterpret_cast<const v4i*>(L + l * planeL + grow * Kpad + kc + qu * 16) : zero;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int c = tid + q * GB_T;
+            const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
+            const int64_t gcol = n_base + col;
+            rb[q] = gcol < Ncols ? *reinterpret_cast<const v4i*>(B + l * planeB + gcol * Kpad + kc + qu * 16) : zero;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = tid + q * GB_T;
+            const int l = c >> 9, rem = c & 511, row = rem >> 2, qu = rem & 3;
+            *reinterpret_cast<v4i*>(&sA[buf][l][row * GB_ROW + qu * 16]) = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int c = tid + q * GB_T;
+            const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
+            *reinterpret_cast<v4i*>(&sB[buf][l][col * GB_ROW + qu * 16]) = rb[q];
+        }
+    };
+
+    v4i acc[7][2][2];
+#pragma unroll
+    for (int s = 0; s < 7; ++s)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
+
+    const int nk = Kpad / GB_K;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int kk = 0; kk < nk; ++kk) {
+        const int buf = kk & 1;
+        if (kk + 1 < nk) gload((kk + 1) * GB_K);
+        v4i bf[4][2];
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+                bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[buf][l][(wn * 32 + ni * 16 + r16) * GB_ROW + 16 * g]);
+#pragma unroll
+        for (int l1 = 0; l1 < 4; ++l1) {
+            v4i af[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+                af[mi] = *reinterpret_cast<const v4i*>(&sA[buf][l1][(wm * 32 + mi * 16 + r16) * GB_ROW + 16 * g]);
+#pragma unroll
+            for (int l2 = 0; l2 < 4; ++l2)
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < 2; ++ni)
+                        acc[l1 + l2][mi][ni] =
+                            __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
+        }
+        if (kk + 1 < nk) lstore(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
+                const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+                if (row < Mrows && col < Ncols) {
+                    int32_t s[7];
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) s[q] = acc[q][mi][ni][i];
+                    uint32_t* cp = C + row * ldc + col;
+                    *cp = fold_sub(CZERO ? 0u : *cp, s, pd, pinv);
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// third form (used): 256 threads, 64 x 64 tile (4 waves of 32 x 32), 64 KiB of
+// double-buffered LDS so that two independent workgroups share a CU and one's
+// prologue / epilogue overlaps the other's MFMA work; the C tile is fetched
+// before the K loop so its latency hides behind the MFMAs.  Same XCD-aware
+// grouped tile order as the second form.
+// ---------------------------------------------------------------------------
+
+constexpr int G3_M = 64, G3_N = 64, G3_T = 256;
+
+inline unsigned gemm3_grid(int64_t Mrows, int64_t Ncols) {
+    const int64_t total = ceil_div(Mrows, G3_M) * ceil_div(Ncols, G3_N);
+    return (unsigned)(ceil_div(total, NUM_XCD) * NUM_XCD);
+}
+
+template <bool CZERO>
+__global__ __launch_bounds__(G3_T, 2) void k_gemm_modp3(uint32_t* __restrict__ C, int64_t ldc, int64_t Mrows,
+                                                        int64_t Ncols, const int8_t* __restrict__ L,
+                                                        int64_t planeL, const int8_t* __restrict__ B,
+                                                        int64_t planeB, int Kpad, double pd, double pinv,
+                                                        const int32_t* kptr, const int32_t* lo_ptr,
+                                                        const int32_t* hi_ptr, int hi_off) {
+    __shared__ __attribute__((aligned(16))) int8_t sA[2][4][G3_M * GB_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[2][4][G3_N * GB_ROW];
+    if (kptr && *kptr == 0) return;
+    int64_t tm, tn;
+    if (!gemm_tile_of(ceil_div(Mrows, G3_M), ceil_div(Ncols, G3_N), &tm, &tn)) return;
+    const int64_t m_base = tm * G3_M, n_base = tn * G3_N;
+    if (lo_ptr && n_base + G3_N <= *lo_ptr) return;
+    if (hi_ptr && n_base >= *hi_ptr + hi_off) return;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int r16 = lane & 15, g = lane >> 4;
+    const v4i zero = {0, 0, 0, 0};
+
+    // C tile values of this lane: fetched while the last K stage runs on the
+    // matrix cores (the stage registers are free by then)
+    uint32_t cpre[2][2][4];
+    auto cload = [&]() {
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
+                    const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+                    cpre[mi][ni][i] = (!CZERO && row < Mrows && col < Ncols) ? C[row * ldc + col] : 0u;
+                }
+    };
+
+    // stage copy: A and B = 4 limbs x 64 rows x 4 sixteen-byte quarters (4 each per thread)
+    v4i ra[4], rb[4];
+    auto gload = [&](int kc) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = tid + q * G3_T;
+            const int l = c >> 8, rem = c & 255, row = rem >> 2, qu = rem & 3;
+            const int64_t grow = m_base + row, gcol = n_base + row;
+            ra[q] = grow < Mrows ? *reinterpret_cast<const v4i*>(L + l * planeL + grow * Kpad + kc + qu * 16) : zero;
+            rb[q] = gcol < Ncols ? *reinterpret_cast<const v4i*>(B + l * planeB + gcol * Kpad + kc + qu * 16) : zero;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = tid + q * G3_T;
+            const int l = c >> 8, rem = c & 255, row = rem >> 2, qu = rem & 3;
+            *reinterpret_cast<v4i*>(&sA[buf][l][row * GB_ROW + qu * 16]) = ra[q];
+            *reinterpret_cast<v4i*>(&sB[buf][l][row * GB_ROW + qu * 16]) = rb[q];
+        }
+    };
+
+    v4i acc[7][2][2];
+#pragma unroll
+    for (int s = 0; s < 7; ++s)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
+
+    const int nk = Kpad / GB_K;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int kk = 0; kk < nk; ++kk) {
+        const int buf = kk & 1;
+        if (kk + 1 < nk) gload((kk + 1) * GB_K);
+        else cload();
+        v4i bf[4][2];
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+                bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[buf][l][(wn * 32 + ni * 16 + r16) * GB_ROW + 16 * g]);
+#pragma unroll
+        for (int l1 = 0; l1 < 4; ++l1) {
+            v4i af[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+                af[mi] = *reinterpret_cast<const v4i*>(&sA[buf][l1][(wm * 32 + mi * 16 + r16) * GB_ROW + 16 * g]);
+#pragma unroll
+            for (int l2 = 0; l2 < 4; ++l2)
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < 2; ++ni)
+                        acc[l1 + l2][mi][ni] =
+                            __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
+        }
+        if (kk + 1 < nk) lstore(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
+                const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+                if (row < Mrows && col < Ncols) {
+                    int32_t s[7];
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) s[q] = acc[q][mi][ni][i];
+                    C[row * ldc + col] = fold_sub(cpre[mi][ni][i], s, pd, pinv);
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// inner-step GEMM on the panel buffer:  X[r, :] -= X[r, c0 + Cp] U  for rows
+// that are not pivots of this step, X[r, :] = U[slot(r), :] for those that
+// are.  K = 64 (k <= 32 pivots, zero padded), one LDS stage.  The A operand's
+// limbs are built from X on the fly (no separate plane buffer), B is the limb
+// planes of U written by k_make_u.  Tiles outside the live column window
+// [c0, BO + q0) only meet zeros and are skipped.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(G3_T, 2) void k_gemm_inner(uint32_t* __restrict__ X, int64_t N,
+                                                        const int8_t* __restrict__ Bpl, int64_t planeB,
+                                                        const uint32_t* __restrict__ U,
+                                                        const BlockState* __restrict__ bs,
+                                                        const int32_t* __restrict__ epoch_of, int epoch,
+                                                        const int32_t* __restrict__ slot_of, uint32_t p, double pd,
+                                                        double pinv) {
+    __shared__ __attribute__((aligned(16))) int8_t sA[4][G3_M * GB_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[4][G3_N * GB_ROW];
+    __shared__ int s_cp[CI];
+    const int k = bs->k;
+    if (k == 0) return;
+    int64_t tm, tn;
+    if (!gemm_tile_of(ceil_div(N, G3_M), XLD / G3_N, &tm, &tn)) return;
+    const int64_t m_base = tm * G3_M, n_base = tn * G3_N;
+    const int c0 = bs->c0_cur;
+    if (n_base + G3_N <= c0 || n_base >= BO + bs->q0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int r16 = lane & 15, g = lane >> 4;
+    if (tid < k) s_cp[tid] = bs->Cp[tid];
+    __syncthreads();
+    // A: thread -> (row = tid / 4, 16 consecutive k), values X[row, c0 + Cp[t]]
+    {
+        const int rl = tid >> 2, kq = (tid & 3) * 16;
+        const int64_t row = m_base + rl;
+        int8_t dg[4][16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int t = kq + i;
+            int8_t d[4] = {0, 0, 0, 0};
+            if (t < k && row < N) balanced_digits(centred(X[row * XLD + c0 + s_cp[t]], p), d);
+#pragma unroll
+            for (int l = 0; l < 4; ++l) dg[l][i] = d[l];
+        }
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            v4i w;
+            __builtin_memcpy(&w, dg[l], 16);
+            *reinterpret_cast<v4i*>(&sA[l][rl * GB_ROW + kq]) = w;
+        }
+    }
+    // B: copy the planes of U for this tile's 64 columns
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = tid + q * G3_T;
+        const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
+        *reinterpret_cast<v4i*>(&sB[l][col * GB_ROW + qu * 16]) =
+            *reinterpret_cast<const v4i*>(Bpl + l * planeB + (n_base + col) * KPI + qu * 16);
+    }
+    __syncthreads();
+    const v4i zero = {0, 0, 0, 0};
+    v4i acc[7][2][2];
+#pragma unroll
+    for (int s = 0; s < 7; ++s)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
+    v4i bf[4][2];
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][(wn * 32 + ni * 16 + r16) * GB_ROW + 16 * g]);
+#pragma unroll
+    for (int l1 = 0; l1 < 4; ++l1) {
+        v4i af[2];
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+            af[mi] = *reinterpret_cast<const v4i*>(&sA[l1][(wm * 32 + mi * 16 + r16) * GB_ROW + 16 * g]);
+#pragma unroll
+        for (int l2 = 0; l2 < 4; ++l2)
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni)
+                    acc[l1 + l2][mi][ni] =
+                        __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
+    }
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
+            if (row >= N) continue;
+            const bool pivot_row = epoch_of[row] == epoch;
+            const int slot = pivot_row ? slot_of[row] : 0;
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+                uint32_t* cp = X + row * XLD + col;
+                if (pivot_row) {
+                    *cp = U[slot * XLD + col];
+                } else {
+                    int32_t sv[7];
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) sv[q] = acc[q][mi][ni][i];
+                    *cp = fold_sub(*cp, sv, pd, pinv);
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
 
 struct BlockedBuffers {
     uint32_t* X;
-    uint8_t* flags;
+    int32_t* slot_of;   // pivot slot of a row within its inner step
     int32_t* epoch_of;
     BlockState* bs;
     int8_t* planesL;  // [4][N][256]
@@ -491,11 +919,11 @@ static int blocked_alloc(DeviceContext* ctx, int64_t N, int64_t M, BlockedBuffer
     b->planesB = (int8_t*)p;
     b->U = (uint32_t*)((char*)p + bytesB);
     b->Gq = (uint32_t*)((char*)p + bytesB + bytesU);
-    const size_t aux = (size_t)round_up(N, 256) + (size_t)round_up(4 * N, 256) + sizeof(BlockState) + 256;
+    const size_t aux = 2 * (size_t)round_up(4 * N, 256) + sizeof(BlockState) + 256;
     LINAC_TRY(ctx->get(SLOT_AUX, aux, &p));
-    b->flags = (uint8_t*)p;
-    b->epoch_of = (int32_t*)((char*)p + round_up(N, 256));
-    b->bs = (BlockState*)((char*)p + round_up(N, 256) + round_up(4 * N, 256));
+    b->slot_of = (int32_t*)p;
+    b->epoch_of = (int32_t*)((char*)p + round_up(4 * N, 256));
+    b->bs = (BlockState*)((char*)p + 2 * round_up(4 * N, 256));
     return LINAC_OK;
 }
 
@@ -508,17 +936,11 @@ static unsigned grid1(int64_t n, int64_t per = 256, int64_t cap = 65536) {
 // complete does nothing.
 static int inner_group(const BlockedBuffers& b, int64_t N, int epoch, int64_t J, const FieldU32& f, double pd,
                        double pinv, hipStream_t s, const ElimWorkspace& ws) {
-    k_row_flags<<<grid1(N, 256, 1 << 30), 256, 0, s>>>(b.X, N, b.bs, ws.used, b.flags);
-    k_inner_factor<<<1, 256, 0, s>>>(b.X, N, b.flags, ws.used, b.epoch_of, epoch, J, f, ws.st, ws.piv_rows,
+    k_inner_factor<<<1, IF_T, 0, s>>>(b.X, N, ws.used, b.slot_of, b.epoch_of, epoch, J, f, ws.st, ws.piv_rows,
                                      ws.piv_cols, b.bs);
     k_make_u<<<XLD / 4, dim3(64, 4), 0, s>>>(b.X, f, b.bs, b.U, b.planesB, b.strideB);
-    k_make_lhs<<<grid1(N * KPI, 256, 1 << 30), 256, 0, s>>>(b.X, XLD, N, nullptr, 0, &b.bs->c0_cur, b.bs->Cp,
-                                                             &b.bs->k, 0, KPI, b.epoch_of, epoch, epoch + 1, 0, f.p,
-                                                             b.planesL, b.strideL);
-    dim3 gg(XLD / 64, (unsigned)ceil_div(N, 64));
-    k_gemm_modp<false><<<gg, 256, 0, s>>>(b.X, XLD, N, XLD, b.planesL, b.strideL, b.planesB, b.strideB, KPI, f.p,
-                                          pd, pinv, &b.bs->k, &b.bs->c0_cur, &b.bs->q0, BO);
-    k_scatter_rows<<<dim3(2, CI), 256, 0, s>>>(b.X, XLD, b.bs->R, &b.bs->k, 0, 0, XLD, b.U, XLD);
+    k_gemm_inner<<<(unsigned)(ceil_div(ceil_div(N, G3_M) * (XLD / G3_N), NUM_XCD) * NUM_XCD), G3_T, 0, s>>>(
+        b.X, N, b.planesB, b.strideB, b.U, b.bs, b.epoch_of, epoch, b.slot_of, f.p, pd, pinv);
     LINAC_HIP(hipGetLastError());
     return LINAC_OK;
 }
@@ -533,7 +955,7 @@ static int blocked_run_u32(uint32_t* A, int64_t ld, int64_t N, int64_t M, const 
     for (int64_t J = 0; J < M; J += BO) {
         const int Bcur = (int)std::min<int64_t>(BO, M - J);
         k_panel_load<<<grid1(N * XLD), 256, 0, s>>>(b.X, A, ld, N, J, Bcur);
-        k_block_state_init<<<1, 1, 0, s>>>(b.bs, Bcur);
+        k_block_state_init<<<1, 1, 0, s>>>(b.bs, Bcur, J == 0);
         const int e0 = epoch;
         const int ninner = (int)ceil_div(Bcur, BI);
         for (int i = 0; i < ninner; ++i) LINAC_TRY(inner_group(b, N, epoch++, J, f, pd, pinv, s, ws));
@@ -565,15 +987,15 @@ static int blocked_run_u32(uint32_t* A, int64_t ld, int64_t N, int64_t M, const 
                                                                               b.planesY, b.strideY);
             k_make_rhs<<<dim3((unsigned)ceil_div(nT, 64), (unsigned)(Kq / 64)), 256, 0, s>>>(
                 A, ld, Q, q0, T0, nT, Kq, f.p, b.planesB, b.strideB);
-            k_gemm_modp<true><<<dim3((unsigned)ceil_div(nT, 64), (unsigned)ceil_div(q0, 64)), 256, 0, s>>>(
-                b.Gq, b.ldg, q0, nT, b.planesY, b.strideY, b.planesB, b.strideB, Kq, f.p, pd, pinv, nullptr, nullptr,
-                nullptr, 0);
+            k_gemm_modp3<true><<<gemm3_grid(q0, nT), G3_T, 0, s>>>(b.Gq, b.ldg, q0, nT, b.planesY, b.strideY,
+                                                                   b.planesB, b.strideB, Kq, pd, pinv, nullptr,
+                                                                   nullptr, nullptr, 0);
             if (prof)
                 LINAC_TRY(prof->sample_begin(s, J, 8.0 * (double)N * (double)nT,
                                              (double)(N - q0) * (double)q0 * (double)nT));
-            k_gemm_modp<false><<<dim3((unsigned)ceil_div(nT, 64), (unsigned)ceil_div(N, 64)), 256, 0, s>>>(
-                A + T0, ld, N, nT, b.planesL, b.strideL, b.planesB, b.strideB, Kq, f.p, pd, pinv, nullptr, nullptr,
-                nullptr, 0);
+            k_gemm_modp3<false><<<gemm3_grid(N, nT), G3_T, 0, s>>>(A + T0, ld, N, nT, b.planesL, b.strideL,
+                                                                   b.planesB, b.strideB, Kq, pd, pinv, nullptr,
+                                                                   nullptr, nullptr, 0);
             if (prof) LINAC_TRY(prof->sample_end(s));
             k_scatter_rows<<<dim3(grid1(nT, 256, 64), (unsigned)q0), 256, 0, s>>>(A, ld, Q, nullptr, q0, T0, nT,
                                                                                 b.Gq, b.ldg);
@@ -582,6 +1004,15 @@ static int blocked_run_u32(uint32_t* A, int64_t ld, int64_t N, int64_t M, const 
         LINAC_HIP(hipGetLastError());
     }
     LINAC_HIP(hipStreamSynchronize(s));
+    if (getenv("LINAC_DEBUG_TIMING")) {
+        unsigned long long tp[8];
+        LINAC_HIP(hipMemcpy(tp, b.bs->tphase, sizeof(tp), hipMemcpyDeviceToHost));
+        int rate_khz = 0;
+        LINAC_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+        const double us = 1e3 / (double)rate_khz;
+        fprintf(stderr, "k_inner_factor phases over %llu calls (us/call): scan %.2f load %.2f gj %.2f norm %.2f out %.2f\n",
+                tp[5], tp[0] * us / tp[5], tp[1] * us / tp[5], tp[2] * us / tp[5], tp[3] * us / tp[5], tp[4] * us / tp[5]);
+    }
     return LINAC_OK;
 }
 
