@@ -87,6 +87,17 @@ int linac_rref_u32(uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t
 int linac_rref_u64(uint64_t* a, int64_t rows, int64_t cols, int64_t ld, uint64_t p,
                    uint32_t flags, void* stream, int64_t* pivots_out, int64_t* rank_out);
 
+/* ---- RREF over float64 / complex128 ----
+ * Replaces cuda_row_reduce(matrix, 0) (linac/pycuda_row_reduce.py:49-56,87-131;
+ * kernels linac/row_reduce.cu:201-270,287-342,404-418): rows are divided once by
+ * their max |x|, the pivot of a column is the unused row of largest |x|, and it
+ * is accepted iff |x| > tol (the reference uses 1e-9).  c128 arrays hold
+ * interleaved (re, im) doubles; ld counts complex elements. */
+int linac_rref_f64(double* a, int64_t rows, int64_t cols, int64_t ld, double tol, uint32_t flags, void* stream,
+                   int64_t* pivots_out, int64_t* rank_out);
+int linac_rref_c128(double* a, int64_t rows, int64_t cols, int64_t ld, double tol, uint32_t flags, void* stream,
+                    int64_t* pivots_out, int64_t* rank_out);
+
 /* ---- monomial-evaluation matrix loader ----
  * Replaces cuda_load_matrix(bases, lindices, shape, field_characteristic)
  * (linac/pycuda_matrix_loader.py:28-124; kernel linac/matrix_loader.cu:88-112):

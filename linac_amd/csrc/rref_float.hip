@@ -1,0 +1,278 @@
+// Gauss-Jordan RREF in float64 and complex128 (field_characteristic == 0).
+//
+// Replaces the REAL / complex branches of linac/row_reduce.cu (SetRowScales
+// and RescaleRows :201-270, ThreadsReduceToMaxIndex :287-342 with |x|,
+// CompareHeadToTolerance :404-418 with tolerance 1e-9, RescaleRow :117-136,
+// RowReduce :138-196) driven by linac/pycuda_row_reduce.py:87-131:
+//   1. every row is divided once by its largest |x| (rows that are all zero
+//      are left alone instead of turning into NaN),
+//   2. per column: the unused row with the largest |x| (first such row on
+//      ties) is the pivot if |x| > tol; every other row r gets
+//      A[r, c] -= (A[r, j] / h) A[piv, c] for c > j and A[r, j] = 0,
+//   3. pivot rows are divided by their head once at the end and gathered in
+//      column order (no row swaps during the sweep, as in the GF(p) path).
+// The arithmetic is float64 throughout (complex: pairs of float64).
+#include <algorithm>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace linac {
+
+struct CplxF {
+    double re, im;
+};
+
+template <class T>
+struct FloatOps;
+
+template <>
+struct FloatOps<double> {
+    __device__ static double abs(double x) { return fabs(x); }
+    __device__ static double div(double a, double b) { return a / b; }
+    __device__ static double fms(double x, double f, double y) { return fma(-f, y, x); }  // x - f*y
+    __device__ static double scale(double x, double s) { return x / s; }
+    __device__ static double zero() { return 0.0; }
+    __device__ static bool is_zero(double x) { return x == 0.0; }
+};
+
+template <>
+struct FloatOps<CplxF> {
+    __device__ static double abs(CplxF z) { return hypot(z.re, z.im); }
+    __device__ static CplxF div(CplxF a, CplxF b) {
+        // Smith's algorithm: no overflow for large |b|
+        if (fabs(b.re) >= fabs(b.im)) {
+            const double r = b.im / b.re, d = b.re + b.im * r;
+            return CplxF{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+        }
+        const double r = b.re / b.im, d = b.im + b.re * r;
+        return CplxF{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+    }
+    __device__ static CplxF fms(CplxF x, CplxF f, CplxF y) {
+        return CplxF{x.re - (f.re * y.re - f.im * y.im), x.im - (f.re * y.im + f.im * y.re)};
+    }
+    __device__ static CplxF scale(CplxF x, double s) { return CplxF{x.re / s, x.im / s}; }
+    __device__ static CplxF zero() { return CplxF{0.0, 0.0}; }
+    __device__ static bool is_zero(CplxF z) { return z.re == 0.0 && z.im == 0.0; }
+};
+
+struct FloatState {
+    int64_t prow;
+    int64_t rank;
+    CplxF factor_unused;
+    int64_t pad[4];
+};
+
+// Each row divided by its max |x| (one workgroup per row).
+template <class T>
+__global__ __launch_bounds__(256) void k_row_scale(T* __restrict__ A, int64_t ld, int64_t N, int64_t M) {
+    __shared__ double s_max[256];
+    const int64_t r = blockIdx.x;
+    T* row = A + r * ld;
+    double m = 0.0;
+    for (int64_t c = threadIdx.x; c < M; c += 256) m = fmax(m, FloatOps<T>::abs(row[c]));
+    s_max[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + w]);
+        __syncthreads();
+    }
+    const double sc = s_max[0];
+    if (sc == 0.0) return;
+    for (int64_t c = threadIdx.x; c < M; c += 256) row[c] = FloatOps<T>::scale(row[c], sc);
+}
+
+__global__ void k_float_init(FloatState* st, uint8_t* used, int64_t N) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x == 0) {
+        st->prow = -1;
+        st->rank = 0;
+    }
+    for (int64_t r = x; r < N; r += (int64_t)gridDim.x * blockDim.x) used[r] = 0;
+}
+
+// Pivot of column j: argmax |A[r, j]| over unused rows (smallest row on ties),
+// accepted iff the maximum exceeds tol.
+template <class T>
+__global__ __launch_bounds__(1024) void k_float_select(const T* __restrict__ A, int64_t ld, int64_t N, int64_t j,
+                                                       double tol, FloatState* st, uint8_t* used,
+                                                       int64_t* piv_rows, int64_t* piv_cols) {
+    __shared__ double s_v[1024];
+    __shared__ int64_t s_r[1024];
+    double best = -1.0;
+    int64_t br = -1;
+    for (int64_t r = threadIdx.x; r < N; r += 1024) {
+        if (used[r]) continue;
+        const double v = FloatOps<T>::abs(A[r * ld + j]);
+        if (v > best) {
+            best = v;
+            br = r;
+        }
+    }
+    s_v[threadIdx.x] = best;
+    s_r[threadIdx.x] = br;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double v2 = s_v[threadIdx.x + w];
+            const int64_t r2 = s_r[threadIdx.x + w];
+            const double v1 = s_v[threadIdx.x];
+            const int64_t r1 = s_r[threadIdx.x];
+            if (v2 > v1 || (v2 == v1 && r2 >= 0 && (r1 < 0 || r2 < r1))) {
+                s_v[threadIdx.x] = v2;
+                s_r[threadIdx.x] = r2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (s_r[0] >= 0 && s_v[0] > tol) {
+            const int64_t r = s_r[0];
+            st->prow = r;
+            used[r] = 1;
+            const int64_t k = st->rank;
+            piv_rows[k] = r;
+            piv_cols[k] = j;
+            st->rank = k + 1;
+        } else {
+            st->prow = -1;
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_float_elim(T* __restrict__ A, int64_t ld, int64_t N, int64_t M, int64_t j,
+                                                    const FloatState* __restrict__ st, int log2G) {
+    const int64_t prow = st->prow;
+    if (prow < 0) return;
+    const int G = 1 << log2G;
+    const int64_t r = ((int64_t)blockIdx.x << (8 - log2G)) + (threadIdx.x >> log2G);
+    const int t = threadIdx.x & (G - 1);
+    if (r >= N || r == prow) return;
+    T* row = A + r * ld;
+    const T a = row[j];
+    if (FloatOps<T>::is_zero(a)) return;
+    const T* pr = A + prow * ld;
+    const T f = FloatOps<T>::div(a, pr[j]);
+    for (int64_t c = j + 1 + t; c < M; c += G) row[c] = FloatOps<T>::fms(row[c], f, pr[c]);
+    if (t == 0) row[j] = FloatOps<T>::zero();
+}
+
+// Row k of the result = k-th pivot row / its head; rows >= rank are zero.
+template <class T>
+__global__ __launch_bounds__(256) void k_float_finalize(const T* A, int64_t ld, T* out, int64_t N, int64_t M,
+                                                        const FloatState* __restrict__ st,
+                                                        const int64_t* __restrict__ piv_rows,
+                                                        const int64_t* __restrict__ piv_cols) {
+    const int64_t k = blockIdx.x;
+    T* o = out + k * ld;
+    if (k >= st->rank) {
+        for (int64_t c = threadIdx.x; c < M; c += 256) o[c] = FloatOps<T>::zero();
+        return;
+    }
+    const T* s = A + piv_rows[k] * ld;
+    const T h = s[piv_cols[k]];
+    for (int64_t c = threadIdx.x; c < M; c += 256) o[c] = FloatOps<T>::div(s[c], h);
+}
+
+template <class T>
+static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double tol, uint32_t flags,
+                            void* stream_arg, int64_t* pivots_out, int64_t* rank_out) {
+    if (rows < 0 || cols < 0) return fail(LINAC_ERR_INVALID, "negative shape");
+    if (ld < cols) return fail(LINAC_ERR_INVALID, "ld (%lld) < cols (%lld)", (long long)ld, (long long)cols);
+    if (!(tol >= 0.0)) return fail(LINAC_ERR_INVALID, "tolerance must be >= 0");
+    if (rank_out) *rank_out = 0;
+    if (rows == 0 || cols == 0) return LINAC_OK;
+    if (!a) return fail(LINAC_ERR_INVALID, "matrix pointer is NULL");
+    const bool on_device = (flags & LINAC_DEVICE_PTR) != 0;
+    DeviceContext* ctx = nullptr;
+    LINAC_TRY(current_context(&ctx));
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    hipStream_t s = stream_arg ? (hipStream_t)stream_arg : ctx->stream;
+    Profiler* prof = (flags & LINAC_PROFILE) ? &thread_profiler() : nullptr;
+
+    T* A = a;
+    int64_t lda = ld;
+    void* p = nullptr;
+    if (!on_device) {
+        lda = round_up(cols, 32);
+        LINAC_TRY(ctx->get(SLOT_MATRIX, (size_t)rows * lda * sizeof(T), &p));
+        A = (T*)p;
+        LINAC_HIP(hipMemcpy2DAsync(A, lda * sizeof(T), a, ld * sizeof(T), cols * sizeof(T), rows,
+                                   hipMemcpyHostToDevice, s));
+    }
+    const int64_t cap = std::max<int64_t>(1, std::min(rows, cols));
+    LINAC_TRY(ctx->get(SLOT_STATE, 256 + 2 * round_up(cap * 8, 256) + round_up(rows, 256), &p));
+    FloatState* st = (FloatState*)p;
+    int64_t* piv_rows = (int64_t*)((char*)p + 256);
+    int64_t* piv_cols = (int64_t*)((char*)p + 256 + round_up(cap * 8, 256));
+    uint8_t* used = (uint8_t*)((char*)p + 256 + 2 * round_up(cap * 8, 256));
+    if (prof) LINAC_TRY(prof->begin(s));
+    k_row_scale<T><<<(unsigned)rows, 256, 0, s>>>(A, lda, rows, cols);
+    k_float_init<<<(unsigned)std::min<int64_t>(1024, ceil_div(rows, 256)), 256, 0, s>>>(st, used, rows);
+    const int64_t stride = std::max<int64_t>(1, cols / 256);
+    for (int64_t j = 0; j < cols; ++j) {
+        k_float_select<T><<<1, 1024, 0, s>>>(A, lda, rows, j, tol, st, used, piv_rows, piv_cols);
+        const int64_t w = cols - j - 1;
+        const int log2G = w >= 1024 ? 8 : (w >= 256 ? 7 : 6);
+        const bool sampled = prof && (j % stride) == 0;
+        if (sampled)
+            LINAC_TRY(prof->sample_begin(s, j, 2.0 * sizeof(T) * (double)(rows - 1) * (double)(cols - j),
+                                         (double)(rows - 1) * (double)(cols - j)));
+        k_float_elim<T><<<(unsigned)ceil_div(rows, 256 >> log2G), 256, 0, s>>>(A, lda, rows, cols, j, st, log2G);
+        if (sampled) LINAC_TRY(prof->sample_end(s));
+    }
+    LINAC_HIP(hipGetLastError());
+    if (prof) {
+        LINAC_TRY(prof->finish(s));
+        prof->result[5] = 1;
+    }
+    void* outp = nullptr;
+    LINAC_TRY(ctx->get(SLOT_OUT, (size_t)rows * lda * sizeof(T), &outp));
+    k_float_finalize<T><<<(unsigned)rows, 256, 0, s>>>(A, lda, (T*)outp, rows, cols, st, piv_rows, piv_cols);
+    LINAC_HIP(hipGetLastError());
+    int64_t rank = 0;
+    LINAC_HIP(hipMemcpyAsync(&rank, &st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    LINAC_HIP(hipStreamSynchronize(s));
+    std::vector<int64_t> pc(rank);
+    if (rank) LINAC_HIP(hipMemcpy(pc.data(), piv_cols, rank * 8, hipMemcpyDeviceToHost));
+    if (prof) {
+        std::vector<char> is_piv(cols, 0);
+        for (int64_t c : pc) is_piv[c] = 1;
+        double bytes = 0, macs = 0;
+        for (auto& x : prof->samples)
+            if (is_piv[x.tag]) { bytes += x.bytes; macs += x.macs; }
+        prof->result[3] = bytes;
+        prof->result[4] = macs;
+    }
+    if (on_device) {
+        LINAC_HIP(hipMemcpy2DAsync(a, ld * sizeof(T), outp, lda * sizeof(T), cols * sizeof(T), rows,
+                                   hipMemcpyDeviceToDevice, s));
+    } else {
+        LINAC_HIP(hipMemcpy2DAsync(a, ld * sizeof(T), outp, lda * sizeof(T), cols * sizeof(T), rows,
+                                   hipMemcpyDeviceToHost, s));
+    }
+    LINAC_HIP(hipStreamSynchronize(s));
+    if (pivots_out)
+        for (int64_t k = 0; k < rank; ++k) pivots_out[k] = pc[k];
+    if (rank_out) *rank_out = rank;
+    return LINAC_OK;
+}
+
+}  // namespace linac
+
+using namespace linac;
+
+extern "C" {
+
+int linac_rref_f64(double* a, int64_t rows, int64_t cols, int64_t ld, double tol, uint32_t flags, void* stream,
+                   int64_t* pivots_out, int64_t* rank_out) {
+    return rref_float_entry<double>(a, rows, cols, ld, tol, flags, stream, pivots_out, rank_out);
+}
+
+int linac_rref_c128(double* a, int64_t rows, int64_t cols, int64_t ld, double tol, uint32_t flags, void* stream,
+                    int64_t* pivots_out, int64_t* rank_out) {
+    return rref_float_entry<CplxF>((CplxF*)a, rows, cols, ld, tol, flags, stream, pivots_out, rank_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+"""float64 / complex128 RREF on the GPU: reference tolerance checks
+(reference tests/test_row_reduce_random_cuda.py:23-49, complex cases), solves
+against LAPACK, and the float oracle on rank-deficient inputs."""
+import numpy
+import pytest
+
+from oracle import crref
+from test_pivot_structure import KNOWN_MEDIUM_PIVOTS, planted_pivot_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_random_complex_gives_identity(gpu, n):
+    rs = numpy.random.RandomState(n)
+    A = rs.rand(n, n) + 1j * rs.rand(n, n)
+    R = gpu.cuda_row_reduce(A, 0)
+    assert R.dtype == numpy.complex128
+    assert numpy.all(numpy.isclose(R - numpy.identity(n), numpy.zeros((n, n))))
+
+
+def test_real_input_uses_float64(gpu):
+    rs = numpy.random.RandomState(1)
+    A = rs.rand(300, 300)
+    R = gpu.cuda_row_reduce(A.astype(complex), 0)
+    assert R.dtype == numpy.float64 and numpy.allclose(R, numpy.identity(300))
+
+
+@pytest.mark.parametrize("n, kind", [(1000, "f64"), (700, "c128"), (1500, "f64")])
+def test_augmented_solution_matches_lapack(gpu, n, kind):
+    rs = numpy.random.RandomState(n)
+    A = rs.random_sample((n, n))
+    b = rs.random_sample((n, 3))
+    if kind == "c128":
+        A = A + 1j * rs.standard_normal((n, n))
+        b = b + 1j * rs.standard_normal((n, 3))
+    R = gpu.cuda_row_reduce(numpy.concatenate([A, b], axis=1), 0)
+    x = numpy.linalg.solve(A, b)
+    rel = numpy.max(numpy.abs(R[:, n:] - x)) / numpy.max(numpy.abs(x))
+    assert rel < 1e-10, rel
+    assert numpy.allclose(R[:, :n], numpy.identity(n), atol=1e-12)
+
+
+@pytest.mark.parametrize("shape, rank, seed, cplx", [((60, 80), 35, 0, False), ((120, 70), 50, 1, True),
+                                                     ((200, 200), 199, 2, False)])
+def test_rank_deficient_matches_float_oracle(gpu, shape, rank, seed, cplx):
+    rs = numpy.random.RandomState(seed)
+    L = rs.standard_normal((shape[0], rank))
+    U = rs.standard_normal((rank, shape[1]))
+    if cplx:
+        L = L + 1j * rs.standard_normal(L.shape)
+        U = U + 1j * rs.standard_normal(U.shape)
+    A = L @ U
+    A[:, 5] = 0
+    R, piv, rk = gpu.rref_with_pivots(A, 0)
+    Ro, po = (crref.rref_c128 if cplx else crref.rref_f64)(A)
+    assert piv == po and rk == len(po)
+    assert numpy.max(numpy.abs(R - Ro)) < 1e-9 * max(1.0, numpy.max(numpy.abs(Ro)))
+
+
+@pytest.mark.parametrize("field", [0, 2 ** 31 - 1])
+def test_known_medium_pivot_structure(gpu, field):
+    """Pivots [0..8, 10, 12..27, 38..41, 59] of a 32 x 86 rank-31 matrix
+    (reference tests/test_null_space.py:26-62; its fixture is a pickle, so the
+    matrix is rebuilt synthetically with the same RREF structure)."""
+    A = planted_pivot_matrix(field, seed=3)
+    R = gpu.cuda_row_reduce(A, field)
+    R = gpu.drop_bottom_zero_rows(R)
+    assert R.shape == (31, 86)
+    assert gpu.pivot_columns_from_row_reduced_echelon_form(R) == KNOWN_MEDIUM_PIVOTS
