@@ -123,6 +123,15 @@ def main():
                     "kernel": "trailing mod-p GEMM (int8 limb MFMA, 16 limb products per modular MAC)",
                     "per_launch": "rows*k*cols modular MACs per panel", "samples": int(prof["samples"])}
 
+    roofline["avg_launch_us"] = round(1e3 * prof["sampled_ms"] / max(1, prof["samples"]), 2)
+    # HBM traffic per launch of the same kernel from the committed rocprofv3 PMC summary
+    # (FETCH_SIZE / WRITE_SIZE, calibrated against a device copy of known size)
+    tfile = os.path.join(ROOT, "profiles", f"rref_{used}_u32_N{N}_traffic.json")
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        roofline["traffic"] = round(t["traffic_bytes_per_launch"])
+        roofline["traffic_algorithmic"] = round(t["algorithmic_bytes_per_launch"])
+        roofline["traffic_source"] = os.path.relpath(tfile, ROOT)
     comparators = {"device_ms_profiled_run": round(prof["device_ms"], 2)}
     if method != "plain" and not args.no_plain and used != "plain":
         t1 = time.perf_counter()
