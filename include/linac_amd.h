@@ -87,6 +87,40 @@ int linac_rref_u32(uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t
 int linac_rref_u64(uint64_t* a, int64_t rows, int64_t cols, int64_t ld, uint64_t p,
                    uint32_t flags, void* stream, int64_t* pivots_out, int64_t* rank_out);
 
+/* ---- batches of GF(p) matrices (blocked method, 2 < p <= 4278124287) ----
+ * A batch advances all its unfinished matrices together, one 256-column panel
+ * per step, in shared kernel launches (one launch per kernel for the whole
+ * batch).  Matrices may be added between steps and taken out as soon as they
+ * are finished, so a serving loop can keep the device busy while requests
+ * arrive and leave (continuous batching).  Each matrix lives in its own device
+ * allocation of linac_batch_footprint_u32(rows, cols) bytes; with a non-zero
+ * cap, linac_batch_add_u32 refuses (LINAC_ERR_NOMEM) a matrix that would
+ * exceed it.  A batch object must be used from one thread at a time.
+ * No counterpart in the reference (batching is new); results equal
+ * linac_rref_u32 on each matrix. */
+typedef struct linac_batch linac_batch;
+int linac_batch_footprint_u32(int64_t rows, int64_t cols, uint64_t* bytes_out);
+int linac_batch_create(linac_batch** out, uint64_t mem_cap_bytes);
+int linac_batch_destroy(linac_batch* b);
+int linac_batch_memory(linac_batch* b, uint64_t* in_use_out, uint64_t* cap_out);
+/* Copies the matrix (host, or device with LINAC_DEVICE_PTR) into the batch. */
+int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p,
+                        uint32_t flags, int64_t* id_out);
+/* One panel for every unfinished matrix; *unfinished_out counts those left. */
+int linac_batch_step(linac_batch* b, int64_t* unfinished_out);
+int linac_batch_run(linac_batch* b);
+/* 1 when finished, 0 while running. */
+int linac_batch_status(linac_batch* b, int64_t id);
+/* Writes the RREF of a finished matrix to out (host, or device with
+ * LINAC_DEVICE_PTR; may be NULL), its pivot columns and rank, and releases it. */
+int linac_batch_take_u32(linac_batch* b, int64_t id, uint32_t* out, int64_t ld_out, uint32_t flags,
+                         int64_t* pivots_out, int64_t* rank_out);
+/* One-shot: RREF of `count` matrices in place (each with its own shape and
+ * prime); pivots_out may be NULL or hold per-matrix pointers (or NULLs). */
+int linac_rref_batch_u32(uint32_t* const* mats, const int64_t* rows, const int64_t* cols, const int64_t* lds,
+                         const uint32_t* primes, int64_t count, uint32_t flags, int64_t* const* pivots_out,
+                         int64_t* ranks_out);
+
 /* ---- RREF over float64 / complex128 ----
  * Replaces cuda_row_reduce(matrix, 0) (linac/pycuda_row_reduce.py:49-56,87-131;
  * kernels linac/row_reduce.cu:201-270,287-342,404-418): rows are divided once by

@@ -8,6 +8,7 @@ Drop-in names (``linac/__init__.py:1-24``): ``cuda_row_reduce``,
 from ._optional import CUDA_FOUND, GALOIS_FOUND, GPU_FOUND
 from .hip_row_reduce import cuda_row_reduce, rref_with_pivots, rref_device
 from .hip_matrix_loader import cuda_load_matrix, load_matrices, load_matrices_stacked
+from .batch import Batch, rref_batch
 from .timeit_decorator import timeit
 from .linear_algebra_tools import pivot_columns_from_row_reduced_echelon_form, drop_bottom_zero_rows
 
@@ -24,6 +25,8 @@ __all__ = [
     "cuda_load_matrix",
     "load_matrices",
     "load_matrices_stacked",
+    "Batch",
+    "rref_batch",
     "timeit",
     "pivot_columns_from_row_reduced_echelon_form",
     "drop_bottom_zero_rows",

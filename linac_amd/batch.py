@@ -1,0 +1,113 @@
+"""Batched GF(p) row reduction (no counterpart in the reference).
+
+Many matrices -- any shapes, any primes 2 < p <= 4278124287 -- advance
+together, one 256-column panel per step, in shared kernel launches.  Matrices
+can be added between steps and taken out as soon as they finish, which is
+what the serving loop (``linac_amd.serving``) builds on.  Each result equals
+``rref_with_pivots`` of that matrix alone.
+"""
+import ctypes
+
+import numpy
+
+from . import _lib
+from .hip_row_reduce import to_field_array
+
+
+def footprint(rows, cols):
+    """Device bytes one matrix occupies inside a batch (matrix + workspace)."""
+    out = ctypes.c_uint64(0)
+    _lib.check(_lib.load().linac_batch_footprint_u32(rows, cols, ctypes.byref(out)))
+    return out.value
+
+
+class Batch:
+    """A set of GF(p) matrices eliminated together.  Use from one thread."""
+
+    def __init__(self, mem_cap_bytes=0):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.linac_batch_create(ctypes.byref(h), int(mem_cap_bytes)))
+        self._h = h
+        self._shapes = {}
+
+    def close(self):
+        if self._h:
+            self._lib.linac_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, matrix, p):
+        """Copy a matrix into the batch; returns its id.  MemoryError when the
+        batch's memory cap would be exceeded (admission control)."""
+        a = to_field_array(matrix, int(p), numpy.dtype(numpy.uint32))
+        rows, cols = a.shape
+        rid = ctypes.c_int64(-1)
+        _lib.check(self._lib.linac_batch_add_u32(self._h, a.ctypes.data_as(ctypes.c_void_p), rows, cols, cols,
+                                                 int(p), 0, ctypes.byref(rid)))
+        self._shapes[rid.value] = (rows, cols)
+        return rid.value
+
+    def add_device(self, ptr, rows, cols, ld, p):
+        """Add a uint32 matrix already in HBM (rows 16-byte aligned)."""
+        rid = ctypes.c_int64(-1)
+        _lib.check(self._lib.linac_batch_add_u32(self._h, ctypes.c_void_p(ptr), rows, cols, ld, int(p),
+                                                 _lib.DEVICE_PTR, ctypes.byref(rid)))
+        self._shapes[rid.value] = (rows, cols)
+        return rid.value
+
+    def step(self):
+        """Advance every unfinished matrix by one panel; returns how many remain."""
+        n = ctypes.c_int64(0)
+        _lib.check(self._lib.linac_batch_step(self._h, ctypes.byref(n)))
+        return n.value
+
+    def run(self):
+        _lib.check(self._lib.linac_batch_run(self._h))
+
+    def done(self, rid):
+        rc = self._lib.linac_batch_status(self._h, int(rid))
+        if rc < 0:
+            _lib.check(rc)
+        return rc == 1
+
+    def memory(self):
+        used, cap = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _lib.check(self._lib.linac_batch_memory(self._h, ctypes.byref(used), ctypes.byref(cap)))
+        return used.value, cap.value
+
+    def take(self, rid):
+        """(R, pivot_columns, rank) of a finished matrix; releases its memory."""
+        rows, cols = self._shapes[rid]
+        out = numpy.empty((rows, cols), dtype=numpy.uint32)
+        piv = numpy.zeros(max(1, min(rows, cols)), dtype=numpy.int64)
+        rank = ctypes.c_int64(0)
+        _lib.check(self._lib.linac_batch_take_u32(self._h, int(rid), out.ctypes.data_as(ctypes.c_void_p), cols, 0,
+                                                  piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                  ctypes.byref(rank)))
+        del self._shapes[rid]
+        return out, piv[:rank.value].tolist(), rank.value
+
+
+def rref_batch(matrices, primes):
+    """RREF of every matrix (each with its own prime) in one batch.
+
+    Returns a list of (R, pivot_columns, rank).
+    """
+    if numpy.isscalar(primes):
+        primes = [primes] * len(matrices)
+    with Batch() as b:
+        ids = [b.add(m, p) for m, p in zip(matrices, primes)]
+        b.run()
+        return [b.take(i) for i in ids]

@@ -36,6 +36,10 @@ struct FieldU32 {
     __device__ __forceinline__ uint32_t sub(uint32_t a, uint32_t b) const {
         return a >= b ? a - b : a + (p - b);
     }
+    // a + b mod p without forming a + b, which wraps 32 bits for p > 2^31
+    __device__ __forceinline__ uint32_t add(uint32_t a, uint32_t b) const {
+        return a >= p - b ? a - (p - b) : a + b;
+    }
     __device__ __forceinline__ uint32_t to_mont(uint32_t a) const { return mont(a, r2); }
     // h^{-1} = h^(p-2) (Fermat), square-and-multiply in the Montgomery domain:
     // a fixed, branch-uniform sequence of about 2 log2(p) products.

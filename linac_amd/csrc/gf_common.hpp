@@ -27,4 +27,11 @@ template <class F>
 int blocked_dispatch(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
                      const ElimWorkspace& ws, DeviceContext* ctx, bool forced, Profiler* prof);
 
+// Shared finalisation (rref_gf.hip): reads the pivot list, optionally sorts it
+// by column and normalises, and gathers the pivot rows to rows 0..rank-1.
+template <class F>
+int gf_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
+                const ElimWorkspace& ws, DeviceContext* ctx, bool normalize, bool sort_by_column,
+                int64_t* rank_host, std::vector<int64_t>* piv_cols_host);
+
 }  // namespace linac

@@ -30,6 +30,11 @@
 // The dominant cost is the second GEMM: N x |Q| x |T| modular multiply-adds
 // with K = |Q| <= 256.
 //
+// Batches: every kernel takes an array of per-matrix descriptors (BMat) and
+// serves a whole set of matrices of any shapes and primes in one launch; each
+// matrix keeps its own panel position, so matrices can join or leave between
+// steps (continuous batching).  A single solve is a batch of one.
+//
 // Rows are never swapped; `used` marks pivot rows, the pivot list records
 // (row, column) and the shared finalisation gathers rows in column order.
 // Pivot rows come out normalised.  Replaces the column loop of
@@ -40,6 +45,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "gf_common.hpp"
 #include "modp_gemm.hpp"
@@ -66,26 +72,64 @@ struct BlockState {
 };
 
 // ---------------------------------------------------------------------------
+// Per-matrix descriptor.  Every kernel below serves a whole batch: the matrix
+// index comes from blockIdx.y (uniform grids) or from a prefix sum of tile
+// counts (GEMMs).  The host rewrites the descriptors before each step.
+// ---------------------------------------------------------------------------
+
+struct BMat {
+    uint32_t* A;
+    int64_t ld, N, M;
+    FieldU32 f;
+    int32_t pad0;
+    double pd, pinv;
+    uint32_t* X;        // [N][XLD] panel buffer [P | Z]
+    int8_t* planesL;    // [4][N][256]
+    int8_t* planesY;    // [4][256][256]
+    int8_t* planesB;    // [4][max(M, XLD)][256]
+    uint32_t* U;        // [CI][XLD]
+    uint32_t* Gq;       // [256][ldg]
+    int64_t strideL, strideB, ldg;
+    BlockState* bs;
+    ElimState* st;
+    int64_t* piv_rows;
+    int64_t* piv_cols;
+    uint8_t* used;
+    int32_t* epoch_of;
+    int32_t* slot_of;
+    // step parameters
+    int64_t J;          // first column of the current outer panel
+    int32_t bcur;       // its width
+    int32_t q0;         // pivots it produced (outer update)
+    int32_t Kq;         // q0 rounded up to 64 (GEMM depth)
+    int32_t e0;         // first epoch of this outer panel
+    int64_t rank_start; // index of its first pivot in piv_rows
+    int64_t T0, nT;     // trailing columns [T0, T0 + nT)
+};
+
+constexpr int64_t STRIDE_Y = 256 * 256;
+
+// ---------------------------------------------------------------------------
 // panel buffer management
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_panel_load(uint32_t* __restrict__ X, const uint32_t* __restrict__ A,
-                                                    int64_t ld, int64_t N, int64_t J, int Bcur) {
-    const int64_t total = N * XLD;
+__global__ __launch_bounds__(256) void k_panel_load(const BMat* __restrict__ D) {
+    const BMat& d = D[blockIdx.y];
+    const int64_t total = d.N * XLD;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = x / XLD;
         const int j = (int)(x - r * XLD);
-        X[x] = j < Bcur ? A[r * ld + J + j] : 0u;
+        d.X[x] = j < d.bcur ? d.A[r * d.ld + d.J + j] : 0u;
     }
 }
 
-__global__ __launch_bounds__(256) void k_panel_store(uint32_t* __restrict__ A, const uint32_t* __restrict__ X,
-                                                     int64_t ld, int64_t N, int64_t J, int Bcur) {
-    const int64_t total = N * Bcur;
+__global__ __launch_bounds__(256) void k_panel_store(const BMat* __restrict__ D) {
+    const BMat& d = D[blockIdx.y];
+    const int64_t total = d.N * d.bcur;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = x / Bcur;
-        const int j = (int)(x - r * Bcur);
-        A[r * ld + J + j] = X[r * XLD + j];
+        const int64_t r = x / d.bcur;
+        const int j = (int)(x - r * d.bcur);
+        d.A[r * d.ld + d.J + j] = d.X[r * XLD + j];
     }
 }
 
@@ -94,32 +138,35 @@ __global__ __launch_bounds__(256) void k_epoch_init(int32_t* epoch, int64_t N) {
         epoch[r] = -1;
 }
 
-__global__ void k_block_state_init(BlockState* bs, int bcur, int reset_timers) {
+__global__ void k_block_state_init(const BMat* __restrict__ D, int reset_timers) {
+    const BMat& d = D[blockIdx.x];
+    BlockState* bs = d.bs;
     if (reset_timers)
         for (int i = 0; i < 8; ++i) bs->tphase[i] = 0;
     bs->k = 0;
     bs->q0 = 0;
     bs->c0_next = 0;
     bs->c0_cur = 0;
-    bs->bcur = bcur;
+    bs->bcur = d.bcur;
 }
 
 // ---------------------------------------------------------------------------
-// inner factorisation: one workgroup
+// inner factorisation: one workgroup per matrix
 // ---------------------------------------------------------------------------
 
 constexpr int IF_T = 1024;  // threads of the inner factorisation: 16 waves, two candidate rows each
 static_assert(IF_T / 64 * 2 == CI, "each wave owns two candidate rows");
 static_assert(BI + CI == 64, "one augmented column per lane");
 
-// One workgroup.  Candidate rows live in LDS as [panel columns | identity];
-// wave w owns rows 2w and 2w+1.  Per column every wave ballots the column
-// itself (all waves see the same data, so they agree on the pivot without a
-// barrier), then updates its own rows with lane j handling column j.  Within
-// a wave the reads of a row precede its writes in program order, so one
-// barrier per column suffices.  No rows are swapped: pivots are recorded by
-// their LDS row, and the identity columns index LDS rows directly.
-__global__ __launch_bounds__(IF_T) void k_inner_factor(uint32_t* __restrict__ X, int64_t N,
+// Candidate rows live in LDS as [panel columns | identity]; wave w owns rows
+// 2w and 2w+1.  Per column every wave ballots the column itself (all waves
+// see the same data, so they agree on the pivot without a barrier), then
+// updates its own rows with lane j handling column j.  Within a wave the
+// reads of a row precede its writes in program order, so one barrier per
+// column suffices.  No rows are swapped: pivots are recorded by their LDS
+// row, and the identity columns index LDS rows directly.
+
+__device__ static void inner_factor_body(uint32_t* __restrict__ X, int64_t N,
                                                       uint8_t* __restrict__ used, int32_t* __restrict__ slot_of,
                                                       int32_t* __restrict__ epoch_of, int epoch,
                                                       int64_t colbase, FieldU32 f, ElimState* st,
@@ -309,371 +356,221 @@ __global__ __launch_bounds__(IF_T) void k_inner_factor(uint32_t* __restrict__ X,
     }
 }
 
-// U[t, n] = sum_t' Apiv^{-1}[t, t'] X[R[t'], n] for every column n of the panel
-// buffer (columns left of the inner panel are zero in the pivot rows, so the
-// product is zero there); writes U (u32) and its limb planes (B operand of
-// the inner GEMM, planes[l][n][t], depth KPI).
-__global__ __launch_bounds__(256) void k_make_u(const uint32_t* __restrict__ X, FieldU32 f,
-                                                const BlockState* __restrict__ bs, uint32_t* __restrict__ U,
-                                                int8_t* __restrict__ planes, int64_t plane_stride) {
+
+__global__ __launch_bounds__(IF_T) void k_inner_factor(const BMat* __restrict__ D, int epoch) {
+    const BMat& d = D[blockIdx.x];
+    inner_factor_body(d.X, d.N, d.used, d.slot_of, d.epoch_of, epoch, d.J, d.f, d.st, d.piv_rows, d.piv_cols, d.bs);
+}
+
+// U = Apiv^{-1} X[R, :] (k x XLD, Montgomery inverse times plain rows gives plain
+// values) and the limb planes of both operands of the inner GEMM:
+//   planesU[l][n][t] = digit l of U[t, n]                (t < k, zero up to KPI)
+//   planesA[l][r][t] = digit l of X[r, c0 + Cp[t]]       (every row r)
+// The GEMM overwrites the pivot columns X[:, c0 + Cp] in place, so its left
+// operand must be captured here, before any of its workgroups runs.
+__device__ static void make_u_body(const uint32_t* __restrict__ X, int64_t N, FieldU32 f,
+                                   const BlockState* __restrict__ bs, uint32_t* __restrict__ U,
+                                   int8_t* __restrict__ planesU, int64_t strideU,
+                                   int8_t* __restrict__ planesA, int64_t strideA) {
     __shared__ uint32_t s_inv[CI * CI];
     __shared__ int64_t s_R[CI];
+    __shared__ int s_cp[CI];
     const int k = bs->k;
+    if (k == 0) return;  // the inner GEMM skips steps without pivots
+    const int c0 = bs->c0_cur;
     for (int x = threadIdx.x + threadIdx.y * 64; x < k * CI; x += 256) s_inv[x] = bs->apinv[x];
-    if (threadIdx.y == 0 && threadIdx.x < k) s_R[threadIdx.x] = bs->R[threadIdx.x];
+    if (threadIdx.y == 0 && threadIdx.x < k) {
+        s_R[threadIdx.x] = bs->R[threadIdx.x];
+        s_cp[threadIdx.x] = bs->Cp[threadIdx.x];
+    }
     __syncthreads();
     const int t = threadIdx.x;  // 0..63
     const int n = blockIdx.x * 4 + threadIdx.y;
     uint32_t u = 0;
     if (t < k) {
         for (int s = 0; s < k; ++s) {
-            const uint32_t prod = f.mont(s_inv[t * CI + s], X[s_R[s] * XLD + n]);
-            const uint32_t v = u + prod;
-            u = v >= f.p ? v - f.p : v;
+            u = f.add(u, f.mont(s_inv[t * CI + s], X[s_R[s] * XLD + n]));
         }
         U[t * XLD + n] = u;
     }
     int8_t d[4] = {0, 0, 0, 0};
     if (t < k) balanced_digits(centred(u, f.p), d);
 #pragma unroll
-    for (int l = 0; l < 4; ++l) planes[l * plane_stride + (int64_t)n * KPI + t] = d[l];
-}
-
-// Limb planes of a left operand: planes[l][m][t] = digit l of (+/-) Y[row(m), col(t)]
-// for t < K; zero for t in [K, Kpad) and for rows whose epoch lies in [e0, e1).
-__global__ __launch_bounds__(256) void k_make_lhs(const uint32_t* __restrict__ Y, int64_t ldy, int64_t Mrows,
-                                                  const int64_t* __restrict__ rowlist, int64_t col0_host,
-                                                  const int32_t* col0ptr,
-                                                  const int32_t* __restrict__ colidx, const int32_t* kptr, int Khost,
-                                                  int Kpad, const int32_t* __restrict__ epoch_of, int e0, int e1,
-                                                  int negate, uint32_t p, int8_t* __restrict__ planes,
-                                                  int64_t plane_stride) {
-    const int K = kptr ? *kptr : Khost;
-    const int64_t col0 = col0ptr ? *col0ptr : col0_host;
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t m = x / Kpad;
-    const int t = (int)(x - m * Kpad);
-    if (m >= Mrows) return;
-    const int64_t r = rowlist ? rowlist[m] : m;
-    int8_t d[4] = {0, 0, 0, 0};
-    const bool zero_row = epoch_of && epoch_of[r] >= e0 && epoch_of[r] < e1;
-    if (t < K && !zero_row) {
-        const int64_t c = col0 + (colidx ? colidx[t] : t);
-        int32_t cv = centred(Y[r * ldy + c], p);
-        balanced_digits(negate ? -cv : cv, d);
-    }
+    for (int l = 0; l < 4; ++l) planesU[l * strideU + (int64_t)n * KPI + t] = d[l];
+    for (int64_t r = (int64_t)blockIdx.x * 4 + threadIdx.y; r < N; r += (int64_t)gridDim.x * 4) {
+        int8_t a[4] = {0, 0, 0, 0};
+        if (t < k) balanced_digits(centred(X[r * XLD + c0 + s_cp[t]], f.p), a);
 #pragma unroll
-    for (int l = 0; l < 4; ++l) planes[l * plane_stride + m * Kpad + t] = d[l];
+        for (int l = 0; l < 4; ++l) planesA[l * strideA + r * KPI + t] = a[l];
+    }
 }
 
-// Limb planes of a right operand, transposed: planes[l][n][t] = digit l of
-// Y[rowlist[t], col0 + n] for t < K, zero for t in [K, Kpad).  64 x 64 tiles
-// through LDS so that both the row reads and the plane writes are coalesced.
-__global__ __launch_bounds__(256) void k_make_rhs(const uint32_t* __restrict__ Y, int64_t ldy,
-                                                  const int64_t* __restrict__ rowlist, int K, int64_t col0,
-                                                  int64_t Ncols, int Kpad, uint32_t p, int8_t* __restrict__ planes,
-                                                  int64_t plane_stride) {
+
+__global__ __launch_bounds__(256) void k_make_u(const BMat* __restrict__ D) {
+    const BMat& d = D[blockIdx.y];
+    make_u_body(d.X, d.N, d.f, d.bs, d.U, d.planesB, d.strideB, d.planesL, d.strideL);
+}
+
+// ---------------------------------------------------------------------------
+// outer update operands
+// ---------------------------------------------------------------------------
+
+// Limb planes of the left operands of the outer update (blockIdx.y = matrix):
+//   mode 0: planesL[l][r][q] = digit l of -Z[r, q] for every row r, zero for the
+//           panel's pivot rows (their epochs lie in [e0, e1));
+//   mode 1: planesY[l][i][q] = digit l of -Z[Q[i], q] for the panel's pivot rows.
+// q < q0 carries data, q in [q0, Kq) is zero padding.
+__global__ __launch_bounds__(256) void k_make_lhs(const BMat* __restrict__ D, int mode, int e1) {
+    const BMat& d = D[blockIdx.y];
+    if (d.q0 == 0 || d.nT == 0) return;
+    const int64_t rows = mode == 0 ? d.N : d.q0;
+    const int Kpad = d.Kq;
+    int8_t* planes = mode == 0 ? d.planesL : d.planesY;
+    const int64_t stride = mode == 0 ? d.strideL : STRIDE_Y;
+    const int64_t* Q = d.piv_rows + d.rank_start;
+    const int64_t total = rows * Kpad;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t m = x / Kpad;
+        const int t = (int)(x - m * Kpad);
+        const int64_t r = mode == 0 ? m : Q[m];
+        int8_t dg[4] = {0, 0, 0, 0};
+        const bool zero_row = mode == 0 && d.epoch_of[r] >= d.e0 && d.epoch_of[r] < e1;
+        if (t < d.q0 && !zero_row) balanced_digits(-centred(d.X[r * XLD + BO + t], d.f.p), dg);
+#pragma unroll
+        for (int l = 0; l < 4; ++l) planes[l * stride + m * Kpad + t] = dg[l];
+    }
+}
+
+// Limb planes of A[Q, T], transposed: planesB[l][n][t] = digit l of
+// A[Q[t], T0 + n] for t < q0, zero for t in [q0, Kq).  64 x 64 tiles through
+// LDS so that both the row reads and the plane writes are coalesced.
+// blockIdx.x strides over column tiles, blockIdx.y = t-tile, blockIdx.z = matrix.
+__global__ __launch_bounds__(256) void k_make_rhs(const BMat* __restrict__ D) {
     __shared__ int32_t tile[64][65];
-    const int64_t nb = (int64_t)blockIdx.x * 64;
+    const BMat& d = D[blockIdx.z];
+    if (d.q0 == 0 || d.nT == 0) return;
     const int tb = blockIdx.y * 64;
-    for (int x = threadIdx.x; x < 64 * 64; x += 256) {
-        const int tt = x >> 6, nn = x & 63;
-        const int t = tb + tt;
+    if (tb >= d.Kq) return;
+    const int64_t* Q = d.piv_rows + d.rank_start;
+    for (int64_t nb = (int64_t)blockIdx.x * 64; nb < d.nT; nb += (int64_t)gridDim.x * 64) {
+        for (int x = threadIdx.x; x < 64 * 64; x += 256) {
+            const int tt = x >> 6, nn = x & 63;
+            const int t = tb + tt;
+            const int64_t n = nb + nn;
+            int32_t v = 0;
+            if (t < d.q0 && n < d.nT) v = centred(d.A[Q[t] * d.ld + d.T0 + n], d.f.p);
+            tile[tt][nn] = v;
+        }
+        __syncthreads();
+        const int nn = threadIdx.x >> 2, tq = (threadIdx.x & 3) * 16;
         const int64_t n = nb + nn;
-        int32_t v = 0;
-        if (t < K && n < Ncols) v = centred(Y[rowlist[t] * ldy + col0 + n], p);
-        tile[tt][nn] = v;
-    }
-    __syncthreads();
-    // each thread writes 16 consecutive t of one column n, for every limb
-    const int nn = threadIdx.x >> 2, tq = (threadIdx.x & 3) * 16;
-    const int64_t n = nb + nn;
-    if (n >= Ncols) return;
-    int8_t buf[4][16];
+        if (n < d.nT) {
+            int8_t buf[4][16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        int8_t d[4];
-        balanced_digits(tile[tq + i][nn], d);
+            for (int i = 0; i < 16; ++i) {
+                int8_t dg[4];
+                balanced_digits(tile[tq + i][nn], dg);
 #pragma unroll
-        for (int l = 0; l < 4; ++l) buf[l][i] = d[l];
-    }
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-        v4i w;
-        __builtin_memcpy(&w, buf[l], 16);
-        *reinterpret_cast<v4i*>(planes + l * plane_stride + n * Kpad + tb + tq) = w;
-    }
-}
-
-// Y[rowlist[t], col0 + n] = U[t, n] for t < K, n < ncols.
-__global__ __launch_bounds__(256) void k_scatter_rows(uint32_t* __restrict__ Y, int64_t ldy,
-                                                      const int64_t* __restrict__ rowlist, const int32_t* kptr,
-                                                      int Khost, int64_t col0, int64_t ncols,
-                                                      const uint32_t* __restrict__ U, int64_t ldu) {
-    const int K = kptr ? *kptr : Khost;
-    const int t = blockIdx.y;
-    if (t >= K) return;
-    const int64_t r = rowlist[t];
-    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < ncols; n += (int64_t)gridDim.x * blockDim.x)
-        Y[r * ldy + col0 + n] = U[t * ldu + n];
-}
-
-// ---------------------------------------------------------------------------
-// the modular GEMM: C <- (C - L B) mod p  (or C <- -L B mod p when CZERO)
-// 256 threads, 64 x 64 tile per workgroup, 32 x 32 per wave (2 x 2 MFMA
-// 16x16x64 tiles), seven diagonal accumulators per tile.
-// ---------------------------------------------------------------------------
-
-template <bool CZERO>
-__global__ __launch_bounds__(256) void k_gemm_modp(uint32_t* __restrict__ C, int64_t ldc, int64_t Mrows,
-                                                   int64_t Ncols, const int8_t* __restrict__ L,
-                                                   int64_t planeL, const int8_t* __restrict__ B,
-                                                   int64_t planeB, int Kpad, uint32_t p, double pd, double pinv,
-                                                   const int32_t* kptr, const int32_t* lo_ptr,
-                                                   const int32_t* hi_ptr, int hi_off) {
-    if (kptr && *kptr == 0) return;
-    // optional column window [lo, hi) read from device memory: tiles wholly
-    // outside it would only add zero products
-    if (lo_ptr && ((int64_t)blockIdx.x + 1) * 64 <= *lo_ptr) return;
-    if (hi_ptr && (int64_t)blockIdx.x * 64 >= *hi_ptr + hi_off) return;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int64_t m0 = (int64_t)blockIdx.y * 64 + (wid >> 1) * 32;
-    const int64_t n0 = (int64_t)blockIdx.x * 64 + (wid & 1) * 32;
-    const int r16 = lane & 15, g = lane >> 4;
-    v4i acc[7][2][2];
-#pragma unroll
-    for (int s = 0; s < 7; ++s)
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[s][i][j] = v4i{0, 0, 0, 0};
-    const v4i zero = {0, 0, 0, 0};
-    for (int kc = 0; kc < Kpad; kc += 64) {
-        v4i a[4][2], b[4][2];
-#pragma unroll
-        for (int mi = 0; mi < 2; ++mi) {
-            const int64_t row = m0 + mi * 16 + r16;
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-                a[l][mi] = row < Mrows ? *reinterpret_cast<const v4i*>(L + l * planeL + row * Kpad + kc + 16 * g) : zero;
-        }
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
-            const int64_t col = n0 + ni * 16 + r16;
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-                b[l][ni] = col < Ncols ? *reinterpret_cast<const v4i*>(B + l * planeB + col * Kpad + kc + 16 * g) : zero;
-        }
-#pragma unroll
-        for (int l1 = 0; l1 < 4; ++l1)
-#pragma unroll
-            for (int l2 = 0; l2 < 4; ++l2)
-#pragma unroll
-                for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-                    for (int ni = 0; ni < 2; ++ni)
-                        acc[l1 + l2][mi][ni] =
-                            __builtin_amdgcn_mfma_i32_16x16x64_i8(a[l1][mi], b[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
-    }
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int64_t row = m0 + mi * 16 + 4 * g + i;
-                const int64_t col = n0 + ni * 16 + r16;
-                if (row < Mrows && col < Ncols) {
-                    int32_t s[7];
-#pragma unroll
-                    for (int q = 0; q < 7; ++q) s[q] = acc[q][mi][ni][i];
-                    uint32_t* cp = C + row * ldc + col;
-                    *cp = fold_sub(CZERO ? 0u : *cp, s, pd, pinv);
-                }
+                for (int l = 0; l < 4; ++l) buf[l][i] = dg[l];
             }
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                v4i w;
+                __builtin_memcpy(&w, buf[l], 16);
+                *reinterpret_cast<v4i*>(d.planesB + l * d.strideB + n * d.Kq + tb + tq) = w;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// A[Q[t], T0 + n] = Gq[t, n] (blockIdx.y = t, blockIdx.z = matrix).
+__global__ __launch_bounds__(256) void k_scatter_q(const BMat* __restrict__ D) {
+    const BMat& d = D[blockIdx.z];
+    const int t = blockIdx.y;
+    if (t >= d.q0 || d.nT == 0) return;
+    const int64_t r = d.piv_rows[d.rank_start + t];
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < d.nT; n += (int64_t)gridDim.x * blockDim.x)
+        d.A[r * d.ld + d.T0 + n] = d.Gq[t * d.ldg + n];
 }
 
 // ---------------------------------------------------------------------------
-// the modular GEMM, second form: 512 threads, 128 x 64 tile per workgroup
-// (8 waves of 32 x 32), operands staged through LDS in 64-deep K stages with
-// double buffering, and a tile order that keeps each XCD's L2 on a small
-// working set: consecutive workgroup ids go to different XCDs, so the tile
-// list is first cut into 8 contiguous ranges (one per XCD), and each range is
-// walked in groups of 8 tile-rows so that a B tile is reused by 8
-// consecutive tiles while the 8 A tiles stay resident.
+// modular GEMMs on the int8 matrix cores (see modp_gemm.hpp for the limb
+// arithmetic): 256 threads, 64 x 64 tile (4 waves of 32 x 32), 64-deep K
+// stages double-buffered in LDS (two workgroups per CU: one's prologue and
+// epilogue overlap the other's MFMAs).  The flattened tile list of the batch
+// is cut into 8 contiguous ranges, one per XCD (consecutive workgroup ids go
+// to different XCDs), and each matrix's tiles are walked in groups of 8 tile
+// rows so that a B tile is reused 8 times while the A tiles stay in L2.
 // ---------------------------------------------------------------------------
 
-constexpr int GB_M = 128, GB_N = 64, GB_K = 64, GB_T = 512;
-constexpr int GB_ROW = GB_K + 16;  // LDS bytes per staged row (padded against bank conflicts)
-constexpr int GB_GROUP = 8;
+constexpr int G_M = 64, G_N = 64, G_K = 64, G_T = 256;
+constexpr int G_ROW = G_K + 16;  // LDS bytes per staged row (padded against bank conflicts)
+constexpr int G_GROUP = 8;
 constexpr int NUM_XCD = 8;
 
-__device__ __forceinline__ bool gemm_tile_of(int64_t mt, int64_t nt, int64_t* tm, int64_t* tn) {
-    const int64_t total = mt * nt;
+// blockIdx.x -> (matrix, tile row, tile column) through the prefix sums of
+// per-matrix tile counts `pre[0..B]` (row-tile counts in `mts`).
+__device__ __forceinline__ bool batch_tile(const int64_t* __restrict__ pre, const int64_t* __restrict__ mts, int B,
+                                           int* m_out, int64_t* tm, int64_t* tn) {
+    const int64_t total = pre[B];
     const int64_t per_xcd = (total + NUM_XCD - 1) / NUM_XCD;
     const int64_t bid = blockIdx.x;
     const int64_t t = (bid % NUM_XCD) * per_xcd + bid / NUM_XCD;
     if (t >= total) return false;
-    const int64_t gsz = (int64_t)GB_GROUP * nt;
-    const int64_t g = t / gsz, tt = t - g * gsz;
-    const int64_t gm = min((int64_t)GB_GROUP, mt - g * GB_GROUP);
-    *tm = g * GB_GROUP + tt % gm;
+    int lo = 0, hi = B;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const int64_t loc = t - pre[lo];
+    const int64_t mt = mts[lo];
+    const int64_t nt = (pre[lo + 1] - pre[lo]) / mt;
+    const int64_t gsz = (int64_t)G_GROUP * nt;
+    const int64_t g = loc / gsz, tt = loc - g * gsz;
+    const int64_t gm = min((int64_t)G_GROUP, mt - g * G_GROUP);
+    *m_out = lo;
+    *tm = g * G_GROUP + tt % gm;
     *tn = tt / gm;
     return true;
 }
 
-inline unsigned gemm2_grid(int64_t Mrows, int64_t Ncols) {
-    const int64_t total = ceil_div(Mrows, GB_M) * ceil_div(Ncols, GB_N);
-    return (unsigned)(ceil_div(total, NUM_XCD) * NUM_XCD);
-}
+struct TileSched {
+    const int64_t* pre;  // [B + 1]
+    const int64_t* mts;  // [B]
+    int B;
+};
 
-template <bool CZERO>
-__global__ __launch_bounds__(GB_T) void k_gemm_modp2(uint32_t* __restrict__ C, int64_t ldc, int64_t Mrows,
-                                                     int64_t Ncols, const int8_t* __restrict__ L, int64_t planeL,
-                                                     const int8_t* __restrict__ B, int64_t planeB, int Kpad,
-                                                     double pd, double pinv, const int32_t* kptr,
-                                                     const int32_t* lo_ptr, const int32_t* hi_ptr, int hi_off) {
-    __shared__ __attribute__((aligned(16))) int8_t sA[2][4][GB_M * GB_ROW];
-    __shared__ __attribute__((aligned(16))) int8_t sB[2][4][GB_N * GB_ROW];
-    if (kptr && *kptr == 0) return;
+// Outer update GEMMs.  MODE 0: Gq = Z[Q] A[Q, T] (C starts at zero, q0 rows);
+// MODE 1: A[:, T] <- A[:, T] + Z A[Q, T] (rows of Q have zero Z rows here).
+// Both are C <- C - L B with L = -Z limbs.
+template <int MODE>
+__global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat* __restrict__ D, TileSched ts) {
+    __shared__ __attribute__((aligned(16))) int8_t sA[2][4][G_M * G_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[2][4][G_N * G_ROW];
+    int mi_;
     int64_t tm, tn;
-    if (!gemm_tile_of(ceil_div(Mrows, GB_M), ceil_div(Ncols, GB_N), &tm, &tn)) return;
-    const int64_t m_base = tm * GB_M, n_base = tn * GB_N;
-    if (lo_ptr && n_base + GB_N <= *lo_ptr) return;
-    if (hi_ptr && n_base >= *hi_ptr + hi_off) return;
+    if (!batch_tile(ts.pre, ts.mts, ts.B, &mi_, &tm, &tn)) return;
+    const BMat& d = D[mi_];
+    uint32_t* C = MODE == 0 ? d.Gq : d.A + d.T0;
+    const int64_t ldc = MODE == 0 ? d.ldg : d.ld;
+    const int64_t Mrows = MODE == 0 ? d.q0 : d.N;
+    const int64_t Ncols = d.nT;
+    const int8_t* L = MODE == 0 ? d.planesY : d.planesL;
+    const int64_t planeL = MODE == 0 ? STRIDE_Y : d.strideL;
+    const int8_t* B = d.planesB;
+    const int64_t planeB = d.strideB;
+    const int Kpad = d.Kq;
+    const double pd = d.pd, pinv = d.pinv;
+    const int64_t m_base = tm * G_M, n_base = tn * G_N;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
     const v4i zero = {0, 0, 0, 0};
 
-    // stage copy: A = 4 limbs x 128 rows x 4 sixteen-byte quarters (4 per thread),
-    //             B = 4 limbs x 64 cols x 4 quarters (2 per thread)
-    v4i ra[4], rb[2];
-    auto gload = [&](int kc) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int c = tid + q * GB_T;
-            const int l = c >> 9, rem = c & 511, row = rem >> 2, qu = rem & 3;
-            const int64_t grow = m_base + row;
-            ra[q] = grow < Mrows ? *reinterpret_cast<const v4i*>(L + l * planeL + grow * Kpad + kc + qu * 16) : zero;
-        }
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int c = tid + q * GB_T;
-            const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
-            const int64_t gcol = n_base + col;
-            rb[q] = gcol < Ncols ? *reinterpret_cast<const v4i*>(B + l * planeB + gcol * Kpad + kc + qu * 16) : zero;
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int c = tid + q * GB_T;
-            const int l = c >> 9, rem = c & 511, row = rem >> 2, qu = rem & 3;
-            *reinterpret_cast<v4i*>(&sA[buf][l][row * GB_ROW + qu * 16]) = ra[q];
-        }
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int c = tid + q * GB_T;
-            const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
-            *reinterpret_cast<v4i*>(&sB[buf][l][col * GB_ROW + qu * 16]) = rb[q];
-        }
-    };
-
-    v4i acc[7][2][2];
-#pragma unroll
-    for (int s = 0; s < 7; ++s)
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
-
-    const int nk = Kpad / GB_K;
-    gload(0);
-    lstore(0);
-    __syncthreads();
-    for (int kk = 0; kk < nk; ++kk) {
-        const int buf = kk & 1;
-        if (kk + 1 < nk) gload((kk + 1) * GB_K);
-        v4i bf[4][2];
-#pragma unroll
-        for (int l = 0; l < 4; ++l)
-#pragma unroll
-            for (int ni = 0; ni < 2; ++ni)
-                bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[buf][l][(wn * 32 + ni * 16 + r16) * GB_ROW + 16 * g]);
-#pragma unroll
-        for (int l1 = 0; l1 < 4; ++l1) {
-            v4i af[2];
-#pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
-                af[mi] = *reinterpret_cast<const v4i*>(&sA[buf][l1][(wm * 32 + mi * 16 + r16) * GB_ROW + 16 * g]);
-#pragma unroll
-            for (int l2 = 0; l2 < 4; ++l2)
-#pragma unroll
-                for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-                    for (int ni = 0; ni < 2; ++ni)
-                        acc[l1 + l2][mi][ni] =
-                            __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
-        }
-        if (kk + 1 < nk) lstore(buf ^ 1);
-        __syncthreads();
-    }
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-                const int64_t col = n_base + wn * 32 + ni * 16 + r16;
-                if (row < Mrows && col < Ncols) {
-                    int32_t s[7];
-#pragma unroll
-                    for (int q = 0; q < 7; ++q) s[q] = acc[q][mi][ni][i];
-                    uint32_t* cp = C + row * ldc + col;
-                    *cp = fold_sub(CZERO ? 0u : *cp, s, pd, pinv);
-                }
-            }
-}
-
-// ---------------------------------------------------------------------------
-// third form (used): 256 threads, 64 x 64 tile (4 waves of 32 x 32), 64 KiB of
-// double-buffered LDS so that two independent workgroups share a CU and one's
-// prologue / epilogue overlaps the other's MFMA work; the C tile is fetched
-// before the K loop so its latency hides behind the MFMAs.  Same XCD-aware
-// grouped tile order as the second form.
-// ---------------------------------------------------------------------------
-
-constexpr int G3_M = 64, G3_N = 64, G3_T = 256;
-
-inline unsigned gemm3_grid(int64_t Mrows, int64_t Ncols) {
-    const int64_t total = ceil_div(Mrows, G3_M) * ceil_div(Ncols, G3_N);
-    return (unsigned)(ceil_div(total, NUM_XCD) * NUM_XCD);
-}
-
-template <bool CZERO>
-__global__ __launch_bounds__(G3_T, 2) void k_gemm_modp3(uint32_t* __restrict__ C, int64_t ldc, int64_t Mrows,
-                                                        int64_t Ncols, const int8_t* __restrict__ L,
-                                                        int64_t planeL, const int8_t* __restrict__ B,
-                                                        int64_t planeB, int Kpad, double pd, double pinv,
-                                                        const int32_t* kptr, const int32_t* lo_ptr,
-                                                        const int32_t* hi_ptr, int hi_off) {
-    __shared__ __attribute__((aligned(16))) int8_t sA[2][4][G3_M * GB_ROW];
-    __shared__ __attribute__((aligned(16))) int8_t sB[2][4][G3_N * GB_ROW];
-    if (kptr && *kptr == 0) return;
-    int64_t tm, tn;
-    if (!gemm_tile_of(ceil_div(Mrows, G3_M), ceil_div(Ncols, G3_N), &tm, &tn)) return;
-    const int64_t m_base = tm * G3_M, n_base = tn * G3_N;
-    if (lo_ptr && n_base + G3_N <= *lo_ptr) return;
-    if (hi_ptr && n_base >= *hi_ptr + hi_off) return;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int wm = wid >> 1, wn = wid & 1;
-    const int r16 = lane & 15, g = lane >> 4;
-    const v4i zero = {0, 0, 0, 0};
-
-    // C tile values of this lane: fetched while the last K stage runs on the
-    // matrix cores (the stage registers are free by then)
+    // C values of this lane, fetched while the last K stage runs on the matrix
+    // cores (the stage registers are free by then)
     uint32_t cpre[2][2][4];
     auto cload = [&]() {
 #pragma unroll
@@ -684,16 +581,15 @@ __global__ __launch_bounds__(G3_T, 2) void k_gemm_modp3(uint32_t* __restrict__ C
                 for (int i = 0; i < 4; ++i) {
                     const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
                     const int64_t col = n_base + wn * 32 + ni * 16 + r16;
-                    cpre[mi][ni][i] = (!CZERO && row < Mrows && col < Ncols) ? C[row * ldc + col] : 0u;
+                    cpre[mi][ni][i] = (MODE == 1 && row < Mrows && col < Ncols) ? C[row * ldc + col] : 0u;
                 }
     };
-
     // stage copy: A and B = 4 limbs x 64 rows x 4 sixteen-byte quarters (4 each per thread)
     v4i ra[4], rb[4];
     auto gload = [&](int kc) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = tid + q * G3_T;
+            const int c = tid + q * G_T;
             const int l = c >> 8, rem = c & 255, row = rem >> 2, qu = rem & 3;
             const int64_t grow = m_base + row, gcol = n_base + row;
             ra[q] = grow < Mrows ? *reinterpret_cast<const v4i*>(L + l * planeL + grow * Kpad + kc + qu * 16) : zero;
@@ -703,10 +599,10 @@ __global__ __launch_bounds__(G3_T, 2) void k_gemm_modp3(uint32_t* __restrict__ C
     auto lstore = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = tid + q * G3_T;
+            const int c = tid + q * G_T;
             const int l = c >> 8, rem = c & 255, row = rem >> 2, qu = rem & 3;
-            *reinterpret_cast<v4i*>(&sA[buf][l][row * GB_ROW + qu * 16]) = ra[q];
-            *reinterpret_cast<v4i*>(&sB[buf][l][row * GB_ROW + qu * 16]) = rb[q];
+            *reinterpret_cast<v4i*>(&sA[buf][l][row * G_ROW + qu * 16]) = ra[q];
+            *reinterpret_cast<v4i*>(&sB[buf][l][row * G_ROW + qu * 16]) = rb[q];
         }
     };
 
@@ -718,26 +614,26 @@ __global__ __launch_bounds__(G3_T, 2) void k_gemm_modp3(uint32_t* __restrict__ C
 #pragma unroll
             for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
 
-    const int nk = Kpad / GB_K;
+    const int nk = Kpad / G_K;
     gload(0);
     lstore(0);
     __syncthreads();
     for (int kk = 0; kk < nk; ++kk) {
         const int buf = kk & 1;
-        if (kk + 1 < nk) gload((kk + 1) * GB_K);
+        if (kk + 1 < nk) gload((kk + 1) * G_K);
         else cload();
         v4i bf[4][2];
 #pragma unroll
         for (int l = 0; l < 4; ++l)
 #pragma unroll
             for (int ni = 0; ni < 2; ++ni)
-                bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[buf][l][(wn * 32 + ni * 16 + r16) * GB_ROW + 16 * g]);
+                bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[buf][l][(wn * 32 + ni * 16 + r16) * G_ROW + 16 * g]);
 #pragma unroll
         for (int l1 = 0; l1 < 4; ++l1) {
             v4i af[2];
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
-                af[mi] = *reinterpret_cast<const v4i*>(&sA[buf][l1][(wm * 32 + mi * 16 + r16) * GB_ROW + 16 * g]);
+                af[mi] = *reinterpret_cast<const v4i*>(&sA[buf][l1][(wm * 32 + mi * 16 + r16) * G_ROW + 16 * g]);
 #pragma unroll
             for (int l2 = 0; l2 < 4; ++l2)
 #pragma unroll
@@ -759,72 +655,48 @@ __global__ __launch_bounds__(G3_T, 2) void k_gemm_modp3(uint32_t* __restrict__ C
                 const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
                 const int64_t col = n_base + wn * 32 + ni * 16 + r16;
                 if (row < Mrows && col < Ncols) {
-                    int32_t s[7];
+                    int32_t sv[7];
 #pragma unroll
-                    for (int q = 0; q < 7; ++q) s[q] = acc[q][mi][ni][i];
-                    C[row * ldc + col] = fold_sub(cpre[mi][ni][i], s, pd, pinv);
+                    for (int q = 0; q < 7; ++q) sv[q] = acc[q][mi][ni][i];
+                    C[row * ldc + col] = fold_sub(cpre[mi][ni][i], sv, pd, pinv);
                 }
             }
 }
 
-// ---------------------------------------------------------------------------
-// inner-step GEMM on the panel buffer:  X[r, :] -= X[r, c0 + Cp] U  for rows
+// Inner-step GEMM on the panel buffer:  X[r, :] -= X[r, c0 + Cp] U  for rows
 // that are not pivots of this step, X[r, :] = U[slot(r), :] for those that
-// are.  K = 64 (k <= 32 pivots, zero padded), one LDS stage.  The A operand's
-// limbs are built from X on the fly (no separate plane buffer), B is the limb
-// planes of U written by k_make_u.  Tiles outside the live column window
-// [c0, BO + q0) only meet zeros and are skipped.
-// ---------------------------------------------------------------------------
-
-__global__ __launch_bounds__(G3_T, 2) void k_gemm_inner(uint32_t* __restrict__ X, int64_t N,
-                                                        const int8_t* __restrict__ Bpl, int64_t planeB,
-                                                        const uint32_t* __restrict__ U,
-                                                        const BlockState* __restrict__ bs,
-                                                        const int32_t* __restrict__ epoch_of, int epoch,
-                                                        const int32_t* __restrict__ slot_of, uint32_t p, double pd,
-                                                        double pinv) {
-    __shared__ __attribute__((aligned(16))) int8_t sA[4][G3_M * GB_ROW];
-    __shared__ __attribute__((aligned(16))) int8_t sB[4][G3_N * GB_ROW];
-    __shared__ int s_cp[CI];
+// are.  K = 64 (k <= 32 pivots, zero padded), one LDS stage.  Both operands
+// come as limb planes from k_make_u (A: X[:, c0 + Cp] captured before this
+// launch, since its tiles overwrite those columns).  Tiles outside the live
+// column window [c0, BO + q0) only meet zeros and are skipped.
+__global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat* __restrict__ D, TileSched ts, int epoch) {
+    __shared__ __attribute__((aligned(16))) int8_t sA[4][G_M * G_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[4][G_N * G_ROW];
+    int mi_;
+    int64_t tm, tn;
+    if (!batch_tile(ts.pre, ts.mts, ts.B, &mi_, &tm, &tn)) return;
+    const BMat& d = D[mi_];
+    const BlockState* bs = d.bs;
     const int k = bs->k;
     if (k == 0) return;
-    int64_t tm, tn;
-    if (!gemm_tile_of(ceil_div(N, G3_M), XLD / G3_N, &tm, &tn)) return;
-    const int64_t m_base = tm * G3_M, n_base = tn * G3_N;
+    const int64_t m_base = tm * G_M, n_base = tn * G_N;
     const int c0 = bs->c0_cur;
-    if (n_base + G3_N <= c0 || n_base >= BO + bs->q0) return;
+    if (n_base + G_N <= c0 || n_base >= BO + bs->q0) return;
+    uint32_t* X = d.X;
+    const int64_t N = d.N;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
-    if (tid < k) s_cp[tid] = bs->Cp[tid];
-    __syncthreads();
-    // A: thread -> (row = tid / 4, 16 consecutive k), values X[row, c0 + Cp[t]]
-    {
-        const int rl = tid >> 2, kq = (tid & 3) * 16;
-        const int64_t row = m_base + rl;
-        int8_t dg[4][16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int t = kq + i;
-            int8_t d[4] = {0, 0, 0, 0};
-            if (t < k && row < N) balanced_digits(centred(X[row * XLD + c0 + s_cp[t]], p), d);
-#pragma unroll
-            for (int l = 0; l < 4; ++l) dg[l][i] = d[l];
-        }
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            v4i w;
-            __builtin_memcpy(&w, dg[l], 16);
-            *reinterpret_cast<v4i*>(&sA[l][rl * GB_ROW + kq]) = w;
-        }
-    }
-    // B: copy the planes of U for this tile's 64 columns
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int c = tid + q * G3_T;
+        const int c = tid + q * G_T;
         const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
-        *reinterpret_cast<v4i*>(&sB[l][col * GB_ROW + qu * 16]) =
-            *reinterpret_cast<const v4i*>(Bpl + l * planeB + (n_base + col) * KPI + qu * 16);
+        *reinterpret_cast<v4i*>(&sB[l][col * G_ROW + qu * 16]) =
+            *reinterpret_cast<const v4i*>(d.planesB + l * d.strideB + (n_base + col) * KPI + qu * 16);
+        const int64_t row = m_base + col;
+        v4i w = {0, 0, 0, 0};
+        if (row < N) w = *reinterpret_cast<const v4i*>(d.planesL + l * d.strideL + row * KPI + qu * 16);
+        *reinterpret_cast<v4i*>(&sA[l][col * G_ROW + qu * 16]) = w;
     }
     __syncthreads();
     const v4i zero = {0, 0, 0, 0};
@@ -840,13 +712,13 @@ __global__ __launch_bounds__(G3_T, 2) void k_gemm_inner(uint32_t* __restrict__ X
     for (int l = 0; l < 4; ++l)
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni)
-            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][(wn * 32 + ni * 16 + r16) * GB_ROW + 16 * g]);
+            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][(wn * 32 + ni * 16 + r16) * G_ROW + 16 * g]);
 #pragma unroll
     for (int l1 = 0; l1 < 4; ++l1) {
         v4i af[2];
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
-            af[mi] = *reinterpret_cast<const v4i*>(&sA[l1][(wm * 32 + mi * 16 + r16) * GB_ROW + 16 * g]);
+            af[mi] = *reinterpret_cast<const v4i*>(&sA[l1][(wm * 32 + mi * 16 + r16) * G_ROW + 16 * g]);
 #pragma unroll
         for (int l2 = 0; l2 < 4; ++l2)
 #pragma unroll
@@ -862,157 +734,369 @@ __global__ __launch_bounds__(G3_T, 2) void k_gemm_inner(uint32_t* __restrict__ X
         for (int i = 0; i < 4; ++i) {
             const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
             if (row >= N) continue;
-            const bool pivot_row = epoch_of[row] == epoch;
-            const int slot = pivot_row ? slot_of[row] : 0;
+            const bool pivot_row = d.epoch_of[row] == epoch;
+            const int slot = pivot_row ? d.slot_of[row] : 0;
 #pragma unroll
             for (int ni = 0; ni < 2; ++ni) {
                 const int64_t col = n_base + wn * 32 + ni * 16 + r16;
                 uint32_t* cp = X + row * XLD + col;
                 if (pivot_row) {
-                    *cp = U[slot * XLD + col];
+                    *cp = d.U[slot * XLD + col];
                 } else {
                     int32_t sv[7];
 #pragma unroll
                     for (int q = 0; q < 7; ++q) sv[q] = acc[q][mi][ni][i];
-                    *cp = fold_sub(*cp, sv, pd, pinv);
+                    *cp = fold_sub(*cp, sv, d.pd, d.pinv);
                 }
             }
         }
 }
 
 // ---------------------------------------------------------------------------
-// host driver
+// per-matrix set-up and status
 // ---------------------------------------------------------------------------
 
-struct BlockedBuffers {
-    uint32_t* X;
-    int32_t* slot_of;   // pivot slot of a row within its inner step
-    int32_t* epoch_of;
-    BlockState* bs;
-    int8_t* planesL;  // [4][N][256]
-    int64_t strideL;
-    int8_t* planesY;  // [4][256][256]
-    int64_t strideY;
-    int8_t* planesB;  // [4][max(M, XLD)][256]
-    int64_t strideB;
-    uint32_t* U;      // [CI][XLD]
-    uint32_t* Gq;     // [256][ldg]
-    int64_t ldg;
+// used = 0, epoch_of = -1, state reset, entries reduced mod p (blockIdx.y = matrix)
+__global__ __launch_bounds__(256) void k_item_init(const BMat* __restrict__ D) {
+    const BMat& d = D[blockIdx.y];
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, gs = (int64_t)gridDim.x * blockDim.x;
+    if (g0 == 0) {
+        d.st->prow = -1;
+        d.st->rank = 0;
+        d.st->first_unused = 0;
+        d.st->factor = 0;
+    }
+    for (int64_t r = g0; r < d.N; r += gs) {
+        d.used[r] = 0;
+        d.epoch_of[r] = -1;
+    }
+    const int64_t total = d.N * d.M;
+    for (int64_t x = g0; x < total; x += gs) {
+        const int64_t r = x / d.M, c = x - r * d.M;
+        uint32_t& v = d.A[r * d.ld + c];
+        v = d.f.reduce(v);
+    }
+}
+
+struct ItemStatus {
+    int32_t c0_next, bcur, q0, pad;
+    int64_t rank;
 };
 
-static int blocked_alloc(DeviceContext* ctx, int64_t N, int64_t M, BlockedBuffers* b) {
-    void* p = nullptr;
-    LINAC_TRY(ctx->get(SLOT_PANEL_A, (size_t)N * XLD * 4, &p));
-    b->X = (uint32_t*)p;
-    b->strideL = round_up(N * 256, 256);
-    b->strideY = 256 * 256;
-    LINAC_TRY(ctx->get(SLOT_PANEL_B, (size_t)(4 * b->strideL + 4 * b->strideY), &p));
-    b->planesL = (int8_t*)p;
-    b->planesY = b->planesL + 4 * b->strideL;
-    const int64_t nb = std::max<int64_t>(M, XLD);
-    b->strideB = round_up(nb * 256, 256);
-    b->ldg = round_up(M, 64);
-    const size_t bytesB = (size_t)(4 * b->strideB);
-    const size_t bytesU = (size_t)CI * XLD * 4;
-    const size_t bytesG = (size_t)256 * b->ldg * 4;
-    LINAC_TRY(ctx->get(SLOT_PANEL_C, bytesB + bytesU + bytesG, &p));
-    b->planesB = (int8_t*)p;
-    b->U = (uint32_t*)((char*)p + bytesB);
-    b->Gq = (uint32_t*)((char*)p + bytesB + bytesU);
-    const size_t aux = 2 * (size_t)round_up(4 * N, 256) + sizeof(BlockState) + 256;
-    LINAC_TRY(ctx->get(SLOT_AUX, aux, &p));
-    b->slot_of = (int32_t*)p;
-    b->epoch_of = (int32_t*)((char*)p + round_up(4 * N, 256));
-    b->bs = (BlockState*)((char*)p + 2 * round_up(4 * N, 256));
+__global__ void k_gather_status(const BMat* __restrict__ D, ItemStatus* out) {
+    const BMat& d = D[blockIdx.x];
+    ItemStatus s;
+    s.c0_next = d.bs->c0_next;
+    s.bcur = d.bs->bcur;
+    s.q0 = d.bs->q0;
+    s.pad = 0;
+    s.rank = d.st->rank;
+    out[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------
+// host engine: a set of matrices advanced together, one outer panel per step
+// ---------------------------------------------------------------------------
+
+struct WorkspaceLayout {
+    size_t X, L, Y, B, U, G, bs, st, piv, used, epoch, slot, total;
+};
+
+static WorkspaceLayout workspace_layout(int64_t N, int64_t M, bool with_elim_state) {
+    auto al = [](size_t x) { return (size_t)round_up((int64_t)x, 256); };
+    WorkspaceLayout w{};
+    const int64_t cap = std::max<int64_t>(1, std::min(N, M));
+    size_t off = 0;
+    w.X = off; off += al((size_t)N * XLD * 4);
+    w.L = off; off += al((size_t)4 * round_up(N * 256, 256));
+    w.Y = off; off += al((size_t)4 * STRIDE_Y);
+    w.B = off; off += al((size_t)4 * round_up(std::max<int64_t>(M, XLD) * 256, 256));
+    w.U = off; off += al((size_t)CI * XLD * 4);
+    w.G = off; off += al((size_t)256 * round_up(M, 64) * 4);
+    w.bs = off; off += al(sizeof(BlockState));
+    w.epoch = off; off += al((size_t)4 * N);
+    w.slot = off; off += al((size_t)4 * N);
+    if (with_elim_state) {
+        w.st = off; off += al(sizeof(ElimState));
+        w.piv = off; off += 2 * al((size_t)cap * 8);
+        w.used = off; off += al((size_t)N);
+    }
+    w.total = off;
+    return w;
+}
+
+static void carve(BMat& d, char* base, const WorkspaceLayout& w, int64_t N, int64_t M, bool with_elim_state) {
+    d.X = (uint32_t*)(base + w.X);
+    d.planesL = (int8_t*)(base + w.L);
+    d.strideL = round_up(N * 256, 256);
+    d.planesY = (int8_t*)(base + w.Y);
+    d.planesB = (int8_t*)(base + w.B);
+    d.strideB = round_up(std::max<int64_t>(M, XLD) * 256, 256);
+    d.U = (uint32_t*)(base + w.U);
+    d.Gq = (uint32_t*)(base + w.G);
+    d.ldg = round_up(M, 64);
+    d.bs = (BlockState*)(base + w.bs);
+    d.epoch_of = (int32_t*)(base + w.epoch);
+    d.slot_of = (int32_t*)(base + w.slot);
+    if (with_elim_state) {
+        const int64_t cap = std::max<int64_t>(1, std::min(N, M));
+        d.st = (ElimState*)(base + w.st);
+        d.piv_rows = (int64_t*)(base + w.piv);
+        d.piv_cols = (int64_t*)(base + w.piv + round_up(cap * 8, 256));
+        d.used = (uint8_t*)(base + w.used);
+    }
+}
+
+struct Item {
+    BMat d{};
+    void* arena = nullptr;     // device memory owned by the item (null when borrowed)
+    size_t bytes = 0;
+    bool finished = false;
+    bool taken = false;
+    int64_t id = -1;
+};
+
+class Engine {
+  public:
+    Engine() = default;
+    ~Engine() {
+        for (int k = 0; k < 2; ++k) {
+            if (d_buf[k]) (void)hipFree(d_buf[k]);
+            if (h_buf[k]) (void)hipHostFree(h_buf[k]);
+        }
+    }
+    // One outer panel for every unfinished item in `items`.
+    int step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof);
+    int init_items(const std::vector<Item*>& items, hipStream_t s);
+    int epoch = 0;
+    bool first_step = true;
+
+  private:
+    int stage(int slot, size_t bytes);
+    int upload(const std::vector<Item*>& act, hipStream_t s, int sched_kind);
+    // Two staging slots (pinned host + device): slot 0 carries the descriptors
+    // of the inner phase, slot 1 those of the outer update.  A slot is
+    // rewritten only after the stream has been synchronised past its last
+    // copy (the per-step status read), so no pending copy reads a buffer the
+    // host is filling.
+    void* d_buf[2] = {nullptr, nullptr};
+    void* h_buf[2] = {nullptr, nullptr};
+    size_t cap[2] = {0, 0};
+    BMat* d_desc = nullptr;
+    int64_t* d_sched = nullptr;
+    ItemStatus* d_status = nullptr;
+    const int64_t* h_sched = nullptr;
+};
+
+int Engine::stage(int slot, size_t bytes) {
+    if (bytes <= cap[slot]) return LINAC_OK;
+    if (d_buf[slot]) LINAC_HIP(hipFree(d_buf[slot]));
+    if (h_buf[slot]) LINAC_HIP(hipHostFree(h_buf[slot]));
+    d_buf[slot] = h_buf[slot] = nullptr;
+    cap[slot] = (size_t)round_up((int64_t)bytes * 2, 4096);
+    LINAC_HIP(hipMalloc(&d_buf[slot], cap[slot]));
+    LINAC_HIP(hipHostMalloc(&h_buf[slot], cap[slot], hipHostMallocDefault));
     return LINAC_OK;
 }
 
-static unsigned grid1(int64_t n, int64_t per = 256, int64_t cap = 65536) {
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cap, ceil_div(n, per)));
+// Staging layout: [descriptors B][sched 6B+3 int64][status B].  sched_kind:
+// 0 = inner GEMM tiles (pre_a, mts_a), 1 = outer GEMM tiles (pre_b, mts_b for
+// Z[Q]A[Q,T]; pre_c, mts_c for the main update).
+int Engine::upload(const std::vector<Item*>& act, hipStream_t s, int sched_kind) {
+    const int slot = sched_kind;
+    const size_t B = act.size();
+    const size_t desc_b = (size_t)round_up((int64_t)(B * sizeof(BMat)), 256);
+    const size_t sched_b = (size_t)round_up((int64_t)((6 * B + 3) * 8), 256);
+    const size_t stat_b = (size_t)round_up((int64_t)(B * sizeof(ItemStatus)), 256);
+    LINAC_TRY(stage(slot, desc_b + sched_b + stat_b));
+    char* h = (char*)h_buf[slot];
+    char* dv = (char*)d_buf[slot];
+    d_desc = (BMat*)dv;
+    d_sched = (int64_t*)(dv + desc_b);
+    d_status = (ItemStatus*)(dv + desc_b + sched_b);
+    BMat* hd = (BMat*)h;
+    int64_t* hs = (int64_t*)(h + desc_b);
+    h_sched = hs;
+    for (size_t m = 0; m < B; ++m) hd[m] = act[m]->d;
+    int64_t* pa = hs;
+    int64_t* ma = pa + (B + 1);
+    int64_t* pb = ma + B;
+    int64_t* mb = pb + (B + 1);
+    int64_t* pc = mb + B;
+    int64_t* mc = pc + (B + 1);
+    pa[0] = pb[0] = pc[0] = 0;
+    for (size_t m = 0; m < B; ++m) {
+        const BMat& d = act[m]->d;
+        if (sched_kind == 0) {
+            ma[m] = ceil_div(d.N, G_M);
+            pa[m + 1] = pa[m] + ma[m] * (XLD / G_N);
+            mb[m] = mc[m] = 1;
+            pb[m + 1] = pc[m + 1] = 0;
+        } else {
+            const bool live = d.q0 > 0 && d.nT > 0;
+            mb[m] = std::max<int64_t>(1, ceil_div(d.q0, G_M));
+            pb[m + 1] = pb[m] + (live ? ceil_div(d.q0, G_M) * ceil_div(d.nT, G_N) : 0);
+            mc[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
+            pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, G_N) : 0);
+            ma[m] = 1;
+            pa[m + 1] = 0;
+        }
+    }
+    LINAC_HIP(hipMemcpyAsync(d_buf[slot], h_buf[slot], desc_b + sched_b, hipMemcpyHostToDevice, s));
+    return LINAC_OK;
 }
 
-// One inner-step group.  Which inner panel it works on is decided on the
-// device (BlockState::c0_next); a group launched after the outer panel is
-// complete does nothing.
-static int inner_group(const BlockedBuffers& b, int64_t N, int epoch, int64_t J, const FieldU32& f, double pd,
-                       double pinv, hipStream_t s, const ElimWorkspace& ws) {
-    k_inner_factor<<<1, IF_T, 0, s>>>(b.X, N, ws.used, b.slot_of, b.epoch_of, epoch, J, f, ws.st, ws.piv_rows,
-                                     ws.piv_cols, b.bs);
-    k_make_u<<<XLD / 4, dim3(64, 4), 0, s>>>(b.X, f, b.bs, b.U, b.planesB, b.strideB);
-    k_gemm_inner<<<(unsigned)(ceil_div(ceil_div(N, G3_M) * (XLD / G3_N), NUM_XCD) * NUM_XCD), G3_T, 0, s>>>(
-        b.X, N, b.planesB, b.strideB, b.U, b.bs, b.epoch_of, epoch, b.slot_of, f.p, pd, pinv);
+static unsigned gs_grid(int64_t work, int64_t cap = 4096) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cap, ceil_div(work, 256)));
+}
+
+int Engine::init_items(const std::vector<Item*>& items, hipStream_t s) {
+    if (items.empty()) return LINAC_OK;
+    LINAC_TRY(upload(items, s, 0));
+    int64_t maxw = 1;
+    for (auto* it : items) maxw = std::max<int64_t>(maxw, it->d.N * it->d.M);
+    k_item_init<<<dim3(gs_grid(maxw), (unsigned)items.size()), 256, 0, s>>>(d_desc);
     LINAC_HIP(hipGetLastError());
+    LINAC_HIP(hipStreamSynchronize(s));  // the next upload reuses this staging slot
     return LINAC_OK;
 }
 
+int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof) {
+    std::vector<Item*> act;
+    for (auto* it : items)
+        if (!it->finished) act.push_back(it);
+    if (act.empty()) return LINAC_OK;
+    const unsigned B = (unsigned)act.size();
+    int64_t maxN = 1, maxNX = 1, ninner = 1;
+    for (auto* it : act) {
+        BMat& d = it->d;
+        d.bcur = (int32_t)std::min<int64_t>(BO, d.M - d.J);
+        d.e0 = epoch;
+        d.q0 = 0;
+        d.Kq = 0;
+        d.nT = 0;
+        maxN = std::max(maxN, d.N);
+        maxNX = std::max(maxNX, d.N * XLD);
+        ninner = std::max<int64_t>(ninner, ceil_div(d.bcur, BI));
+    }
+    LINAC_TRY(upload(act, s, 0));
+    k_panel_load<<<dim3(gs_grid(maxNX, 8192), B), 256, 0, s>>>(d_desc);
+    k_block_state_init<<<B, 1, 0, s>>>(d_desc, first_step ? 1 : 0);
+    first_step = false;
+    const int64_t inner_tiles = h_sched[B];
+    TileSched ti{d_sched, d_sched + (B + 1), (int)B};
+    auto group = [&]() -> int {
+        k_inner_factor<<<B, IF_T, 0, s>>>(d_desc, epoch);
+        k_make_u<<<dim3(XLD / 4, B), dim3(64, 4), 0, s>>>(d_desc);
+        k_gemm_inner<<<(unsigned)(ceil_div(inner_tiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc, ti, epoch);
+        ++epoch;
+        LINAC_HIP(hipGetLastError());
+        return LINAC_OK;
+    };
+    for (int64_t i = 0; i < ninner; ++i) LINAC_TRY(group());
+    // one synchronisation per step; extra groups only for rank-deficient panels
+    std::vector<ItemStatus> st(B);
+    for (int64_t guard = 0;; ++guard) {
+        k_gather_status<<<B, 1, 0, s>>>(d_desc, d_status);
+        LINAC_HIP(hipMemcpyAsync(st.data(), d_status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipStreamSynchronize(s));
+        int64_t behind = 0;
+        for (unsigned m = 0; m < B; ++m)
+            if (st[m].c0_next < st[m].bcur) behind = std::max<int64_t>(behind, ceil_div(st[m].bcur - st[m].c0_next, BI));
+        if (behind == 0) break;
+        if (guard > (int64_t)BO * (BI + 1)) return fail(LINAC_ERR_DEVICE, "blocked elimination did not converge");
+        for (int64_t i = 0; i < behind; ++i) LINAC_TRY(group());
+    }
+    // outer update of the trailing columns
+    bool any_outer = false;
+    int64_t maxnT = 1;
+    double macs = 0, bytes = 0;
+    for (unsigned m = 0; m < B; ++m) {
+        BMat& d = act[m]->d;
+        d.q0 = st[m].q0;
+        d.rank_start = st[m].rank - st[m].q0;
+        d.T0 = d.J + d.bcur;
+        d.nT = d.M - d.T0;
+        d.Kq = (int32_t)round_up(std::max(1, d.q0), 64);
+        if (d.q0 > 0 && d.nT > 0) {
+            any_outer = true;
+            maxnT = std::max(maxnT, d.nT);
+            macs += (double)(d.N - d.q0) * d.q0 * d.nT;
+            bytes += 8.0 * d.N * d.nT;
+        }
+    }
+    LINAC_TRY(upload(act, s, 1));
+    if (any_outer) {
+        const int64_t* hs = h_sched;
+        const int64_t gq_tiles = hs[(B + 1) + B + B];
+        const int64_t main_tiles = hs[2 * (B + 1) + 2 * B + B];
+        TileSched tq{d_sched + (B + 1) + B, d_sched + 2 * (B + 1) + B, (int)B};
+        TileSched tm{d_sched + 2 * (B + 1) + 2 * B, d_sched + 3 * (B + 1) + 2 * B, (int)B};
+        k_make_lhs<<<dim3(gs_grid(maxN * 256, 16384), B), 256, 0, s>>>(d_desc, 0, epoch);
+        k_make_lhs<<<dim3(gs_grid(256 * 256), B), 256, 0, s>>>(d_desc, 1, epoch);
+        k_make_rhs<<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(maxnT, 64)), 4, B), 256, 0, s>>>(d_desc);
+        if (gq_tiles > 0)
+            k_gemm_outer<0><<<(unsigned)(ceil_div(gq_tiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc, tq);
+        if (prof) LINAC_TRY(prof->sample_begin(s, act[0]->d.J, bytes, macs));
+        if (main_tiles > 0)
+            k_gemm_outer<1><<<(unsigned)(ceil_div(main_tiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc, tm);
+        if (prof) LINAC_TRY(prof->sample_end(s));
+        k_scatter_q<<<dim3((unsigned)std::min<int64_t>(64, ceil_div(maxnT, 256)), 256, B), 256, 0, s>>>(d_desc);
+    }
+    int64_t maxpanel = 1;
+    for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
+    k_panel_store<<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, s>>>(d_desc);
+    LINAC_HIP(hipGetLastError());
+    for (auto* it : act) {
+        it->d.J += it->d.bcur;
+        if (it->d.J >= it->d.M) it->finished = true;
+    }
+    return LINAC_OK;
+}
+
+static void describe(BMat& d, uint32_t* A, int64_t ld, int64_t N, int64_t M, const FieldU32& f) {
+    d.A = A;
+    d.ld = ld;
+    d.N = N;
+    d.M = M;
+    d.f = f;
+    d.pd = (double)f.p;
+    d.pinv = 1.0 / (double)f.p;
+    d.J = 0;
+}
+
+static void print_timers(const BMat& d, DeviceContext* ctx) {
+    unsigned long long tp[8];
+    if (hipMemcpy(tp, d.bs->tphase, sizeof(tp), hipMemcpyDeviceToHost) != hipSuccess || tp[5] == 0) return;
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess) return;
+    const double us = 1e3 / (double)rate_khz;
+    fprintf(stderr, "k_inner_factor phases over %llu calls (us/call): scan %.2f load %.2f gj %.2f norm %.2f out %.2f\n",
+            tp[5], tp[0] * us / tp[5], tp[1] * us / tp[5], tp[2] * us / tp[5], tp[3] * us / tp[5],
+            tp[4] * us / tp[5]);
+}
+
+// Single matrix through the engine (a batch of one); workspace from the
+// context's cached scratch, elimination state from `ws`.
 static int blocked_run_u32(uint32_t* A, int64_t ld, int64_t N, int64_t M, const FieldU32& f, hipStream_t s,
                            const ElimWorkspace& ws, DeviceContext* ctx, Profiler* prof) {
-    BlockedBuffers b;
-    LINAC_TRY(blocked_alloc(ctx, N, M, &b));
-    const double pd = (double)f.p, pinv = 1.0 / (double)f.p;
-    k_epoch_init<<<grid1(N), 256, 0, s>>>(b.epoch_of, N);
-    int epoch = 0;
-    for (int64_t J = 0; J < M; J += BO) {
-        const int Bcur = (int)std::min<int64_t>(BO, M - J);
-        k_panel_load<<<grid1(N * XLD), 256, 0, s>>>(b.X, A, ld, N, J, Bcur);
-        k_block_state_init<<<1, 1, 0, s>>>(b.bs, Bcur, J == 0);
-        const int e0 = epoch;
-        const int ninner = (int)ceil_div(Bcur, BI);
-        for (int i = 0; i < ninner; ++i) LINAC_TRY(inner_group(b, N, epoch++, J, f, pd, pinv, s, ws));
-        // one synchronisation per outer panel; extra groups only when an inner
-        // panel needed more than one round (rank-deficient panels)
-        BlockState hs;
-        for (int64_t guard = 0;; ++guard) {
-            LINAC_HIP(hipMemcpyAsync(&hs, b.bs, offsetof(BlockState, R), hipMemcpyDeviceToHost, s));
-            LINAC_HIP(hipStreamSynchronize(s));
-            if (hs.c0_next >= Bcur) break;
-            if (guard > (int64_t)BO * (BI + 1)) return fail(LINAC_ERR_DEVICE, "blocked elimination did not converge");
-            const int extra = std::max(1, (int)ceil_div(Bcur - hs.c0_next, BI));
-            for (int i = 0; i < extra; ++i) LINAC_TRY(inner_group(b, N, epoch++, J, f, pd, pinv, s, ws));
-        }
-        LINAC_HIP(hipMemcpyAsync(ctx->pinned, &ws.st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        LINAC_HIP(hipStreamSynchronize(s));
-        const int64_t rank_end = ctx->pinned[0];
-        const int q0 = hs.q0;
-        const int64_t rank_start = rank_end - q0;
-        const int64_t T0 = J + Bcur, nT = M - T0;
-        if (q0 > 0 && nT > 0) {
-            const int Kq = (int)round_up(q0, 64);
-            const int64_t* Q = ws.piv_rows + rank_start;
-            k_make_lhs<<<grid1(N * Kq, 256, 1 << 30), 256, 0, s>>>(b.X, XLD, N, nullptr, BO, nullptr, nullptr, nullptr, q0, Kq,
-                                                                    b.epoch_of, e0, epoch, 1, f.p, b.planesL,
-                                                                    b.strideL);
-            k_make_lhs<<<grid1((int64_t)q0 * Kq, 256, 1 << 30), 256, 0, s>>>(b.X, XLD, q0, Q, BO, nullptr, nullptr,
-                                                                              nullptr, q0, Kq, nullptr, 0, 0, 1, f.p,
-                                                                              b.planesY, b.strideY);
-            k_make_rhs<<<dim3((unsigned)ceil_div(nT, 64), (unsigned)(Kq / 64)), 256, 0, s>>>(
-                A, ld, Q, q0, T0, nT, Kq, f.p, b.planesB, b.strideB);
-            k_gemm_modp3<true><<<gemm3_grid(q0, nT), G3_T, 0, s>>>(b.Gq, b.ldg, q0, nT, b.planesY, b.strideY,
-                                                                   b.planesB, b.strideB, Kq, pd, pinv, nullptr,
-                                                                   nullptr, nullptr, 0);
-            if (prof)
-                LINAC_TRY(prof->sample_begin(s, J, 8.0 * (double)N * (double)nT,
-                                             (double)(N - q0) * (double)q0 * (double)nT));
-            k_gemm_modp3<false><<<gemm3_grid(N, nT), G3_T, 0, s>>>(A + T0, ld, N, nT, b.planesL, b.strideL,
-                                                                   b.planesB, b.strideB, Kq, pd, pinv, nullptr,
-                                                                   nullptr, nullptr, 0);
-            if (prof) LINAC_TRY(prof->sample_end(s));
-            k_scatter_rows<<<dim3(grid1(nT, 256, 64), (unsigned)q0), 256, 0, s>>>(A, ld, Q, nullptr, q0, T0, nT,
-                                                                                b.Gq, b.ldg);
-        }
-        k_panel_store<<<grid1(N * Bcur), 256, 0, s>>>(A, b.X, ld, N, J, Bcur);
-        LINAC_HIP(hipGetLastError());
-    }
+    const WorkspaceLayout w = workspace_layout(N, M, false);
+    void* p = nullptr;
+    LINAC_TRY(ctx->get(SLOT_PANEL_A, w.total, &p));
+    Item it;
+    describe(it.d, A, ld, N, M, f);
+    carve(it.d, (char*)p, w, N, M, false);
+    it.d.st = ws.st;
+    it.d.piv_rows = ws.piv_rows;
+    it.d.piv_cols = ws.piv_cols;
+    it.d.used = ws.used;
+    Engine eng;
+    std::vector<Item*> items{&it};
+    // state and row flags were initialised by the caller; epochs here
+    k_epoch_init<<<gs_grid(N), 256, 0, s>>>(it.d.epoch_of, N);
+    while (!it.finished) LINAC_TRY(eng.step(items, s, prof));
     LINAC_HIP(hipStreamSynchronize(s));
-    if (getenv("LINAC_DEBUG_TIMING")) {
-        unsigned long long tp[8];
-        LINAC_HIP(hipMemcpy(tp, b.bs->tphase, sizeof(tp), hipMemcpyDeviceToHost));
-        int rate_khz = 0;
-        LINAC_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
-        const double us = 1e3 / (double)rate_khz;
-        fprintf(stderr, "k_inner_factor phases over %llu calls (us/call): scan %.2f load %.2f gj %.2f norm %.2f out %.2f\n",
-                tp[5], tp[0] * us / tp[5], tp[1] * us / tp[5], tp[2] * us / tp[5], tp[3] * us / tp[5], tp[4] * us / tp[5]);
-    }
+    if (getenv("LINAC_DEBUG_TIMING")) print_timers(it.d, ctx);
     return LINAC_OK;
 }
 
@@ -1036,4 +1120,201 @@ template int blocked_dispatch<FieldU64>(uint64_t*, int64_t, int64_t, int64_t, co
 template int blocked_dispatch<FieldGF2>(uint32_t*, int64_t, int64_t, int64_t, const FieldGF2&, hipStream_t,
                                         const ElimWorkspace&, DeviceContext*, bool, Profiler*);
 
+// ---------------------------------------------------------------------------
+// batch objects: matrices join and leave between steps (continuous batching)
+// ---------------------------------------------------------------------------
+
 }  // namespace linac
+
+struct linac_batch {
+    linac::DeviceContext* ctx = nullptr;
+    linac::Engine* eng = nullptr;
+    std::vector<linac::Item*> items;   // not yet taken
+    std::vector<linac::Item*> fresh;   // added since the last step (need init)
+    int64_t next_id = 0;
+    uint64_t cap = 0, in_use = 0;
+};
+
+namespace linac {
+
+static size_t item_bytes(int64_t rows, int64_t cols) {
+    const int64_t ldp = round_up(cols, 64);
+    return (size_t)round_up(rows * ldp * 4, 256) + workspace_layout(rows, cols, true).total;
+}
+
+static Item* find_item(linac_batch* b, int64_t id) {
+    for (auto* it : b->items)
+        if (it->id == id) return it;
+    return nullptr;
+}
+
+}  // namespace linac
+
+using namespace linac;
+
+extern "C" {
+
+int linac_batch_footprint_u32(int64_t rows, int64_t cols, uint64_t* bytes_out) {
+    if (rows < 0 || cols < 0 || !bytes_out) return fail(LINAC_ERR_INVALID, "bad arguments");
+    *bytes_out = (rows == 0 || cols == 0) ? 0 : (uint64_t)item_bytes(rows, cols);
+    return LINAC_OK;
+}
+
+int linac_batch_create(linac_batch** out, uint64_t mem_cap_bytes) {
+    if (!out) return fail(LINAC_ERR_INVALID, "out is NULL");
+    DeviceContext* ctx = nullptr;
+    LINAC_TRY(current_context(&ctx));
+    auto* b = new linac_batch();
+    b->ctx = ctx;
+    b->eng = new Engine();
+    b->cap = mem_cap_bytes;
+    *out = b;
+    return LINAC_OK;
+}
+
+int linac_batch_destroy(linac_batch* b) {
+    if (!b) return LINAC_OK;
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (auto* it : b->items) {
+        if (it->arena) (void)hipFree(it->arena);
+        delete it;
+    }
+    delete b->eng;
+    delete b;
+    return LINAC_OK;
+}
+
+int linac_batch_memory(linac_batch* b, uint64_t* in_use_out, uint64_t* cap_out) {
+    if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
+    if (in_use_out) *in_use_out = b->in_use;
+    if (cap_out) *cap_out = b->cap;
+    return LINAC_OK;
+}
+
+int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p,
+                        uint32_t flags, int64_t* id_out) {
+    if (!b || !id_out) return fail(LINAC_ERR_INVALID, "batch or id_out is NULL");
+    if (rows <= 0 || cols <= 0) return fail(LINAC_ERR_INVALID, "empty matrix");
+    if (ld < cols) return fail(LINAC_ERR_INVALID, "ld < cols");
+    if (!a) return fail(LINAC_ERR_INVALID, "matrix pointer is NULL");
+    if (!is_prime_u64(p) || p == 2) return fail(LINAC_ERR_INVALID, "field characteristic %u is not an odd prime", p);
+    if (p > LIMB_P_MAX) return fail(LINAC_ERR_INVALID, "p = %u exceeds the batched path's limit %llu", p,
+                                    (unsigned long long)LIMB_P_MAX);
+    const size_t need = item_bytes(rows, cols);
+    if (b->cap && b->in_use + need > b->cap)
+        return fail(LINAC_ERR_NOMEM, "admission: %zu bytes needed, %llu of %llu in use", need,
+                    (unsigned long long)b->in_use, (unsigned long long)b->cap);
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    hipStream_t s = b->ctx->stream;
+    auto* it = new Item();
+    hipError_t e = hipMalloc(&it->arena, need);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        delete it;
+        return fail(LINAC_ERR_NOMEM, "device allocation of %zu bytes failed", need);
+    }
+    it->bytes = need;
+    const int64_t ldp = round_up(cols, 64);
+    char* base = (char*)it->arena;
+    uint32_t* A = (uint32_t*)base;
+    hipError_t ce = hipMemcpy2DAsync(A, ldp * 4, a, ld * 4, cols * 4, rows,
+                                     (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
+    if (ce != hipSuccess) {
+        (void)hipFree(it->arena);
+        delete it;
+        return fail(LINAC_ERR_DEVICE, "copy into the batch failed: %s", hipGetErrorString(ce));
+    }
+    describe(it->d, A, ldp, rows, cols, make_field_u32(p));
+    carve(it->d, base + round_up(rows * ldp * 4, 256), workspace_layout(rows, cols, true), rows, cols, true);
+    it->id = b->next_id++;
+    b->in_use += need;
+    b->items.push_back(it);
+    b->fresh.push_back(it);
+    *id_out = it->id;
+    return LINAC_OK;
+}
+
+int linac_batch_step(linac_batch* b, int64_t* unfinished_out) {
+    if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    hipStream_t s = b->ctx->stream;
+    LINAC_TRY(b->eng->init_items(b->fresh, s));
+    b->fresh.clear();
+    Profiler* prof = nullptr;
+    LINAC_TRY(b->eng->step(b->items, s, prof));
+    LINAC_HIP(hipStreamSynchronize(s));
+    int64_t n = 0;
+    for (auto* it : b->items) n += it->finished ? 0 : 1;
+    if (unfinished_out) *unfinished_out = n;
+    return LINAC_OK;
+}
+
+int linac_batch_run(linac_batch* b) {
+    int64_t n = 1;
+    while (n > 0) LINAC_TRY(linac_batch_step(b, &n));
+    return LINAC_OK;
+}
+
+int linac_batch_status(linac_batch* b, int64_t id) {
+    if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
+    Item* it = find_item(b, id);
+    if (!it) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)id);
+    return it->finished ? 1 : 0;
+}
+
+int linac_batch_take_u32(linac_batch* b, int64_t id, uint32_t* out, int64_t ld_out, uint32_t flags,
+                         int64_t* pivots_out, int64_t* rank_out) {
+    if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
+    Item* it = find_item(b, id);
+    if (!it) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)id);
+    if (!it->finished) return fail(LINAC_ERR_INVALID, "matrix %lld is not finished", (long long)id);
+    if (out && ld_out < it->d.M) return fail(LINAC_ERR_INVALID, "ld_out < cols");
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    hipStream_t s = b->ctx->stream;
+    ElimWorkspace ws{it->d.st, it->d.piv_rows, it->d.piv_cols, it->d.used, std::max<int64_t>(1, std::min(it->d.N, it->d.M))};
+    int64_t rank = 0;
+    std::vector<int64_t> pc;
+    LINAC_TRY(gf_finalize(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, true, &rank, &pc));
+    if (out) {
+        LINAC_HIP(hipMemcpy2DAsync(out, ld_out * 4, it->d.A, it->d.ld * 4, it->d.M * 4, it->d.N,
+                                   (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipStreamSynchronize(s));
+    }
+    if (pivots_out)
+        for (int64_t k = 0; k < rank; ++k) pivots_out[k] = pc[k];
+    if (rank_out) *rank_out = rank;
+    b->in_use -= it->bytes;
+    (void)hipFree(it->arena);
+    b->items.erase(std::find(b->items.begin(), b->items.end(), it));
+    delete it;
+    return LINAC_OK;
+}
+
+int linac_rref_batch_u32(uint32_t* const* mats, const int64_t* rows, const int64_t* cols, const int64_t* lds,
+                         const uint32_t* primes, int64_t count, uint32_t flags, int64_t* const* pivots_out,
+                         int64_t* ranks_out) {
+    if (count < 0 || (count > 0 && (!mats || !rows || !cols || !lds || !primes)))
+        return fail(LINAC_ERR_INVALID, "bad batch arguments");
+    linac_batch* b = nullptr;
+    LINAC_TRY(linac_batch_create(&b, 0));
+    std::vector<int64_t> ids(count, -1);
+    int rc = LINAC_OK;
+    for (int64_t i = 0; i < count && rc == LINAC_OK; ++i) {
+        if (rows[i] == 0 || cols[i] == 0) {
+            if (ranks_out) ranks_out[i] = 0;
+            continue;
+        }
+        rc = linac_batch_add_u32(b, mats[i], rows[i], cols[i], lds[i], primes[i], flags, &ids[i]);
+    }
+    if (rc == LINAC_OK) rc = linac_batch_run(b);
+    for (int64_t i = 0; i < count && rc == LINAC_OK; ++i) {
+        if (ids[i] < 0) continue;
+        rc = linac_batch_take_u32(b, ids[i], mats[i], lds[i], flags, pivots_out ? pivots_out[i] : nullptr,
+                                  ranks_out ? ranks_out + i : nullptr);
+    }
+    const int rc2 = linac_batch_destroy(b);
+    return rc != LINAC_OK ? rc : rc2;
+}
+
+}  // extern "C"

@@ -284,6 +284,9 @@ int gf_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, 
     return LINAC_OK;
 }
 
+template int gf_finalize<FieldU32>(uint32_t*, int64_t, int64_t, int64_t, const FieldU32&, hipStream_t,
+                                   const ElimWorkspace&, DeviceContext*, bool, bool, int64_t*, std::vector<int64_t>*);
+
 // Shared entry: validation, host<->device staging, method dispatch.
 template <class F>
 static int rref_gf_entry(typename F::T* a, int64_t rows, int64_t cols, int64_t ld, const F& f, uint32_t flags,

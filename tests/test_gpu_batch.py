@@ -1,0 +1,106 @@
+"""Batched GF(p) elimination: every result equals the matrix solved alone and
+the oracle, across uneven shapes, different primes and joins between steps."""
+import numpy
+import pytest
+
+from oracle import crref, modp
+
+pytestmark = pytest.mark.gpu
+
+PRIMES = [2 ** 31 - 1, 2 ** 31 - 19, 2147483629, 2147483587, 2 ** 31 + 11, 65537, 3, 4278124237]  # last: largest limb prime
+
+
+def _mat(rows, cols, rank, p, seed):
+    if p < 2 ** 31:
+        return modp.low_rank(rows, cols, rank, p, seed).astype(numpy.uint32)
+    rs = numpy.random.RandomState(seed)
+    L = rs.randint(0, p, size=(rows, rank), dtype=numpy.int64).astype(object)
+    U = rs.randint(0, p, size=(rank, cols), dtype=numpy.int64).astype(object)
+    return numpy.array((L @ U) % p, dtype=numpy.uint32)
+
+
+def test_mixed_shapes_and_primes(gpu):
+    rs = numpy.random.RandomState(0)
+    mats, primes = [], []
+    for i in range(14):
+        rows, cols = int(rs.randint(1, 700)), int(rs.randint(1, 700))
+        rank = int(rs.randint(0, min(rows, cols) + 1))
+        p = PRIMES[i % len(PRIMES)]
+        A = _mat(rows, cols, max(rank, 1), p, i)
+        if i % 3 == 0 and cols > 5:
+            A[:, 2] = 0
+        mats.append(A)
+        primes.append(p)
+    results = gpu.rref_batch(mats, primes)
+    for A, p, (R, piv, rank) in zip(mats, primes, results):
+        Ro, po = crref.rref_modp(A, p)
+        assert piv == po and rank == len(po)
+        assert numpy.array_equal(R.astype(numpy.uint64), Ro.astype(numpy.uint64))
+
+
+def test_same_result_alone_and_in_the_mix(gpu):
+    p = 2 ** 31 - 1
+    rs = numpy.random.RandomState(1)
+    mats = [_mat(n, n + 7, n - 3, p, 10 + k) for k, n in enumerate([300, 1200, 64, 900, 1500])]
+    alone = [gpu.rref_with_pivots(A, p, method="blocked") for A in mats]
+    mixed = gpu.rref_batch(mats, p)
+    for (Ra, pa, ra), (Rm, pm, rm) in zip(alone, mixed):
+        assert pa == pm and ra == rm and numpy.array_equal(Ra, Rm)
+
+
+def test_continuous_joins_and_early_exits(gpu):
+    p = 2 ** 31 - 19
+    big = _mat(1300, 1300, 1250, p, 3)
+    small = [_mat(150, 170, 120, p, 4 + k) for k in range(4)]
+    with gpu.Batch() as b:
+        ib = b.add(big, p)
+        b.step()
+        b.step()
+        ids = [b.add(m, p) for m in small]  # join while the big one is half way
+        left = b.step()
+        assert all(b.done(i) for i in ids)   # 170 columns: one panel
+        taken = {i: b.take(i) for i in ids}  # leave before the big one finishes
+        assert not b.done(ib) and left >= 1
+        b.run()
+        Rb, pb, rb = b.take(ib)
+    for m, i in zip(small, ids):
+        Ro, po = crref.rref_modp(m, p)
+        assert taken[i][1] == po and numpy.array_equal(taken[i][0], Ro)
+    Ro, po = crref.rref_modp(big, p)
+    assert pb == po and numpy.array_equal(Rb, Ro)
+
+
+def test_admission_cap_and_errors(gpu):
+    from linac_amd.batch import footprint
+    fp = footprint(500, 500)
+    with gpu.Batch(mem_cap_bytes=2 * fp + 1) as b:
+        b.add(numpy.eye(500, dtype=numpy.uint32), 7)
+        b.add(numpy.eye(500, dtype=numpy.uint32), 7)
+        with pytest.raises(MemoryError):
+            b.add(numpy.eye(500, dtype=numpy.uint32), 7)
+        used, cap = b.memory()
+        assert used == 2 * fp and cap == 2 * fp + 1
+        with pytest.raises(ValueError):
+            b.take(0)  # not finished yet
+        with pytest.raises(ValueError):
+            b.done(99)
+        b.run()
+        R, piv, rank = b.take(0)
+        assert rank == 500 and numpy.array_equal(R, numpy.eye(500, dtype=numpy.uint32))
+        b.add(numpy.eye(500, dtype=numpy.uint32), 7)  # memory released by take
+    with pytest.raises(ValueError):
+        gpu.rref_batch([numpy.eye(3, dtype=numpy.uint32)], [2 ** 32 - 5])  # beyond the limb limit
+
+
+def test_loader_sweep_batched(gpu):
+    """Loader output for a sweep of evaluation points row-reduced as one batch."""
+    p = 2 ** 31 - 1
+    rs = numpy.random.RandomState(7)
+    S, rows, bl = 6, 400, 9
+    stack = rs.randint(0, p, size=(S, rows, bl)).astype(numpy.uint32)
+    lind = rs.randint(0, bl, size=(380, 6)).astype(numpy.uint32)
+    mats = gpu.load_matrices_stacked(stack, lind, p)
+    res = gpu.rref_batch(list(mats), p)
+    for A, (R, piv, rank) in zip(mats, res):
+        Ro, po = crref.rref_modp(A, p)
+        assert piv == po and numpy.array_equal(R, Ro)

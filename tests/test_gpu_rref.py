@@ -66,6 +66,7 @@ def test_full_rank_gives_identity(gpu, p, n, method):
     ((300, 300), 250, 2 ** 31 - 1, 0), ((257, 513), 200, 2 ** 31 - 19, 1), ((600, 170), 170, 65537, 2),
     ((129, 1000), 129, 2 ** 31 + 11, 3), ((400, 400), 17, 2 ** 61 - 1, 4), ((333, 444), 300, 2, 5),
     ((200, 300), 150, 3, 6), ((1000, 1000), 700, 2 ** 31 - 1, 7), ((64, 4096), 64, 2 ** 32 - 5, 8),
+    ((300, 700), 200, 4278124237, 9), ((520, 520), 520, 4278124237, 10),  # largest limb prime: sums wrap 32 bits
 ])
 @pytest.mark.parametrize("method", ["plain", "auto", "blocked"])
 def test_matches_oracle_bit_exact(gpu, shape, rank, p, seed, method):
