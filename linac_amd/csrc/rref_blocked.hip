@@ -368,6 +368,11 @@ __global__ __launch_bounds__(IF_T) void k_inner_factor(const BMat* __restrict__ 
 //   planesA[l][r][t] = digit l of X[r, c0 + Cp[t]]       (every row r)
 // The GEMM overwrites the pivot columns X[:, c0 + Cp] in place, so its left
 // operand must be captured here, before any of its workgroups runs.
+// Workgroups [0, MU_BLOCKS) build U (4 columns each), the rest planesA
+// (MU_ROWS rows each, grid-strided).
+constexpr int MU_BLOCKS = XLD / 4;
+constexpr int MU_ROWS = 16;
+
 __device__ static void make_u_body(const uint32_t* __restrict__ X, int64_t N, FieldU32 f,
                                    const BlockState* __restrict__ bs, uint32_t* __restrict__ U,
                                    int8_t* __restrict__ planesU, int64_t strideU,
@@ -377,32 +382,45 @@ __device__ static void make_u_body(const uint32_t* __restrict__ X, int64_t N, Fi
     __shared__ int s_cp[CI];
     const int k = bs->k;
     if (k == 0) return;  // the inner GEMM skips steps without pivots
-    const int c0 = bs->c0_cur;
-    for (int x = threadIdx.x + threadIdx.y * 64; x < k * CI; x += 256) s_inv[x] = bs->apinv[x];
-    if (threadIdx.y == 0 && threadIdx.x < k) {
-        s_R[threadIdx.x] = bs->R[threadIdx.x];
-        s_cp[threadIdx.x] = bs->Cp[threadIdx.x];
-    }
-    __syncthreads();
     const int t = threadIdx.x;  // 0..63
+    if (blockIdx.x >= MU_BLOCKS) {
+        const int c0 = bs->c0_cur;
+        if (threadIdx.y == 0 && t < k) s_cp[t] = bs->Cp[t];
+        __syncthreads();
+        const int64_t col = t < k ? c0 + s_cp[t] : 0;
+        for (int64_t r0 = (int64_t)(blockIdx.x - MU_BLOCKS) * MU_ROWS + threadIdx.y; r0 < N;
+             r0 += (int64_t)(gridDim.x - MU_BLOCKS) * MU_ROWS) {
+            uint32_t v[MU_ROWS / 4];
+#pragma unroll
+            for (int i = 0; i < MU_ROWS / 4; ++i) {
+                const int64_t r = r0 + 4 * i;
+                v[i] = (t < k && r < N) ? X[r * XLD + col] : 0u;
+            }
+#pragma unroll
+            for (int i = 0; i < MU_ROWS / 4; ++i) {
+                const int64_t r = r0 + 4 * i;
+                if (r >= N) break;
+                int8_t a[4] = {0, 0, 0, 0};
+                if (t < k) balanced_digits(centred(v[i], f.p), a);
+#pragma unroll
+                for (int l = 0; l < 4; ++l) planesA[l * strideA + r * KPI + t] = a[l];
+            }
+        }
+        return;
+    }
+    for (int x = threadIdx.x + threadIdx.y * 64; x < k * CI; x += 256) s_inv[x] = bs->apinv[x];
+    if (threadIdx.y == 0 && t < k) s_R[t] = bs->R[t];
+    __syncthreads();
     const int n = blockIdx.x * 4 + threadIdx.y;
     uint32_t u = 0;
     if (t < k) {
-        for (int s = 0; s < k; ++s) {
-            u = f.add(u, f.mont(s_inv[t * CI + s], X[s_R[s] * XLD + n]));
-        }
+        for (int s = 0; s < k; ++s) u = f.add(u, f.mont(s_inv[t * CI + s], X[s_R[s] * XLD + n]));
         U[t * XLD + n] = u;
     }
     int8_t d[4] = {0, 0, 0, 0};
     if (t < k) balanced_digits(centred(u, f.p), d);
 #pragma unroll
     for (int l = 0; l < 4; ++l) planesU[l * strideU + (int64_t)n * KPI + t] = d[l];
-    for (int64_t r = (int64_t)blockIdx.x * 4 + threadIdx.y; r < N; r += (int64_t)gridDim.x * 4) {
-        int8_t a[4] = {0, 0, 0, 0};
-        if (t < k) balanced_digits(centred(X[r * XLD + c0 + s_cp[t]], f.p), a);
-#pragma unroll
-        for (int l = 0; l < 4; ++l) planesA[l * strideA + r * KPI + t] = a[l];
-    }
 }
 
 
@@ -984,10 +1002,11 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     k_block_state_init<<<B, 1, 0, s>>>(d_desc, first_step ? 1 : 0);
     first_step = false;
     const int64_t inner_tiles = h_sched[B];
+    const unsigned mu_grid = (unsigned)(MU_BLOCKS + std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)));
     TileSched ti{d_sched, d_sched + (B + 1), (int)B};
     auto group = [&]() -> int {
         k_inner_factor<<<B, IF_T, 0, s>>>(d_desc, epoch);
-        k_make_u<<<dim3(XLD / 4, B), dim3(64, 4), 0, s>>>(d_desc);
+        k_make_u<<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(d_desc);
         k_gemm_inner<<<(unsigned)(ceil_div(inner_tiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc, ti, epoch);
         ++epoch;
         LINAC_HIP(hipGetLastError());
