@@ -46,6 +46,7 @@ extern "C" {
 #define LINAC_METHOD_BLOCKED  0x20u  /* panel + exact mod-p MFMA trailing update    */
 #define LINAC_NO_FINALIZE     0x100u /* leave rows unpermuted (pivots_out says where)*/
 #define LINAC_PROFILE         0x200u /* time the call with HIP events (linac_profile_last) */
+#define LINAC_IN_PLACE        0x400u /* batch: work on the caller's device matrix, no copy */
 
 /* ---- library ---- */
 const char* linac_last_error(void);
@@ -103,7 +104,12 @@ int linac_batch_footprint_u32(int64_t rows, int64_t cols, uint64_t* bytes_out);
 int linac_batch_create(linac_batch** out, uint64_t mem_cap_bytes);
 int linac_batch_destroy(linac_batch* b);
 int linac_batch_memory(linac_batch* b, uint64_t* in_use_out, uint64_t* cap_out);
-/* Copies the matrix (host, or device with LINAC_DEVICE_PTR) into the batch. */
+/* Copies the matrix (host, or device with LINAC_DEVICE_PTR) into the batch.
+ * With LINAC_DEVICE_PTR | LINAC_IN_PLACE the batch works on the caller's
+ * device matrix itself (no copy; the footprint then excludes the matrix,
+ * linac_batch_footprint_u32 minus rows*round_up(cols,64)*4 rounded to 256):
+ * the buffer must stay allocated until the matrix is taken, and holds its RREF
+ * afterwards. */
 int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p,
                         uint32_t flags, int64_t* id_out);
 /* One panel for every unfinished matrix; *unfinished_out counts those left. */

@@ -5,6 +5,7 @@ There is no JIT and no CPU fallback: if the library is missing, every compute
 call raises ``RuntimeError`` naming the file it looked for.
 """
 import ctypes
+import importlib.util
 import os
 import threading
 
@@ -19,6 +20,7 @@ ERR_NOT_INVERTIBLE = -4
 ERR_FORKED = -5
 
 DEVICE_PTR = 0x1
+IN_PLACE = 0x400
 METHOD_AUTO = 0x00
 METHOD_PLAIN = 0x10
 METHOD_BLOCKED = 0x20
@@ -72,6 +74,22 @@ class LibraryMissing(RuntimeError):
     pass
 
 
+def _preload_torch_hip_runtime():
+    """Make torch's bundled HIP runtime the process's HIP runtime.
+
+    torch ships its own libamdhip64/libhsa-runtime64 with the same sonames as
+    /opt/rocm's, so whichever is loaded first serves the whole process.  torch
+    does not work on a runtime other than its own (it reports no GPU), while
+    this library runs on either; loading torch's first (without importing
+    torch) keeps both usable in one process whatever the import order."""
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
 def load():
     """Load (once) and return the ctypes library; raise if it is not built."""
     global _lib, _load_error
@@ -80,6 +98,7 @@ def load():
             return _lib
         if not os.path.exists(LIB_PATH):
             raise LibraryMissing(f"{LIB_PATH} is not built; run linac_amd.build.build() (hipcc, gfx950)")
+        _preload_torch_hip_runtime()
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name, None)

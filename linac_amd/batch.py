@@ -56,15 +56,19 @@ class Batch:
         rid = ctypes.c_int64(-1)
         _lib.check(self._lib.linac_batch_add_u32(self._h, a.ctypes.data_as(ctypes.c_void_p), rows, cols, cols,
                                                  int(p), 0, ctypes.byref(rid)))
-        self._shapes[rid.value] = (rows, cols)
+        self._shapes[rid.value] = (rows, cols, False)
         return rid.value
 
-    def add_device(self, ptr, rows, cols, ld, p):
-        """Add a uint32 matrix already in HBM (rows 16-byte aligned)."""
+    def add_device(self, ptr, rows, cols, ld, p, in_place=False):
+        """Add a uint32 matrix already in HBM (device address ``ptr``, row
+        stride ``ld`` elements).  ``in_place``: reduce it where it lies (no
+        copy); the buffer must outlive the matrix's stay in the batch and
+        holds its RREF once taken."""
         rid = ctypes.c_int64(-1)
+        flags = _lib.DEVICE_PTR | (_lib.IN_PLACE if in_place else 0)
         _lib.check(self._lib.linac_batch_add_u32(self._h, ctypes.c_void_p(ptr), rows, cols, ld, int(p),
-                                                 _lib.DEVICE_PTR, ctypes.byref(rid)))
-        self._shapes[rid.value] = (rows, cols)
+                                                 flags, ctypes.byref(rid)))
+        self._shapes[rid.value] = (rows, cols, bool(in_place))
         return rid.value
 
     def step(self):
@@ -88,13 +92,16 @@ class Batch:
         return used.value, cap.value
 
     def take(self, rid):
-        """(R, pivot_columns, rank) of a finished matrix; releases its memory."""
-        rows, cols = self._shapes[rid]
-        out = numpy.empty((rows, cols), dtype=numpy.uint32)
+        """(R, pivot_columns, rank) of a finished matrix; releases its memory.
+        For a matrix added in place, R is None: the RREF is in the caller's
+        device buffer."""
+        rows, cols, in_place = self._shapes[rid]
+        out = None if in_place else numpy.empty((rows, cols), dtype=numpy.uint32)
         piv = numpy.zeros(max(1, min(rows, cols)), dtype=numpy.int64)
         rank = ctypes.c_int64(0)
-        _lib.check(self._lib.linac_batch_take_u32(self._h, int(rid), out.ctypes.data_as(ctypes.c_void_p), cols, 0,
-                                                  piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+        _lib.check(self._lib.linac_batch_take_u32(self._h, int(rid),
+                                                  None if out is None else out.ctypes.data_as(ctypes.c_void_p),
+                                                  cols, 0, piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                                   ctypes.byref(rank)))
         del self._shapes[rid]
         return out, piv[:rank.value].tolist(), rank.value

@@ -1156,9 +1156,10 @@ struct linac_batch {
 
 namespace linac {
 
+static size_t matrix_bytes(int64_t rows, int64_t cols) { return (size_t)round_up(rows * round_up(cols, 64) * 4, 256); }
+
 static size_t item_bytes(int64_t rows, int64_t cols) {
-    const int64_t ldp = round_up(cols, 64);
-    return (size_t)round_up(rows * ldp * 4, 256) + workspace_layout(rows, cols, true).total;
+    return matrix_bytes(rows, cols) + workspace_layout(rows, cols, true).total;
 }
 
 static Item* find_item(linac_batch* b, int64_t id) {
@@ -1220,7 +1221,9 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
     if (!is_prime_u64(p) || p == 2) return fail(LINAC_ERR_INVALID, "field characteristic %u is not an odd prime", p);
     if (p > LIMB_P_MAX) return fail(LINAC_ERR_INVALID, "p = %u exceeds the batched path's limit %llu", p,
                                     (unsigned long long)LIMB_P_MAX);
-    const size_t need = item_bytes(rows, cols);
+    const bool in_place = (flags & LINAC_IN_PLACE) != 0;
+    if (in_place && !(flags & LINAC_DEVICE_PTR)) return fail(LINAC_ERR_INVALID, "LINAC_IN_PLACE needs LINAC_DEVICE_PTR");
+    const size_t need = item_bytes(rows, cols) - (in_place ? matrix_bytes(rows, cols) : 0);
     if (b->cap && b->in_use + need > b->cap)
         return fail(LINAC_ERR_NOMEM, "admission: %zu bytes needed, %llu of %llu in use", need,
                     (unsigned long long)b->in_use, (unsigned long long)b->cap);
@@ -1234,18 +1237,23 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
         return fail(LINAC_ERR_NOMEM, "device allocation of %zu bytes failed", need);
     }
     it->bytes = need;
-    const int64_t ldp = round_up(cols, 64);
     char* base = (char*)it->arena;
-    uint32_t* A = (uint32_t*)base;
-    hipError_t ce = hipMemcpy2DAsync(A, ldp * 4, a, ld * 4, cols * 4, rows,
-                                     (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
-    if (ce != hipSuccess) {
-        (void)hipFree(it->arena);
-        delete it;
-        return fail(LINAC_ERR_DEVICE, "copy into the batch failed: %s", hipGetErrorString(ce));
+    if (in_place) {
+        describe(it->d, const_cast<uint32_t*>(a), ld, rows, cols, make_field_u32(p));
+        carve(it->d, base, workspace_layout(rows, cols, true), rows, cols, true);
+    } else {
+        const int64_t ldp = round_up(cols, 64);
+        uint32_t* A = (uint32_t*)base;
+        hipError_t ce = hipMemcpy2DAsync(A, ldp * 4, a, ld * 4, cols * 4, rows,
+                                         (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
+        if (ce != hipSuccess) {
+            (void)hipFree(it->arena);
+            delete it;
+            return fail(LINAC_ERR_DEVICE, "copy into the batch failed: %s", hipGetErrorString(ce));
+        }
+        describe(it->d, A, ldp, rows, cols, make_field_u32(p));
+        carve(it->d, base + matrix_bytes(rows, cols), workspace_layout(rows, cols, true), rows, cols, true);
     }
-    describe(it->d, A, ldp, rows, cols, make_field_u32(p));
-    carve(it->d, base + round_up(rows * ldp * 4, 256), workspace_layout(rows, cols, true), rows, cols, true);
     it->id = b->next_id++;
     b->in_use += need;
     b->items.push_back(it);

@@ -143,20 +143,27 @@ __global__ __launch_bounds__(256) void k_finalize(const typename F::T* A, int64_
                                                   int64_t ld_out, int64_t N, int64_t M, F f,
                                                   const ElimState* __restrict__ st,
                                                   const int64_t* __restrict__ piv_rows,
-                                                  const int64_t* __restrict__ piv_cols, int normalize) {
+                                                  const int64_t* __restrict__ piv_cols, int normalize,
+                                                  int vec) {
     using T = typename F::T;
     constexpr int V = 16 / sizeof(T);
     using VT = Vec16<T>;
     __shared__ T s_scale;
     const int64_t k = blockIdx.x;
     const int64_t rank = st->rank;
+    // vec: rows are 16-byte aligned and padded to whole vectors (ld % V == 0),
+    // so a row may be written as round_up(M, V) elements; otherwise per element
     const int64_t nv = (M + V - 1) / V;
-    VT* o = reinterpret_cast<VT*>(out + k * ld_out);
+    T* orow = out + k * ld_out;
     if (k >= rank) {
-        VT z;
+        if (vec) {
+            VT z;
 #pragma unroll
-        for (int e = 0; e < V; ++e) z.v[e] = 0;
-        for (int64_t v = threadIdx.x; v < nv; v += blockDim.x) o[v] = z;
+            for (int e = 0; e < V; ++e) z.v[e] = 0;
+            for (int64_t v = threadIdx.x; v < nv; v += blockDim.x) reinterpret_cast<VT*>(orow)[v] = z;
+        } else {
+            for (int64_t c = threadIdx.x; c < M; c += blockDim.x) orow[c] = 0;
+        }
         return;
     }
     const T* s = A + piv_rows[k] * ld;
@@ -165,13 +172,17 @@ __global__ __launch_bounds__(256) void k_finalize(const typename F::T* A, int64_
         __syncthreads();
     }
     const T sc = normalize ? s_scale : (T)0;
-    for (int64_t v = threadIdx.x; v < nv; v += blockDim.x) {
-        VT x = reinterpret_cast<const VT*>(s)[v];
-        if (normalize) {
+    if (vec) {
+        for (int64_t v = threadIdx.x; v < nv; v += blockDim.x) {
+            VT x = reinterpret_cast<const VT*>(s)[v];
+            if (normalize) {
 #pragma unroll
-            for (int e = 0; e < V; ++e) x.v[e] = f.mont(sc, x.v[e]);
+                for (int e = 0; e < V; ++e) x.v[e] = f.mont(sc, x.v[e]);
+            }
+            reinterpret_cast<VT*>(orow)[v] = x;
         }
-        o[v] = x;
+    } else {
+        for (int64_t c = threadIdx.x; c < M; c += blockDim.x) orow[c] = normalize ? f.mont(sc, s[c]) : s[c];
     }
 }
 
@@ -265,16 +276,18 @@ int gf_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, 
     }
     bool identity = true;
     for (int64_t k = 0; k < rank && identity; ++k) identity = rows[k] == k;
+    constexpr int V = 16 / sizeof(T);
+    const int vec = (ld % V == 0 && ((uintptr_t)A & 15) == 0) ? 1 : 0;
     if (identity) {
         if (normalize || rank < N)
             k_finalize<F><<<(unsigned)N, 256, 0, s>>>(A, ld, A, ld, N, M, f, ws.st, ws.piv_rows, ws.piv_cols,
-                                                       normalize ? 1 : 0);
+                                                       normalize ? 1 : 0, vec);
     } else {
         void* outp = nullptr;
         LINAC_TRY(ctx->get(SLOT_OUT, (size_t)N * ld * sizeof(T), &outp));
         T* out = (T*)outp;
         k_finalize<F><<<(unsigned)N, 256, 0, s>>>(A, ld, out, ld, N, M, f, ws.st, ws.piv_rows, ws.piv_cols,
-                                                   normalize ? 1 : 0);
+                                                   normalize ? 1 : 0, vec);
         LINAC_HIP(hipMemcpyAsync(A, out, (size_t)N * ld * sizeof(T), hipMemcpyDeviceToDevice, s));
     }
     LINAC_HIP(hipGetLastError());
