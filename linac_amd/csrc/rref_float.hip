@@ -15,46 +15,14 @@
 #include <algorithm>
 #include <vector>
 
+#include "float_ops.hpp"
 #include "runtime.hpp"
 
 namespace linac {
 
-struct CplxF {
-    double re, im;
-};
-
 template <class T>
-struct FloatOps;
-
-template <>
-struct FloatOps<double> {
-    __device__ static double abs(double x) { return fabs(x); }
-    __device__ static double div(double a, double b) { return a / b; }
-    __device__ static double fms(double x, double f, double y) { return fma(-f, y, x); }  // x - f*y
-    __device__ static double scale(double x, double s) { return x / s; }
-    __device__ static double zero() { return 0.0; }
-    __device__ static bool is_zero(double x) { return x == 0.0; }
-};
-
-template <>
-struct FloatOps<CplxF> {
-    __device__ static double abs(CplxF z) { return hypot(z.re, z.im); }
-    __device__ static CplxF div(CplxF a, CplxF b) {
-        // Smith's algorithm: no overflow for large |b|
-        if (fabs(b.re) >= fabs(b.im)) {
-            const double r = b.im / b.re, d = b.re + b.im * r;
-            return CplxF{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
-        }
-        const double r = b.re / b.im, d = b.im + b.re * r;
-        return CplxF{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
-    }
-    __device__ static CplxF fms(CplxF x, CplxF f, CplxF y) {
-        return CplxF{x.re - (f.re * y.re - f.im * y.im), x.im - (f.re * y.im + f.im * y.re)};
-    }
-    __device__ static CplxF scale(CplxF x, double s) { return CplxF{x.re / s, x.im / s}; }
-    __device__ static CplxF zero() { return CplxF{0.0, 0.0}; }
-    __device__ static bool is_zero(CplxF z) { return z.re == 0.0 && z.im == 0.0; }
-};
+int float_blocked_run(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStream_t s, int64_t* rank,
+                      uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, Profiler* prof);
 
 struct FloatState {
     int64_t prow;
@@ -210,8 +178,25 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
     if (prof) LINAC_TRY(prof->begin(s));
     k_row_scale<T><<<(unsigned)rows, 256, 0, s>>>(A, lda, rows, cols);
     k_float_init<<<(unsigned)std::min<int64_t>(1024, ceil_div(rows, 256)), 256, 0, s>>>(st, used, rows);
+    const uint32_t method = flags & LINAC_METHOD_MASK;
+    if (method != LINAC_METHOD_PLAIN && method != LINAC_METHOD_AUTO && method != LINAC_METHOD_BLOCKED)
+        return fail(LINAC_ERR_INVALID, "unknown method flag 0x%x", method);
+    // blocked: panels of 64 columns with a cooperative pivot kernel and an
+    // FP64 matrix-core trailing update; the plain sweep stays the method for
+    // small matrices, where per-column launches cost less than the panels
+    bool blocked = method == LINAC_METHOD_BLOCKED || (method == LINAC_METHOD_AUTO && rows >= 256 && cols >= 128);
+    if (blocked) {
+        const int rc = float_blocked_run<T>(A, lda, rows, cols, tol, s, &st->rank, used, piv_rows, piv_cols, ctx, prof);
+        if (rc < 0) return rc;
+        if (rc == 1) {
+            if (method == LINAC_METHOD_BLOCKED)
+                return fail(LINAC_ERR_INVALID, "blocked float method does not fit this device for %lld rows",
+                            (long long)rows);
+            blocked = false;
+        }
+    }
     const int64_t stride = std::max<int64_t>(1, cols / 256);
-    for (int64_t j = 0; j < cols; ++j) {
+    for (int64_t j = 0; j < cols && !blocked; ++j) {
         k_float_select<T><<<1, 1024, 0, s>>>(A, lda, rows, j, tol, st, used, piv_rows, piv_cols);
         const int64_t w = cols - j - 1;
         const int log2G = w >= 1024 ? 8 : (w >= 256 ? 7 : 6);
@@ -225,7 +210,7 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
     LINAC_HIP(hipGetLastError());
     if (prof) {
         LINAC_TRY(prof->finish(s));
-        prof->result[5] = 1;
+        prof->result[5] = blocked ? 2 : 1;
     }
     void* outp = nullptr;
     LINAC_TRY(ctx->get(SLOT_OUT, (size_t)rows * lda * sizeof(T), &outp));
@@ -236,7 +221,7 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
     LINAC_HIP(hipStreamSynchronize(s));
     std::vector<int64_t> pc(rank);
     if (rank) LINAC_HIP(hipMemcpy(pc.data(), piv_cols, rank * 8, hipMemcpyDeviceToHost));
-    if (prof) {
+    if (prof && !blocked) {
         std::vector<char> is_piv(cols, 0);
         for (int64_t c : pc) is_piv[c] = 1;
         double bytes = 0, macs = 0;

@@ -24,7 +24,7 @@ def rref_float_host(a, method="auto", tol=TOLERANCE):
     piv = numpy.zeros(max(1, min(rows, cols)), dtype=numpy.int64)
     rank = ctypes.c_int64(0)
     fn = lib.linac_rref_f64 if a.dtype == numpy.float64 else lib.linac_rref_c128
-    flags = 0 if method in ("auto", "plain") else _lib.METHODS[method]
+    flags = _lib.METHODS[method]
     rc = fn(a.ctypes.data_as(ctypes.c_void_p), rows, cols, cols, tol, flags, None,
             piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(rank))
     _lib.check(rc)

@@ -127,17 +127,26 @@ def rref_device(ptr, rows, cols, ld, field_characteristic, dtype, method="auto",
                 profile=False):
     """In-place RREF of a matrix already in HBM (``ptr`` = device address).
 
-    ``dtype`` is numpy.uint32 or numpy.uint64; rows must be 16-byte aligned.
-    Returns (pivot_columns, rank).
+    ``dtype``: numpy.uint32 / numpy.uint64 (field_characteristic = p, rows
+    16-byte aligned) or numpy.float64 / numpy.complex128 (field_characteristic
+    0, pivot tolerance 1e-9).  Returns (pivot_columns, rank).
     """
     lib = _lib.load()
     p = int(field_characteristic)
+    dt = numpy.dtype(dtype)
     piv = numpy.zeros(max(1, min(rows, cols)), dtype=numpy.int64)
     rank = ctypes.c_int64(0)
     flags = _lib.DEVICE_PTR | _lib.METHODS[method] | (0 if finalize else _lib.NO_FINALIZE) | \
         (_lib.PROFILE if profile else 0)
-    fn = lib.linac_rref_u32 if numpy.dtype(dtype) == numpy.uint32 else lib.linac_rref_u64
-    rc = fn(ctypes.c_void_p(ptr), rows, cols, ld, p, flags, ctypes.c_void_p(stream) if stream else None,
-            piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(rank))
+    st = ctypes.c_void_p(stream) if stream else None
+    out = (piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(rank))
+    if dt.kind in 'fc':
+        if p != 0:
+            raise ValueError("float matrices need field_characteristic 0")
+        fn = lib.linac_rref_f64 if dt == numpy.float64 else lib.linac_rref_c128
+        rc = fn(ctypes.c_void_p(ptr), rows, cols, ld, 1e-9, flags, st, *out)
+    else:
+        fn = lib.linac_rref_u32 if dt == numpy.uint32 else lib.linac_rref_u64
+        rc = fn(ctypes.c_void_p(ptr), rows, cols, ld, p, flags, st, *out)
     _lib.check(rc)
     return piv[:rank.value].tolist(), rank.value

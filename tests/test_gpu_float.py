@@ -26,24 +26,31 @@ def test_real_input_uses_float64(gpu):
     assert R.dtype == numpy.float64 and numpy.allclose(R, numpy.identity(300))
 
 
+@pytest.mark.parametrize("method", ["plain", "blocked"])
 @pytest.mark.parametrize("n, kind", [(1000, "f64"), (700, "c128"), (1500, "f64")])
-def test_augmented_solution_matches_lapack(gpu, n, kind):
+def test_augmented_solution_matches_lapack(gpu, n, kind, method):
     rs = numpy.random.RandomState(n)
     A = rs.random_sample((n, n))
     b = rs.random_sample((n, 3))
     if kind == "c128":
         A = A + 1j * rs.standard_normal((n, n))
         b = b + 1j * rs.standard_normal((n, 3))
-    R = gpu.cuda_row_reduce(numpy.concatenate([A, b], axis=1), 0)
+    R = gpu.cuda_row_reduce(numpy.concatenate([A, b], axis=1), 0, method=method)
     x = numpy.linalg.solve(A, b)
     rel = numpy.max(numpy.abs(R[:, n:] - x)) / numpy.max(numpy.abs(x))
     assert rel < 1e-10, rel
     assert numpy.allclose(R[:, :n], numpy.identity(n), atol=1e-12)
 
 
-@pytest.mark.parametrize("shape, rank, seed, cplx", [((60, 80), 35, 0, False), ((120, 70), 50, 1, True),
-                                                     ((200, 200), 199, 2, False)])
-def test_rank_deficient_matches_float_oracle(gpu, shape, rank, seed, cplx):
+@pytest.mark.parametrize("method", ["plain", "blocked"])
+@pytest.mark.parametrize("shape, rank, seed, cplx", [
+    ((60, 80), 35, 0, False), ((120, 70), 50, 1, True), ((200, 200), 199, 2, False),
+    ((700, 900), 400, 3, False),   # several panels, rank-deficient panels
+    ((333, 517), 333, 4, True),    # ragged sizes (not multiples of 64)
+    ((3000, 150), 140, 5, False),  # tall: many workgroups in the panel kernel
+    ((90, 1500), 90, 6, False),    # wide: many panels, few rows
+])
+def test_rank_deficient_matches_float_oracle(gpu, shape, rank, seed, cplx, method):
     rs = numpy.random.RandomState(seed)
     L = rs.standard_normal((shape[0], rank))
     U = rs.standard_normal((rank, shape[1]))
@@ -52,7 +59,7 @@ def test_rank_deficient_matches_float_oracle(gpu, shape, rank, seed, cplx):
         U = U + 1j * rs.standard_normal(U.shape)
     A = L @ U
     A[:, 5] = 0
-    R, piv, rk = gpu.rref_with_pivots(A, 0)
+    R, piv, rk = gpu.rref_with_pivots(A, 0, method=method)
     Ro, po = (crref.rref_c128 if cplx else crref.rref_f64)(A)
     assert piv == po and rk == len(po)
     assert numpy.max(numpy.abs(R - Ro)) < 1e-9 * max(1.0, numpy.max(numpy.abs(Ro)))
@@ -68,3 +75,30 @@ def test_known_medium_pivot_structure(gpu, field):
     R = gpu.drop_bottom_zero_rows(R)
     assert R.shape == (31, 86)
     assert gpu.pivot_columns_from_row_reduced_echelon_form(R) == KNOWN_MEDIUM_PIVOTS
+
+
+@pytest.mark.parametrize("method", ["plain", "blocked"])
+def test_integer_valued_ties_and_zero_columns(gpu, method):
+    """Small integers make equal |x| common: the smallest row wins ties, as in
+    the oracle; whole zero columns are skipped.  (No zero rows here: the
+    oracle keeps the reference's division of a zero row by its zero maximum,
+    whose NaNs stop the elimination; this library leaves such rows at zero.)"""
+    rs = numpy.random.RandomState(7)
+    A = rs.randint(-2, 3, size=(400, 300)).astype(numpy.float64)
+    A[:, 10:20] = 0
+    R, piv, rk = gpu.rref_with_pivots(A, 0, method=method)
+    Ro, po = crref.rref_f64(A)
+    assert piv == po and rk == len(po) == 290
+    assert numpy.max(numpy.abs(R - Ro)) < 1e-9 * max(1.0, numpy.max(numpy.abs(Ro)))
+
+
+def test_blocked_equals_plain_on_rank_deficient_sweep(gpu):
+    """Both methods give the same pivots and agree to rounding on a matrix
+    whose rank deficiency spans several panels."""
+    rs = numpy.random.RandomState(8)
+    A = rs.standard_normal((640, 37)) @ rs.standard_normal((37, 700))
+    A = numpy.concatenate([A, rs.standard_normal((640, 5))], axis=1)
+    Rp, pp, rp = gpu.rref_with_pivots(A, 0, method="plain")
+    Rb, pb, rb = gpu.rref_with_pivots(A, 0, method="blocked")
+    assert pp == pb and rp == rb == 42
+    assert numpy.max(numpy.abs(Rp - Rb)) < 1e-9 * max(1.0, numpy.max(numpy.abs(Rp)))
