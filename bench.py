@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=1200)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched loader sweep (64 x 3000^2)")
     ap.add_argument("--sweep-systems", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run (2000 mixed requests)")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     return ap.parse_args()
 
 
@@ -113,6 +116,40 @@ def sweep_bench(n_systems, p=2 ** 31 - 1):
         "sequential_plain_s_per_solve": round(seq["plain"], 4),
         "sequential_blocked_s_per_solve": round(seq["blocked"], 4),
         "speedup_vs_sequential_plain": round(seq["plain"] * n_systems / total, 2),
+    }
+
+
+def serve_bench(n, rate):
+    """Serving: n requests with Poisson arrivals at `rate`/s, N log-uniform in
+    [200, 8000], cols N * U[0.8, 1.2], p from 8 seeded 31-bit primes, 10%
+    float64 uniform [0, 1); each matrix generated in HBM at its arrival;
+    admission by HBM footprint under 200 GB.  Saturation throughput (all
+    arrivals at t = 0) beside it, and sequential single plain calls on an
+    evenly spaced sample of the same requests as the comparator."""
+    import numpy
+    from linac_amd import serving
+    serving.serve(serving.make_requests(8, 0, seed=99, n_min=200, n_max=2000), seed=99)  # warm-up
+    reqs = serving.make_requests(n, rate, seed=0)
+    res, wall, st = serving.serve(reqs, seed=0)
+    timed = serving.latency_summary(res, wall)
+    sat_reqs = serving.make_requests(n, 0, seed=0)
+    res_s, wall_s, _ = serving.serve(sat_reqs, seed=0)
+    sat = serving.latency_summary(res_s, wall_s)
+    step = max(1, n // 50)
+    sample = [serving.Request(r.rid, r.rows, r.cols, r.p) for r in sat_reqs[::step]]
+    t_plain = serving.sequential_plain(sample, seed=0, method="plain")
+    t_auto = serving.sequential_plain(sample, seed=0, method="auto")
+    plain_rate = 1.0 / float(numpy.mean(t_plain))
+    return {
+        "workload": f"{n} requests, Poisson {rate}/s, N log-uniform [200, 8000], cols N*U[0.8,1.2], "
+                    "8 seeded 31-bit primes, 10% float64, matrices generated in HBM at arrival, 200 GB cap",
+        "at_rate": timed,
+        "saturated": sat,
+        "max_admitted_bytes": st["max_in_use_bytes"],
+        "sequential_plain_solves_per_s": round(plain_rate, 2),
+        "sequential_single_auto_solves_per_s": round(1.0 / float(numpy.mean(t_auto)), 2),
+        "sequential_sample": f"every {step}th request ({len(sample)}), same matrices",
+        "speedup_vs_sequential_plain": round(sat["solves_per_s"] / plain_rate, 2),
     }
 
 
@@ -218,6 +255,8 @@ def main():
     }
     if not args.no_sweep:
         out["batched"] = sweep_bench(args.sweep_systems)
+    if not args.no_serve:
+        out["serving"] = serve_bench(args.serve_requests, args.serve_rate)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, args.cpu_n)
     print(json.dumps(out), flush=True)

@@ -1,0 +1,66 @@
+"""Continuous-batching serving loop: every request's result is the same alone
+and in the mix (bit-identical for GF(p), float64 too: the float path runs
+each request by itself), admission holds the memory cap, and requests that
+arrive while others run join the batch."""
+import numpy
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(req, seed):
+    from linac_amd import serving
+    import torch
+    serving.materialise(req, seed)
+    if req.p == 0:
+        A = req.tensor.cpu().numpy()
+    else:
+        A = req.tensor.cpu().numpy().view(numpy.uint32)
+    req.tensor = None
+    from linac_amd import rref_with_pivots
+    method = "blocked" if req.p == 0 and req.rows >= 256 and req.cols >= 128 else \
+        ("plain" if req.p == 0 else "blocked")
+    return rref_with_pivots(A, req.p, method=method)
+
+
+def test_results_same_alone_and_in_the_mix(gpu):
+    from linac_amd import serving
+    reqs = serving.make_requests(40, rate=400.0, seed=5, n_min=20, n_max=700, float_share=0.2)
+    assert any(r.p == 0 for r in reqs) and any(r.p for r in reqs)
+    results, wall, stats = serving.serve(reqs, seed=5, keep=True)
+    assert [r.rid for r in results] == list(range(40))
+    for req, res in zip(sorted(reqs, key=lambda r: r.rid), results):
+        R, piv, rank = _alone(serving.Request(req.rid, req.rows, req.cols, req.p), 5)
+        got = res.tensor.cpu().numpy()
+        if req.p:
+            got = got.view(numpy.uint32)
+            assert numpy.array_equal(got, R), req
+        else:
+            assert numpy.array_equal(got, R), (req, float(numpy.max(numpy.abs(got - R))))
+        assert res.pivots == piv and res.rank == rank
+        assert res.finished >= res.arrival
+
+
+def test_admission_holds_the_cap_and_queues(gpu):
+    """A cap that fits about two of the requests: the loop still finishes
+    every request, never exceeds the cap, and later arrivals wait."""
+    from linac_amd import serving
+    reqs = serving.make_requests(12, rate=0, seed=7, n_min=300, n_max=400, float_share=0.25)
+    one = max(serving.Server(1)._bytes(r) for r in reqs)
+    cap = int(2.2 * one)
+    results, wall, stats = serving.serve(reqs, mem_cap_bytes=cap, seed=7)
+    assert len(results) == 12 and stats["max_in_use_bytes"] <= cap
+    assert stats["max_in_use_bytes"] >= one  # it did run requests side by side or back to back
+    for r in results:
+        assert r.rank > 0
+
+
+def test_request_larger_than_the_cap_is_rejected(gpu):
+    """A request whose footprint alone exceeds the cap can never be admitted:
+    the server reports it instead of spinning."""
+    from linac_amd import serving
+    srv = serving.Server(mem_cap_bytes=1000)
+    req = serving.materialise(serving.Request(0, 64, 64, serving.SERVING_PRIMES[0]))
+    with pytest.raises(MemoryError):
+        srv.submit(req)
+    srv.close()
