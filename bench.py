@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run (2000 mixed requests)")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--configs", action="store_true",
+                    help="instead of the headline line: one JSON line measuring every GF(p) and float configuration")
+    ap.add_argument("--with-100k", action="store_true", help="with --configs: include the 100000^2 u32 run (~40 GB)")
     return ap.parse_args()
 
 
@@ -153,8 +156,118 @@ def serve_bench(n, rate):
     }
 
 
+def _gf_device_case(n, p, rank_frac, method, seed=0, dtype="u32"):
+    """Seconds of an in-place device RREF of an n x n matrix uniform in [0, p)
+    (rank n w.h.p.), or of rank round(rank_frac * n): its last rows are
+    multiples c_i * row_i of the first ones (a low-rank product L U with
+    L = [I; diag(c)] selecting rows).  Checks the rank and that the result is
+    [I | *] over the pivot rows (the unique RREF when the first r columns are
+    independent, as they are w.h.p.)."""
+    import numpy
+    import torch
+    import linac_amd
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    tdt = torch.int32 if dtype == "u32" else torch.int64
+    A = torch.empty((n, n), dtype=tdt, device="cuda")
+    chunk = max(1, (1 << 28) // n)
+    for r0 in range(0, n, chunk):  # int64 draws in slices (a whole int64 copy would double the footprint)
+        A[r0:r0 + chunk] = torch.randint(0, p, (min(chunk, n - r0), n), generator=g, device="cuda",
+                                         dtype=torch.int64).to(tdt)
+    r = n if rank_frac >= 1 else int(round(rank_frac * n))
+    if r < n:
+        c = torch.randint(1, 1 << 20, (n - r, 1), generator=g, device="cuda", dtype=torch.int64)
+        A[r:] = ((A[:n - r].to(torch.int64) * c) % p).to(tdt)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    piv, rank = linac_amd.rref_device(A.data_ptr(), n, n, n, p, numpy.uint32 if dtype == "u32" else numpy.uint64,
+                                      method=method)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t
+    ok = rank == r and piv == list(range(r))
+    if ok:
+        top = A[:r, :r]
+        ok = bool(top.diagonal().eq(1).all().item()) and int(top.count_nonzero().item()) == r and \
+            int(A[r:].count_nonzero().item()) == 0
+    del A
+    torch.cuda.empty_cache()
+    return {"n": n, "p": p, "rank": rank, "method": method, "seconds": round(sec, 4), "correct": ok}
+
+
+def _float_case(kind, n, seed=0):
+    """[A | b] on the device through the blocked float path; error of the
+    solution column against numpy's FP64 LAPACK solve (the FP64 RREF of
+    [A | b] for square non-singular A), plus max |R[:, :n] - I|."""
+    import numpy
+    import torch
+    import linac_amd
+    rs = numpy.random.RandomState(seed)
+    if kind == "f64":
+        A = rs.random_sample((n, n + 1))
+    elif kind == "c128":
+        A = rs.standard_normal((n, n + 1)) + 1j * rs.standard_normal((n, n + 1))
+    else:  # ill-conditioned: U diag(logspace(0, -8)) V^T, U and V Haar (QR of Gaussians, sign-fixed)
+        def haar(m):
+            q, r = numpy.linalg.qr(rs.standard_normal((m, m)))
+            return q * numpy.sign(numpy.diag(r))
+        A = numpy.empty((n, n + 1))
+        A[:, :n] = (haar(n) * numpy.logspace(0, -8, n)) @ haar(n).T
+        A[:, n] = rs.random_sample(n)
+    x = numpy.linalg.solve(A[:, :n], A[:, n])
+    dt = torch.complex128 if kind == "c128" else torch.float64
+    d = torch.from_numpy(A).to("cuda", dtype=dt)
+    torch.cuda.synchronize()
+    out = {}
+    for method in ("blocked", "plain"):
+        w = d.clone()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        piv, rank = linac_amd.rref_device(w.data_ptr(), n, n + 1, n + 1, 0,
+                                          numpy.complex128 if kind == "c128" else numpy.float64, method=method)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t
+        R = w.cpu().numpy()
+        err = float(numpy.max(numpy.abs(R[:, n] - x)) / numpy.max(numpy.abs(x)))
+        out[method] = {"seconds": round(sec, 4), "rank": rank, "rel_err_solution_vs_lapack": float(f"{err:.3g}"),
+                       "max_abs_R_minus_I": float(f"{float(numpy.max(numpy.abs(R[:, :n] - numpy.eye(n)))):.3g}")}
+        del w
+    del d
+    torch.cuda.empty_cache()
+    cond = None
+    if kind == "illcond":
+        cond = 1e8
+    return {"kind": kind, "n": n, "cond_2": cond, **out}
+
+
+def configs_main(args):
+    """Every configuration of the GF(p) and float forms, device-resident."""
+    import torch
+    from linac_amd import _lib
+    torch.cuda.set_device(0)
+    _lib.check(_lib.load().linac_init(0))
+    p31, p61 = 2 ** 31 - 1, 2 ** 61 - 1
+    _gf_device_case(2000, p31, 1, "auto")  # warm-up
+    gf = []
+    for n in (5000, 10000, 20000):
+        for method in ("blocked", "plain"):
+            gf.append(_gf_device_case(n, p31, 1, method))
+    gf.append(_gf_device_case(20000, p31, 0.9, "blocked"))
+    gf.append(_gf_device_case(20000, p31, 0.9, "plain"))
+    gf.append(_gf_device_case(10000, p61, 1, "plain", dtype="u64"))
+    if args.with_100k:
+        gf.append(_gf_device_case(100000, p31, 1, "blocked"))
+    fl = [_float_case("f64", 8000), _float_case("c128", 6000), _float_case("illcond", 4000)]
+    print(json.dumps({"configs": {"gf": gf, "float": fl}, "host": platform.node()}), flush=True)
+
+
 def main():
     args = parse()
+    if args.configs:
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            sys.exit(2)
+        configs_main(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(independent solves shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
