@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--configs", action="store_true",
                     help="instead of the headline line: one JSON line measuring every GF(p) and float configuration")
     ap.add_argument("--with-100k", action="store_true", help="with --configs: include the 100000^2 u32 run (~40 GB)")
+    ap.add_argument("--only", default="", help="with --configs: comma list of cases, e.g. u32-20000-blocked,f64-8000")
     return ap.parse_args()
 
 
@@ -246,17 +247,21 @@ def configs_main(args):
     torch.cuda.set_device(0)
     _lib.check(_lib.load().linac_init(0))
     p31, p61 = 2 ** 31 - 1, 2 ** 61 - 1
-    _gf_device_case(2000, p31, 1, "auto")  # warm-up
-    gf = []
-    for n in (5000, 10000, 20000):
-        for method in ("blocked", "plain"):
-            gf.append(_gf_device_case(n, p31, 1, method))
-    gf.append(_gf_device_case(20000, p31, 0.9, "blocked"))
-    gf.append(_gf_device_case(20000, p31, 0.9, "plain"))
-    gf.append(_gf_device_case(10000, p61, 1, "plain", dtype="u64"))
+    only = set(x for x in args.only.split(",") if x)
+    gf_cases = [(f"u32-{n}-{m}", (n, p31, 1, m, 0, "u32")) for n in (5000, 10000, 20000) for m in ("blocked", "plain")]
+    gf_cases += [("u32-20000-rank0.9-blocked", (20000, p31, 0.9, "blocked", 0, "u32")),
+                 ("u32-20000-rank0.9-plain", (20000, p31, 0.9, "plain", 0, "u32")),
+                 ("u64-10000-plain", (10000, p61, 1, "plain", 0, "u64"))]
     if args.with_100k:
-        gf.append(_gf_device_case(100000, p31, 1, "blocked"))
-    fl = [_float_case("f64", 8000), _float_case("c128", 6000), _float_case("illcond", 4000)]
+        gf_cases.append(("u32-100000-blocked", (100000, p31, 1, "blocked", 0, "u32")))
+    fl_cases = [("f64-8000", ("f64", 8000)), ("c128-6000", ("c128", 6000)), ("illcond-4000", ("illcond", 4000))]
+    unknown = only - {k for k, _ in gf_cases + fl_cases}
+    if unknown:
+        print(f"unknown cases: {sorted(unknown)}", file=sys.stderr)
+        sys.exit(2)
+    _gf_device_case(2000, p31, 1, "auto")  # warm-up
+    gf = [_gf_device_case(*a) for k, a in gf_cases if not only or k in only]
+    fl = [_float_case(*a) for k, a in fl_cases if not only or k in only]
     print(json.dumps({"configs": {"gf": gf, "float": fl}, "host": platform.node()}), flush=True)
 
 
