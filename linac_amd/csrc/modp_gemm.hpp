@@ -17,10 +17,11 @@
 // The seven S_s are folded modulo p in double precision, where every value is
 // an integer below 2^53 and therefore exact:
 //   P0 = S0 + 256 S1, P1 = S2 + 256 S3, P2 = S4 + 256 S5, P3 = S6  (|P| < 2^33)
-//   r = P3;  r = red(r * 2^16 + P2);  r = red(r * 2^16 + P1);  r = red(r * 2^16 + P0)
+//   r = P3;  r = red(r * 2^16 + P2);  r = red(r * 2^16 + P1);  r = r * 2^16 + P0
 // with red(x) = x - rint(x / p) * p (an FMA, exact because the result is a
 // small integer); |r| stays below ~1.5 p < 2^32, so r * 2^16 + P < 2^49.
-// A final floor-based reduction brings C - r into [0, p).
+// One floor-based reduction of C - r (|C - r| < 2^50) brings it into [0, p):
+// q = floor((C - r) / p) is off by at most one, which the two corrections fix.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -61,7 +62,7 @@ __device__ __forceinline__ uint32_t fold_sub(uint32_t cval, const int32_t s[7], 
     double r = (double)s[6];
     r = red_rint(fma(r, 65536.0, P2), p, pinv);
     r = red_rint(fma(r, 65536.0, P1), p, pinv);
-    r = red_rint(fma(r, 65536.0, P0), p, pinv);
+    r = fma(r, 65536.0, P0);  // |r| < 2^48: reduced together with C below
     double t = (double)cval - r;
     const double q = floor(t * pinv);
     t = fma(-q, p, t);

@@ -523,7 +523,13 @@ __global__ __launch_bounds__(256) void k_scatter_q(const BMat* __restrict__ D) {
 // ---------------------------------------------------------------------------
 
 constexpr int G_M = 64, G_N = 64, G_K = 64, G_T = 256;
-constexpr int G_ROW = G_K + 16;  // LDS bytes per staged row (padded against bank conflicts)
+constexpr int G_ROW = G_K;  // LDS bytes per staged row (no padding: the 16-byte columns are swizzled)
+
+// LDS offset of the 16-byte column `qu` of staged row `row`.  The XOR with
+// bits 2-3 of the row makes both access patterns conflict-free: the MFMA
+// fragment reads (16 consecutive rows, one column) and the stage stores
+// (4 consecutive rows, all 4 columns, per 16 lanes).
+__device__ __forceinline__ int lds_at(int row, int qu) { return row * G_ROW + ((qu ^ ((row >> 2) & 3)) << 4); }
 constexpr int G_GROUP = 8;
 constexpr int NUM_XCD = 8;
 
@@ -602,25 +608,26 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat* __restrict__ 
                     cpre[mi][ni][i] = (MODE == 1 && row < Mrows && col < Ncols) ? C[row * ldc + col] : 0u;
                 }
     };
-    // stage copy: A and B = 4 limbs x 64 rows x 4 sixteen-byte quarters (4 each per thread)
+    // stage copy: A and B = 4 limbs x 64 rows x 4 sixteen-byte quarters (4
+    // each per thread); the row/limb part of each address is fixed per tile
     v4i ra[4], rb[4];
+    const int srow = tid >> 2, squ = tid & 3;  // the same for the 4 limbs (q = limb)
+    const bool a_ok = m_base + srow < Mrows, b_ok = n_base + srow < Ncols;
+    const int8_t* pa = L + (a_ok ? (m_base + srow) * Kpad : 0) + squ * 16;
+    const int8_t* pb = B + (b_ok ? (n_base + srow) * Kpad : 0) + squ * 16;
+    const int soff = lds_at(srow, squ);
     auto gload = [&](int kc) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = tid + q * G_T;
-            const int l = c >> 8, rem = c & 255, row = rem >> 2, qu = rem & 3;
-            const int64_t grow = m_base + row, gcol = n_base + row;
-            ra[q] = grow < Mrows ? *reinterpret_cast<const v4i*>(L + l * planeL + grow * Kpad + kc + qu * 16) : zero;
-            rb[q] = gcol < Ncols ? *reinterpret_cast<const v4i*>(B + l * planeB + gcol * Kpad + kc + qu * 16) : zero;
+            ra[q] = a_ok ? *reinterpret_cast<const v4i*>(pa + q * planeL + kc) : zero;
+            rb[q] = b_ok ? *reinterpret_cast<const v4i*>(pb + q * planeB + kc) : zero;
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = tid + q * G_T;
-            const int l = c >> 8, rem = c & 255, row = rem >> 2, qu = rem & 3;
-            *reinterpret_cast<v4i*>(&sA[buf][l][row * G_ROW + qu * 16]) = ra[q];
-            *reinterpret_cast<v4i*>(&sB[buf][l][row * G_ROW + qu * 16]) = rb[q];
+            *reinterpret_cast<v4i*>(&sA[buf][q][soff]) = ra[q];
+            *reinterpret_cast<v4i*>(&sB[buf][q][soff]) = rb[q];
         }
     };
 
@@ -645,13 +652,13 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat* __restrict__ 
         for (int l = 0; l < 4; ++l)
 #pragma unroll
             for (int ni = 0; ni < 2; ++ni)
-                bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[buf][l][(wn * 32 + ni * 16 + r16) * G_ROW + 16 * g]);
+                bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[buf][l][lds_at(wn * 32 + ni * 16 + r16, g)]);
 #pragma unroll
         for (int l1 = 0; l1 < 4; ++l1) {
             v4i af[2];
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
-                af[mi] = *reinterpret_cast<const v4i*>(&sA[buf][l1][(wm * 32 + mi * 16 + r16) * G_ROW + 16 * g]);
+                af[mi] = *reinterpret_cast<const v4i*>(&sA[buf][l1][lds_at(wm * 32 + mi * 16 + r16, g)]);
 #pragma unroll
             for (int l2 = 0; l2 < 4; ++l2)
 #pragma unroll
@@ -709,12 +716,12 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat* __restrict__ 
     for (int q = 0; q < 4; ++q) {
         const int c = tid + q * G_T;
         const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
-        *reinterpret_cast<v4i*>(&sB[l][col * G_ROW + qu * 16]) =
+        *reinterpret_cast<v4i*>(&sB[l][lds_at(col, qu)]) =
             *reinterpret_cast<const v4i*>(d.planesB + l * d.strideB + (n_base + col) * KPI + qu * 16);
         const int64_t row = m_base + col;
         v4i w = {0, 0, 0, 0};
         if (row < N) w = *reinterpret_cast<const v4i*>(d.planesL + l * d.strideL + row * KPI + qu * 16);
-        *reinterpret_cast<v4i*>(&sA[l][col * G_ROW + qu * 16]) = w;
+        *reinterpret_cast<v4i*>(&sA[l][lds_at(col, qu)]) = w;
     }
     __syncthreads();
     const v4i zero = {0, 0, 0, 0};
@@ -730,13 +737,13 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat* __restrict__ 
     for (int l = 0; l < 4; ++l)
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni)
-            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][(wn * 32 + ni * 16 + r16) * G_ROW + 16 * g]);
+            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][lds_at(wn * 32 + ni * 16 + r16, g)]);
 #pragma unroll
     for (int l1 = 0; l1 < 4; ++l1) {
         v4i af[2];
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
-            af[mi] = *reinterpret_cast<const v4i*>(&sA[l1][(wm * 32 + mi * 16 + r16) * G_ROW + 16 * g]);
+            af[mi] = *reinterpret_cast<const v4i*>(&sA[l1][lds_at(wm * 32 + mi * 16 + r16, g)]);
 #pragma unroll
         for (int l2 = 0; l2 < 4; ++l2)
 #pragma unroll
