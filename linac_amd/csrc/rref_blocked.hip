@@ -87,9 +87,11 @@ struct BMat {
     int8_t* planesL;    // [4][N][256]
     int8_t* planesY;    // [4][256][256]
     int8_t* planesB;    // [4][max(M, XLD)][256]
+    int8_t* planesAi;   // [4][N][KPI]   left operand of the inner GEMM
+    int8_t* planesUi;   // [4][XLD][KPI] right operand of the inner GEMM
     uint32_t* U;        // [CI][XLD]
     uint32_t* Gq;       // [256][ldg]
-    int64_t strideL, strideB, ldg;
+    int64_t strideL, strideB, strideAi, ldg;
     BlockState* bs;
     ElimState* st;
     int64_t* piv_rows;
@@ -426,7 +428,7 @@ __device__ static void make_u_body(const uint32_t* __restrict__ X, int64_t N, Fi
 
 __global__ __launch_bounds__(256) void k_make_u(const BMat* __restrict__ D) {
     const BMat& d = D[blockIdx.y];
-    make_u_body(d.X, d.N, d.f, d.bs, d.U, d.planesB, d.strideB, d.planesL, d.strideL);
+    make_u_body(d.X, d.N, d.f, d.bs, d.U, d.planesUi, (int64_t)XLD * KPI, d.planesAi, d.strideAi);
 }
 
 // ---------------------------------------------------------------------------
@@ -717,10 +719,10 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat* __restrict__ 
         const int c = tid + q * G_T;
         const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
         *reinterpret_cast<v4i*>(&sB[l][lds_at(col, qu)]) =
-            *reinterpret_cast<const v4i*>(d.planesB + l * d.strideB + (n_base + col) * KPI + qu * 16);
+            *reinterpret_cast<const v4i*>(d.planesUi + l * (int64_t)XLD * KPI + (n_base + col) * KPI + qu * 16);
         const int64_t row = m_base + col;
         v4i w = {0, 0, 0, 0};
-        if (row < N) w = *reinterpret_cast<const v4i*>(d.planesL + l * d.strideL + row * KPI + qu * 16);
+        if (row < N) w = *reinterpret_cast<const v4i*>(d.planesAi + l * d.strideAi + row * KPI + qu * 16);
         *reinterpret_cast<v4i*>(&sA[l][lds_at(col, qu)]) = w;
     }
     __syncthreads();
@@ -824,7 +826,7 @@ __global__ void k_gather_status(const BMat* __restrict__ D, ItemStatus* out) {
 // ---------------------------------------------------------------------------
 
 struct WorkspaceLayout {
-    size_t X, L, Y, B, U, G, bs, st, piv, used, epoch, slot, total;
+    size_t X, L, Y, B, Ai, Ui, U, G, bs, st, piv, used, epoch, slot, total;
 };
 
 static WorkspaceLayout workspace_layout(int64_t N, int64_t M, bool with_elim_state) {
@@ -836,6 +838,8 @@ static WorkspaceLayout workspace_layout(int64_t N, int64_t M, bool with_elim_sta
     w.L = off; off += al((size_t)4 * round_up(N * 256, 256));
     w.Y = off; off += al((size_t)4 * STRIDE_Y);
     w.B = off; off += al((size_t)4 * round_up(std::max<int64_t>(M, XLD) * 256, 256));
+    w.Ai = off; off += al((size_t)4 * round_up(N * KPI, 256));
+    w.Ui = off; off += al((size_t)4 * XLD * KPI);
     w.U = off; off += al((size_t)CI * XLD * 4);
     w.G = off; off += al((size_t)256 * round_up(M, 64) * 4);
     w.bs = off; off += al(sizeof(BlockState));
@@ -857,6 +861,9 @@ static void carve(BMat& d, char* base, const WorkspaceLayout& w, int64_t N, int6
     d.planesY = (int8_t*)(base + w.Y);
     d.planesB = (int8_t*)(base + w.B);
     d.strideB = round_up(std::max<int64_t>(M, XLD) * 256, 256);
+    d.planesAi = (int8_t*)(base + w.Ai);
+    d.strideAi = round_up(N * KPI, 256);
+    d.planesUi = (int8_t*)(base + w.Ui);
     d.U = (uint32_t*)(base + w.U);
     d.Gq = (uint32_t*)(base + w.G);
     d.ldg = round_up(M, 64);
@@ -885,36 +892,66 @@ class Engine {
   public:
     Engine() = default;
     ~Engine() {
-        for (int k = 0; k < 2; ++k) {
+        if (so) (void)hipStreamSynchronize(so);
+        for (int k = 0; k < NSLOT; ++k) {
             if (d_buf[k]) (void)hipFree(d_buf[k]);
             if (h_buf[k]) (void)hipHostFree(h_buf[k]);
         }
+        if (ev_ready) (void)hipEventDestroy(ev_ready);
+        if (so) (void)hipStreamDestroy(so);
     }
-    // One outer panel for every unfinished item in `items`.
+    // One outer panel for every unfinished item in `items`.  The inner phase
+    // runs on `s`; the trailing update on the engine's own stream, where the
+    // part beyond the next panel keeps running after step() returns.
     int step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof);
     int init_items(const std::vector<Item*>& items, hipStream_t s);
+    // Waits for every trailing update the engine has issued (before results
+    // are read or an item's memory is released).
+    int drain();
     int epoch = 0;
     bool first_step = true;
 
   private:
+    // Staging slots (pinned host + device): 0 = inner phase; 1..4 = the two
+    // parts of the trailing update of even and odd steps.  A slot is refilled
+    // two steps after its last use, when its copies and readers are done (the
+    // inner phase of a step waits for the previous step's first part, which
+    // follows the part before it on the same stream).
+    static constexpr int NSLOT = 5;
+    struct Staged {
+        BMat* desc;
+        TileSched inner, gq, main;
+        int64_t inner_tiles, gq_tiles, main_tiles;
+        ItemStatus* status;
+    };
     int stage(int slot, size_t bytes);
-    int upload(const std::vector<Item*>& act, hipStream_t s, int sched_kind);
-    // Two staging slots (pinned host + device): slot 0 carries the descriptors
-    // of the inner phase, slot 1 those of the outer update.  A slot is
-    // rewritten only after the stream has been synchronised past its last
-    // copy (the per-step status read), so no pending copy reads a buffer the
-    // host is filling.
-    void* d_buf[2] = {nullptr, nullptr};
-    void* h_buf[2] = {nullptr, nullptr};
-    size_t cap[2] = {0, 0};
-    BMat* d_desc = nullptr;
-    int64_t* d_sched = nullptr;
-    ItemStatus* d_status = nullptr;
-    const int64_t* h_sched = nullptr;
+    // part 0: inner phase; 1: trailing columns of the next panel; 2: the rest
+    int upload(const std::vector<Item*>& act, hipStream_t s, int slot, int part, Staged* out);
+    int ensure_stream();
+    hipStream_t so = nullptr;
+    hipEvent_t ev_ready = nullptr;  // on `so`: next panel's columns updated, panel buffer free
+    bool ev_pending = false;
+    int parity = 0;
+    void* d_buf[NSLOT] = {};
+    void* h_buf[NSLOT] = {};
+    size_t cap[NSLOT] = {};
 };
+
+int Engine::ensure_stream() {
+    if (so) return LINAC_OK;
+    LINAC_HIP(hipStreamCreateWithFlags(&so, hipStreamNonBlocking));
+    LINAC_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    return LINAC_OK;
+}
+
+int Engine::drain() {
+    if (so) LINAC_HIP(hipStreamSynchronize(so));
+    return LINAC_OK;
+}
 
 int Engine::stage(int slot, size_t bytes) {
     if (bytes <= cap[slot]) return LINAC_OK;
+    if (so) LINAC_HIP(hipStreamSynchronize(so));
     if (d_buf[slot]) LINAC_HIP(hipFree(d_buf[slot]));
     if (h_buf[slot]) LINAC_HIP(hipHostFree(h_buf[slot]));
     d_buf[slot] = h_buf[slot] = nullptr;
@@ -924,11 +961,10 @@ int Engine::stage(int slot, size_t bytes) {
     return LINAC_OK;
 }
 
-// Staging layout: [descriptors B][sched 6B+3 int64][status B].  sched_kind:
-// 0 = inner GEMM tiles (pre_a, mts_a), 1 = outer GEMM tiles (pre_b, mts_b for
-// Z[Q]A[Q,T]; pre_c, mts_c for the main update).
-int Engine::upload(const std::vector<Item*>& act, hipStream_t s, int sched_kind) {
-    const int slot = sched_kind;
+// Staging layout: [descriptors B][sched 6B+3 int64][status B].  The sched
+// arrays are prefix sums of per-matrix tile counts: (pa, ma) inner GEMM,
+// (pb, mb) Z[Q] A[Q, T], (pc, mc) main trailing update.
+int Engine::upload(const std::vector<Item*>& act, hipStream_t s, int slot, int part, Staged* out) {
     const size_t B = act.size();
     const size_t desc_b = (size_t)round_up((int64_t)(B * sizeof(BMat)), 256);
     const size_t sched_b = (size_t)round_up((int64_t)((6 * B + 3) * 8), 256);
@@ -936,13 +972,22 @@ int Engine::upload(const std::vector<Item*>& act, hipStream_t s, int sched_kind)
     LINAC_TRY(stage(slot, desc_b + sched_b + stat_b));
     char* h = (char*)h_buf[slot];
     char* dv = (char*)d_buf[slot];
-    d_desc = (BMat*)dv;
-    d_sched = (int64_t*)(dv + desc_b);
-    d_status = (ItemStatus*)(dv + desc_b + sched_b);
     BMat* hd = (BMat*)h;
     int64_t* hs = (int64_t*)(h + desc_b);
-    h_sched = hs;
-    for (size_t m = 0; m < B; ++m) hd[m] = act[m]->d;
+    int64_t* ds = (int64_t*)(dv + desc_b);
+    for (size_t m = 0; m < B; ++m) {
+        BMat d = act[m]->d;
+        if (part == 2) {  // the columns beyond the next panel
+            const int64_t n1 = std::min<int64_t>(BO, d.nT);
+            d.T0 += n1;
+            d.nT -= n1;
+            d.planesB += n1 * d.Kq;
+            d.Gq += n1;
+        } else if (part == 1) {
+            d.nT = std::min<int64_t>(BO, d.nT);
+        }
+        hd[m] = d;
+    }
     int64_t* pa = hs;
     int64_t* ma = pa + (B + 1);
     int64_t* pb = ma + B;
@@ -951,23 +996,24 @@ int Engine::upload(const std::vector<Item*>& act, hipStream_t s, int sched_kind)
     int64_t* mc = pc + (B + 1);
     pa[0] = pb[0] = pc[0] = 0;
     for (size_t m = 0; m < B; ++m) {
-        const BMat& d = act[m]->d;
-        if (sched_kind == 0) {
-            ma[m] = ceil_div(d.N, G_M);
-            pa[m + 1] = pa[m] + ma[m] * (XLD / G_N);
-            mb[m] = mc[m] = 1;
-            pb[m + 1] = pc[m + 1] = 0;
-        } else {
-            const bool live = d.q0 > 0 && d.nT > 0;
-            mb[m] = std::max<int64_t>(1, ceil_div(d.q0, G_M));
-            pb[m + 1] = pb[m] + (live ? ceil_div(d.q0, G_M) * ceil_div(d.nT, G_N) : 0);
-            mc[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
-            pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, G_N) : 0);
-            ma[m] = 1;
-            pa[m + 1] = 0;
-        }
+        const BMat& d = hd[m];
+        ma[m] = ceil_div(d.N, G_M);
+        pa[m + 1] = pa[m] + (part == 0 ? ma[m] * (XLD / G_N) : 0);
+        const bool live = part != 0 && d.q0 > 0 && d.nT > 0;
+        mb[m] = std::max<int64_t>(1, ceil_div(d.q0, G_M));
+        pb[m + 1] = pb[m] + (live ? ceil_div(d.q0, G_M) * ceil_div(d.nT, G_N) : 0);
+        mc[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
+        pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, G_N) : 0);
     }
-    LINAC_HIP(hipMemcpyAsync(d_buf[slot], h_buf[slot], desc_b + sched_b, hipMemcpyHostToDevice, s));
+    out->desc = (BMat*)dv;
+    out->inner = TileSched{ds, ds + (B + 1), (int)B};
+    out->gq = TileSched{ds + (B + 1) + B, ds + 2 * (B + 1) + B, (int)B};
+    out->main = TileSched{ds + 2 * (B + 1) + 2 * B, ds + 3 * (B + 1) + 2 * B, (int)B};
+    out->inner_tiles = pa[B];
+    out->gq_tiles = pb[B];
+    out->main_tiles = pc[B];
+    out->status = (ItemStatus*)(dv + desc_b + sched_b);
+    LINAC_HIP(hipMemcpyAsync(dv, h, desc_b + sched_b, hipMemcpyHostToDevice, s));
     return LINAC_OK;
 }
 
@@ -977,10 +1023,11 @@ static unsigned gs_grid(int64_t work, int64_t cap = 4096) {
 
 int Engine::init_items(const std::vector<Item*>& items, hipStream_t s) {
     if (items.empty()) return LINAC_OK;
-    LINAC_TRY(upload(items, s, 0));
+    Staged st;
+    LINAC_TRY(upload(items, s, 0, 0, &st));
     int64_t maxw = 1;
     for (auto* it : items) maxw = std::max<int64_t>(maxw, it->d.N * it->d.M);
-    k_item_init<<<dim3(gs_grid(maxw), (unsigned)items.size()), 256, 0, s>>>(d_desc);
+    k_item_init<<<dim3(gs_grid(maxw), (unsigned)items.size()), 256, 0, s>>>(st.desc);
     LINAC_HIP(hipGetLastError());
     LINAC_HIP(hipStreamSynchronize(s));  // the next upload reuses this staging slot
     return LINAC_OK;
@@ -991,6 +1038,7 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     for (auto* it : items)
         if (!it->finished) act.push_back(it);
     if (act.empty()) return LINAC_OK;
+    LINAC_TRY(ensure_stream());
     const unsigned B = (unsigned)act.size();
     int64_t maxN = 1, maxNX = 1, ninner = 1;
     for (auto* it : act) {
@@ -1004,17 +1052,19 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
         maxNX = std::max(maxNX, d.N * XLD);
         ninner = std::max<int64_t>(ninner, ceil_div(d.bcur, BI));
     }
-    LINAC_TRY(upload(act, s, 0));
-    k_panel_load<<<dim3(gs_grid(maxNX, 8192), B), 256, 0, s>>>(d_desc);
-    k_block_state_init<<<B, 1, 0, s>>>(d_desc, first_step ? 1 : 0);
+    // --- inner phase on s, after the previous step's first trailing part
+    if (ev_pending) LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
+    Staged in;
+    LINAC_TRY(upload(act, s, 0, 0, &in));
+    k_panel_load<<<dim3(gs_grid(maxNX, 8192), B), 256, 0, s>>>(in.desc);
+    k_block_state_init<<<B, 1, 0, s>>>(in.desc, first_step ? 1 : 0);
     first_step = false;
-    const int64_t inner_tiles = h_sched[B];
     const unsigned mu_grid = (unsigned)(MU_BLOCKS + std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)));
-    TileSched ti{d_sched, d_sched + (B + 1), (int)B};
     auto group = [&]() -> int {
-        k_inner_factor<<<B, IF_T, 0, s>>>(d_desc, epoch);
-        k_make_u<<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(d_desc);
-        k_gemm_inner<<<(unsigned)(ceil_div(inner_tiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc, ti, epoch);
+        k_inner_factor<<<B, IF_T, 0, s>>>(in.desc, epoch);
+        k_make_u<<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(in.desc);
+        k_gemm_inner<<<(unsigned)(ceil_div(in.inner_tiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(in.desc, in.inner,
+                                                                                             epoch);
         ++epoch;
         LINAC_HIP(hipGetLastError());
         return LINAC_OK;
@@ -1023,8 +1073,8 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     // one synchronisation per step; extra groups only for rank-deficient panels
     std::vector<ItemStatus> st(B);
     for (int64_t guard = 0;; ++guard) {
-        k_gather_status<<<B, 1, 0, s>>>(d_desc, d_status);
-        LINAC_HIP(hipMemcpyAsync(st.data(), d_status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
+        k_gather_status<<<B, 1, 0, s>>>(in.desc, in.status);
+        LINAC_HIP(hipMemcpyAsync(st.data(), in.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
         LINAC_HIP(hipStreamSynchronize(s));
         int64_t behind = 0;
         for (unsigned m = 0; m < B; ++m)
@@ -1033,10 +1083,9 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
         if (guard > (int64_t)BO * (BI + 1)) return fail(LINAC_ERR_DEVICE, "blocked elimination did not converge");
         for (int64_t i = 0; i < behind; ++i) LINAC_TRY(group());
     }
-    // outer update of the trailing columns
-    bool any_outer = false;
+    // --- trailing update on so: first the next panel's columns (then the
+    // panel goes back to A and the next inner phase may start), then the rest
     int64_t maxnT = 1;
-    double macs = 0, bytes = 0;
     for (unsigned m = 0; m < B; ++m) {
         BMat& d = act[m]->d;
         d.q0 = st[m].q0;
@@ -1044,35 +1093,52 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
         d.T0 = d.J + d.bcur;
         d.nT = d.M - d.T0;
         d.Kq = (int32_t)round_up(std::max(1, d.q0), 64);
-        if (d.q0 > 0 && d.nT > 0) {
-            any_outer = true;
-            maxnT = std::max(maxnT, d.nT);
-            macs += (double)(d.N - d.q0) * d.q0 * d.nT;
-            bytes += 8.0 * d.N * d.nT;
+        if (d.q0 > 0 && d.nT > 0) maxnT = std::max(maxnT, d.nT);
+    }
+    const int p1 = 1 + 2 * parity, p2 = 2 + 2 * parity;
+    parity ^= 1;
+    Staged t1, t2;
+    LINAC_TRY(upload(act, so, p1, 1, &t1));
+    LINAC_TRY(upload(act, so, p2, 2, &t2));
+    // algorithmic work of a part's main GEMM: (N - q0) q0 |T| modular MACs, C read + written
+    auto part_work = [&](int part, double* pmacs, double* pbytes) {
+        *pmacs = *pbytes = 0;
+        for (unsigned m = 0; m < B; ++m) {
+            const BMat& d = act[m]->d;
+            if (d.q0 == 0 || d.nT == 0) continue;
+            const int64_t n1 = std::min<int64_t>(BO, d.nT), n = part == 1 ? n1 : d.nT - n1;
+            *pmacs += (double)(d.N - d.q0) * d.q0 * n;
+            *pbytes += 8.0 * d.N * n;
         }
+    };
+    auto trailing = [&](const Staged& t, int part, int64_t cols) -> int {
+        if (t.main_tiles == 0 && t.gq_tiles == 0) return LINAC_OK;
+        k_make_rhs<<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(cols, 64)), 4, B), 256, 0, so>>>(t.desc);
+        if (t.gq_tiles > 0)
+            k_gemm_outer<0><<<(unsigned)(ceil_div(t.gq_tiles, NUM_XCD) * NUM_XCD), G_T, 0, so>>>(t.desc, t.gq);
+        if (t.main_tiles > 0) {
+            double pm, pbts;
+            part_work(part, &pm, &pbts);
+            if (prof) LINAC_TRY(prof->sample_begin(so, act[0]->d.J, pbts, pm));
+            k_gemm_outer<1><<<(unsigned)(ceil_div(t.main_tiles, NUM_XCD) * NUM_XCD), G_T, 0, so>>>(t.desc, t.main);
+            if (prof) LINAC_TRY(prof->sample_end(so));
+        }
+        k_scatter_q<<<dim3((unsigned)std::min<int64_t>(64, ceil_div(cols, 256)), 256, B), 256, 0, so>>>(t.desc);
+        LINAC_HIP(hipGetLastError());
+        return LINAC_OK;
+    };
+    if (t1.main_tiles > 0 || t1.gq_tiles > 0) {
+        k_make_lhs<<<dim3(gs_grid(maxN * 256, 16384), B), 256, 0, so>>>(t1.desc, 0, epoch);
+        k_make_lhs<<<dim3(gs_grid(256 * 256), B), 256, 0, so>>>(t1.desc, 1, epoch);
     }
-    LINAC_TRY(upload(act, s, 1));
-    if (any_outer) {
-        const int64_t* hs = h_sched;
-        const int64_t gq_tiles = hs[(B + 1) + B + B];
-        const int64_t main_tiles = hs[2 * (B + 1) + 2 * B + B];
-        TileSched tq{d_sched + (B + 1) + B, d_sched + 2 * (B + 1) + B, (int)B};
-        TileSched tm{d_sched + 2 * (B + 1) + 2 * B, d_sched + 3 * (B + 1) + 2 * B, (int)B};
-        k_make_lhs<<<dim3(gs_grid(maxN * 256, 16384), B), 256, 0, s>>>(d_desc, 0, epoch);
-        k_make_lhs<<<dim3(gs_grid(256 * 256), B), 256, 0, s>>>(d_desc, 1, epoch);
-        k_make_rhs<<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(maxnT, 64)), 4, B), 256, 0, s>>>(d_desc);
-        if (gq_tiles > 0)
-            k_gemm_outer<0><<<(unsigned)(ceil_div(gq_tiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc, tq);
-        if (prof) LINAC_TRY(prof->sample_begin(s, act[0]->d.J, bytes, macs));
-        if (main_tiles > 0)
-            k_gemm_outer<1><<<(unsigned)(ceil_div(main_tiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc, tm);
-        if (prof) LINAC_TRY(prof->sample_end(s));
-        k_scatter_q<<<dim3((unsigned)std::min<int64_t>(64, ceil_div(maxnT, 256)), 256, B), 256, 0, s>>>(d_desc);
-    }
+    LINAC_TRY(trailing(t1, 1, std::min<int64_t>(maxnT, BO)));
     int64_t maxpanel = 1;
     for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
-    k_panel_store<<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, s>>>(d_desc);
+    k_panel_store<<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, so>>>(t1.desc);
     LINAC_HIP(hipGetLastError());
+    LINAC_HIP(hipEventRecord(ev_ready, so));
+    ev_pending = true;
+    LINAC_TRY(trailing(t2, 2, std::max<int64_t>(1, maxnT - BO)));
     for (auto* it : act) {
         it->d.J += it->d.bcur;
         if (it->d.J >= it->d.M) it->finished = true;
@@ -1121,6 +1187,7 @@ static int blocked_run_u32(uint32_t* A, int64_t ld, int64_t N, int64_t M, const 
     // state and row flags were initialised by the caller; epochs here
     k_epoch_init<<<gs_grid(N), 256, 0, s>>>(it.d.epoch_of, N);
     while (!it.finished) LINAC_TRY(eng.step(items, s, prof));
+    LINAC_TRY(eng.drain());
     LINAC_HIP(hipStreamSynchronize(s));
     if (getenv("LINAC_DEBUG_TIMING")) print_timers(it.d, ctx);
     return LINAC_OK;
@@ -1202,6 +1269,7 @@ int linac_batch_create(linac_batch** out, uint64_t mem_cap_bytes) {
 int linac_batch_destroy(linac_batch* b) {
     if (!b) return LINAC_OK;
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    (void)b->eng->drain();  // no trailing update may still run on an item's memory
     (void)hipStreamSynchronize(b->ctx->stream);
     for (auto* it : b->items) {
         if (it->arena) (void)hipFree(it->arena);
@@ -1306,6 +1374,7 @@ int linac_batch_take_u32(linac_batch* b, int64_t id, uint32_t* out, int64_t ld_o
     if (out && ld_out < it->d.M) return fail(LINAC_ERR_INVALID, "ld_out < cols");
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
     hipStream_t s = b->ctx->stream;
+    LINAC_TRY(b->eng->drain());  // the item's last trailing-update work has finished
     ElimWorkspace ws{it->d.st, it->d.piv_rows, it->d.piv_cols, it->d.used, std::max<int64_t>(1, std::min(it->d.N, it->d.M))};
     int64_t rank = 0;
     std::vector<int64_t> pc;
