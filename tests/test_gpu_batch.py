@@ -104,3 +104,19 @@ def test_loader_sweep_batched(gpu):
     for A, (R, piv, rank) in zip(mats, res):
         Ro, po = crref.rref_modp(A, p)
         assert piv == po and numpy.array_equal(R, Ro)
+
+
+def test_edge_shapes_in_one_batch(gpu):
+    """1x1, all-zero, single row, single column and late-pivot matrices
+    together with a regular one: each equals the oracle."""
+    rs = numpy.random.RandomState(12)
+    p = 2 ** 31 - 19
+    late = numpy.zeros((150, 600), dtype=numpy.int64)
+    late[:, 520:] = rs.randint(0, p, size=(150, 80))
+    mats = [numpy.array([[3]]), numpy.zeros((90, 310), dtype=numpy.int64), rs.randint(0, p, size=(1, 513)),
+            rs.randint(0, p, size=(513, 1)), late, rs.randint(0, p, size=(700, 650))]
+    results = gpu.rref_batch([m.astype(numpy.uint32) for m in mats], p)
+    for A, (R, piv, rank) in zip(mats, results):
+        Ro, po = crref.rref_modp(A, p)
+        assert piv == po and rank == len(po)
+        assert numpy.array_equal(R.astype(numpy.uint64), Ro.astype(numpy.uint64))

@@ -181,3 +181,40 @@ def test_blocked_structured_inputs(gpu, shape, structure, seed):
     Ro, po = crref.rref_modp(A, p)
     assert pg == po and rk == len(po)
     assert numpy.array_equal(Rg, Ro)
+
+
+@pytest.mark.parametrize("case", ["1x1", "1x1zero", "zeros", "row", "column", "late_pivots", "zero_panel",
+                                  "one_tile", "few_rows", "identity_perm"])
+def test_blocked_edge_shapes(gpu, case):
+    """Shapes and structures at the edges of the panel / tile logic, blocked
+    method against the oracle (bit-exact)."""
+    p = 2 ** 31 - 1
+    rs = numpy.random.RandomState(11)
+    if case == "1x1":
+        A = numpy.array([[5]])
+    elif case == "1x1zero":
+        A = numpy.array([[0]])
+    elif case == "zeros":
+        A = numpy.zeros((300, 530), dtype=numpy.int64)
+    elif case == "row":
+        A = rs.randint(0, p, size=(1, 700))
+        A[0, :3] = 0
+    elif case == "column":
+        A = rs.randint(0, p, size=(700, 1))
+    elif case == "late_pivots":  # the first 600 columns are zero: pivots only in the third panel
+        A = numpy.zeros((200, 700), dtype=numpy.int64)
+        A[:, 600:] = rs.randint(0, p, size=(200, 100))
+    elif case == "zero_panel":  # a whole 256-column panel without pivots between two full ones
+        A = rs.randint(0, p, size=(400, 800))
+        A[:, 256:512] = 0
+    elif case == "one_tile":
+        A = rs.randint(0, p, size=(64, 64))
+    elif case == "few_rows":
+        A = rs.randint(0, p, size=(5, 900))
+    else:  # a permuted identity: pivot rows far from their columns
+        A = numpy.identity(333, dtype=numpy.int64)[rs.permutation(333)] * 7
+    A = numpy.asarray(A, dtype=numpy.int64)
+    Rg, pg, rk = gpu.rref_with_pivots(A, p, method="blocked")
+    Ro, po = crref.rref_modp(A, p)
+    assert pg == po and rk == len(po)
+    assert numpy.array_equal(Rg.astype(numpy.uint64), Ro.astype(numpy.uint64))
