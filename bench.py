@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run (2000 mixed requests)")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--configs", action="store_true",
                     help="instead of the headline line: one JSON line measuring every GF(p) and float configuration")
     ap.add_argument("--with-100k", action="store_true", help="with --configs: include the 100000^2 u32 run (~40 GB)")
@@ -286,6 +287,14 @@ def main():
     p = 2 ** 31 - 1
     torch.cuda.set_device(0)
     _lib.check(_lib.load().linac_init(0))
+    tuning = None
+    if not args.no_tune:
+        from linac_amd import autotune
+        t0 = time.perf_counter()
+        tuned = autotune.tune(save_result=False)
+        tuning = {"knobs": tuned["knobs"], "seconds": round(time.perf_counter() - t0, 2),
+                  "gemm_candidates": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in m.items()}
+                                      for m in tuned["measurements"]["gemm"]]}
     g = torch.Generator(device="cuda")
     g.manual_seed(0)
     src = torch.randint(0, p, (N, N), generator=g, device="cuda", dtype=torch.int64).to(torch.int32)
@@ -369,6 +378,7 @@ def main():
                    "input": "uniform [0,p) from torch.randint on cuda:0, seed 0; full rank, result checked == I"},
         "roofline": roofline,
         "comparators": comparators,
+        "tuning": tuning,
         "host": platform.node(),
     }
     if not args.no_sweep:

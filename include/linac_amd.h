@@ -73,6 +73,23 @@ int linac_is_prime(uint64_t n);
  *   out[5] the method that ran (1 plain, 2 blocked).
  * Writes min(n, 6) values and returns how many were written. */
 int linac_profile_last(double* out, int n);
+/* Run-time tuning knobs, process-wide (no counterpart in the reference, whose
+ * kernels are specialised by JIT per shape).  linac_amd.autotune measures and
+ * sets them.  Keys and defaults:
+ *   "gf_blocked_min_rows" 1024, "gf_blocked_min_cols" 512: LINAC_METHOD_AUTO
+ *       picks the blocked GF(p) method from this size on;
+ *   "float_blocked_min_rows" 256, "float_blocked_min_cols" 128: the same for
+ *       float64 / complex128;
+ *   "gemm_group" 8: tile rows walked together in the GEMM tile order (a right
+ *       tile is reused that many times while the left tiles stay in L2), 1..64;
+ *   "lookahead" 1: the trailing update of a panel overlaps the next panel's
+ *       inner steps (0: one after the other).
+ * Unknown keys and out-of-range values return LINAC_ERR_INVALID. */
+int linac_tune_set(const char* key, int64_t value);
+int linac_tune_get(const char* key, int64_t* value);
+/* Architecture and marketing name of the current device ("gfx950: AMD
+ * Instinct MI355X"), NUL-terminated in out[0..n); keys tuning files. */
+int linac_device_name(char* out, int n);
 
 /* ---- RREF over GF(p) ----
  * Replaces cuda_row_reduce(matrix, field_characteristic=p) for p > 0

@@ -41,6 +41,9 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 
 # name -> (restype, argtypes); every entry point declared in include/linac_amd.h
 SIGNATURES = {
+    "linac_tune_set": (ctypes.c_int, [ctypes.c_char_p, _i64]),
+    "linac_tune_get": (ctypes.c_int, [ctypes.c_char_p, _i64p]),
+    "linac_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "linac_last_error": (ctypes.c_char_p, []),
     "linac_version": (ctypes.c_int, []),
     "linac_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
@@ -107,7 +110,14 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = lib
-        return lib
+    # a saved tuning for this device (linac_amd.autotune) applies from the start
+    try:
+        from . import autotune
+        if lib.linac_available():
+            autotune.load()
+    except (OSError, ValueError, RuntimeError):
+        pass
+    return lib
 
 
 def last_error():

@@ -520,8 +520,9 @@ __global__ __launch_bounds__(256) void k_scatter_q(const BMat* __restrict__ D) {
 // stages double-buffered in LDS (two workgroups per CU: one's prologue and
 // epilogue overlap the other's MFMAs).  The flattened tile list of the batch
 // is cut into 8 contiguous ranges, one per XCD (consecutive workgroup ids go
-// to different XCDs), and each matrix's tiles are walked in groups of 8 tile
-// rows so that a B tile is reused 8 times while the A tiles stay in L2.
+// to different XCDs), and each matrix's tiles are walked in groups of tile
+// rows (tuning knob "gemm_group", default 8) so that a B tile is reused that
+// many times while the A tiles stay in L2.
 // ---------------------------------------------------------------------------
 
 constexpr int G_M = 64, G_N = 64, G_K = 64, G_T = 256;
@@ -532,13 +533,12 @@ constexpr int G_ROW = G_K;  // LDS bytes per staged row (no padding: the 16-byte
 // fragment reads (16 consecutive rows, one column) and the stage stores
 // (4 consecutive rows, all 4 columns, per 16 lanes).
 __device__ __forceinline__ int lds_at(int row, int qu) { return row * G_ROW + ((qu ^ ((row >> 2) & 3)) << 4); }
-constexpr int G_GROUP = 8;
 constexpr int NUM_XCD = 8;
 
 // blockIdx.x -> (matrix, tile row, tile column) through the prefix sums of
 // per-matrix tile counts `pre[0..B]` (row-tile counts in `mts`).
 __device__ __forceinline__ bool batch_tile(const int64_t* __restrict__ pre, const int64_t* __restrict__ mts, int B,
-                                           int* m_out, int64_t* tm, int64_t* tn) {
+                                           int group, int* m_out, int64_t* tm, int64_t* tn) {
     const int64_t total = pre[B];
     const int64_t per_xcd = (total + NUM_XCD - 1) / NUM_XCD;
     const int64_t bid = blockIdx.x;
@@ -553,11 +553,11 @@ __device__ __forceinline__ bool batch_tile(const int64_t* __restrict__ pre, cons
     const int64_t loc = t - pre[lo];
     const int64_t mt = mts[lo];
     const int64_t nt = (pre[lo + 1] - pre[lo]) / mt;
-    const int64_t gsz = (int64_t)G_GROUP * nt;
+    const int64_t gsz = (int64_t)group * nt;
     const int64_t g = loc / gsz, tt = loc - g * gsz;
-    const int64_t gm = min((int64_t)G_GROUP, mt - g * G_GROUP);
+    const int64_t gm = min((int64_t)group, mt - g * group);
     *m_out = lo;
-    *tm = g * G_GROUP + tt % gm;
+    *tm = g * group + tt % gm;
     *tn = tt / gm;
     return true;
 }
@@ -566,6 +566,7 @@ struct TileSched {
     const int64_t* pre;  // [B + 1]
     const int64_t* mts;  // [B]
     int B;
+    int group;           // tile rows walked together (tuning knob "gemm_group")
 };
 
 // Outer update GEMMs.  MODE 0: Gq = Z[Q] A[Q, T] (C starts at zero, q0 rows);
@@ -577,7 +578,7 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat* __restrict__ 
     __shared__ __attribute__((aligned(16))) int8_t sB[2][4][G_N * G_ROW];
     int mi_;
     int64_t tm, tn;
-    if (!batch_tile(ts.pre, ts.mts, ts.B, &mi_, &tm, &tn)) return;
+    if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
     const BMat& d = D[mi_];
     uint32_t* C = MODE == 0 ? d.Gq : d.A + d.T0;
     const int64_t ldc = MODE == 0 ? d.ldg : d.ld;
@@ -701,7 +702,7 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat* __restrict__ 
     __shared__ __attribute__((aligned(16))) int8_t sB[4][G_N * G_ROW];
     int mi_;
     int64_t tm, tn;
-    if (!batch_tile(ts.pre, ts.mts, ts.B, &mi_, &tm, &tn)) return;
+    if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
     const BMat& d = D[mi_];
     const BlockState* bs = d.bs;
     const int k = bs->k;
@@ -1006,9 +1007,10 @@ int Engine::upload(const std::vector<Item*>& act, hipStream_t s, int slot, int p
         pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, G_N) : 0);
     }
     out->desc = (BMat*)dv;
-    out->inner = TileSched{ds, ds + (B + 1), (int)B};
-    out->gq = TileSched{ds + (B + 1) + B, ds + 2 * (B + 1) + B, (int)B};
-    out->main = TileSched{ds + 2 * (B + 1) + 2 * B, ds + 3 * (B + 1) + 2 * B, (int)B};
+    const int grp = (int)tuning().gemm_group;
+    out->inner = TileSched{ds, ds + (B + 1), (int)B, grp};
+    out->gq = TileSched{ds + (B + 1) + B, ds + 2 * (B + 1) + B, (int)B, grp};
+    out->main = TileSched{ds + 2 * (B + 1) + 2 * B, ds + 3 * (B + 1) + 2 * B, (int)B, grp};
     out->inner_tiles = pa[B];
     out->gq_tiles = pb[B];
     out->main_tiles = pc[B];
@@ -1136,9 +1138,11 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
     k_panel_store<<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, so>>>(t1.desc);
     LINAC_HIP(hipGetLastError());
-    LINAC_HIP(hipEventRecord(ev_ready, so));
-    ev_pending = true;
+    const bool lookahead = tuning().lookahead != 0;
+    if (lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));
     LINAC_TRY(trailing(t2, 2, std::max<int64_t>(1, maxnT - BO)));
+    if (!lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));  // the next inner phase waits for all of it
+    ev_pending = true;
     for (auto* it : act) {
         it->d.J += it->d.bcur;
         if (it->d.J >= it->d.M) it->finished = true;
@@ -1198,7 +1202,7 @@ int blocked_dispatch(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F
                      const ElimWorkspace& ws, DeviceContext* ctx, bool forced, Profiler* prof) {
     if constexpr (std::is_same<F, FieldU32>::value) {
         if (f.p > LIMB_P_MAX) return NOT_APPLICABLE;
-        if (!forced && (N < 1024 || M < 512)) return NOT_APPLICABLE;
+        if (!forced && (N < tuning().gf_blocked_min_rows || M < tuning().gf_blocked_min_cols)) return NOT_APPLICABLE;
         return blocked_run_u32(A, ld, N, M, f, s, ws, ctx, prof);
     } else {
         (void)A; (void)ld; (void)N; (void)M; (void)f; (void)s; (void)ws; (void)ctx; (void)forced; (void)prof;

@@ -184,7 +184,9 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
     // blocked: panels of 64 columns with a cooperative pivot kernel and an
     // FP64 matrix-core trailing update; the plain sweep stays the method for
     // small matrices, where per-column launches cost less than the panels
-    bool blocked = method == LINAC_METHOD_BLOCKED || (method == LINAC_METHOD_AUTO && rows >= 256 && cols >= 128);
+    bool blocked = method == LINAC_METHOD_BLOCKED ||
+                   (method == LINAC_METHOD_AUTO && rows >= tuning().float_blocked_min_rows &&
+                    cols >= tuning().float_blocked_min_cols);
     if (blocked) {
         const int rc = float_blocked_run<T>(A, lda, rows, cols, tol, s, &st->rank, used, piv_rows, piv_cols, ctx, prof);
         if (rc < 0) return rc;

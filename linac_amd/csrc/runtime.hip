@@ -211,6 +211,11 @@ int Profiler::finish(hipStream_t s) {
     return LINAC_OK;
 }
 
+Tuning& tuning() {
+    static Tuning t;
+    return t;
+}
+
 }  // namespace linac
 
 using namespace linac;
@@ -256,6 +261,59 @@ int linac_release_cache(void) {
 }
 
 int linac_is_prime(uint64_t n) { return is_prime_u64(n) ? 1 : 0; }
+
+int linac_tune_set(const char* key, int64_t value) {
+    if (!key) return fail(LINAC_ERR_INVALID, "tuning key is NULL");
+    Tuning& t = tuning();
+    struct Knob {
+        const char* name;
+        int64_t* slot;
+        int64_t lo, hi;
+    } knobs[] = {{"gf_blocked_min_rows", &t.gf_blocked_min_rows, 1, INT64_MAX},
+                 {"gf_blocked_min_cols", &t.gf_blocked_min_cols, 1, INT64_MAX},
+                 {"float_blocked_min_rows", &t.float_blocked_min_rows, 1, INT64_MAX},
+                 {"float_blocked_min_cols", &t.float_blocked_min_cols, 1, INT64_MAX},
+                 {"gemm_group", &t.gemm_group, 1, 64},
+                 {"lookahead", &t.lookahead, 0, 1}};
+    for (auto& k : knobs)
+        if (!strcmp(key, k.name)) {
+            if (value < k.lo || value > k.hi)
+                return fail(LINAC_ERR_INVALID, "%s = %lld outside [%lld, %lld]", key, (long long)value,
+                            (long long)k.lo, (long long)k.hi);
+            *k.slot = value;
+            return LINAC_OK;
+        }
+    return fail(LINAC_ERR_INVALID, "unknown tuning key '%s'", key);
+}
+
+int linac_tune_get(const char* key, int64_t* value) {
+    if (!key || !value) return fail(LINAC_ERR_INVALID, "NULL argument");
+    const Tuning& t = tuning();
+    const struct {
+        const char* name;
+        int64_t v;
+    } knobs[] = {{"gf_blocked_min_rows", t.gf_blocked_min_rows}, {"gf_blocked_min_cols", t.gf_blocked_min_cols},
+                 {"float_blocked_min_rows", t.float_blocked_min_rows},
+                 {"float_blocked_min_cols", t.float_blocked_min_cols}, {"gemm_group", t.gemm_group},
+                 {"lookahead", t.lookahead}};
+    for (auto& k : knobs)
+        if (!strcmp(key, k.name)) {
+            *value = k.v;
+            return LINAC_OK;
+        }
+    return fail(LINAC_ERR_INVALID, "unknown tuning key '%s'", key);
+}
+
+int linac_device_name(char* out, int n) {
+    if (!out || n <= 0) return fail(LINAC_ERR_INVALID, "bad output buffer");
+    LINAC_TRY(check_process());
+    int dev = 0;
+    LINAC_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    LINAC_HIP(hipGetDeviceProperties(&prop, dev));
+    snprintf(out, (size_t)n, "%s: %s", prop.gcnArchName, prop.name);
+    return LINAC_OK;
+}
 
 int linac_profile_last(double* out, int n) {
     if (!out || n <= 0) return 0;

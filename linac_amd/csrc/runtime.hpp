@@ -103,6 +103,15 @@ struct Profiler {
 };
 Profiler& thread_profiler();
 
+// Process-wide tuning knobs (linac_tune_set); read at launch time.
+struct Tuning {
+    int64_t gf_blocked_min_rows = 1024, gf_blocked_min_cols = 512;
+    int64_t float_blocked_min_rows = 256, float_blocked_min_cols = 128;
+    int64_t gemm_group = 8;
+    int64_t lookahead = 1;
+};
+Tuning& tuning();
+
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 __host__ __device__ inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 

@@ -125,3 +125,39 @@ def test_loader_dtype_rules_follow_reference():
     assert loader_dtype(7, _mod64=True) == numpy.uint64
     assert loader_dtype(0, numpy.ones((2, 2))) == numpy.float64
     assert loader_dtype(0, numpy.ones((2, 2)) * 1j) == numpy.complex128
+
+
+def test_tuning_knobs_set_get_and_validation():
+    """Knob API (host only): defaults, round trip, range and key checks."""
+    from linac_amd import autotune
+    before = autotune.current()
+    assert set(before) == set(autotune.KNOBS)
+    try:
+        autotune.set_knob("gemm_group", 4)
+        assert autotune.get("gemm_group") == 4
+        autotune.set_knob("lookahead", 0)
+        assert autotune.get("lookahead") == 0
+        with pytest.raises(ValueError):
+            autotune.set_knob("gemm_group", 0)
+        with pytest.raises(ValueError):
+            autotune.set_knob("lookahead", 2)
+        with pytest.raises(ValueError):
+            autotune.set_knob("no_such_knob", 1)
+        with pytest.raises(ValueError):
+            autotune.get("no_such_knob")
+    finally:
+        autotune.apply(before)
+    assert autotune.current() == before
+
+
+def test_tuning_file_round_trip(tmp_path):
+    from linac_amd import autotune
+    before = autotune.current()
+    path = tmp_path / "t.json"
+    try:
+        autotune.save({"knobs": dict(before, gf_blocked_min_rows=777)}, str(path))
+        data = autotune.load(str(path))
+        assert data["knobs"]["gf_blocked_min_rows"] == 777 and autotune.get("gf_blocked_min_rows") == 777
+        assert autotune.load(str(tmp_path / "missing.json")) is None
+    finally:
+        autotune.apply(before)

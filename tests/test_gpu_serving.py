@@ -17,9 +17,12 @@ def _alone(req, seed):
     else:
         A = req.tensor.cpu().numpy().view(numpy.uint32)
     req.tensor = None
-    from linac_amd import rref_with_pivots
-    method = "blocked" if req.p == 0 and req.rows >= 256 and req.cols >= 128 else \
-        ("plain" if req.p == 0 else "blocked")
+    from linac_amd import rref_with_pivots, autotune
+    if req.p:
+        method = "blocked"  # GF(p): any method gives the unique RREF
+    else:  # float: the method the serving loop's "auto" picks for this shape
+        big = req.rows >= autotune.get("float_blocked_min_rows") and req.cols >= autotune.get("float_blocked_min_cols")
+        method = "blocked" if big else "plain"
     return rref_with_pivots(A, req.p, method=method)
 
 
