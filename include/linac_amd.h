@@ -138,6 +138,12 @@ int linac_batch_status(linac_batch* b, int64_t id);
  * LINAC_DEVICE_PTR; may be NULL), its pivot columns and rank, and releases it. */
 int linac_batch_take_u32(linac_batch* b, int64_t id, uint32_t* out, int64_t ld_out, uint32_t flags,
                          int64_t* pivots_out, int64_t* rank_out);
+/* Many finished matrices at once: as linac_batch_take_u32 for each ids[i]
+ * (outs[i] / pivots_out[i] may be NULL, as may the arrays themselves), with two
+ * host round trips in all instead of several per matrix. */
+int linac_batch_take_many_u32(linac_batch* b, const int64_t* ids, int64_t count, uint32_t* const* outs,
+                              const int64_t* ld_outs, uint32_t flags, int64_t* const* pivots_out,
+                              int64_t* ranks_out);
 /* One-shot: RREF of `count` matrices in place (each with its own shape and
  * prime); pivots_out may be NULL or hold per-matrix pointers (or NULLs). */
 int linac_rref_batch_u32(uint32_t* const* mats, const int64_t* rows, const int64_t* cols, const int64_t* lds,

@@ -95,6 +95,8 @@ class Batch:
         """(R, pivot_columns, rank) of a finished matrix; releases its memory.
         For a matrix added in place, R is None: the RREF is in the caller's
         device buffer."""
+        if rid not in self._shapes:
+            raise ValueError(f"unknown or already taken id {rid}")
         rows, cols, in_place = self._shapes[rid]
         out = None if in_place else numpy.empty((rows, cols), dtype=numpy.uint32)
         piv = numpy.zeros(max(1, min(rows, cols)), dtype=numpy.int64)
@@ -107,6 +109,33 @@ class Batch:
         return out, piv[:rank.value].tolist(), rank.value
 
 
+    def take_many(self, rids):
+        """take() for several finished matrices at once (two host round trips
+        in all); returns a list in the order of ``rids``."""
+        rids = [int(r) for r in rids]
+        n = len(rids)
+        if n == 0:
+            return []
+        outs, piv = [], []
+        for rid in rids:
+            if rid not in self._shapes:
+                raise ValueError(f"unknown or already taken id {rid}")
+            rows, cols, in_place = self._shapes[rid]
+            outs.append(None if in_place else numpy.empty((rows, cols), dtype=numpy.uint32))
+            piv.append(numpy.zeros(max(1, min(rows, cols)), dtype=numpy.int64))
+        ids = (ctypes.c_int64 * n)(*rids)
+        out_p = (ctypes.c_void_p * n)(*[None if o is None else o.ctypes.data for o in outs])
+        lds = (ctypes.c_int64 * n)(*[self._shapes[r][1] for r in rids])
+        piv_p = (ctypes.c_void_p * n)(*[x.ctypes.data for x in piv])
+        ranks = (ctypes.c_int64 * n)()
+        _lib.check(self._lib.linac_batch_take_many_u32(self._h, ids, n, out_p, lds, 0, piv_p, ranks))
+        res = []
+        for rid, o, x, r in zip(rids, outs, piv, ranks):
+            del self._shapes[rid]
+            res.append((o, x[:r].tolist(), int(r)))
+        return res
+
+
 def rref_batch(matrices, primes):
     """RREF of every matrix (each with its own prime) in one batch.
 
@@ -117,4 +146,4 @@ def rref_batch(matrices, primes):
     with Batch() as b:
         ids = [b.add(m, p) for m, p in zip(matrices, primes)]
         b.run()
-        return [b.take(i) for i in ids]
+        return b.take_many(ids)

@@ -33,5 +33,12 @@ template <class F>
 int gf_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
                 const ElimWorkspace& ws, DeviceContext* ctx, bool normalize, bool sort_by_column,
                 int64_t* rank_host, std::vector<int64_t>* piv_cols_host);
+// The gather alone, enqueued without synchronisation, for a pivot list the
+// caller has already read (and sorted: `resorted` rewrites the device copy).
+template <class F>
+int gf_finalize_apply(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
+                      const ElimWorkspace& ws, DeviceContext* ctx, bool normalize, const std::vector<int64_t>& rows,
+                      const std::vector<int64_t>& cols, bool resorted);
+bool sort_pivots_by_column(std::vector<int64_t>& rows, std::vector<int64_t>& cols);
 
 }  // namespace linac

@@ -811,6 +811,22 @@ struct ItemStatus {
     int64_t rank;
 };
 
+// Ranks, then pivot lists, of many finished matrices in one launch each
+// (blockIdx.x = matrix); `offs` are the prefix sums of the ranks.
+__global__ void k_collect_rank(const BMat* __restrict__ D, int64_t* out) {
+    if (threadIdx.x == 0) out[blockIdx.x] = D[blockIdx.x].st->rank;
+}
+
+__global__ __launch_bounds__(256) void k_collect_pivots(const BMat* __restrict__ D, const int64_t* __restrict__ offs,
+                                                        int64_t* __restrict__ rows, int64_t* __restrict__ cols) {
+    const BMat& d = D[blockIdx.x];
+    const int64_t o = offs[blockIdx.x], n = offs[blockIdx.x + 1] - o;
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+        rows[o + k] = d.piv_rows[k];
+        cols[o + k] = d.piv_cols[k];
+    }
+}
+
 __global__ void k_gather_status(const BMat* __restrict__ D, ItemStatus* out) {
     const BMat& d = D[blockIdx.x];
     ItemStatus s;
@@ -1398,6 +1414,78 @@ int linac_batch_take_u32(linac_batch* b, int64_t id, uint32_t* out, int64_t ld_o
     return LINAC_OK;
 }
 
+int linac_batch_take_many_u32(linac_batch* b, const int64_t* ids, int64_t count, uint32_t* const* outs,
+                              const int64_t* ld_outs, uint32_t flags, int64_t* const* pivots_out,
+                              int64_t* ranks_out) {
+    if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
+    if (count < 0 || (count > 0 && !ids)) return fail(LINAC_ERR_INVALID, "bad id list");
+    if (count == 0) return LINAC_OK;
+    std::vector<Item*> its((size_t)count);
+    for (int64_t i = 0; i < count; ++i) {
+        its[i] = find_item(b, ids[i]);
+        if (!its[i]) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)ids[i]);
+        if (!its[i]->finished) return fail(LINAC_ERR_INVALID, "matrix %lld is not finished", (long long)ids[i]);
+        for (int64_t j = 0; j < i; ++j)
+            if (its[j] == its[i]) return fail(LINAC_ERR_INVALID, "id %lld listed twice", (long long)ids[i]);
+        if (outs && outs[i] && (!ld_outs || ld_outs[i] < its[i]->d.M))
+            return fail(LINAC_ERR_INVALID, "ld_out < cols for id %lld", (long long)ids[i]);
+    }
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    hipStream_t s = b->ctx->stream;
+    LINAC_TRY(b->eng->drain());
+    // descriptors, ranks and pivot lists in one scratch block: two round trips in all
+    int64_t cap_total = 0;
+    for (auto* it : its) cap_total += std::max<int64_t>(1, std::min(it->d.N, it->d.M));
+    const size_t desc_b = (size_t)round_up(count * (int64_t)sizeof(BMat), 256);
+    const size_t offs_b = (size_t)round_up((count + 1) * 8, 256);
+    void* scratch = nullptr;
+    LINAC_TRY(b->ctx->get(SLOT_AUX, desc_b + offs_b + 2 * (size_t)round_up(cap_total * 8, 256), &scratch));
+    BMat* d_desc = (BMat*)scratch;
+    int64_t* d_offs = (int64_t*)((char*)scratch + desc_b);
+    int64_t* d_rows = (int64_t*)((char*)scratch + desc_b + offs_b);
+    int64_t* d_cols = d_rows + round_up(cap_total * 8, 256) / 8;
+    std::vector<BMat> hd((size_t)count);
+    for (int64_t i = 0; i < count; ++i) hd[i] = its[i]->d;
+    LINAC_HIP(hipMemcpyAsync(d_desc, hd.data(), count * sizeof(BMat), hipMemcpyHostToDevice, s));
+    k_collect_rank<<<(unsigned)count, 64, 0, s>>>(d_desc, d_offs);
+    std::vector<int64_t> rk((size_t)count), offs((size_t)count + 1, 0);
+    LINAC_HIP(hipMemcpyAsync(rk.data(), d_offs, count * 8, hipMemcpyDeviceToHost, s));
+    LINAC_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < count; ++i) offs[i + 1] = offs[i] + rk[i];
+    std::vector<int64_t> prow((size_t)std::max<int64_t>(1, offs[count])), pcol(prow.size());
+    if (offs[count] > 0) {
+        LINAC_HIP(hipMemcpyAsync(d_offs, offs.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
+        k_collect_pivots<<<(unsigned)count, 256, 0, s>>>(d_desc, d_offs, d_rows, d_cols);
+        LINAC_HIP(hipMemcpyAsync(prow.data(), d_rows, offs[count] * 8, hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipMemcpyAsync(pcol.data(), d_cols, offs[count] * 8, hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipStreamSynchronize(s));
+    }
+    for (int64_t i = 0; i < count; ++i) {
+        Item* it = its[i];
+        std::vector<int64_t> r(prow.begin() + offs[i], prow.begin() + offs[i + 1]);
+        std::vector<int64_t> c(pcol.begin() + offs[i], pcol.begin() + offs[i + 1]);
+        const bool resorted = sort_pivots_by_column(r, c);
+        ElimWorkspace ws{it->d.st, it->d.piv_rows, it->d.piv_cols, it->d.used,
+                         std::max<int64_t>(1, std::min(it->d.N, it->d.M))};
+        LINAC_TRY(gf_finalize_apply(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, r, c, resorted));
+        if (outs && outs[i])
+            LINAC_HIP(hipMemcpy2DAsync(outs[i], ld_outs[i] * 4, it->d.A, it->d.ld * 4, it->d.M * 4, it->d.N,
+                                       (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                       s));
+        if (pivots_out && pivots_out[i])
+            for (int64_t k = 0; k < (int64_t)c.size(); ++k) pivots_out[i][k] = c[k];
+        if (ranks_out) ranks_out[i] = rk[i];
+    }
+    LINAC_HIP(hipStreamSynchronize(s));
+    for (auto* it : its) {
+        b->in_use -= it->bytes;
+        (void)hipFree(it->arena);
+        b->items.erase(std::find(b->items.begin(), b->items.end(), it));
+        delete it;
+    }
+    return LINAC_OK;
+}
+
 int linac_rref_batch_u32(uint32_t* const* mats, const int64_t* rows, const int64_t* cols, const int64_t* lds,
                          const uint32_t* primes, int64_t count, uint32_t flags, int64_t* const* pivots_out,
                          int64_t* ranks_out) {
@@ -1415,10 +1503,24 @@ int linac_rref_batch_u32(uint32_t* const* mats, const int64_t* rows, const int64
         rc = linac_batch_add_u32(b, mats[i], rows[i], cols[i], lds[i], primes[i], flags, &ids[i]);
     }
     if (rc == LINAC_OK) rc = linac_batch_run(b);
-    for (int64_t i = 0; i < count && rc == LINAC_OK; ++i) {
-        if (ids[i] < 0) continue;
-        rc = linac_batch_take_u32(b, ids[i], mats[i], lds[i], flags, pivots_out ? pivots_out[i] : nullptr,
-                                  ranks_out ? ranks_out + i : nullptr);
+    if (rc == LINAC_OK) {
+        std::vector<int64_t> tid, tld, trk;
+        std::vector<uint32_t*> tout;
+        std::vector<int64_t*> tpiv;
+        std::vector<int64_t> where;
+        for (int64_t i = 0; i < count; ++i) {
+            if (ids[i] < 0) continue;
+            tid.push_back(ids[i]);
+            tout.push_back(mats[i]);
+            tld.push_back(lds[i]);
+            tpiv.push_back(pivots_out ? pivots_out[i] : nullptr);
+            where.push_back(i);
+        }
+        trk.resize(tid.size());
+        rc = linac_batch_take_many_u32(b, tid.data(), (int64_t)tid.size(), tout.data(), tld.data(), flags,
+                                       tpiv.data(), trk.data());
+        if (rc == LINAC_OK && ranks_out)
+            for (size_t k = 0; k < where.size(); ++k) ranks_out[where[k]] = trk[k];
     }
     const int rc2 = linac_batch_destroy(b);
     return rc != LINAC_OK ? rc : rc2;

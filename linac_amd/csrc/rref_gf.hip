@@ -247,32 +247,18 @@ int plain_run(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hi
     return LINAC_OK;
 }
 
+// Enqueues the final gather on s (no synchronisation): rows/cols are the
+// pivot list in the order the result rows take (the device copy of the list
+// is rewritten when `resorted`).
 template <class F>
-int gf_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
-                const ElimWorkspace& ws, DeviceContext* ctx, bool normalize, bool sort_by_column,
-                int64_t* rank_host, std::vector<int64_t>* piv_cols_host) {
+int gf_finalize_apply(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
+                      const ElimWorkspace& ws, DeviceContext* ctx, bool normalize, const std::vector<int64_t>& rows,
+                      const std::vector<int64_t>& cols, bool resorted) {
     using T = typename F::T;
-    LINAC_HIP(hipMemcpyAsync(ctx->pinned, &ws.st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    LINAC_HIP(hipStreamSynchronize(s));
-    const int64_t rank = ctx->pinned[0];
-    std::vector<int64_t> rows(rank), cols(rank);
-    if (rank > 0) {
-        LINAC_HIP(hipMemcpyAsync(rows.data(), ws.piv_rows, rank * 8, hipMemcpyDeviceToHost, s));
-        LINAC_HIP(hipMemcpyAsync(cols.data(), ws.piv_cols, rank * 8, hipMemcpyDeviceToHost, s));
-        LINAC_HIP(hipStreamSynchronize(s));
-    }
-    if (sort_by_column) {
-        std::vector<int64_t> order(rank);
-        for (int64_t k = 0; k < rank; ++k) order[k] = k;
-        std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cols[a] < cols[b]; });
-        std::vector<int64_t> r2(rank), c2(rank);
-        for (int64_t k = 0; k < rank; ++k) { r2[k] = rows[order[k]]; c2[k] = cols[order[k]]; }
-        rows.swap(r2);
-        cols.swap(c2);
-        if (rank > 0) {
-            LINAC_HIP(hipMemcpyAsync(ws.piv_rows, rows.data(), rank * 8, hipMemcpyHostToDevice, s));
-            LINAC_HIP(hipMemcpyAsync(ws.piv_cols, cols.data(), rank * 8, hipMemcpyHostToDevice, s));
-        }
+    const int64_t rank = (int64_t)rows.size();
+    if (resorted && rank > 0) {
+        LINAC_HIP(hipMemcpyAsync(ws.piv_rows, rows.data(), rank * 8, hipMemcpyHostToDevice, s));
+        LINAC_HIP(hipMemcpyAsync(ws.piv_cols, cols.data(), rank * 8, hipMemcpyHostToDevice, s));
     }
     bool identity = true;
     for (int64_t k = 0; k < rank && identity; ++k) identity = rows[k] == k;
@@ -291,12 +277,52 @@ int gf_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, 
         LINAC_HIP(hipMemcpyAsync(A, out, (size_t)N * ld * sizeof(T), hipMemcpyDeviceToDevice, s));
     }
     LINAC_HIP(hipGetLastError());
+    return LINAC_OK;
+}
+
+// Sorts a pivot list by column in place; true when the order changed.
+bool sort_pivots_by_column(std::vector<int64_t>& rows, std::vector<int64_t>& cols) {
+    const int64_t rank = (int64_t)rows.size();
+    bool sorted = true;
+    for (int64_t k = 1; k < rank && sorted; ++k) sorted = cols[k - 1] < cols[k];
+    if (sorted) return false;
+    std::vector<int64_t> order(rank);
+    for (int64_t k = 0; k < rank; ++k) order[k] = k;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return cols[a] < cols[b]; });
+    std::vector<int64_t> r2(rank), c2(rank);
+    for (int64_t k = 0; k < rank; ++k) {
+        r2[k] = rows[order[k]];
+        c2[k] = cols[order[k]];
+    }
+    rows.swap(r2);
+    cols.swap(c2);
+    return true;
+}
+
+template <class F>
+int gf_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
+                const ElimWorkspace& ws, DeviceContext* ctx, bool normalize, bool sort_by_column,
+                int64_t* rank_host, std::vector<int64_t>* piv_cols_host) {
+    LINAC_HIP(hipMemcpyAsync(ctx->pinned, &ws.st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    LINAC_HIP(hipStreamSynchronize(s));
+    const int64_t rank = ctx->pinned[0];
+    std::vector<int64_t> rows(rank), cols(rank);
+    if (rank > 0) {
+        LINAC_HIP(hipMemcpyAsync(rows.data(), ws.piv_rows, rank * 8, hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipMemcpyAsync(cols.data(), ws.piv_cols, rank * 8, hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipStreamSynchronize(s));
+    }
+    const bool resorted = sort_by_column && sort_pivots_by_column(rows, cols);
+    LINAC_TRY(gf_finalize_apply(A, ld, N, M, f, s, ws, ctx, normalize, rows, cols, resorted));
     LINAC_HIP(hipStreamSynchronize(s));
     *rank_host = rank;
     *piv_cols_host = cols;
     return LINAC_OK;
 }
 
+template int gf_finalize_apply<FieldU32>(uint32_t*, int64_t, int64_t, int64_t, const FieldU32&, hipStream_t,
+                                         const ElimWorkspace&, DeviceContext*, bool, const std::vector<int64_t>&,
+                                         const std::vector<int64_t>&, bool);
 template int gf_finalize<FieldU32>(uint32_t*, int64_t, int64_t, int64_t, const FieldU32&, hipStream_t,
                                    const ElimWorkspace&, DeviceContext*, bool, bool, int64_t*, std::vector<int64_t>*);
 

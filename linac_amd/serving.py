@@ -121,10 +121,11 @@ class Server:
             self.in_use -= need
         if self.active:
             self.batch.step()
+            fin = [b for b in self.active if self.batch.done(b)]
+            taken = self.batch.take_many(fin)
             now = clock()
-            for bid in [b for b in self.active if self.batch.done(b)]:
+            for bid, (_, piv, rank) in zip(fin, taken):
                 req, need = self.active.pop(bid)
-                _, piv, rank = self.batch.take(bid)
                 done.append(Result(req.rid, piv, rank, req.arrival, now, req.tensor))
                 self.in_use -= need
         return done

@@ -128,8 +128,7 @@ def reduce_sweep(bases, lindices, p, mem_cap_bytes=0):
         item = rows * cols * 4
         ids = [b.add_device(d_out.data_ptr() + s * item, rows, cols, cols, p, in_place=True) for s in range(S)]
         b.run()
-        for i in ids:
-            _, piv, rank = b.take(i)
+        for _, piv, rank in b.take_many(ids):
             pivots.append(piv)
             ranks.append(rank)
     t2 = time.perf_counter()

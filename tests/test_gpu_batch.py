@@ -120,3 +120,31 @@ def test_edge_shapes_in_one_batch(gpu):
         Ro, po = crref.rref_modp(A, p)
         assert piv == po and rank == len(po)
         assert numpy.array_equal(R.astype(numpy.uint64), Ro.astype(numpy.uint64))
+
+
+def test_take_many_order_subsets_and_errors(gpu):
+    """take_many returns results in the order asked, leaves the others in the
+    batch, and refuses duplicates and unfinished matrices."""
+    p = 2 ** 31 - 1
+    rs = numpy.random.RandomState(13)
+    mats = [rs.randint(0, p, size=(n, n + 3)).astype(numpy.uint32) for n in (40, 300, 90, 520)]
+    with gpu.Batch() as b:
+        ids = [b.add(m, p) for m in mats]
+        b.run()
+        with pytest.raises(ValueError):
+            b.take_many([ids[0], ids[0]])
+        got = b.take_many([ids[3], ids[1]])
+        rest = b.take_many([ids[2], ids[0]])
+        with pytest.raises(ValueError):
+            b.take(ids[1])  # already taken
+    for A, (R, piv, rank) in zip([mats[3], mats[1], mats[2], mats[0]], got + rest):
+        Ro, po = crref.rref_modp(A, p)
+        assert piv == po and rank == len(po) and numpy.array_equal(R, Ro)
+    with gpu.Batch() as b:
+        big = b.add(rs.randint(0, p, size=(600, 600)).astype(numpy.uint32), p)
+        small = b.add(rs.randint(0, p, size=(30, 30)).astype(numpy.uint32), p)
+        b.step()
+        assert b.done(small) and not b.done(big)
+        with pytest.raises(ValueError):
+            b.take_many([small, big])  # big is not finished
+        assert b.take_many([small])[0][2] == 30
