@@ -71,4 +71,22 @@ __device__ __forceinline__ uint32_t fold_sub(uint32_t cval, const int32_t s[7], 
     return (uint32_t)t;
 }
 
+// fold_sub for sums of at most 32 limb products per diagonal entry (the
+// inner GEMM: k <= 32 pivots), where |S_s| <= 4 * 32 * 2^14 = 2^21, so the
+// pairs S_2i + 256 S_2i+1 are exact in int32 (|.| < 2^30) and only four values
+// go through double precision.
+__device__ __forceinline__ uint32_t fold_sub_k32(uint32_t cval, const int32_t s[7], double p, double pinv) {
+    const int32_t Q0 = s[0] + s[1] * 256, Q1 = s[2] + s[3] * 256, Q2 = s[4] + s[5] * 256;
+    double r = (double)s[6];
+    r = red_rint(fma(r, 65536.0, (double)Q2), p, pinv);
+    r = red_rint(fma(r, 65536.0, (double)Q1), p, pinv);
+    r = fma(r, 65536.0, (double)Q0);
+    double t = (double)cval - r;
+    const double q = floor(t * pinv);
+    t = fma(-q, p, t);
+    t = t < 0.0 ? t + p : t;
+    t = t >= p ? t - p : t;
+    return (uint32_t)t;
+}
+
 }  // namespace linac

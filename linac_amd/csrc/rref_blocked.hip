@@ -696,8 +696,10 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat* __restrict__ 
 // are.  K = 64 (k <= 32 pivots, zero padded), one LDS stage.  Both operands
 // come as limb planes from k_make_u (A: X[:, c0 + Cp] captured before this
 // launch, since its tiles overwrite those columns).  Tiles outside the live
-// column window [c0, BO + q0) only meet zeros and are skipped.
-__global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat* __restrict__ D, TileSched ts, int epoch) {
+// column window [c0, BO + q0) only meet zeros and are skipped.  The kernel is
+// latency-bound (one K stage per tile): four workgroups per CU (a 20-byte
+// register spill in the epilogue) measured faster than three without.
+__global__ __launch_bounds__(G_T, 4) void k_gemm_inner(const BMat* __restrict__ D, TileSched ts, int epoch) {
     __shared__ __attribute__((aligned(16))) int8_t sA[4][G_M * G_ROW];
     __shared__ __attribute__((aligned(16))) int8_t sB[4][G_N * G_ROW];
     int mi_;
@@ -774,7 +776,7 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat* __restrict__ 
                     int32_t sv[7];
 #pragma unroll
                     for (int q = 0; q < 7; ++q) sv[q] = acc[q][mi][ni][i];
-                    *cp = fold_sub(*cp, sv, d.pd, d.pinv);
+                    *cp = fold_sub_k32(*cp, sv, d.pd, d.pinv);
                 }
             }
         }
