@@ -252,6 +252,7 @@ def configs_main(args):
     gf_cases = [(f"u32-{n}-{m}", (n, p31, 1, m, 0, "u32")) for n in (5000, 10000, 20000) for m in ("blocked", "plain")]
     gf_cases += [("u32-20000-rank0.9-blocked", (20000, p31, 0.9, "blocked", 0, "u32")),
                  ("u32-20000-rank0.9-plain", (20000, p31, 0.9, "plain", 0, "u32")),
+                 ("u64-10000-blocked", (10000, p61, 1, "blocked", 0, "u64")),
                  ("u64-10000-plain", (10000, p61, 1, "plain", 0, "u64"))]
     if args.with_100k:
         gf_cases.append(("u32-100000-blocked", (100000, p31, 1, "blocked", 0, "u32")))

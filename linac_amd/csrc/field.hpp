@@ -92,7 +92,14 @@ struct FieldU64 {
     __device__ __forceinline__ uint64_t sub(uint64_t a, uint64_t b) const {
         return a >= b ? a - b : a + (p - b);
     }
+    __device__ __forceinline__ uint64_t add(uint64_t a, uint64_t b) const {
+        return a >= p - b ? a - (p - b) : a + b;
+    }
     __device__ __forceinline__ uint64_t to_mont(uint64_t a) const { return mont(a, r2); }
+    // (h x - a y) R^{-1} mod p (the fraction-free row update of FieldU32).
+    __device__ __forceinline__ uint64_t ff_update(uint64_t h, uint64_t x, uint64_t a, uint64_t y) const {
+        return sub(mont(h, x), mont(a, y));
+    }
     __device__ __forceinline__ uint64_t inverse(uint64_t h) const {
         const uint64_t e = p - 2ull;
         const uint64_t hm = to_mont(h);

@@ -89,4 +89,54 @@ __device__ __forceinline__ uint32_t fold_sub_k32(uint32_t cval, const int32_t s[
     return (uint32_t)t;
 }
 
+// ---------------------------------------------------------------------------
+// 64-bit storage: 2^32 < p < 2^63.  Centred values (|c| < 2^62) take eight
+// balanced base-256 digits (the top one within [-65, 65]), so a modular
+// product needs 64 limb products whose sums fall on 15 diagonals
+//   S_s = sum_{l1+l2=s} D_{l1,l2},  |S_s| <= 8 * K * 2^14 = 2^25  (K <= 256).
+// The fold groups them in base-2^24 triples
+//   T_i = S_3i + 2^8 S_3i+1 + 2^16 S_3i+2   (|T_i| < 2^42, exact in int64 and double)
+// and runs Horner r <- r 2^24 + T_i with a reduction at every step: the
+// quotient q = rint((r 2^24 + T_i) / p) comes from double precision (|q| <
+// 2^24 once |r| <= p/2, since |r| stays within (1/2 + 2^-27) p), and the
+// remainder is formed exactly in wrapping 64-bit integers, which is correct
+// because its true value is below p in magnitude.
+// ---------------------------------------------------------------------------
+
+constexpr uint64_t LIMB64_P_MIN = 1ull << 32;   // smaller primes use 32-bit storage
+constexpr uint64_t LIMB64_P_MAX = 1ull << 63;   // exclusive; the field arithmetic needs p < 2^63
+
+__device__ __forceinline__ int64_t centred(uint64_t v, uint64_t p) {
+    return v > (p - 1) / 2 ? (int64_t)(v - p) : (int64_t)v;
+}
+
+__device__ __forceinline__ void balanced_digits(int64_t c, int8_t d[8]) {
+    int64_t x = c;
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+        const int8_t dl = (int8_t)(x & 0xFF);
+        d[l] = dl;
+        x = (x - dl) >> 8;
+    }
+    d[7] = (int8_t)x;
+}
+
+// (cval - sum_{s<15} 256^s S_s) mod p for 2^32 <= p < 2^63.
+__device__ __forceinline__ uint64_t fold_sub64(uint64_t cval, const int32_t s[15], uint64_t p, double pinv) {
+    int64_t T[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        T[i] = (int64_t)s[3 * i] + (int64_t)s[3 * i + 1] * 256 + (int64_t)s[3 * i + 2] * 65536;
+    int64_t r = T[4];
+    r -= (int64_t)(int32_t)rint((double)r * pinv) * (int64_t)p;  // |T_4| / p < 2^10
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+        const double xd = fma((double)r, 16777216.0, (double)T[i]);
+        const int32_t q = (int32_t)rint(xd * pinv);
+        r = (int64_t)(((uint64_t)r << 24) + (uint64_t)T[i] - (uint64_t)((int64_t)q * (int64_t)p));
+    }
+    const uint64_t ur = r < 0 ? (uint64_t)(r + (int64_t)p) : (uint64_t)r;
+    return cval >= ur ? cval - ur : cval + (p - ur);
+}
+
 }  // namespace linac

@@ -58,6 +58,24 @@ constexpr int CI = 32;         // candidate rows per inner step
 constexpr int XLD = 2 * BO;    // leading dimension of the panel buffer [P | Z]
 constexpr int KPI = 64;        // padded depth of the inner GEMM
 
+// The two storage widths: element type, field, signed type of centred values
+// and the number of balanced base-256 digits (int8 limb planes) per entry.
+// 32-bit: p <= 4278124287, 4 digits, 16 limb products on 7 diagonals;
+// 64-bit: 2^32 <= p < 2^63, 8 digits, 64 limb products on 15 diagonals.
+struct LimbsU32 {
+    using T = uint32_t;
+    using F = FieldU32;
+    using S = int32_t;
+    static constexpr int NL = 4;
+};
+struct LimbsU64 {
+    using T = uint64_t;
+    using F = FieldU64;
+    using S = int64_t;
+    static constexpr int NL = 8;
+};
+
+template <class T>
 struct BlockState {
     int32_t k;           // pivots found by the last inner step
     int32_t q0;          // pivot slots used in the current outer panel
@@ -67,7 +85,7 @@ struct BlockState {
     int32_t pad_;
     int64_t R[CI];       // pivot rows of the last inner step (pivot order)
     int32_t Cp[CI];      // their columns relative to the inner panel start
-    uint32_t apinv[CI * CI];  // Montgomery form of (X[R, c0 + Cp])^{-1}
+    T apinv[CI * CI];    // Montgomery form of (X[R, c0 + Cp])^{-1}
     unsigned long long tphase[8];  // accumulated wall-clock ticks per phase of k_inner_factor
 };
 
@@ -77,22 +95,24 @@ struct BlockState {
 // counts (GEMMs).  The host rewrites the descriptors before each step.
 // ---------------------------------------------------------------------------
 
+template <class LT>
 struct BMat {
-    uint32_t* A;
+    using T = typename LT::T;
+    T* A;
     int64_t ld, N, M;
-    FieldU32 f;
+    typename LT::F f;
     int32_t pad0;
     double pd, pinv;
-    uint32_t* X;        // [N][XLD] panel buffer [P | Z]
-    int8_t* planesL;    // [4][N][256]
-    int8_t* planesY;    // [4][256][256]
-    int8_t* planesB;    // [4][max(M, XLD)][256]
-    int8_t* planesAi;   // [4][N][KPI]   left operand of the inner GEMM
-    int8_t* planesUi;   // [4][XLD][KPI] right operand of the inner GEMM
-    uint32_t* U;        // [CI][XLD]
-    uint32_t* Gq;       // [256][ldg]
+    T* X;               // [N][XLD] panel buffer [P | Z]
+    int8_t* planesL;    // [NL][N][256]
+    int8_t* planesY;    // [NL][256][256]
+    int8_t* planesB;    // [NL][max(M, XLD)][256]
+    int8_t* planesAi;   // [NL][N][KPI]   left operand of the inner GEMM
+    int8_t* planesUi;   // [NL][XLD][KPI] right operand of the inner GEMM
+    T* U;               // [CI][XLD]
+    T* Gq;              // [256][ldg]
     int64_t strideL, strideB, strideAi, ldg;
-    BlockState* bs;
+    BlockState<T>* bs;
     ElimState* st;
     int64_t* piv_rows;
     int64_t* piv_cols;
@@ -115,8 +135,9 @@ constexpr int64_t STRIDE_Y = 256 * 256;
 // panel buffer management
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_panel_load(const BMat* __restrict__ D) {
-    const BMat& d = D[blockIdx.y];
+template <class LT>
+__global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__ D) {
+    const BMat<LT>& d = D[blockIdx.y];
     const int64_t total = d.N * XLD;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = x / XLD;
@@ -125,8 +146,9 @@ __global__ __launch_bounds__(256) void k_panel_load(const BMat* __restrict__ D) 
     }
 }
 
-__global__ __launch_bounds__(256) void k_panel_store(const BMat* __restrict__ D) {
-    const BMat& d = D[blockIdx.y];
+template <class LT>
+__global__ __launch_bounds__(256) void k_panel_store(const BMat<LT>* __restrict__ D) {
+    const BMat<LT>& d = D[blockIdx.y];
     const int64_t total = d.N * d.bcur;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = x / d.bcur;
@@ -140,9 +162,10 @@ __global__ __launch_bounds__(256) void k_epoch_init(int32_t* epoch, int64_t N) {
         epoch[r] = -1;
 }
 
-__global__ void k_block_state_init(const BMat* __restrict__ D, int reset_timers) {
-    const BMat& d = D[blockIdx.x];
-    BlockState* bs = d.bs;
+template <class LT>
+__global__ void k_block_state_init(const BMat<LT>* __restrict__ D, int reset_timers) {
+    const BMat<LT>& d = D[blockIdx.x];
+    auto* bs = d.bs;
     if (reset_timers)
         for (int i = 0; i < 8; ++i) bs->tphase[i] = 0;
     bs->k = 0;
@@ -168,14 +191,16 @@ static_assert(BI + CI == 64, "one augmented column per lane");
 // column suffices.  No rows are swapped: pivots are recorded by their LDS
 // row, and the identity columns index LDS rows directly.
 
-__device__ static void inner_factor_body(uint32_t* __restrict__ X, int64_t N,
-                                                      uint8_t* __restrict__ used, int32_t* __restrict__ slot_of,
-                                                      int32_t* __restrict__ epoch_of, int epoch,
-                                                      int64_t colbase, FieldU32 f, ElimState* st,
-                                                      int64_t* __restrict__ piv_rows, int64_t* __restrict__ piv_cols,
-                                                      BlockState* bs) {
+template <class LT>
+__device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t N,
+                                         uint8_t* __restrict__ used, int32_t* __restrict__ slot_of,
+                                         int32_t* __restrict__ epoch_of, int epoch,
+                                         int64_t colbase, typename LT::F f, ElimState* st,
+                                         int64_t* __restrict__ piv_rows, int64_t* __restrict__ piv_cols,
+                                         BlockState<typename LT::T>* bs) {
+    using T = typename LT::T;
     constexpr int W2 = BI + CI;
-    __shared__ uint32_t aug[CI][W2 + 1];
+    __shared__ T aug[CI][W2 + 1];
     __shared__ int64_t cand[CI + 1];
     __shared__ int prow_of[CI];   // LDS row of the t-th pivot
     __shared__ int pcol[CI];      // its column within the inner panel
@@ -198,18 +223,17 @@ __device__ static void inner_factor_body(uint32_t* __restrict__ X, int64_t N,
         const int64_t r = base + tid;
         bool fl = false;
         if (r < N && !used[r]) {  // unused row with a non-zero entry in the inner panel
-            const uint4* row = reinterpret_cast<const uint4*>(X + r * XLD + c0);  // 128-byte aligned
-            uint4 v[BI / 4];
+            constexpr int PER = 16 / (int)sizeof(T);  // elements per 16-byte vector
+            const Vec16<T>* row = reinterpret_cast<const Vec16<T>*>(X + r * XLD + c0);  // 128-byte aligned
+            Vec16<T> v[BI / PER];
 #pragma unroll
-            for (int q = 0; q < BI / 4; ++q) v[q] = row[q];
-            uint32_t acc = 0;
+            for (int q = 0; q < BI / PER; ++q) v[q] = row[q];
+            T acc = 0;
 #pragma unroll
-            for (int q = 0; q < BI / 4; ++q) {
-                const int j = 4 * q;
-                acc |= (j + 0 < w ? v[q].x : 0u) | (j + 1 < w ? v[q].y : 0u) | (j + 2 < w ? v[q].z : 0u) |
-                       (j + 3 < w ? v[q].w : 0u);
-            }
-            fl = acc != 0u;
+            for (int q = 0; q < BI / PER; ++q)
+#pragma unroll
+                for (int e = 0; e < PER; ++e) acc |= (PER * q + e < w ? v[q].v[e] : (T)0);
+            fl = acc != (T)0;
         }
         const unsigned long long m = __ballot(fl);
         if (lane == 0) s_wcnt[wid] = __popcll(m);
@@ -246,10 +270,10 @@ __device__ static void inner_factor_body(uint32_t* __restrict__ X, int64_t N,
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             const int ii = i + 16 * half;
-            uint32_t v = 0u;
+            T v = 0;
             if (ii < nC) {
                 if (j < w) v = X[cand[ii] * XLD + c0 + j];
-                else if (j >= BI) v = (j - BI == ii) ? 1u : 0u;
+                else if (j >= BI) v = (j - BI == ii) ? 1 : 0;
             }
             aug[ii][j] = v;
         }
@@ -263,18 +287,18 @@ __device__ static void inner_factor_body(uint32_t* __restrict__ X, int64_t N,
     int t = 0;
     const int r0 = 2 * wid, r1 = 2 * wid + 1;
     for (int c = 0; c < w && t < nC; ++c) {
-        const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != 0u;
+        const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
         const unsigned long long m = __ballot(nz);
         if (m == 0ull) continue;  // no pivot in this column (every wave sees the same)
         const int pv = __ffsll((long long)m) - 1;
-        const uint32_t h = aug[pv][c];
-        const uint32_t y = aug[pv][lane];
+        const T h = aug[pv][c];
+        const T y = aug[pv][lane];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int r = q ? r1 : r0;
             if (r < nC && r != pv) {
-                const uint32_t a = aug[r][c];
-                const uint32_t x = aug[r][lane];
+                const T a = aug[r][c];
+                const T x = aug[r][lane];
                 aug[r][lane] = f.ff_update(h, x, a, y);
             }
         }
@@ -293,7 +317,7 @@ __device__ static void inner_factor_body(uint32_t* __restrict__ X, int64_t N,
     // 4. normalise the pivot rows.  The k inverses run side by side in the
     //    lanes of wave 0 (one Fermat chain in total instead of one per wave,
     //    which would serialise on the SIMDs), then each wave scales its rows.
-    __shared__ uint32_t s_invm[CI];
+    __shared__ T s_invm[CI];
     __shared__ int s_tof[CI];
     if (wid == 0 && lane < k) {
         const int r = prow_of[lane];
@@ -325,7 +349,7 @@ __device__ static void inner_factor_body(uint32_t* __restrict__ X, int64_t N,
         slot_of[r] = tid;
         piv_rows[rank0 + tid] = r;
         piv_cols[rank0 + tid] = colbase + c0 + pcol[tid];
-        X[r * XLD + BO + q0 + tid] = 1u;  // Z[r, slot] = 1
+        X[r * XLD + BO + q0 + tid] = 1;  // Z[r, slot] = 1
     }
     __syncthreads();
     // advance first_unused past the rows that are now pivots, 1024 rows per probe
@@ -359,9 +383,10 @@ __device__ static void inner_factor_body(uint32_t* __restrict__ X, int64_t N,
 }
 
 
-__global__ __launch_bounds__(IF_T) void k_inner_factor(const BMat* __restrict__ D, int epoch) {
-    const BMat& d = D[blockIdx.x];
-    inner_factor_body(d.X, d.N, d.used, d.slot_of, d.epoch_of, epoch, d.J, d.f, d.st, d.piv_rows, d.piv_cols, d.bs);
+template <class LT>
+__global__ __launch_bounds__(IF_T) void k_inner_factor(const BMat<LT>* __restrict__ D, int epoch) {
+    const BMat<LT>& d = D[blockIdx.x];
+    inner_factor_body<LT>(d.X, d.N, d.used, d.slot_of, d.epoch_of, epoch, d.J, d.f, d.st, d.piv_rows, d.piv_cols, d.bs);
 }
 
 // U = Apiv^{-1} X[R, :] (k x XLD, Montgomery inverse times plain rows gives plain
@@ -375,11 +400,14 @@ __global__ __launch_bounds__(IF_T) void k_inner_factor(const BMat* __restrict__ 
 constexpr int MU_BLOCKS = XLD / 4;
 constexpr int MU_ROWS = 16;
 
-__device__ static void make_u_body(const uint32_t* __restrict__ X, int64_t N, FieldU32 f,
-                                   const BlockState* __restrict__ bs, uint32_t* __restrict__ U,
+template <class LT>
+__device__ static void make_u_body(const typename LT::T* __restrict__ X, int64_t N, typename LT::F f,
+                                   const BlockState<typename LT::T>* __restrict__ bs, typename LT::T* __restrict__ U,
                                    int8_t* __restrict__ planesU, int64_t strideU,
                                    int8_t* __restrict__ planesA, int64_t strideA) {
-    __shared__ uint32_t s_inv[CI * CI];
+    using T = typename LT::T;
+    constexpr int NL = LT::NL;
+    __shared__ T s_inv[CI * CI];
     __shared__ int64_t s_R[CI];
     __shared__ int s_cp[CI];
     const int k = bs->k;
@@ -392,20 +420,20 @@ __device__ static void make_u_body(const uint32_t* __restrict__ X, int64_t N, Fi
         const int64_t col = t < k ? c0 + s_cp[t] : 0;
         for (int64_t r0 = (int64_t)(blockIdx.x - MU_BLOCKS) * MU_ROWS + threadIdx.y; r0 < N;
              r0 += (int64_t)(gridDim.x - MU_BLOCKS) * MU_ROWS) {
-            uint32_t v[MU_ROWS / 4];
+            T v[MU_ROWS / 4];
 #pragma unroll
             for (int i = 0; i < MU_ROWS / 4; ++i) {
                 const int64_t r = r0 + 4 * i;
-                v[i] = (t < k && r < N) ? X[r * XLD + col] : 0u;
+                v[i] = (t < k && r < N) ? X[r * XLD + col] : (T)0;
             }
 #pragma unroll
             for (int i = 0; i < MU_ROWS / 4; ++i) {
                 const int64_t r = r0 + 4 * i;
                 if (r >= N) break;
-                int8_t a[4] = {0, 0, 0, 0};
+                int8_t a[NL] = {};
                 if (t < k) balanced_digits(centred(v[i], f.p), a);
 #pragma unroll
-                for (int l = 0; l < 4; ++l) planesA[l * strideA + r * KPI + t] = a[l];
+                for (int l = 0; l < NL; ++l) planesA[l * strideA + r * KPI + t] = a[l];
             }
         }
         return;
@@ -414,21 +442,22 @@ __device__ static void make_u_body(const uint32_t* __restrict__ X, int64_t N, Fi
     if (threadIdx.y == 0 && t < k) s_R[t] = bs->R[t];
     __syncthreads();
     const int n = blockIdx.x * 4 + threadIdx.y;
-    uint32_t u = 0;
+    T u = 0;
     if (t < k) {
         for (int s = 0; s < k; ++s) u = f.add(u, f.mont(s_inv[t * CI + s], X[s_R[s] * XLD + n]));
         U[t * XLD + n] = u;
     }
-    int8_t d[4] = {0, 0, 0, 0};
+    int8_t d[NL] = {};
     if (t < k) balanced_digits(centred(u, f.p), d);
 #pragma unroll
-    for (int l = 0; l < 4; ++l) planesU[l * strideU + (int64_t)n * KPI + t] = d[l];
+    for (int l = 0; l < NL; ++l) planesU[l * strideU + (int64_t)n * KPI + t] = d[l];
 }
 
 
-__global__ __launch_bounds__(256) void k_make_u(const BMat* __restrict__ D) {
-    const BMat& d = D[blockIdx.y];
-    make_u_body(d.X, d.N, d.f, d.bs, d.U, d.planesUi, (int64_t)XLD * KPI, d.planesAi, d.strideAi);
+template <class LT>
+__global__ __launch_bounds__(256) void k_make_u(const BMat<LT>* __restrict__ D) {
+    const BMat<LT>& d = D[blockIdx.y];
+    make_u_body<LT>(d.X, d.N, d.f, d.bs, d.U, d.planesUi, (int64_t)XLD * KPI, d.planesAi, d.strideAi);
 }
 
 // ---------------------------------------------------------------------------
@@ -440,8 +469,10 @@ __global__ __launch_bounds__(256) void k_make_u(const BMat* __restrict__ D) {
 //           panel's pivot rows (their epochs lie in [e0, e1));
 //   mode 1: planesY[l][i][q] = digit l of -Z[Q[i], q] for the panel's pivot rows.
 // q < q0 carries data, q in [q0, Kq) is zero padding.
-__global__ __launch_bounds__(256) void k_make_lhs(const BMat* __restrict__ D, int mode, int e1) {
-    const BMat& d = D[blockIdx.y];
+template <class LT>
+__global__ __launch_bounds__(256) void k_make_lhs(const BMat<LT>* __restrict__ D, int mode, int e1) {
+    constexpr int NL = LT::NL;
+    const BMat<LT>& d = D[blockIdx.y];
     if (d.q0 == 0 || d.nT == 0) return;
     const int64_t rows = mode == 0 ? d.N : d.q0;
     const int Kpad = d.Kq;
@@ -453,11 +484,11 @@ __global__ __launch_bounds__(256) void k_make_lhs(const BMat* __restrict__ D, in
         const int64_t m = x / Kpad;
         const int t = (int)(x - m * Kpad);
         const int64_t r = mode == 0 ? m : Q[m];
-        int8_t dg[4] = {0, 0, 0, 0};
+        int8_t dg[NL] = {};
         const bool zero_row = mode == 0 && d.epoch_of[r] >= d.e0 && d.epoch_of[r] < e1;
         if (t < d.q0 && !zero_row) balanced_digits(-centred(d.X[r * XLD + BO + t], d.f.p), dg);
 #pragma unroll
-        for (int l = 0; l < 4; ++l) planes[l * stride + m * Kpad + t] = dg[l];
+        for (int l = 0; l < NL; ++l) planes[l * stride + m * Kpad + t] = dg[l];
     }
 }
 
@@ -465,9 +496,12 @@ __global__ __launch_bounds__(256) void k_make_lhs(const BMat* __restrict__ D, in
 // A[Q[t], T0 + n] for t < q0, zero for t in [q0, Kq).  64 x 64 tiles through
 // LDS so that both the row reads and the plane writes are coalesced.
 // blockIdx.x strides over column tiles, blockIdx.y = t-tile, blockIdx.z = matrix.
-__global__ __launch_bounds__(256) void k_make_rhs(const BMat* __restrict__ D) {
-    __shared__ int32_t tile[64][65];
-    const BMat& d = D[blockIdx.z];
+template <class LT>
+__global__ __launch_bounds__(256) void k_make_rhs(const BMat<LT>* __restrict__ D) {
+    using S = typename LT::S;
+    constexpr int NL = LT::NL;
+    __shared__ S tile[64][65];
+    const BMat<LT>& d = D[blockIdx.z];
     if (d.q0 == 0 || d.nT == 0) return;
     const int tb = blockIdx.y * 64;
     if (tb >= d.Kq) return;
@@ -477,7 +511,7 @@ __global__ __launch_bounds__(256) void k_make_rhs(const BMat* __restrict__ D) {
             const int tt = x >> 6, nn = x & 63;
             const int t = tb + tt;
             const int64_t n = nb + nn;
-            int32_t v = 0;
+            S v = 0;
             if (t < d.q0 && n < d.nT) v = centred(d.A[Q[t] * d.ld + d.T0 + n], d.f.p);
             tile[tt][nn] = v;
         }
@@ -485,16 +519,16 @@ __global__ __launch_bounds__(256) void k_make_rhs(const BMat* __restrict__ D) {
         const int nn = threadIdx.x >> 2, tq = (threadIdx.x & 3) * 16;
         const int64_t n = nb + nn;
         if (n < d.nT) {
-            int8_t buf[4][16];
+            int8_t buf[NL][16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                int8_t dg[4];
+                int8_t dg[NL];
                 balanced_digits(tile[tq + i][nn], dg);
 #pragma unroll
-                for (int l = 0; l < 4; ++l) buf[l][i] = dg[l];
+                for (int l = 0; l < NL; ++l) buf[l][i] = dg[l];
             }
 #pragma unroll
-            for (int l = 0; l < 4; ++l) {
+            for (int l = 0; l < NL; ++l) {
                 v4i w;
                 __builtin_memcpy(&w, buf[l], 16);
                 *reinterpret_cast<v4i*>(d.planesB + l * d.strideB + n * d.Kq + tb + tq) = w;
@@ -505,8 +539,9 @@ __global__ __launch_bounds__(256) void k_make_rhs(const BMat* __restrict__ D) {
 }
 
 // A[Q[t], T0 + n] = Gq[t, n] (blockIdx.y = t, blockIdx.z = matrix).
-__global__ __launch_bounds__(256) void k_scatter_q(const BMat* __restrict__ D) {
-    const BMat& d = D[blockIdx.z];
+template <class LT>
+__global__ __launch_bounds__(256) void k_scatter_q(const BMat<LT>* __restrict__ D) {
+    const BMat<LT>& d = D[blockIdx.z];
     const int t = blockIdx.y;
     if (t >= d.q0 || d.nT == 0) return;
     const int64_t r = d.piv_rows[d.rank_start + t];
@@ -573,13 +608,13 @@ struct TileSched {
 // MODE 1: A[:, T] <- A[:, T] + Z A[Q, T] (rows of Q have zero Z rows here).
 // Both are C <- C - L B with L = -Z limbs.
 template <int MODE>
-__global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat* __restrict__ D, TileSched ts) {
+__global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __restrict__ D, TileSched ts) {
     __shared__ __attribute__((aligned(16))) int8_t sA[2][4][G_M * G_ROW];
     __shared__ __attribute__((aligned(16))) int8_t sB[2][4][G_N * G_ROW];
     int mi_;
     int64_t tm, tn;
     if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
-    const BMat& d = D[mi_];
+    const BMat<LimbsU32>& d = D[mi_];
     uint32_t* C = MODE == 0 ? d.Gq : d.A + d.T0;
     const int64_t ldc = MODE == 0 ? d.ldg : d.ld;
     const int64_t Mrows = MODE == 0 ? d.q0 : d.N;
@@ -699,14 +734,14 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat* __restrict__ 
 // column window [c0, BO + q0) only meet zeros and are skipped.  The kernel is
 // latency-bound (one K stage per tile): four workgroups per CU (a 20-byte
 // register spill in the epilogue) measured faster than three without.
-__global__ __launch_bounds__(G_T, 4) void k_gemm_inner(const BMat* __restrict__ D, TileSched ts, int epoch) {
+__global__ __launch_bounds__(G_T, 4) void k_gemm_inner(const BMat<LimbsU32>* __restrict__ D, TileSched ts, int epoch) {
     __shared__ __attribute__((aligned(16))) int8_t sA[4][G_M * G_ROW];
     __shared__ __attribute__((aligned(16))) int8_t sB[4][G_N * G_ROW];
     int mi_;
     int64_t tm, tn;
     if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
-    const BMat& d = D[mi_];
-    const BlockState* bs = d.bs;
+    const BMat<LimbsU32>& d = D[mi_];
+    const BlockState<uint32_t>* bs = d.bs;
     const int k = bs->k;
     if (k == 0) return;
     const int64_t m_base = tm * G_M, n_base = tn * G_N;
@@ -783,12 +818,208 @@ __global__ __launch_bounds__(G_T, 4) void k_gemm_inner(const BMat* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// 64-bit storage: the same 64 x 64 tiles (4 waves of 32 x 32) with eight limb
+// planes per operand.  A 64-deep K stage is 8 x 8 x 4 = 256 MFMAs per wave,
+// long enough to hide the next stage's global loads behind a single LDS
+// buffer (64 KB: 8 limbs x 64 rows x 64 bytes per operand).  The 15 x 4
+// accumulators take 240 registers, so one workgroup runs per CU.
+// ---------------------------------------------------------------------------
+
+constexpr int NL64 = LimbsU64::NL, ND64 = 2 * NL64 - 1;
+
+// acc[l1 + l2] += A_l1 B_l2 over one staged K slice (all 64 limb pairs).
+__device__ __forceinline__ void mfma_stage64(const int8_t (*sA)[G_M * G_ROW], const int8_t (*sB)[G_N * G_ROW],
+                                             int wm, int wn, int r16, int g, v4i (&acc)[ND64][2][2]) {
+    v4i bf[NL64][2];
+#pragma unroll
+    for (int l = 0; l < NL64; ++l)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][lds_at(wn * 32 + ni * 16 + r16, g)]);
+#pragma unroll
+    for (int l1 = 0; l1 < NL64; ++l1) {
+        v4i af[2];
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+            af[mi] = *reinterpret_cast<const v4i*>(&sA[l1][lds_at(wm * 32 + mi * 16 + r16, g)]);
+#pragma unroll
+        for (int l2 = 0; l2 < NL64; ++l2)
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni)
+                    acc[l1 + l2][mi][ni] =
+                        __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
+    }
+}
+
+// Outer update GEMMs for 64-bit storage (MODE as in k_gemm_outer).
+template <int MODE>
+__global__ __launch_bounds__(G_T, 1) void k_gemm_outer64(const BMat<LimbsU64>* __restrict__ D, TileSched ts) {
+    __shared__ __attribute__((aligned(16))) int8_t sA[NL64][G_M * G_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[NL64][G_N * G_ROW];
+    int mi_;
+    int64_t tm, tn;
+    if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
+    const BMat<LimbsU64>& d = D[mi_];
+    uint64_t* C = MODE == 0 ? d.Gq : d.A + d.T0;
+    const int64_t ldc = MODE == 0 ? d.ldg : d.ld;
+    const int64_t Mrows = MODE == 0 ? d.q0 : d.N;
+    const int64_t Ncols = d.nT;
+    const int8_t* L = MODE == 0 ? d.planesY : d.planesL;
+    const int64_t planeL = MODE == 0 ? STRIDE_Y : d.strideL;
+    const int8_t* B = d.planesB;
+    const int64_t planeB = d.strideB;
+    const int Kpad = d.Kq;
+    const uint64_t p = d.f.p;
+    const double pinv = d.pinv;
+    const int64_t m_base = tm * G_M, n_base = tn * G_N;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int r16 = lane & 15, g = lane >> 4;
+    const v4i zero = {0, 0, 0, 0};
+
+    uint64_t cpre[2][2][4];
+    auto cload = [&]() {
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
+                    const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+                    cpre[mi][ni][i] = (MODE == 1 && row < Mrows && col < Ncols) ? C[row * ldc + col] : 0ull;
+                }
+    };
+    v4i ra[NL64], rb[NL64];
+    const int srow = tid >> 2, squ = tid & 3;
+    const bool a_ok = m_base + srow < Mrows, b_ok = n_base + srow < Ncols;
+    const int8_t* pa = L + (a_ok ? (m_base + srow) * Kpad : 0) + squ * 16;
+    const int8_t* pb = B + (b_ok ? (n_base + srow) * Kpad : 0) + squ * 16;
+    const int soff = lds_at(srow, squ);
+    auto gload = [&](int kc) {
+#pragma unroll
+        for (int q = 0; q < NL64; ++q) {
+            ra[q] = a_ok ? *reinterpret_cast<const v4i*>(pa + q * planeL + kc) : zero;
+            rb[q] = b_ok ? *reinterpret_cast<const v4i*>(pb + q * planeB + kc) : zero;
+        }
+    };
+
+    v4i acc[ND64][2][2];
+#pragma unroll
+    for (int s = 0; s < ND64; ++s)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
+
+    const int nk = Kpad / G_K;
+    gload(0);
+    for (int kk = 0; kk < nk; ++kk) {
+#pragma unroll
+        for (int q = 0; q < NL64; ++q) {
+            *reinterpret_cast<v4i*>(&sA[q][soff]) = ra[q];
+            *reinterpret_cast<v4i*>(&sB[q][soff]) = rb[q];
+        }
+        __syncthreads();
+        if (kk + 1 < nk) gload((kk + 1) * G_K);
+        else cload();
+        mfma_stage64(sA, sB, wm, wn, r16, g, acc);
+        __syncthreads();  // the next stage overwrites the buffer
+    }
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
+                const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+                if (row < Mrows && col < Ncols) {
+                    int32_t sv[ND64];
+#pragma unroll
+                    for (int q = 0; q < ND64; ++q) sv[q] = acc[q][mi][ni][i];
+                    C[row * ldc + col] = fold_sub64(cpre[mi][ni][i], sv, p, pinv);
+                }
+            }
+}
+
+// Inner-step GEMM for 64-bit storage (as k_gemm_inner).
+__global__ __launch_bounds__(G_T, 1) void k_gemm_inner64(const BMat<LimbsU64>* __restrict__ D, TileSched ts,
+                                                         int epoch) {
+    __shared__ __attribute__((aligned(16))) int8_t sA[NL64][G_M * G_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[NL64][G_N * G_ROW];
+    int mi_;
+    int64_t tm, tn;
+    if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
+    const BMat<LimbsU64>& d = D[mi_];
+    const BlockState<uint64_t>* bs = d.bs;
+    const int k = bs->k;
+    if (k == 0) return;
+    const int64_t m_base = tm * G_M, n_base = tn * G_N;
+    const int c0 = bs->c0_cur;
+    if (n_base + G_N <= c0 || n_base >= BO + bs->q0) return;
+    uint64_t* X = d.X;
+    const int64_t N = d.N;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int r16 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int q = 0; q < NL64; ++q) {
+        const int c = tid + q * G_T;
+        const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
+        *reinterpret_cast<v4i*>(&sB[l][lds_at(col, qu)]) =
+            *reinterpret_cast<const v4i*>(d.planesUi + l * (int64_t)XLD * KPI + (n_base + col) * KPI + qu * 16);
+        const int64_t row = m_base + col;
+        v4i w = {0, 0, 0, 0};
+        if (row < N) w = *reinterpret_cast<const v4i*>(d.planesAi + l * d.strideAi + row * KPI + qu * 16);
+        *reinterpret_cast<v4i*>(&sA[l][lds_at(col, qu)]) = w;
+    }
+    __syncthreads();
+    const v4i zero = {0, 0, 0, 0};
+    v4i acc[ND64][2][2];
+#pragma unroll
+    for (int s = 0; s < ND64; ++s)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
+    mfma_stage64(sA, sB, wm, wn, r16, g, acc);
+    const uint64_t p = d.f.p;
+    const double pinv = d.pinv;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
+            if (row >= N) continue;
+            const bool pivot_row = d.epoch_of[row] == epoch;
+            const int slot = pivot_row ? d.slot_of[row] : 0;
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+                uint64_t* cp = X + row * XLD + col;
+                if (pivot_row) {
+                    *cp = d.U[slot * XLD + col];
+                } else {
+                    int32_t sv[ND64];
+#pragma unroll
+                    for (int q = 0; q < ND64; ++q) sv[q] = acc[q][mi][ni][i];
+                    *cp = fold_sub64(*cp, sv, p, pinv);
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
 // per-matrix set-up and status
 // ---------------------------------------------------------------------------
 
 // used = 0, epoch_of = -1, state reset, entries reduced mod p (blockIdx.y = matrix)
-__global__ __launch_bounds__(256) void k_item_init(const BMat* __restrict__ D) {
-    const BMat& d = D[blockIdx.y];
+template <class LT>
+__global__ __launch_bounds__(256) void k_item_init(const BMat<LT>* __restrict__ D) {
+    const BMat<LT>& d = D[blockIdx.y];
     const int64_t g0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, gs = (int64_t)gridDim.x * blockDim.x;
     if (g0 == 0) {
         d.st->prow = -1;
@@ -803,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_item_init(const BMat* __restrict__ D) {
     const int64_t total = d.N * d.M;
     for (int64_t x = g0; x < total; x += gs) {
         const int64_t r = x / d.M, c = x - r * d.M;
-        uint32_t& v = d.A[r * d.ld + c];
+        auto& v = d.A[r * d.ld + c];
         v = d.f.reduce(v);
     }
 }
@@ -815,13 +1046,15 @@ struct ItemStatus {
 
 // Ranks, then pivot lists, of many finished matrices in one launch each
 // (blockIdx.x = matrix); `offs` are the prefix sums of the ranks.
-__global__ void k_collect_rank(const BMat* __restrict__ D, int64_t* out) {
+template <class LT>
+__global__ void k_collect_rank(const BMat<LT>* __restrict__ D, int64_t* out) {
     if (threadIdx.x == 0) out[blockIdx.x] = D[blockIdx.x].st->rank;
 }
 
-__global__ __launch_bounds__(256) void k_collect_pivots(const BMat* __restrict__ D, const int64_t* __restrict__ offs,
+template <class LT>
+__global__ __launch_bounds__(256) void k_collect_pivots(const BMat<LT>* __restrict__ D, const int64_t* __restrict__ offs,
                                                         int64_t* __restrict__ rows, int64_t* __restrict__ cols) {
-    const BMat& d = D[blockIdx.x];
+    const BMat<LT>& d = D[blockIdx.x];
     const int64_t o = offs[blockIdx.x], n = offs[blockIdx.x + 1] - o;
     for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
         rows[o + k] = d.piv_rows[k];
@@ -829,8 +1062,9 @@ __global__ __launch_bounds__(256) void k_collect_pivots(const BMat* __restrict__
     }
 }
 
-__global__ void k_gather_status(const BMat* __restrict__ D, ItemStatus* out) {
-    const BMat& d = D[blockIdx.x];
+template <class LT>
+__global__ void k_gather_status(const BMat<LT>* __restrict__ D, ItemStatus* out) {
+    const BMat<LT>& d = D[blockIdx.x];
     ItemStatus s;
     s.c0_next = d.bs->c0_next;
     s.bcur = d.bs->bcur;
@@ -848,20 +1082,23 @@ struct WorkspaceLayout {
     size_t X, L, Y, B, Ai, Ui, U, G, bs, st, piv, used, epoch, slot, total;
 };
 
+template <class LT>
 static WorkspaceLayout workspace_layout(int64_t N, int64_t M, bool with_elim_state) {
+    using T = typename LT::T;
+    constexpr size_t NL = LT::NL, E = sizeof(T);
     auto al = [](size_t x) { return (size_t)round_up((int64_t)x, 256); };
     WorkspaceLayout w{};
     const int64_t cap = std::max<int64_t>(1, std::min(N, M));
     size_t off = 0;
-    w.X = off; off += al((size_t)N * XLD * 4);
-    w.L = off; off += al((size_t)4 * round_up(N * 256, 256));
-    w.Y = off; off += al((size_t)4 * STRIDE_Y);
-    w.B = off; off += al((size_t)4 * round_up(std::max<int64_t>(M, XLD) * 256, 256));
-    w.Ai = off; off += al((size_t)4 * round_up(N * KPI, 256));
-    w.Ui = off; off += al((size_t)4 * XLD * KPI);
-    w.U = off; off += al((size_t)CI * XLD * 4);
-    w.G = off; off += al((size_t)256 * round_up(M, 64) * 4);
-    w.bs = off; off += al(sizeof(BlockState));
+    w.X = off; off += al((size_t)N * XLD * E);
+    w.L = off; off += al(NL * round_up(N * 256, 256));
+    w.Y = off; off += al(NL * STRIDE_Y);
+    w.B = off; off += al(NL * round_up(std::max<int64_t>(M, XLD) * 256, 256));
+    w.Ai = off; off += al(NL * round_up(N * KPI, 256));
+    w.Ui = off; off += al(NL * XLD * KPI);
+    w.U = off; off += al((size_t)CI * XLD * E);
+    w.G = off; off += al((size_t)256 * round_up(M, 64) * E);
+    w.bs = off; off += al(sizeof(BlockState<T>));
     w.epoch = off; off += al((size_t)4 * N);
     w.slot = off; off += al((size_t)4 * N);
     if (with_elim_state) {
@@ -873,8 +1110,10 @@ static WorkspaceLayout workspace_layout(int64_t N, int64_t M, bool with_elim_sta
     return w;
 }
 
-static void carve(BMat& d, char* base, const WorkspaceLayout& w, int64_t N, int64_t M, bool with_elim_state) {
-    d.X = (uint32_t*)(base + w.X);
+template <class LT>
+static void carve(BMat<LT>& d, char* base, const WorkspaceLayout& w, int64_t N, int64_t M, bool with_elim_state) {
+    using T = typename LT::T;
+    d.X = (T*)(base + w.X);
     d.planesL = (int8_t*)(base + w.L);
     d.strideL = round_up(N * 256, 256);
     d.planesY = (int8_t*)(base + w.Y);
@@ -883,10 +1122,10 @@ static void carve(BMat& d, char* base, const WorkspaceLayout& w, int64_t N, int6
     d.planesAi = (int8_t*)(base + w.Ai);
     d.strideAi = round_up(N * KPI, 256);
     d.planesUi = (int8_t*)(base + w.Ui);
-    d.U = (uint32_t*)(base + w.U);
-    d.Gq = (uint32_t*)(base + w.G);
+    d.U = (T*)(base + w.U);
+    d.Gq = (T*)(base + w.G);
     d.ldg = round_up(M, 64);
-    d.bs = (BlockState*)(base + w.bs);
+    d.bs = (BlockState<T>*)(base + w.bs);
     d.epoch_of = (int32_t*)(base + w.epoch);
     d.slot_of = (int32_t*)(base + w.slot);
     if (with_elim_state) {
@@ -898,8 +1137,9 @@ static void carve(BMat& d, char* base, const WorkspaceLayout& w, int64_t N, int6
     }
 }
 
+template <class LT>
 struct Item {
-    BMat d{};
+    BMat<LT> d{};
     void* arena = nullptr;     // device memory owned by the item (null when borrowed)
     size_t bytes = 0;
     bool finished = false;
@@ -907,8 +1147,11 @@ struct Item {
     int64_t id = -1;
 };
 
+template <class LT>
 class Engine {
   public:
+    using BM = BMat<LT>;
+    using It = Item<LT>;
     Engine() = default;
     ~Engine() {
         if (so) (void)hipStreamSynchronize(so);
@@ -922,8 +1165,8 @@ class Engine {
     // One outer panel for every unfinished item in `items`.  The inner phase
     // runs on `s`; the trailing update on the engine's own stream, where the
     // part beyond the next panel keeps running after step() returns.
-    int step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof);
-    int init_items(const std::vector<Item*>& items, hipStream_t s);
+    int step(const std::vector<It*>& items, hipStream_t s, Profiler* prof);
+    int init_items(const std::vector<It*>& items, hipStream_t s);
     // Waits for every trailing update the engine has issued (before results
     // are read or an item's memory is released).
     int drain();
@@ -938,14 +1181,14 @@ class Engine {
     // follows the part before it on the same stream).
     static constexpr int NSLOT = 5;
     struct Staged {
-        BMat* desc;
+        BM* desc;
         TileSched inner, gq, main;
         int64_t inner_tiles, gq_tiles, main_tiles;
         ItemStatus* status;
     };
     int stage(int slot, size_t bytes);
     // part 0: inner phase; 1: trailing columns of the next panel; 2: the rest
-    int upload(const std::vector<Item*>& act, hipStream_t s, int slot, int part, Staged* out);
+    int upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out);
     int ensure_stream();
     hipStream_t so = nullptr;
     hipEvent_t ev_ready = nullptr;  // on `so`: next panel's columns updated, panel buffer free
@@ -956,19 +1199,22 @@ class Engine {
     size_t cap[NSLOT] = {};
 };
 
-int Engine::ensure_stream() {
+template <class LT>
+int Engine<LT>::ensure_stream() {
     if (so) return LINAC_OK;
     LINAC_HIP(hipStreamCreateWithFlags(&so, hipStreamNonBlocking));
     LINAC_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     return LINAC_OK;
 }
 
-int Engine::drain() {
+template <class LT>
+int Engine<LT>::drain() {
     if (so) LINAC_HIP(hipStreamSynchronize(so));
     return LINAC_OK;
 }
 
-int Engine::stage(int slot, size_t bytes) {
+template <class LT>
+int Engine<LT>::stage(int slot, size_t bytes) {
     if (bytes <= cap[slot]) return LINAC_OK;
     if (so) LINAC_HIP(hipStreamSynchronize(so));
     if (d_buf[slot]) LINAC_HIP(hipFree(d_buf[slot]));
@@ -983,19 +1229,20 @@ int Engine::stage(int slot, size_t bytes) {
 // Staging layout: [descriptors B][sched 6B+3 int64][status B].  The sched
 // arrays are prefix sums of per-matrix tile counts: (pa, ma) inner GEMM,
 // (pb, mb) Z[Q] A[Q, T], (pc, mc) main trailing update.
-int Engine::upload(const std::vector<Item*>& act, hipStream_t s, int slot, int part, Staged* out) {
+template <class LT>
+int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out) {
     const size_t B = act.size();
-    const size_t desc_b = (size_t)round_up((int64_t)(B * sizeof(BMat)), 256);
+    const size_t desc_b = (size_t)round_up((int64_t)(B * sizeof(BM)), 256);
     const size_t sched_b = (size_t)round_up((int64_t)((6 * B + 3) * 8), 256);
     const size_t stat_b = (size_t)round_up((int64_t)(B * sizeof(ItemStatus)), 256);
     LINAC_TRY(stage(slot, desc_b + sched_b + stat_b));
     char* h = (char*)h_buf[slot];
     char* dv = (char*)d_buf[slot];
-    BMat* hd = (BMat*)h;
+    BM* hd = (BM*)h;
     int64_t* hs = (int64_t*)(h + desc_b);
     int64_t* ds = (int64_t*)(dv + desc_b);
     for (size_t m = 0; m < B; ++m) {
-        BMat d = act[m]->d;
+        BM d = act[m]->d;
         if (part == 2) {  // the columns beyond the next panel
             const int64_t n1 = std::min<int64_t>(BO, d.nT);
             d.T0 += n1;
@@ -1015,7 +1262,7 @@ int Engine::upload(const std::vector<Item*>& act, hipStream_t s, int slot, int p
     int64_t* mc = pc + (B + 1);
     pa[0] = pb[0] = pc[0] = 0;
     for (size_t m = 0; m < B; ++m) {
-        const BMat& d = hd[m];
+        const BM& d = hd[m];
         ma[m] = ceil_div(d.N, G_M);
         pa[m + 1] = pa[m] + (part == 0 ? ma[m] * (XLD / G_N) : 0);
         const bool live = part != 0 && d.q0 > 0 && d.nT > 0;
@@ -1024,7 +1271,7 @@ int Engine::upload(const std::vector<Item*>& act, hipStream_t s, int slot, int p
         mc[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
         pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, G_N) : 0);
     }
-    out->desc = (BMat*)dv;
+    out->desc = (BM*)dv;
     const int grp = (int)tuning().gemm_group;
     out->inner = TileSched{ds, ds + (B + 1), (int)B, grp};
     out->gq = TileSched{ds + (B + 1) + B, ds + 2 * (B + 1) + B, (int)B, grp};
@@ -1041,20 +1288,23 @@ static unsigned gs_grid(int64_t work, int64_t cap = 4096) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cap, ceil_div(work, 256)));
 }
 
-int Engine::init_items(const std::vector<Item*>& items, hipStream_t s) {
+template <class LT>
+int Engine<LT>::init_items(const std::vector<It*>& items, hipStream_t s) {
     if (items.empty()) return LINAC_OK;
     Staged st;
     LINAC_TRY(upload(items, s, 0, 0, &st));
     int64_t maxw = 1;
     for (auto* it : items) maxw = std::max<int64_t>(maxw, it->d.N * it->d.M);
-    k_item_init<<<dim3(gs_grid(maxw), (unsigned)items.size()), 256, 0, s>>>(st.desc);
+    k_item_init<LT><<<dim3(gs_grid(maxw), (unsigned)items.size()), 256, 0, s>>>(st.desc);
     LINAC_HIP(hipGetLastError());
     LINAC_HIP(hipStreamSynchronize(s));  // the next upload reuses this staging slot
     return LINAC_OK;
 }
 
-int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof) {
-    std::vector<Item*> act;
+template <class LT>
+int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* prof) {
+    constexpr bool wide = std::is_same<LT, LimbsU64>::value;
+    std::vector<It*> act;
     for (auto* it : items)
         if (!it->finished) act.push_back(it);
     if (act.empty()) return LINAC_OK;
@@ -1062,7 +1312,7 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     const unsigned B = (unsigned)act.size();
     int64_t maxN = 1, maxNX = 1, ninner = 1;
     for (auto* it : act) {
-        BMat& d = it->d;
+        BM& d = it->d;
         d.bcur = (int32_t)std::min<int64_t>(BO, d.M - d.J);
         d.e0 = epoch;
         d.q0 = 0;
@@ -1076,15 +1326,16 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     if (ev_pending) LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
     Staged in;
     LINAC_TRY(upload(act, s, 0, 0, &in));
-    k_panel_load<<<dim3(gs_grid(maxNX, 8192), B), 256, 0, s>>>(in.desc);
-    k_block_state_init<<<B, 1, 0, s>>>(in.desc, first_step ? 1 : 0);
+    k_panel_load<LT><<<dim3(gs_grid(maxNX, 8192), B), 256, 0, s>>>(in.desc);
+    k_block_state_init<LT><<<B, 1, 0, s>>>(in.desc, first_step ? 1 : 0);
     first_step = false;
     const unsigned mu_grid = (unsigned)(MU_BLOCKS + std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)));
     auto group = [&]() -> int {
-        k_inner_factor<<<B, IF_T, 0, s>>>(in.desc, epoch);
-        k_make_u<<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(in.desc);
-        k_gemm_inner<<<(unsigned)(ceil_div(in.inner_tiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(in.desc, in.inner,
-                                                                                             epoch);
+        k_inner_factor<LT><<<B, IF_T, 0, s>>>(in.desc, epoch);
+        k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(in.desc);
+        const unsigned grid = (unsigned)(ceil_div(in.inner_tiles, NUM_XCD) * NUM_XCD);
+        if constexpr (wide) k_gemm_inner64<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch);
+        else k_gemm_inner<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch);
         ++epoch;
         LINAC_HIP(hipGetLastError());
         return LINAC_OK;
@@ -1093,7 +1344,7 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     // one synchronisation per step; extra groups only for rank-deficient panels
     std::vector<ItemStatus> st(B);
     for (int64_t guard = 0;; ++guard) {
-        k_gather_status<<<B, 1, 0, s>>>(in.desc, in.status);
+        k_gather_status<LT><<<B, 1, 0, s>>>(in.desc, in.status);
         LINAC_HIP(hipMemcpyAsync(st.data(), in.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
         LINAC_HIP(hipStreamSynchronize(s));
         int64_t behind = 0;
@@ -1107,7 +1358,7 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     // panel goes back to A and the next inner phase may start), then the rest
     int64_t maxnT = 1;
     for (unsigned m = 0; m < B; ++m) {
-        BMat& d = act[m]->d;
+        BM& d = act[m]->d;
         d.q0 = st[m].q0;
         d.rank_start = st[m].rank - st[m].q0;
         d.T0 = d.J + d.bcur;
@@ -1124,37 +1375,42 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     auto part_work = [&](int part, double* pmacs, double* pbytes) {
         *pmacs = *pbytes = 0;
         for (unsigned m = 0; m < B; ++m) {
-            const BMat& d = act[m]->d;
+            const BM& d = act[m]->d;
             if (d.q0 == 0 || d.nT == 0) continue;
             const int64_t n1 = std::min<int64_t>(BO, d.nT), n = part == 1 ? n1 : d.nT - n1;
             *pmacs += (double)(d.N - d.q0) * d.q0 * n;
-            *pbytes += 8.0 * d.N * n;
+            *pbytes += 2.0 * sizeof(typename LT::T) * d.N * n;
         }
     };
     auto trailing = [&](const Staged& t, int part, int64_t cols) -> int {
         if (t.main_tiles == 0 && t.gq_tiles == 0) return LINAC_OK;
-        k_make_rhs<<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(cols, 64)), 4, B), 256, 0, so>>>(t.desc);
-        if (t.gq_tiles > 0)
-            k_gemm_outer<0><<<(unsigned)(ceil_div(t.gq_tiles, NUM_XCD) * NUM_XCD), G_T, 0, so>>>(t.desc, t.gq);
+        k_make_rhs<LT><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(cols, 64)), 4, B), 256, 0, so>>>(t.desc);
+        if (t.gq_tiles > 0) {
+            const unsigned grid = (unsigned)(ceil_div(t.gq_tiles, NUM_XCD) * NUM_XCD);
+            if constexpr (wide) k_gemm_outer64<0><<<grid, G_T, 0, so>>>(t.desc, t.gq);
+            else k_gemm_outer<0><<<grid, G_T, 0, so>>>(t.desc, t.gq);
+        }
         if (t.main_tiles > 0) {
             double pm, pbts;
             part_work(part, &pm, &pbts);
             if (prof) LINAC_TRY(prof->sample_begin(so, act[0]->d.J, pbts, pm));
-            k_gemm_outer<1><<<(unsigned)(ceil_div(t.main_tiles, NUM_XCD) * NUM_XCD), G_T, 0, so>>>(t.desc, t.main);
+            const unsigned grid = (unsigned)(ceil_div(t.main_tiles, NUM_XCD) * NUM_XCD);
+            if constexpr (wide) k_gemm_outer64<1><<<grid, G_T, 0, so>>>(t.desc, t.main);
+            else k_gemm_outer<1><<<grid, G_T, 0, so>>>(t.desc, t.main);
             if (prof) LINAC_TRY(prof->sample_end(so));
         }
-        k_scatter_q<<<dim3((unsigned)std::min<int64_t>(64, ceil_div(cols, 256)), 256, B), 256, 0, so>>>(t.desc);
+        k_scatter_q<LT><<<dim3((unsigned)std::min<int64_t>(64, ceil_div(cols, 256)), 256, B), 256, 0, so>>>(t.desc);
         LINAC_HIP(hipGetLastError());
         return LINAC_OK;
     };
     if (t1.main_tiles > 0 || t1.gq_tiles > 0) {
-        k_make_lhs<<<dim3(gs_grid(maxN * 256, 16384), B), 256, 0, so>>>(t1.desc, 0, epoch);
-        k_make_lhs<<<dim3(gs_grid(256 * 256), B), 256, 0, so>>>(t1.desc, 1, epoch);
+        k_make_lhs<LT><<<dim3(gs_grid(maxN * 256, 16384), B), 256, 0, so>>>(t1.desc, 0, epoch);
+        k_make_lhs<LT><<<dim3(gs_grid(256 * 256), B), 256, 0, so>>>(t1.desc, 1, epoch);
     }
     LINAC_TRY(trailing(t1, 1, std::min<int64_t>(maxnT, BO)));
     int64_t maxpanel = 1;
     for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
-    k_panel_store<<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, so>>>(t1.desc);
+    k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, so>>>(t1.desc);
     LINAC_HIP(hipGetLastError());
     const bool lookahead = tuning().lookahead != 0;
     if (lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));
@@ -1168,7 +1424,8 @@ int Engine::step(const std::vector<Item*>& items, hipStream_t s, Profiler* prof)
     return LINAC_OK;
 }
 
-static void describe(BMat& d, uint32_t* A, int64_t ld, int64_t N, int64_t M, const FieldU32& f) {
+template <class LT>
+static void describe(BMat<LT>& d, typename LT::T* A, int64_t ld, int64_t N, int64_t M, const typename LT::F& f) {
     d.A = A;
     d.ld = ld;
     d.N = N;
@@ -1179,7 +1436,8 @@ static void describe(BMat& d, uint32_t* A, int64_t ld, int64_t N, int64_t M, con
     d.J = 0;
 }
 
-static void print_timers(const BMat& d, DeviceContext* ctx) {
+template <class LT>
+static void print_timers(const BMat<LT>& d, DeviceContext* ctx) {
     unsigned long long tp[8];
     if (hipMemcpy(tp, d.bs->tphase, sizeof(tp), hipMemcpyDeviceToHost) != hipSuccess || tp[5] == 0) return;
     int rate_khz = 0;
@@ -1192,20 +1450,21 @@ static void print_timers(const BMat& d, DeviceContext* ctx) {
 
 // Single matrix through the engine (a batch of one); workspace from the
 // context's cached scratch, elimination state from `ws`.
-static int blocked_run_u32(uint32_t* A, int64_t ld, int64_t N, int64_t M, const FieldU32& f, hipStream_t s,
-                           const ElimWorkspace& ws, DeviceContext* ctx, Profiler* prof) {
-    const WorkspaceLayout w = workspace_layout(N, M, false);
+template <class LT>
+static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, const typename LT::F& f, hipStream_t s,
+                       const ElimWorkspace& ws, DeviceContext* ctx, Profiler* prof) {
+    const WorkspaceLayout w = workspace_layout<LT>(N, M, false);
     void* p = nullptr;
     LINAC_TRY(ctx->get(SLOT_PANEL_A, w.total, &p));
-    Item it;
+    Item<LT> it;
     describe(it.d, A, ld, N, M, f);
     carve(it.d, (char*)p, w, N, M, false);
     it.d.st = ws.st;
     it.d.piv_rows = ws.piv_rows;
     it.d.piv_cols = ws.piv_cols;
     it.d.used = ws.used;
-    Engine eng;
-    std::vector<Item*> items{&it};
+    Engine<LT> eng;
+    std::vector<Item<LT>*> items{&it};
     // state and row flags were initialised by the caller; epochs here
     k_epoch_init<<<gs_grid(N), 256, 0, s>>>(it.d.epoch_of, N);
     while (!it.finished) LINAC_TRY(eng.step(items, s, prof));
@@ -1221,7 +1480,11 @@ int blocked_dispatch(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F
     if constexpr (std::is_same<F, FieldU32>::value) {
         if (f.p > LIMB_P_MAX) return NOT_APPLICABLE;
         if (!forced && (N < tuning().gf_blocked_min_rows || M < tuning().gf_blocked_min_cols)) return NOT_APPLICABLE;
-        return blocked_run_u32(A, ld, N, M, f, s, ws, ctx, prof);
+        return blocked_run<LimbsU32>(A, ld, N, M, f, s, ws, ctx, prof);
+    } else if constexpr (std::is_same<F, FieldU64>::value) {
+        if (f.p < LIMB64_P_MIN || f.p >= LIMB64_P_MAX) return NOT_APPLICABLE;
+        if (!forced && (N < tuning().gf_blocked_min_rows || M < tuning().gf_blocked_min_cols)) return NOT_APPLICABLE;
+        return blocked_run<LimbsU64>(A, ld, N, M, f, s, ws, ctx, prof);
     } else {
         (void)A; (void)ld; (void)N; (void)M; (void)f; (void)s; (void)ws; (void)ctx; (void)forced; (void)prof;
         return NOT_APPLICABLE;
@@ -1243,22 +1506,25 @@ template int blocked_dispatch<FieldGF2>(uint32_t*, int64_t, int64_t, int64_t, co
 
 struct linac_batch {
     linac::DeviceContext* ctx = nullptr;
-    linac::Engine* eng = nullptr;
-    std::vector<linac::Item*> items;   // not yet taken
-    std::vector<linac::Item*> fresh;   // added since the last step (need init)
+    linac::Engine<linac::LimbsU32>* eng = nullptr;
+    std::vector<linac::Item<linac::LimbsU32>*> items;   // not yet taken
+    std::vector<linac::Item<linac::LimbsU32>*> fresh;   // added since the last step (need init)
     int64_t next_id = 0;
     uint64_t cap = 0, in_use = 0;
 };
 
 namespace linac {
 
+using Item32 = Item<LimbsU32>;
+using BMat32 = BMat<LimbsU32>;
+
 static size_t matrix_bytes(int64_t rows, int64_t cols) { return (size_t)round_up(rows * round_up(cols, 64) * 4, 256); }
 
 static size_t item_bytes(int64_t rows, int64_t cols) {
-    return matrix_bytes(rows, cols) + workspace_layout(rows, cols, true).total;
+    return matrix_bytes(rows, cols) + workspace_layout<LimbsU32>(rows, cols, true).total;
 }
 
-static Item* find_item(linac_batch* b, int64_t id) {
+static Item32* find_item(linac_batch* b, int64_t id) {
     for (auto* it : b->items)
         if (it->id == id) return it;
     return nullptr;
@@ -1282,7 +1548,7 @@ int linac_batch_create(linac_batch** out, uint64_t mem_cap_bytes) {
     LINAC_TRY(current_context(&ctx));
     auto* b = new linac_batch();
     b->ctx = ctx;
-    b->eng = new Engine();
+    b->eng = new Engine<LimbsU32>();
     b->cap = mem_cap_bytes;
     *out = b;
     return LINAC_OK;
@@ -1326,7 +1592,7 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
                     (unsigned long long)b->in_use, (unsigned long long)b->cap);
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
     hipStream_t s = b->ctx->stream;
-    auto* it = new Item();
+    auto* it = new Item32();
     hipError_t e = hipMalloc(&it->arena, need);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -1337,7 +1603,7 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
     char* base = (char*)it->arena;
     if (in_place) {
         describe(it->d, const_cast<uint32_t*>(a), ld, rows, cols, make_field_u32(p));
-        carve(it->d, base, workspace_layout(rows, cols, true), rows, cols, true);
+        carve(it->d, base, workspace_layout<LimbsU32>(rows, cols, true), rows, cols, true);
     } else {
         const int64_t ldp = round_up(cols, 64);
         uint32_t* A = (uint32_t*)base;
@@ -1349,7 +1615,7 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
             return fail(LINAC_ERR_DEVICE, "copy into the batch failed: %s", hipGetErrorString(ce));
         }
         describe(it->d, A, ldp, rows, cols, make_field_u32(p));
-        carve(it->d, base + matrix_bytes(rows, cols), workspace_layout(rows, cols, true), rows, cols, true);
+        carve(it->d, base + matrix_bytes(rows, cols), workspace_layout<LimbsU32>(rows, cols, true), rows, cols, true);
     }
     it->id = b->next_id++;
     b->in_use += need;
@@ -1382,7 +1648,7 @@ int linac_batch_run(linac_batch* b) {
 
 int linac_batch_status(linac_batch* b, int64_t id) {
     if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
-    Item* it = find_item(b, id);
+    Item32* it = find_item(b, id);
     if (!it) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)id);
     return it->finished ? 1 : 0;
 }
@@ -1390,7 +1656,7 @@ int linac_batch_status(linac_batch* b, int64_t id) {
 int linac_batch_take_u32(linac_batch* b, int64_t id, uint32_t* out, int64_t ld_out, uint32_t flags,
                          int64_t* pivots_out, int64_t* rank_out) {
     if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
-    Item* it = find_item(b, id);
+    Item32* it = find_item(b, id);
     if (!it) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)id);
     if (!it->finished) return fail(LINAC_ERR_INVALID, "matrix %lld is not finished", (long long)id);
     if (out && ld_out < it->d.M) return fail(LINAC_ERR_INVALID, "ld_out < cols");
@@ -1422,7 +1688,7 @@ int linac_batch_take_many_u32(linac_batch* b, const int64_t* ids, int64_t count,
     if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
     if (count < 0 || (count > 0 && !ids)) return fail(LINAC_ERR_INVALID, "bad id list");
     if (count == 0) return LINAC_OK;
-    std::vector<Item*> its((size_t)count);
+    std::vector<Item32*> its((size_t)count);
     for (int64_t i = 0; i < count; ++i) {
         its[i] = find_item(b, ids[i]);
         if (!its[i]) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)ids[i]);
@@ -1438,18 +1704,18 @@ int linac_batch_take_many_u32(linac_batch* b, const int64_t* ids, int64_t count,
     // descriptors, ranks and pivot lists in one scratch block: two round trips in all
     int64_t cap_total = 0;
     for (auto* it : its) cap_total += std::max<int64_t>(1, std::min(it->d.N, it->d.M));
-    const size_t desc_b = (size_t)round_up(count * (int64_t)sizeof(BMat), 256);
+    const size_t desc_b = (size_t)round_up(count * (int64_t)sizeof(BMat32), 256);
     const size_t offs_b = (size_t)round_up((count + 1) * 8, 256);
     void* scratch = nullptr;
     LINAC_TRY(b->ctx->get(SLOT_AUX, desc_b + offs_b + 2 * (size_t)round_up(cap_total * 8, 256), &scratch));
-    BMat* d_desc = (BMat*)scratch;
+    BMat32* d_desc = (BMat32*)scratch;
     int64_t* d_offs = (int64_t*)((char*)scratch + desc_b);
     int64_t* d_rows = (int64_t*)((char*)scratch + desc_b + offs_b);
     int64_t* d_cols = d_rows + round_up(cap_total * 8, 256) / 8;
-    std::vector<BMat> hd((size_t)count);
+    std::vector<BMat32> hd((size_t)count);
     for (int64_t i = 0; i < count; ++i) hd[i] = its[i]->d;
-    LINAC_HIP(hipMemcpyAsync(d_desc, hd.data(), count * sizeof(BMat), hipMemcpyHostToDevice, s));
-    k_collect_rank<<<(unsigned)count, 64, 0, s>>>(d_desc, d_offs);
+    LINAC_HIP(hipMemcpyAsync(d_desc, hd.data(), count * sizeof(BMat32), hipMemcpyHostToDevice, s));
+    k_collect_rank<LimbsU32><<<(unsigned)count, 64, 0, s>>>(d_desc, d_offs);
     std::vector<int64_t> rk((size_t)count), offs((size_t)count + 1, 0);
     LINAC_HIP(hipMemcpyAsync(rk.data(), d_offs, count * 8, hipMemcpyDeviceToHost, s));
     LINAC_HIP(hipStreamSynchronize(s));
@@ -1457,13 +1723,13 @@ int linac_batch_take_many_u32(linac_batch* b, const int64_t* ids, int64_t count,
     std::vector<int64_t> prow((size_t)std::max<int64_t>(1, offs[count])), pcol(prow.size());
     if (offs[count] > 0) {
         LINAC_HIP(hipMemcpyAsync(d_offs, offs.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
-        k_collect_pivots<<<(unsigned)count, 256, 0, s>>>(d_desc, d_offs, d_rows, d_cols);
+        k_collect_pivots<LimbsU32><<<(unsigned)count, 256, 0, s>>>(d_desc, d_offs, d_rows, d_cols);
         LINAC_HIP(hipMemcpyAsync(prow.data(), d_rows, offs[count] * 8, hipMemcpyDeviceToHost, s));
         LINAC_HIP(hipMemcpyAsync(pcol.data(), d_cols, offs[count] * 8, hipMemcpyDeviceToHost, s));
         LINAC_HIP(hipStreamSynchronize(s));
     }
     for (int64_t i = 0; i < count; ++i) {
-        Item* it = its[i];
+        Item32* it = its[i];
         std::vector<int64_t> r(prow.begin() + offs[i], prow.begin() + offs[i + 1]);
         std::vector<int64_t> c(pcol.begin() + offs[i], pcol.begin() + offs[i + 1]);
         const bool resorted = sort_pivots_by_column(r, c);

@@ -13,11 +13,16 @@ from oracle import crref, modp
 pytestmark = pytest.mark.gpu
 
 METHODS = ["plain", "blocked"]
-LIMB_P_MAX = 4278124287  # largest prime of the int8-limb GEMM (blocked method)
+LIMB_P_MAX = 4278124287  # largest prime of the 32-bit int8-limb GEMM (blocked method)
+P63 = 9223372036854775783  # largest prime below 2**63 (64-bit blocked method: 2**32 <= p < 2**63)
+P32_UP = 4294967311  # smallest prime above 2**32
 
 
-def blocked_applies(p):
-    return 2 < p <= LIMB_P_MAX
+def blocked_applies(p, mod64=None):
+    """Whether the blocked method serves p with the storage the dtype rules pick
+    (uint64 for p > 2**32 - 1, or when forced)."""
+    wide = p > 2 ** 32 - 1 if mod64 is None else mod64
+    return (2 ** 32 <= p < 2 ** 63) if wide else (2 < p <= LIMB_P_MAX)
 
 
 def _rref(gpu, A, p, method, **kw):
@@ -53,7 +58,7 @@ def test_sympy_fixtures(gpu, case):
 
 
 @pytest.mark.parametrize("p, n", [(2 ** 31 - 1, 1024), (2 ** 31 - 1, 2048), (2 ** 31 - 1, 3072), (2 ** 31 + 11, 3072),
-                                  (2 ** 61 - 1, 2048)])
+                                  (2 ** 61 - 1, 2048), (P63, 1100)])
 @pytest.mark.parametrize("method", METHODS)
 def test_full_rank_gives_identity(gpu, p, n, method):
     """reference tests/test_row_reduce_random_cuda.py:23-49 (seeded here)."""
@@ -67,6 +72,7 @@ def test_full_rank_gives_identity(gpu, p, n, method):
     ((129, 1000), 129, 2 ** 31 + 11, 3), ((400, 400), 17, 2 ** 61 - 1, 4), ((333, 444), 300, 2, 5),
     ((200, 300), 150, 3, 6), ((1000, 1000), 700, 2 ** 31 - 1, 7), ((64, 4096), 64, 2 ** 32 - 5, 8),
     ((300, 700), 200, 4278124237, 9), ((520, 520), 520, 4278124237, 10),  # largest limb prime: sums wrap 32 bits
+    ((300, 420), 60, P63, 11), ((150, 330), 40, P32_UP, 12),  # ends of the 64-bit limb range
 ])
 @pytest.mark.parametrize("method", ["plain", "auto", "blocked"])
 def test_matches_oracle_bit_exact(gpu, shape, rank, p, seed, method):
@@ -123,6 +129,65 @@ def test_u64_planted(gpu):
     R = gpu.cuda_row_reduce(Ab, p)
     assert R.dtype == numpy.uint64
     assert R[:, -1].tolist() == X[:, 0].tolist()
+
+
+def _u64_structured(rows, cols, p, seed):
+    """Uniform rows over GF(p), then every fifth row replaced by the sum of two
+    earlier rows, three zero columns and two duplicated columns: a
+    rank-deficient input with irregular pivots, built without big-integer
+    products."""
+    rs = numpy.random.RandomState(seed)
+    hi = numpy.uint64(p)
+    A = (rs.randint(0, 2 ** 62, size=(rows, cols), dtype=numpy.int64).astype(numpy.uint64) % hi)
+    for r in range(10, rows, 5):
+        i, j = rs.randint(0, r, size=2)
+        A[r] = (A[i] + A[j]) % hi  # both < p < 2**63: the sum fits 64 bits
+    A[:, [3, cols // 2, cols - 2]] = 0
+    A[:, 7] = A[:, 1]
+    A[:, cols - 1] = A[:, cols // 3]
+    return A
+
+
+@pytest.mark.parametrize("p", [2 ** 61 - 1, P63, P32_UP])
+@pytest.mark.parametrize("shape", [(700, 900), (1100, 530)])
+def test_u64_blocked_matches_oracle(gpu, p, shape):
+    """64-bit storage through the blocked method (eight int8 limbs, integer
+    fold) against the C oracle, bit-exact, over several 256-column panels."""
+    A = _u64_structured(shape[0], shape[1], p, seed=shape[0] + p % 1000)
+    Rg, pg, rk = gpu.rref_with_pivots(A, p, method="blocked")
+    Ro, po = crref.rref_modp(A, p)
+    assert pg == po and rk == len(po) < min(shape)
+    assert Rg.dtype == numpy.uint64 and numpy.array_equal(Rg, Ro.astype(numpy.uint64))
+
+
+@pytest.mark.parametrize("p", [2 ** 61 - 1, P63])
+def test_u64_blocked_planted(gpu, p):
+    """[A | A x] over GF(p) with N = 2600 (eleven panels): the solution column
+    comes back exactly, and the blocked and plain kernels agree bit for bit."""
+    n = 2600
+    rs = numpy.random.RandomState(31)
+    A = rs.randint(0, 2 ** 62, size=(n, n), dtype=numpy.int64).astype(numpy.uint64) % numpy.uint64(p)
+    x = [int(v) for v in rs.randint(0, 2 ** 62, size=n, dtype=numpy.int64) % p]
+    b = numpy.array([sum(int(a) * xi for a, xi in zip(row.tolist(), x)) % p for row in A], dtype=numpy.uint64)
+    Ab = numpy.concatenate([A, b[:, None]], axis=1)
+    Rb, pb, rb = gpu.rref_with_pivots(Ab, p, method="blocked")
+    assert rb == n and pb == list(range(n))
+    assert [int(v) for v in Rb[:, -1]] == x
+    assert numpy.array_equal(Rb[:, :n], numpy.eye(n, dtype=numpy.uint64))
+    Rp, pp, rp = gpu.rref_with_pivots(Ab, p, method="plain")
+    assert numpy.array_equal(Rp, Rb)
+
+
+def test_u64_blocked_thresholds(gpu):
+    """64-bit storage of a prime below 2**32 (forced by the caller) is outside
+    the 64-bit limb range: "blocked" refuses it, "auto" takes the plain kernel."""
+    p = 2 ** 31 - 1
+    A = numpy.random.RandomState(3).randint(0, p, size=(40, 50)).astype(numpy.uint64)
+    with pytest.raises(ValueError):
+        gpu.rref_with_pivots(A, p, _mod64=True, method="blocked")
+    R, piv, rank = gpu.rref_with_pivots(A, p, _mod64=True, method="auto")
+    Ro, po = crref.rref_modp(A, p)
+    assert R.dtype == numpy.uint64 and piv == po and numpy.array_equal(R, Ro.astype(numpy.uint64))
 
 
 def test_edge_cases(gpu):
