@@ -96,9 +96,17 @@ struct FieldU64 {
         return a >= p - b ? a - (p - b) : a + b;
     }
     __device__ __forceinline__ uint64_t to_mont(uint64_t a) const { return mont(a, r2); }
-    // (h x - a y) R^{-1} mod p (the fraction-free row update of FieldU32).
+    // (h x - a y) R^{-1} mod p, the fraction-free row update of FieldU32 with
+    // one shared reduction: t = h x + (p - a) y < 2 p^2 < p 2^64 (p < 2^63) is
+    // a valid REDC input, summed in 128 bits.
     __device__ __forceinline__ uint64_t ff_update(uint64_t h, uint64_t x, uint64_t a, uint64_t y) const {
-        return sub(mont(h, x), mont(a, y));
+        const uint64_t b = p - a;
+        const uint64_t l1 = h * x, l2 = b * y;
+        const uint64_t t_lo = l1 + l2;
+        const uint64_t t_hi = __umul64hi(h, x) + __umul64hi(b, y) + (t_lo < l1 ? 1ull : 0ull);
+        const uint64_t m = t_lo * pneg;
+        const uint64_t u = t_hi + __umul64hi(m, p) + (t_lo != 0ull);
+        return u >= p ? u - p : u;
     }
     __device__ __forceinline__ uint64_t inverse(uint64_t h) const {
         const uint64_t e = p - 2ull;

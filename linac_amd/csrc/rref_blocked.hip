@@ -62,17 +62,24 @@ constexpr int KPI = 64;        // padded depth of the inner GEMM
 // and the number of balanced base-256 digits (int8 limb planes) per entry.
 // 32-bit: p <= 4278124287, 4 digits, 16 limb products on 7 diagonals;
 // 64-bit: 2^32 <= p < 2^63, 8 digits, 64 limb products on 15 diagonals.
+// IF_WAVES: minimum waves per SIMD of the inner factorisation, i.e. its
+// register cap (512 / IF_WAVES).  The 64-bit one is held to 64 registers so
+// that its 16 waves fit in the space one GEMM workgroup frees on a CU; the
+// trailing update of the previous panel runs beside it and otherwise refills
+// every freed slot first.
 struct LimbsU32 {
     using T = uint32_t;
     using F = FieldU32;
     using S = int32_t;
     static constexpr int NL = 4;
+    static constexpr int IF_WAVES = 1;
 };
 struct LimbsU64 {
     using T = uint64_t;
     using F = FieldU64;
     using S = int64_t;
     static constexpr int NL = 8;
+    static constexpr int IF_WAVES = 8;
 };
 
 template <class T>
@@ -384,7 +391,7 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
 
 
 template <class LT>
-__global__ __launch_bounds__(IF_T) void k_inner_factor(const BMat<LT>* __restrict__ D, int epoch) {
+__global__ __launch_bounds__(IF_T, LT::IF_WAVES) void k_inner_factor(const BMat<LT>* __restrict__ D, int epoch) {
     const BMat<LT>& d = D[blockIdx.x];
     inner_factor_body<LT>(d.X, d.N, d.used, d.slot_of, d.epoch_of, epoch, d.J, d.f, d.st, d.piv_rows, d.piv_cols, d.bs);
 }
@@ -818,24 +825,30 @@ __global__ __launch_bounds__(G_T, 4) void k_gemm_inner(const BMat<LimbsU32>* __r
 }
 
 // ---------------------------------------------------------------------------
-// 64-bit storage: the same 64 x 64 tiles (4 waves of 32 x 32) with eight limb
-// planes per operand.  A 64-deep K stage is 8 x 8 x 4 = 256 MFMAs per wave,
-// long enough to hide the next stage's global loads behind a single LDS
-// buffer (64 KB: 8 limbs x 64 rows x 64 bytes per operand).  The 15 x 4
-// accumulators take 240 registers, so one workgroup runs per CU.
+// 64-bit storage: eight limb planes per operand and 15 diagonals.  A tile is
+// 64 x TN64 (4 waves of 32 x 16 NI64); a 64-deep K stage is 8 x 8 x 2 NI64
+// MFMAs per wave, long enough to hide the next stage's global loads behind a
+// single LDS buffer (8 limbs x (64 + TN64) rows x 64 bytes).  With NI64 = 1
+// the accumulators take 120 registers, so two workgroups share a CU and one's
+// exact fold (about 150 VALU instructions per entry, mostly 64-bit integer
+// work and int64 <-> double conversions) overlaps the other's MFMAs.
 // ---------------------------------------------------------------------------
 
 constexpr int NL64 = LimbsU64::NL, ND64 = 2 * NL64 - 1;
+constexpr int NI64 = 1;             // 16-column blocks per wave
+constexpr int TN64 = 32 * NI64;     // tile width
+constexpr int BQ64 = TN64 / 8;      // B-stage 16-byte chunks per thread (TN64 rows x 4 x 8 limbs / 256)
+constexpr int GW64 = NI64 == 1 ? 2 : 1;  // workgroups per CU (accumulators: 120 NI64 registers)
 
 // acc[l1 + l2] += A_l1 B_l2 over one staged K slice (all 64 limb pairs).
-__device__ __forceinline__ void mfma_stage64(const int8_t (*sA)[G_M * G_ROW], const int8_t (*sB)[G_N * G_ROW],
-                                             int wm, int wn, int r16, int g, v4i (&acc)[ND64][2][2]) {
-    v4i bf[NL64][2];
+__device__ __forceinline__ void mfma_stage64(const int8_t (*sA)[G_M * G_ROW], const int8_t (*sB)[TN64 * G_ROW],
+                                             int wm, int wn, int r16, int g, v4i (&acc)[ND64][2][NI64]) {
+    v4i bf[NL64][NI64];
 #pragma unroll
     for (int l = 0; l < NL64; ++l)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][lds_at(wn * 32 + ni * 16 + r16, g)]);
+        for (int ni = 0; ni < NI64; ++ni)
+            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][lds_at(wn * 16 * NI64 + ni * 16 + r16, g)]);
 #pragma unroll
     for (int l1 = 0; l1 < NL64; ++l1) {
         v4i af[2];
@@ -847,17 +860,22 @@ __device__ __forceinline__ void mfma_stage64(const int8_t (*sA)[G_M * G_ROW], co
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-                for (int ni = 0; ni < 2; ++ni)
+                for (int ni = 0; ni < NI64; ++ni)
                     acc[l1 + l2][mi][ni] =
                         __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
     }
 }
 
+// B-stage chunks of a thread: chunk q is limb BL0 + q * BLS of row BROW,
+// 16-byte column BQU (the same row and column for every q).
+constexpr int BLS64 = G_T / (TN64 * 4);
+static_assert(G_T % (TN64 * 4) == 0, "whole limbs per B-stage pass");
+
 // Outer update GEMMs for 64-bit storage (MODE as in k_gemm_outer).
 template <int MODE>
-__global__ __launch_bounds__(G_T, 1) void k_gemm_outer64(const BMat<LimbsU64>* __restrict__ D, TileSched ts) {
+__global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>* __restrict__ D, TileSched ts) {
     __shared__ __attribute__((aligned(16))) int8_t sA[NL64][G_M * G_ROW];
-    __shared__ __attribute__((aligned(16))) int8_t sB[NL64][G_N * G_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[NL64][TN64 * G_ROW];
     int mi_;
     int64_t tm, tn;
     if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
@@ -873,55 +891,58 @@ __global__ __launch_bounds__(G_T, 1) void k_gemm_outer64(const BMat<LimbsU64>* _
     const int Kpad = d.Kq;
     const uint64_t p = d.f.p;
     const double pinv = d.pinv;
-    const int64_t m_base = tm * G_M, n_base = tn * G_N;
+    const int64_t m_base = tm * G_M, n_base = tn * TN64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
     const v4i zero = {0, 0, 0, 0};
 
-    uint64_t cpre[2][2][4];
+    uint64_t cpre[2][NI64][4];
     auto cload = [&]() {
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-            for (int ni = 0; ni < 2; ++ni)
+            for (int ni = 0; ni < NI64; ++ni)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-                    const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+                    const int64_t col = n_base + wn * 16 * NI64 + ni * 16 + r16;
                     cpre[mi][ni][i] = (MODE == 1 && row < Mrows && col < Ncols) ? C[row * ldc + col] : 0ull;
                 }
     };
-    v4i ra[NL64], rb[NL64];
+    // A stage: limb q of row tid/4, quarter tid%4; B stage: BQ64 chunks
+    v4i ra[NL64], rb[BQ64];
     const int srow = tid >> 2, squ = tid & 3;
-    const bool a_ok = m_base + srow < Mrows, b_ok = n_base + srow < Ncols;
+    const bool a_ok = m_base + srow < Mrows;
     const int8_t* pa = L + (a_ok ? (m_base + srow) * Kpad : 0) + squ * 16;
-    const int8_t* pb = B + (b_ok ? (n_base + srow) * Kpad : 0) + squ * 16;
     const int soff = lds_at(srow, squ);
+    const int bl0 = tid / (TN64 * 4), brow = (tid % (TN64 * 4)) >> 2, bqu = tid & 3;
+    const bool b_ok = n_base + brow < Ncols;
+    const int8_t* pb = B + bl0 * planeB + (b_ok ? (n_base + brow) * Kpad : 0) + bqu * 16;
+    const int boff = lds_at(brow, bqu);
     auto gload = [&](int kc) {
 #pragma unroll
-        for (int q = 0; q < NL64; ++q) {
-            ra[q] = a_ok ? *reinterpret_cast<const v4i*>(pa + q * planeL + kc) : zero;
-            rb[q] = b_ok ? *reinterpret_cast<const v4i*>(pb + q * planeB + kc) : zero;
-        }
+        for (int q = 0; q < NL64; ++q) ra[q] = a_ok ? *reinterpret_cast<const v4i*>(pa + q * planeL + kc) : zero;
+#pragma unroll
+        for (int q = 0; q < BQ64; ++q)
+            rb[q] = b_ok ? *reinterpret_cast<const v4i*>(pb + q * BLS64 * planeB + kc) : zero;
     };
 
-    v4i acc[ND64][2][2];
+    v4i acc[ND64][2][NI64];
 #pragma unroll
     for (int s = 0; s < ND64; ++s)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
+            for (int j = 0; j < NI64; ++j) acc[s][i][j] = zero;
 
     const int nk = Kpad / G_K;
     gload(0);
     for (int kk = 0; kk < nk; ++kk) {
 #pragma unroll
-        for (int q = 0; q < NL64; ++q) {
-            *reinterpret_cast<v4i*>(&sA[q][soff]) = ra[q];
-            *reinterpret_cast<v4i*>(&sB[q][soff]) = rb[q];
-        }
+        for (int q = 0; q < NL64; ++q) *reinterpret_cast<v4i*>(&sA[q][soff]) = ra[q];
+#pragma unroll
+        for (int q = 0; q < BQ64; ++q) *reinterpret_cast<v4i*>(&sB[bl0 + q * BLS64][boff]) = rb[q];
         __syncthreads();
         if (kk + 1 < nk) gload((kk + 1) * G_K);
         else cload();
@@ -931,11 +952,11 @@ __global__ __launch_bounds__(G_T, 1) void k_gemm_outer64(const BMat<LimbsU64>* _
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
+        for (int ni = 0; ni < NI64; ++ni)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-                const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+                const int64_t col = n_base + wn * 16 * NI64 + ni * 16 + r16;
                 if (row < Mrows && col < Ncols) {
                     int32_t sv[ND64];
 #pragma unroll
@@ -945,11 +966,13 @@ __global__ __launch_bounds__(G_T, 1) void k_gemm_outer64(const BMat<LimbsU64>* _
             }
 }
 
-// Inner-step GEMM for 64-bit storage (as k_gemm_inner).
-__global__ __launch_bounds__(G_T, 1) void k_gemm_inner64(const BMat<LimbsU64>* __restrict__ D, TileSched ts,
+// Inner-step GEMM for 64-bit storage (as k_gemm_inner, 64 x TN64 tiles).  The
+// panel entries, row epochs and slots the epilogue needs are fetched before
+// the MFMAs, so their latency hides behind the matrix cores.
+__global__ __launch_bounds__(G_T, GW64) void k_gemm_inner64(const BMat<LimbsU64>* __restrict__ D, TileSched ts,
                                                          int epoch) {
     __shared__ __attribute__((aligned(16))) int8_t sA[NL64][G_M * G_ROW];
-    __shared__ __attribute__((aligned(16))) int8_t sB[NL64][G_N * G_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[NL64][TN64 * G_ROW];
     int mi_;
     int64_t tm, tn;
     if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
@@ -957,34 +980,56 @@ __global__ __launch_bounds__(G_T, 1) void k_gemm_inner64(const BMat<LimbsU64>* _
     const BlockState<uint64_t>* bs = d.bs;
     const int k = bs->k;
     if (k == 0) return;
-    const int64_t m_base = tm * G_M, n_base = tn * G_N;
+    const int64_t m_base = tm * G_M, n_base = tn * TN64;
     const int c0 = bs->c0_cur;
-    if (n_base + G_N <= c0 || n_base >= BO + bs->q0) return;
+    if (n_base + TN64 <= c0 || n_base >= BO + bs->q0) return;
     uint64_t* X = d.X;
     const int64_t N = d.N;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
 #pragma unroll
-    for (int q = 0; q < NL64; ++q) {
-        const int c = tid + q * G_T;
-        const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
-        *reinterpret_cast<v4i*>(&sB[l][lds_at(col, qu)]) =
-            *reinterpret_cast<const v4i*>(d.planesUi + l * (int64_t)XLD * KPI + (n_base + col) * KPI + qu * 16);
-        const int64_t row = m_base + col;
+    for (int q = 0; q < NL64; ++q) {  // A: limb q, row tid / 4
+        const int row_l = tid >> 2, qu = tid & 3;
+        const int64_t row = m_base + row_l;
         v4i w = {0, 0, 0, 0};
-        if (row < N) w = *reinterpret_cast<const v4i*>(d.planesAi + l * d.strideAi + row * KPI + qu * 16);
-        *reinterpret_cast<v4i*>(&sA[l][lds_at(col, qu)]) = w;
+        if (row < N) w = *reinterpret_cast<const v4i*>(d.planesAi + q * d.strideAi + row * KPI + qu * 16);
+        *reinterpret_cast<v4i*>(&sA[q][lds_at(row_l, qu)]) = w;
     }
+    {
+        const int bl0 = tid / (TN64 * 4), bcol = (tid % (TN64 * 4)) >> 2, bqu = tid & 3;
+#pragma unroll
+        for (int q = 0; q < BQ64; ++q) {
+            const int l = bl0 + q * BLS64;
+            *reinterpret_cast<v4i*>(&sB[l][lds_at(bcol, bqu)]) = *reinterpret_cast<const v4i*>(
+                d.planesUi + l * (int64_t)XLD * KPI + (n_base + bcol) * KPI + bqu * 16);
+        }
+    }
+    // epilogue operands, in flight during the MFMAs
+    uint64_t cv[2][NI64][4];
+    int slot[2][4];
+    bool prow[2][4];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
+            const bool live = row < N;
+            prow[mi][i] = live && d.epoch_of[row] == epoch;
+            slot[mi][i] = live ? d.slot_of[row] : 0;
+#pragma unroll
+            for (int ni = 0; ni < NI64; ++ni)
+                cv[mi][ni][i] = live ? X[row * XLD + n_base + wn * 16 * NI64 + ni * 16 + r16] : 0ull;
+        }
     __syncthreads();
     const v4i zero = {0, 0, 0, 0};
-    v4i acc[ND64][2][2];
+    v4i acc[ND64][2][NI64];
 #pragma unroll
     for (int s = 0; s < ND64; ++s)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
+            for (int j = 0; j < NI64; ++j) acc[s][i][j] = zero;
     mfma_stage64(sA, sB, wm, wn, r16, g, acc);
     const uint64_t p = d.f.p;
     const double pinv = d.pinv;
@@ -994,19 +1039,17 @@ __global__ __launch_bounds__(G_T, 1) void k_gemm_inner64(const BMat<LimbsU64>* _
         for (int i = 0; i < 4; ++i) {
             const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
             if (row >= N) continue;
-            const bool pivot_row = d.epoch_of[row] == epoch;
-            const int slot = pivot_row ? d.slot_of[row] : 0;
 #pragma unroll
-            for (int ni = 0; ni < 2; ++ni) {
-                const int64_t col = n_base + wn * 32 + ni * 16 + r16;
+            for (int ni = 0; ni < NI64; ++ni) {
+                const int64_t col = n_base + wn * 16 * NI64 + ni * 16 + r16;
                 uint64_t* cp = X + row * XLD + col;
-                if (pivot_row) {
-                    *cp = d.U[slot * XLD + col];
+                if (prow[mi][i]) {
+                    *cp = d.U[slot[mi][i] * XLD + col];
                 } else {
                     int32_t sv[ND64];
 #pragma unroll
                     for (int q = 0; q < ND64; ++q) sv[q] = acc[q][mi][ni][i];
-                    *cp = fold_sub64(*cp, sv, p, pinv);
+                    *cp = fold_sub64(cv[mi][ni][i], sv, p, pinv);
                 }
             }
         }
@@ -1254,6 +1297,7 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
         }
         hd[m] = d;
     }
+    constexpr int64_t TN = std::is_same<LT, LimbsU64>::value ? TN64 : G_N;  // GEMM tile width
     int64_t* pa = hs;
     int64_t* ma = pa + (B + 1);
     int64_t* pb = ma + B;
@@ -1264,12 +1308,12 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
     for (size_t m = 0; m < B; ++m) {
         const BM& d = hd[m];
         ma[m] = ceil_div(d.N, G_M);
-        pa[m + 1] = pa[m] + (part == 0 ? ma[m] * (XLD / G_N) : 0);
+        pa[m + 1] = pa[m] + (part == 0 ? ma[m] * (XLD / TN) : 0);
         const bool live = part != 0 && d.q0 > 0 && d.nT > 0;
         mb[m] = std::max<int64_t>(1, ceil_div(d.q0, G_M));
-        pb[m + 1] = pb[m] + (live ? ceil_div(d.q0, G_M) * ceil_div(d.nT, G_N) : 0);
+        pb[m + 1] = pb[m] + (live ? ceil_div(d.q0, G_M) * ceil_div(d.nT, TN) : 0);
         mc[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
-        pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, G_N) : 0);
+        pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, TN) : 0);
     }
     out->desc = (BM*)dv;
     const int grp = (int)tuning().gemm_group;
