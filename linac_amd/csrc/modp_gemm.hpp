@@ -33,6 +33,16 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 // Largest prime handled by the limb GEMM (centred values fit 4 balanced digits).
 constexpr uint64_t LIMB_P_MAX = 4278124287ull;
 
+// Loads through the global address space.  Pointers read from a descriptor
+// are generic, and generic (flat) loads count against lgkmcnt as well as
+// vmcnt, so the wait for an LDS fragment would also wait for every global
+// load in flight (the next K stage's prefetch); global loads count in vmcnt
+// only.
+template <class V>
+__device__ __forceinline__ V gld(const void* p) {
+    return *(const __attribute__((address_space(1))) V*)p;
+}
+
 __device__ __forceinline__ int32_t centred(uint32_t v, uint32_t p) {
     return v > (p - 1) / 2 ? (int32_t)((int64_t)v - (int64_t)p) : (int32_t)v;
 }

@@ -638,6 +638,10 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
     const int r16 = lane & 15, g = lane >> 4;
     const v4i zero = {0, 0, 0, 0};
 
+    // Loads are unconditional: rows and columns beyond the matrix read a
+    // clamped (valid) address, and the accumulators they feed are never
+    // stored.  (A select or branch around a load makes the compiler wait for
+    // it where the value is merged, not where it is used.)
     // C values of this lane, fetched while the last K stage runs on the matrix
     // cores (the stage registers are free by then)
     uint32_t cpre[2][2][4];
@@ -648,24 +652,23 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
             for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-                    const int64_t col = n_base + wn * 32 + ni * 16 + r16;
-                    cpre[mi][ni][i] = (MODE == 1 && row < Mrows && col < Ncols) ? C[row * ldc + col] : 0u;
+                    const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, Mrows - 1);
+                    const int64_t col = min(n_base + wn * 32 + ni * 16 + r16, Ncols - 1);
+                    cpre[mi][ni][i] = MODE == 1 ? gld<uint32_t>(C + row * ldc + col) : 0u;
                 }
     };
     // stage copy: A and B = 4 limbs x 64 rows x 4 sixteen-byte quarters (4
     // each per thread); the row/limb part of each address is fixed per tile
     v4i ra[4], rb[4];
     const int srow = tid >> 2, squ = tid & 3;  // the same for the 4 limbs (q = limb)
-    const bool a_ok = m_base + srow < Mrows, b_ok = n_base + srow < Ncols;
-    const int8_t* pa = L + (a_ok ? (m_base + srow) * Kpad : 0) + squ * 16;
-    const int8_t* pb = B + (b_ok ? (n_base + srow) * Kpad : 0) + squ * 16;
+    const int8_t* pa = L + min(m_base + srow, Mrows - 1) * Kpad + squ * 16;
+    const int8_t* pb = B + min(n_base + srow, Ncols - 1) * Kpad + squ * 16;
     const int soff = lds_at(srow, squ);
     auto gload = [&](int kc) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            ra[q] = a_ok ? *reinterpret_cast<const v4i*>(pa + q * planeL + kc) : zero;
-            rb[q] = b_ok ? *reinterpret_cast<const v4i*>(pb + q * planeB + kc) : zero;
+            ra[q] = gld<v4i>(pa + q * planeL + kc);
+            rb[q] = gld<v4i>(pb + q * planeB + kc);
         }
     };
     auto lstore = [&](int buf) {
@@ -897,35 +900,19 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
     const int r16 = lane & 15, g = lane >> 4;
     const v4i zero = {0, 0, 0, 0};
 
-    uint64_t cpre[2][NI64][4];
-    auto cload = [&]() {
-#pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-            for (int ni = 0; ni < NI64; ++ni)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-                    const int64_t col = n_base + wn * 16 * NI64 + ni * 16 + r16;
-                    cpre[mi][ni][i] = (MODE == 1 && row < Mrows && col < Ncols) ? C[row * ldc + col] : 0ull;
-                }
-    };
     // A stage: limb q of row tid/4, quarter tid%4; B stage: BQ64 chunks
     v4i ra[NL64], rb[BQ64];
     const int srow = tid >> 2, squ = tid & 3;
-    const bool a_ok = m_base + srow < Mrows;
-    const int8_t* pa = L + (a_ok ? (m_base + srow) * Kpad : 0) + squ * 16;
+    const int8_t* pa = L + min(m_base + srow, Mrows - 1) * Kpad + squ * 16;
     const int soff = lds_at(srow, squ);
     const int bl0 = tid / (TN64 * 4), brow = (tid % (TN64 * 4)) >> 2, bqu = tid & 3;
-    const bool b_ok = n_base + brow < Ncols;
-    const int8_t* pb = B + bl0 * planeB + (b_ok ? (n_base + brow) * Kpad : 0) + bqu * 16;
+    const int8_t* pb = B + bl0 * planeB + min(n_base + brow, Ncols - 1) * Kpad + bqu * 16;
     const int boff = lds_at(brow, bqu);
     auto gload = [&](int kc) {
 #pragma unroll
-        for (int q = 0; q < NL64; ++q) ra[q] = a_ok ? *reinterpret_cast<const v4i*>(pa + q * planeL + kc) : zero;
+        for (int q = 0; q < NL64; ++q) ra[q] = gld<v4i>(pa + q * planeL + kc);
 #pragma unroll
-        for (int q = 0; q < BQ64; ++q)
-            rb[q] = b_ok ? *reinterpret_cast<const v4i*>(pb + q * BLS64 * planeB + kc) : zero;
+        for (int q = 0; q < BQ64; ++q) rb[q] = gld<v4i>(pb + q * BLS64 * planeB + kc);
     };
 
     v4i acc[ND64][2][NI64];
@@ -945,10 +932,24 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
         for (int q = 0; q < BQ64; ++q) *reinterpret_cast<v4i*>(&sB[bl0 + q * BLS64][boff]) = rb[q];
         __syncthreads();
         if (kk + 1 < nk) gload((kk + 1) * G_K);
-        else cload();
         mfma_stage64(sA, sB, wm, wn, r16, g, acc);
         __syncthreads();  // the next stage overwrites the buffer
     }
+    // C values: issued together after the last stage (prefetching them into
+    // the stage registers would keep 16 more addresses live through the loop,
+    // beyond the register budget of two workgroups per CU); the other
+    // workgroup's MFMAs cover the latency
+    uint64_t cv[2][NI64][4];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI64; ++ni)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {  // unconditional loads, as in k_gemm_outer
+                const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, Mrows - 1);
+                const int64_t col = min(n_base + wn * 16 * NI64 + ni * 16 + r16, Ncols - 1);
+                cv[mi][ni][i] = MODE == 1 ? gld<uint64_t>(C + row * ldc + col) : 0ull;
+            }
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -961,7 +962,7 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
                     int32_t sv[ND64];
 #pragma unroll
                     for (int q = 0; q < ND64; ++q) sv[q] = acc[q][mi][ni][i];
-                    C[row * ldc + col] = fold_sub64(cpre[mi][ni][i], sv, p, pinv);
+                    C[row * ldc + col] = fold_sub64(cv[mi][ni][i], sv, p, pinv);
                 }
             }
 }
@@ -991,18 +992,16 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_inner64(const BMat<LimbsU64>
 #pragma unroll
     for (int q = 0; q < NL64; ++q) {  // A: limb q, row tid / 4
         const int row_l = tid >> 2, qu = tid & 3;
-        const int64_t row = m_base + row_l;
-        v4i w = {0, 0, 0, 0};
-        if (row < N) w = *reinterpret_cast<const v4i*>(d.planesAi + q * d.strideAi + row * KPI + qu * 16);
-        *reinterpret_cast<v4i*>(&sA[q][lds_at(row_l, qu)]) = w;
+        const int64_t row = min(m_base + row_l, N - 1);  // rows beyond N: clamped, results not stored
+        *reinterpret_cast<v4i*>(&sA[q][lds_at(row_l, qu)]) = gld<v4i>(d.planesAi + q * d.strideAi + row * KPI + qu * 16);
     }
     {
         const int bl0 = tid / (TN64 * 4), bcol = (tid % (TN64 * 4)) >> 2, bqu = tid & 3;
 #pragma unroll
         for (int q = 0; q < BQ64; ++q) {
             const int l = bl0 + q * BLS64;
-            *reinterpret_cast<v4i*>(&sB[l][lds_at(bcol, bqu)]) = *reinterpret_cast<const v4i*>(
-                d.planesUi + l * (int64_t)XLD * KPI + (n_base + bcol) * KPI + bqu * 16);
+            *reinterpret_cast<v4i*>(&sB[l][lds_at(bcol, bqu)]) =
+                gld<v4i>(d.planesUi + l * (int64_t)XLD * KPI + (n_base + bcol) * KPI + bqu * 16);
         }
     }
     // epilogue operands, in flight during the MFMAs
@@ -1013,13 +1012,12 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_inner64(const BMat<LimbsU64>
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-            const bool live = row < N;
-            prow[mi][i] = live && d.epoch_of[row] == epoch;
-            slot[mi][i] = live ? d.slot_of[row] : 0;
+            const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, N - 1);
+            prow[mi][i] = gld<int32_t>(d.epoch_of + row) == epoch;
+            slot[mi][i] = gld<int32_t>(d.slot_of + row);
 #pragma unroll
             for (int ni = 0; ni < NI64; ++ni)
-                cv[mi][ni][i] = live ? X[row * XLD + n_base + wn * 16 * NI64 + ni * 16 + r16] : 0ull;
+                cv[mi][ni][i] = gld<uint64_t>(X + row * XLD + n_base + wn * 16 * NI64 + ni * 16 + r16);
         }
     __syncthreads();
     const v4i zero = {0, 0, 0, 0};
