@@ -12,11 +12,11 @@
 // With D_{l1,l2} = sum_k L_l1[m, k] * B_l2[k, n] (one MFMA chain per limb pair),
 //   sum_k L[m,k] B[k,n] = sum_{s=0..6} 256^s S_s,   S_s = sum_{l1+l2=s} D_{l1,l2}.
 // Exactness of the int32 accumulators: |digit products| <= 2^14, so
-// |S_s| <= 4 * K * 2^14 <= 2^24 for K <= 256 (2^31 would allow K < 2^15).
+// |S_s| <= 4 * K * 2^14 <= 2^25 for K <= 512 (2^31 would allow K < 2^15).
 //
 // The seven S_s are folded modulo p in double precision, where every value is
 // an integer below 2^53 and therefore exact:
-//   P0 = S0 + 256 S1, P1 = S2 + 256 S3, P2 = S4 + 256 S5, P3 = S6  (|P| < 2^33)
+//   P0 = S0 + 256 S1, P1 = S2 + 256 S3, P2 = S4 + 256 S5, P3 = S6  (|P| < 2^34)
 //   r = P3;  r = red(r * 2^16 + P2);  r = red(r * 2^16 + P1);  r = r * 2^16 + P0
 // with red(x) = x - rint(x / p) * p (an FMA, exact because the result is a
 // small integer); |r| stays below ~1.5 p < 2^32, so r * 2^16 + P < 2^49.
@@ -103,9 +103,9 @@ __device__ __forceinline__ uint32_t fold_sub_k32(uint32_t cval, const int32_t s[
 // 64-bit storage: 2^32 < p < 2^63.  Centred values (|c| < 2^62) take eight
 // balanced base-256 digits (the top one within [-65, 65]), so a modular
 // product needs 64 limb products whose sums fall on 15 diagonals
-//   S_s = sum_{l1+l2=s} D_{l1,l2},  |S_s| <= 8 * K * 2^14 = 2^25  (K <= 256).
+//   S_s = sum_{l1+l2=s} D_{l1,l2},  |S_s| <= 8 * K * 2^14 = 2^26  (K <= 512).
 // The fold groups them in base-2^24 triples
-//   T_i = S_3i + 2^8 S_3i+1 + 2^16 S_3i+2   (|T_i| < 2^42, exact in int64 and double)
+//   T_i = S_3i + 2^8 S_3i+1 + 2^16 S_3i+2   (|T_i| < 2^43, exact in int64 and double)
 // and runs Horner r <- r 2^24 + T_i with a reduction at every step: the
 // quotient q = rint((r 2^24 + T_i) / p) comes from double precision (|q| <
 // 2^24 once |r| <= p/2, since |r| stays within (1/2 + 2^-27) p), and the
@@ -138,7 +138,7 @@ __device__ __forceinline__ uint64_t fold_sub64(uint64_t cval, const int32_t s[15
     for (int i = 0; i < 5; ++i)
         T[i] = (int64_t)s[3 * i] + (int64_t)s[3 * i + 1] * 256 + (int64_t)s[3 * i + 2] * 65536;
     int64_t r = T[4];
-    r -= (int64_t)(int32_t)rint((double)r * pinv) * (int64_t)p;  // |T_4| / p < 2^10
+    r -= (int64_t)(int32_t)rint((double)r * pinv) * (int64_t)p;  // |T_4| / p < 2^11
 #pragma unroll
     for (int i = 3; i >= 0; --i) {
         const double xd = fma((double)r, 16777216.0, (double)T[i]);

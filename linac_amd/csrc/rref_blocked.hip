@@ -111,14 +111,16 @@ struct BMat {
     int32_t pad0;
     double pd, pinv;
     T* X;               // [N][XLD] panel buffer [P | Z]
-    int8_t* planesL;    // [NL][N][256]
-    int8_t* planesY;    // [NL][256][256]
-    int8_t* planesB;    // [NL][max(M, XLD)][256]
+    int8_t* planesL;    // [NL][N][2 BO]  left operand of the trailing update, row stride Kq
+    int8_t* planesW;    // [NL][BO][BO]   right operand of the composite transformation, row stride Kq - KqA
+    int8_t* planesB;    // [NL][max(M, XLD)][2 BO] right operand of the trailing update, row stride Kq
     int8_t* planesAi;   // [NL][N][KPI]   left operand of the inner GEMM
     int8_t* planesUi;   // [NL][XLD][KPI] right operand of the inner GEMM
     T* U;               // [CI][XLD]
-    T* Gq;              // [256][ldg]
-    int64_t strideL, strideB, strideAi, ldg;
+    T* W;               // [N][BO]  transformation of the deferred panel (ld BO)
+    int8_t* planesL1;   // [NL][N][BO]  operands of a deferring panel's own update (the
+    int8_t* planesB1;   // [NL][BO][BO]  composite operands stay live for its second half)
+    int64_t strideL, strideB, strideAi, strideL1;
     BlockState<T>* bs;
     ElimState* st;
     int64_t* piv_rows;
@@ -130,13 +132,17 @@ struct BMat {
     int64_t J;          // first column of the current outer panel
     int32_t bcur;       // its width
     int32_t q0;         // pivots it produced (outer update)
-    int32_t Kq;         // q0 rounded up to 64 (GEMM depth)
+    int32_t kA;         // pivots of the deferred previous panel (0: none)
+    int32_t KqA;        // kA rounded up to 64: depth of the deferred part of the trailing update
+    int32_t Kq;         // KqA + q0 rounded up to 64 (GEMM depth, row stride of planesL / planesB)
+    int32_t save_w;     // this panel's update beyond the next panel is deferred: keep W
     int32_t e0;         // first epoch of this outer panel
-    int64_t rank_start; // index of its first pivot in piv_rows
+    int32_t pad1;
+    int64_t rank_start; // index of its first pivot in piv_rows (the deferred panel's kA precede it)
     int64_t T0, nT;     // trailing columns [T0, T0 + nT)
 };
 
-constexpr int64_t STRIDE_Y = 256 * 256;
+constexpr int64_t STRIDE_W = (int64_t)BO * BO;
 
 // ---------------------------------------------------------------------------
 // panel buffer management
@@ -471,61 +477,105 @@ __global__ __launch_bounds__(256) void k_make_u(const BMat<LT>* __restrict__ D) 
 // outer update operands
 // ---------------------------------------------------------------------------
 
-// Limb planes of the left operands of the outer update (blockIdx.y = matrix):
-//   mode 0: planesL[l][r][q] = digit l of -Z[r, q] for every row r, zero for the
-//           panel's pivot rows (their epochs lie in [e0, e1));
-//   mode 1: planesY[l][i][q] = digit l of -Z[Q[i], q] for the panel's pivot rows.
-// q < q0 carries data, q in [q0, Kq) is zero padding.
+// The trailing update in the form P = I + W S: S selects the panel's pivot
+// rows Q and W = Z - E, where E has a 1 at (Q[i], i), so that one product
+//   A[:, T] <- A[:, T] + W A[Q, T]
+// also gives the pivot rows their new values Z[Q] A[Q, T].  Two consecutive
+// panels compose into P' P = I + [W + W' W[Q'] | W'] [S; S'], so the update of
+// the columns beyond the next panel is deferred by one panel and then done as
+// one GEMM of depth |Q| + |Q'| <= 512 (half the folds and C traffic per MAC).
+// K layout of both operands: [deferred part, KqA = kA rounded up to 64 |
+// current panel, q0 rounded up to 64], zero padded.
+
+// slot_of[Q[i]] = i for the current panel's pivot rows (blockIdx.y = matrix),
+// which identifies the 1 of E in each pivot row.
 template <class LT>
-__global__ __launch_bounds__(256) void k_make_lhs(const BMat<LT>* __restrict__ D, int mode, int e1) {
+__global__ __launch_bounds__(256) void k_mark_slots(const BMat<LT>* __restrict__ D) {
+    const BMat<LT>& d = D[blockIdx.y];
+    const int64_t* Q = d.piv_rows + d.rank_start;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.q0; i += gridDim.x * blockDim.x) d.slot_of[Q[i]] = i;
+}
+
+// Limb planes of -W, the left operand (blockIdx.y = matrix):
+//   src 0: the current panel, W[m, t] = Z[m, t] - [m = Q[t]] (Z = X[:, BO + t], t < q0),
+//          written at K offset KqA; kept in d.W when the panel's update is deferred;
+//   src 1: the deferred panel's composite transformation d.W[:, t] (t < kA), at K offset 0.
+template <class LT>
+__global__ __launch_bounds__(256) void k_make_lhs(const BMat<LT>* __restrict__ D, int src, int e1) {
+    using T = typename LT::T;
     constexpr int NL = LT::NL;
     const BMat<LT>& d = D[blockIdx.y];
-    if (d.q0 == 0 || d.nT == 0) return;
-    const int64_t rows = mode == 0 ? d.N : d.q0;
-    const int Kpad = d.Kq;
-    int8_t* planes = mode == 0 ? d.planesL : d.planesY;
-    const int64_t stride = mode == 0 ? d.strideL : STRIDE_Y;
-    const int64_t* Q = d.piv_rows + d.rank_start;
-    const int64_t total = rows * Kpad;
+    if (d.nT == 0 && !(src == 0 && d.save_w)) return;
+    const int width = src == 0 ? d.q0 : d.kA;
+    const int kw = src == 0 ? d.Kq - d.KqA : d.KqA;
+    const int koff = src == 0 ? d.KqA : 0;
+    if (width == 0) return;
+    const bool save = src == 0 && d.save_w;
+    const int64_t total = d.N * kw;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t m = x / Kpad;
-        const int t = (int)(x - m * Kpad);
-        const int64_t r = mode == 0 ? m : Q[m];
+        const int64_t m = x / kw;
+        const int t = (int)(x - m * kw);
         int8_t dg[NL] = {};
-        const bool zero_row = mode == 0 && d.epoch_of[r] >= d.e0 && d.epoch_of[r] < e1;
-        if (t < d.q0 && !zero_row) balanced_digits(-centred(d.X[r * XLD + BO + t], d.f.p), dg);
+        if (t < width) {
+            T w;
+            if (src == 0) {
+                w = d.X[m * XLD + BO + t];
+                if (d.epoch_of[m] >= d.e0 && d.epoch_of[m] < e1 && d.slot_of[m] == t) w = d.f.sub(w, (T)1);
+                if (save) d.W[m * BO + t] = w;
+            } else {
+                w = d.W[m * BO + t];
+            }
+            balanced_digits(-centred(w, d.f.p), dg);
+        }
 #pragma unroll
-        for (int l = 0; l < NL; ++l) planes[l * stride + m * Kpad + t] = dg[l];
+        for (int l = 0; l < NL; ++l) d.planesL[l * d.strideL + m * d.Kq + koff + t] = dg[l];
     }
 }
 
-// Limb planes of A[Q, T], transposed: planesB[l][n][t] = digit l of
-// A[Q[t], T0 + n] for t < q0, zero for t in [q0, Kq).  64 x 64 tiles through
-// LDS so that both the row reads and the plane writes are coalesced.
-// blockIdx.x strides over column tiles, blockIdx.y = t-tile, blockIdx.z = matrix.
+// Limb planes of the right operand, transposed (blockIdx.x strides over column
+// tiles, blockIdx.y = 64-deep K tile, blockIdx.z = matrix), 64 x 64 tiles
+// through LDS so that both the row reads and the plane writes are coalesced.
+//   job 0: planesB[l][n][t] = digit l of A[row(t), T0 + n], row(t) = the
+//          deferred panel's pivot Q_A[t] (t < kA) or the current panel's
+//          Q[t - KqA] (t - KqA < q0); zero elsewhere.  Row stride Kq.
+//   job 1: planesW[l][n][t] = digit l of W[Q[t], n] (n < kA, t < q0): the
+//          factor W[Q'] of the composite transformation.  Row stride Kq - KqA.
 template <class LT>
-__global__ __launch_bounds__(256) void k_make_rhs(const BMat<LT>* __restrict__ D) {
+__global__ __launch_bounds__(256) void k_make_rhs(const BMat<LT>* __restrict__ D, int job) {
     using S = typename LT::S;
     constexpr int NL = LT::NL;
     __shared__ S tile[64][65];
     const BMat<LT>& d = D[blockIdx.z];
-    if (d.q0 == 0 || d.nT == 0) return;
+    const int64_t ncols = job == 0 ? d.nT : (d.q0 > 0 ? d.kA : 0);
+    const int depth = job == 0 ? d.Kq : d.Kq - d.KqA;
+    if (ncols == 0) return;
     const int tb = blockIdx.y * 64;
-    if (tb >= d.Kq) return;
+    if (tb >= depth) return;
     const int64_t* Q = d.piv_rows + d.rank_start;
-    for (int64_t nb = (int64_t)blockIdx.x * 64; nb < d.nT; nb += (int64_t)gridDim.x * 64) {
+    const int64_t* QA = Q - d.kA;
+    int8_t* planes = job == 0 ? d.planesB : d.planesW;
+    const int64_t pstride = job == 0 ? d.strideB : STRIDE_W;
+    for (int64_t nb = (int64_t)blockIdx.x * 64; nb < ncols; nb += (int64_t)gridDim.x * 64) {
         for (int x = threadIdx.x; x < 64 * 64; x += 256) {
             const int tt = x >> 6, nn = x & 63;
             const int t = tb + tt;
             const int64_t n = nb + nn;
             S v = 0;
-            if (t < d.q0 && n < d.nT) v = centred(d.A[Q[t] * d.ld + d.T0 + n], d.f.p);
+            if (n < ncols) {
+                if (job == 1) {
+                    if (t < d.q0) v = centred(d.W[Q[t] * BO + n], d.f.p);
+                } else if (t < d.KqA) {
+                    if (t < d.kA) v = centred(d.A[QA[t] * d.ld + d.T0 + n], d.f.p);
+                } else if (t - d.KqA < d.q0) {
+                    v = centred(d.A[Q[t - d.KqA] * d.ld + d.T0 + n], d.f.p);
+                }
+            }
             tile[tt][nn] = v;
         }
         __syncthreads();
         const int nn = threadIdx.x >> 2, tq = (threadIdx.x & 3) * 16;
         const int64_t n = nb + nn;
-        if (n < d.nT) {
+        if (n < ncols) {
             int8_t buf[NL][16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
@@ -538,22 +588,11 @@ __global__ __launch_bounds__(256) void k_make_rhs(const BMat<LT>* __restrict__ D
             for (int l = 0; l < NL; ++l) {
                 v4i w;
                 __builtin_memcpy(&w, buf[l], 16);
-                *reinterpret_cast<v4i*>(d.planesB + l * d.strideB + n * d.Kq + tb + tq) = w;
+                *reinterpret_cast<v4i*>(planes + l * pstride + n * depth + tb + tq) = w;
             }
         }
         __syncthreads();
     }
-}
-
-// A[Q[t], T0 + n] = Gq[t, n] (blockIdx.y = t, blockIdx.z = matrix).
-template <class LT>
-__global__ __launch_bounds__(256) void k_scatter_q(const BMat<LT>* __restrict__ D) {
-    const BMat<LT>& d = D[blockIdx.z];
-    const int t = blockIdx.y;
-    if (t >= d.q0 || d.nT == 0) return;
-    const int64_t r = d.piv_rows[d.rank_start + t];
-    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < d.nT; n += (int64_t)gridDim.x * blockDim.x)
-        d.A[r * d.ld + d.T0 + n] = d.Gq[t * d.ldg + n];
 }
 
 // ---------------------------------------------------------------------------
@@ -611,10 +650,11 @@ struct TileSched {
     int group;           // tile rows walked together (tuning knob "gemm_group")
 };
 
-// Outer update GEMMs.  MODE 0: Gq = Z[Q] A[Q, T] (C starts at zero, q0 rows);
-// MODE 1: A[:, T] <- A[:, T] + Z A[Q, T] (rows of Q have zero Z rows here).
-// Both are C <- C - L B with L = -Z limbs.
-template <int MODE>
+// Trailing-update GEMMs, C <- C - L B with L = limbs of -W (see k_make_lhs):
+//   JOB 0: A[:, T] <- A[:, T] + W A[Qc, T]   (the whole depth Kq, every row);
+//   JOB 1: W <- W + W' W[Q']                 (the composite transformation of a
+//          deferred panel: L = the current panel's part, B = planesW).
+template <int JOB>
 __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __restrict__ D, TileSched ts) {
     __shared__ __attribute__((aligned(16))) int8_t sA[2][4][G_M * G_ROW];
     __shared__ __attribute__((aligned(16))) int8_t sB[2][4][G_N * G_ROW];
@@ -622,15 +662,16 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
     int64_t tm, tn;
     if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
     const BMat<LimbsU32>& d = D[mi_];
-    uint32_t* C = MODE == 0 ? d.Gq : d.A + d.T0;
-    const int64_t ldc = MODE == 0 ? d.ldg : d.ld;
-    const int64_t Mrows = MODE == 0 ? d.q0 : d.N;
-    const int64_t Ncols = d.nT;
-    const int8_t* L = MODE == 0 ? d.planesY : d.planesL;
-    const int64_t planeL = MODE == 0 ? STRIDE_Y : d.strideL;
-    const int8_t* B = d.planesB;
-    const int64_t planeB = d.strideB;
-    const int Kpad = d.Kq;
+    uint32_t* C = JOB == 0 ? d.A + d.T0 : d.W;
+    const int64_t ldc = JOB == 0 ? d.ld : BO;
+    const int64_t Mrows = d.N;
+    const int64_t Ncols = JOB == 0 ? d.nT : d.kA;
+    const int64_t ldL = d.Kq;
+    const int8_t* L = d.planesL + (JOB == 0 ? 0 : d.KqA);
+    const int64_t planeL = d.strideL;
+    const int8_t* B = JOB == 0 ? d.planesB : d.planesW;
+    const int64_t planeB = JOB == 0 ? d.strideB : STRIDE_W;
+    const int Kpad = JOB == 0 ? d.Kq : d.Kq - d.KqA;
     const double pd = d.pd, pinv = d.pinv;
     const int64_t m_base = tm * G_M, n_base = tn * G_N;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -654,14 +695,14 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
                 for (int i = 0; i < 4; ++i) {
                     const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, Mrows - 1);
                     const int64_t col = min(n_base + wn * 32 + ni * 16 + r16, Ncols - 1);
-                    cpre[mi][ni][i] = MODE == 1 ? gld<uint32_t>(C + row * ldc + col) : 0u;
+                    cpre[mi][ni][i] = gld<uint32_t>(C + row * ldc + col);
                 }
     };
     // stage copy: A and B = 4 limbs x 64 rows x 4 sixteen-byte quarters (4
     // each per thread); the row/limb part of each address is fixed per tile
     v4i ra[4], rb[4];
     const int srow = tid >> 2, squ = tid & 3;  // the same for the 4 limbs (q = limb)
-    const int8_t* pa = L + min(m_base + srow, Mrows - 1) * Kpad + squ * 16;
+    const int8_t* pa = L + min(m_base + srow, Mrows - 1) * ldL + squ * 16;
     const int8_t* pb = B + min(n_base + srow, Ncols - 1) * Kpad + squ * 16;
     const int soff = lds_at(srow, squ);
     auto gload = [&](int kc) {
@@ -874,8 +915,8 @@ __device__ __forceinline__ void mfma_stage64(const int8_t (*sA)[G_M * G_ROW], co
 constexpr int BLS64 = G_T / (TN64 * 4);
 static_assert(G_T % (TN64 * 4) == 0, "whole limbs per B-stage pass");
 
-// Outer update GEMMs for 64-bit storage (MODE as in k_gemm_outer).
-template <int MODE>
+// Trailing-update GEMMs for 64-bit storage (JOB as in k_gemm_outer).
+template <int JOB>
 __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>* __restrict__ D, TileSched ts) {
     __shared__ __attribute__((aligned(16))) int8_t sA[NL64][G_M * G_ROW];
     __shared__ __attribute__((aligned(16))) int8_t sB[NL64][TN64 * G_ROW];
@@ -883,15 +924,16 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
     int64_t tm, tn;
     if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
     const BMat<LimbsU64>& d = D[mi_];
-    uint64_t* C = MODE == 0 ? d.Gq : d.A + d.T0;
-    const int64_t ldc = MODE == 0 ? d.ldg : d.ld;
-    const int64_t Mrows = MODE == 0 ? d.q0 : d.N;
-    const int64_t Ncols = d.nT;
-    const int8_t* L = MODE == 0 ? d.planesY : d.planesL;
-    const int64_t planeL = MODE == 0 ? STRIDE_Y : d.strideL;
-    const int8_t* B = d.planesB;
-    const int64_t planeB = d.strideB;
-    const int Kpad = d.Kq;
+    uint64_t* C = JOB == 0 ? d.A + d.T0 : d.W;
+    const int64_t ldc = JOB == 0 ? d.ld : BO;
+    const int64_t Mrows = d.N;
+    const int64_t Ncols = JOB == 0 ? d.nT : d.kA;
+    const int64_t ldL = d.Kq;
+    const int8_t* L = d.planesL + (JOB == 0 ? 0 : d.KqA);
+    const int64_t planeL = d.strideL;
+    const int8_t* B = JOB == 0 ? d.planesB : d.planesW;
+    const int64_t planeB = JOB == 0 ? d.strideB : STRIDE_W;
+    const int Kpad = JOB == 0 ? d.Kq : d.Kq - d.KqA;
     const uint64_t p = d.f.p;
     const double pinv = d.pinv;
     const int64_t m_base = tm * G_M, n_base = tn * TN64;
@@ -903,7 +945,7 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
     // A stage: limb q of row tid/4, quarter tid%4; B stage: BQ64 chunks
     v4i ra[NL64], rb[BQ64];
     const int srow = tid >> 2, squ = tid & 3;
-    const int8_t* pa = L + min(m_base + srow, Mrows - 1) * Kpad + squ * 16;
+    const int8_t* pa = L + min(m_base + srow, Mrows - 1) * ldL + squ * 16;
     const int soff = lds_at(srow, squ);
     const int bl0 = tid / (TN64 * 4), brow = (tid % (TN64 * 4)) >> 2, bqu = tid & 3;
     const int8_t* pb = B + bl0 * planeB + min(n_base + brow, Ncols - 1) * Kpad + bqu * 16;
@@ -948,7 +990,7 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
             for (int i = 0; i < 4; ++i) {  // unconditional loads, as in k_gemm_outer
                 const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, Mrows - 1);
                 const int64_t col = min(n_base + wn * 16 * NI64 + ni * 16 + r16, Ncols - 1);
-                cv[mi][ni][i] = MODE == 1 ? gld<uint64_t>(C + row * ldc + col) : 0ull;
+                cv[mi][ni][i] = gld<uint64_t>(C + row * ldc + col);
             }
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
@@ -1120,7 +1162,7 @@ __global__ void k_gather_status(const BMat<LT>* __restrict__ D, ItemStatus* out)
 // ---------------------------------------------------------------------------
 
 struct WorkspaceLayout {
-    size_t X, L, Y, B, Ai, Ui, U, G, bs, st, piv, used, epoch, slot, total;
+    size_t X, L, Wp, B, L1, B1, Ai, Ui, U, W, bs, st, piv, used, epoch, slot, total;
 };
 
 template <class LT>
@@ -1132,13 +1174,15 @@ static WorkspaceLayout workspace_layout(int64_t N, int64_t M, bool with_elim_sta
     const int64_t cap = std::max<int64_t>(1, std::min(N, M));
     size_t off = 0;
     w.X = off; off += al((size_t)N * XLD * E);
-    w.L = off; off += al(NL * round_up(N * 256, 256));
-    w.Y = off; off += al(NL * STRIDE_Y);
-    w.B = off; off += al(NL * round_up(std::max<int64_t>(M, XLD) * 256, 256));
+    w.L = off; off += al(NL * round_up(N * 2 * BO, 256));
+    w.Wp = off; off += al(NL * STRIDE_W);
+    w.B = off; off += al(NL * round_up(std::max<int64_t>(M, XLD) * 2 * BO, 256));
+    w.L1 = off; off += al(NL * round_up(N * BO, 256));
+    w.B1 = off; off += al(NL * STRIDE_W);
     w.Ai = off; off += al(NL * round_up(N * KPI, 256));
     w.Ui = off; off += al(NL * XLD * KPI);
     w.U = off; off += al((size_t)CI * XLD * E);
-    w.G = off; off += al((size_t)256 * round_up(M, 64) * E);
+    w.W = off; off += al((size_t)N * BO * E);
     w.bs = off; off += al(sizeof(BlockState<T>));
     w.epoch = off; off += al((size_t)4 * N);
     w.slot = off; off += al((size_t)4 * N);
@@ -1156,16 +1200,18 @@ static void carve(BMat<LT>& d, char* base, const WorkspaceLayout& w, int64_t N, 
     using T = typename LT::T;
     d.X = (T*)(base + w.X);
     d.planesL = (int8_t*)(base + w.L);
-    d.strideL = round_up(N * 256, 256);
-    d.planesY = (int8_t*)(base + w.Y);
+    d.strideL = round_up(N * 2 * BO, 256);
+    d.planesW = (int8_t*)(base + w.Wp);
     d.planesB = (int8_t*)(base + w.B);
-    d.strideB = round_up(std::max<int64_t>(M, XLD) * 256, 256);
+    d.strideB = round_up(std::max<int64_t>(M, XLD) * 2 * BO, 256);
+    d.planesL1 = (int8_t*)(base + w.L1);
+    d.strideL1 = round_up(N * BO, 256);
+    d.planesB1 = (int8_t*)(base + w.B1);
     d.planesAi = (int8_t*)(base + w.Ai);
     d.strideAi = round_up(N * KPI, 256);
     d.planesUi = (int8_t*)(base + w.Ui);
     d.U = (T*)(base + w.U);
-    d.Gq = (T*)(base + w.G);
-    d.ldg = round_up(M, 64);
+    d.W = (T*)(base + w.W);
     d.bs = (BlockState<T>*)(base + w.bs);
     d.epoch_of = (int32_t*)(base + w.epoch);
     d.slot_of = (int32_t*)(base + w.slot);
@@ -1215,25 +1261,32 @@ class Engine {
     bool first_step = true;
 
   private:
-    // Staging slots (pinned host + device): 0 = inner phase; 1..4 = the two
-    // parts of the trailing update of even and odd steps.  A slot is refilled
-    // two steps after its last use, when its copies and readers are done (the
-    // inner phase of a step waits for the previous step's first part, which
-    // follows the part before it on the same stream).
-    static constexpr int NSLOT = 5;
+    // Staging slots (pinned host + device): 0 = inner phase; 1..6 = the three
+    // parts of the trailing update of even and odd steps.  A slot's device
+    // copy is refilled on `so` after every kernel that read it, and its host
+    // side two steps after its copy was queued (the inner phase of a step
+    // waits for the previous step's first part, which follows the parts
+    // before it on the same stream).
+    static constexpr int NSLOT = 7;
     struct Staged {
         BM* desc;
-        TileSched inner, gq, main;
-        int64_t inner_tiles, gq_tiles, main_tiles;
+        TileSched inner, wq, main;  // inner GEMM, composite W GEMM, trailing update
+        int64_t inner_tiles, wq_tiles, main_tiles;
         ItemStatus* status;
+        double macs, bytes;         // algorithmic work of the trailing update (profiler)
+        int64_t cols;               // widest column range of the part
     };
     int stage(int slot, size_t bytes);
-    // part 0: inner phase; 1: trailing columns of the next panel; 2: the rest
+    // part 0: inner phase; 1: the next panel's columns; 2, 3: the two halves
+    // of the columns beyond it (composite updates only)
     int upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out);
+    int trailing(const Staged& t, Profiler* prof, int64_t tag);
     int ensure_stream();
     hipStream_t so = nullptr;
     hipEvent_t ev_ready = nullptr;  // on `so`: next panel's columns updated, panel buffer free
     bool ev_pending = false;
+    Staged pend{};                  // second half of the last composite update, not yet queued
+    bool has_pend = false;
     int parity = 0;
     void* d_buf[NSLOT] = {};
     void* h_buf[NSLOT] = {};
@@ -1250,7 +1303,27 @@ int Engine<LT>::ensure_stream() {
 
 template <class LT>
 int Engine<LT>::drain() {
+    if (has_pend) {
+        has_pend = false;
+        LINAC_TRY(trailing(pend, nullptr, 0));
+    }
     if (so) LINAC_HIP(hipStreamSynchronize(so));
+    return LINAC_OK;
+}
+
+// The trailing GEMM of one staged part on `so`: B operand planes, then the update.
+template <class LT>
+int Engine<LT>::trailing(const Staged& t, Profiler* prof, int64_t tag) {
+    if (t.main_tiles == 0) return LINAC_OK;
+    const unsigned B = (unsigned)t.inner.B;
+    k_make_rhs<LT><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(t.cols, 64)), 2 * BO / 64, B), 256, 0, so>>>(
+        t.desc, 0);
+    if (prof) LINAC_TRY(prof->sample_begin(so, tag, t.bytes, t.macs));
+    const unsigned grid = (unsigned)(ceil_div(t.main_tiles, NUM_XCD) * NUM_XCD);
+    if constexpr (std::is_same<LT, LimbsU64>::value) k_gemm_outer64<0><<<grid, G_T, 0, so>>>(t.desc, t.main);
+    else k_gemm_outer<0><<<grid, G_T, 0, so>>>(t.desc, t.main);
+    if (prof) LINAC_TRY(prof->sample_end(so));
+    LINAC_HIP(hipGetLastError());
     return LINAC_OK;
 }
 
@@ -1269,7 +1342,7 @@ int Engine<LT>::stage(int slot, size_t bytes) {
 
 // Staging layout: [descriptors B][sched 6B+3 int64][status B].  The sched
 // arrays are prefix sums of per-matrix tile counts: (pa, ma) inner GEMM,
-// (pb, mb) Z[Q] A[Q, T], (pc, mc) main trailing update.
+// (pb, mb) composite W of a deferred panel, (pc, mc) trailing update.
 template <class LT>
 int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out) {
     const size_t B = act.size();
@@ -1282,16 +1355,36 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
     BM* hd = (BM*)h;
     int64_t* hs = (int64_t*)(h + desc_b);
     int64_t* ds = (int64_t*)(dv + desc_b);
+    out->macs = out->bytes = 0;
+    out->cols = 1;
     for (size_t m = 0; m < B; ++m) {
         BM d = act[m]->d;
-        if (part == 2) {  // the columns beyond the next panel
-            const int64_t n1 = std::min<int64_t>(BO, d.nT);
-            d.T0 += n1;
-            d.nT -= n1;
-            d.planesB += n1 * d.Kq;
-            d.Gq += n1;
-        } else if (part == 1) {
-            d.nT = std::min<int64_t>(BO, d.nT);
+        if (part != 0) {
+            // the next panel's n1 columns, then (composite updates only) the rest
+            // in two halves; the first half covers at least the panel after next
+            const int64_t n1 = std::min<int64_t>(BO, d.nT), rest = d.nT - n1;
+            const int64_t h = rest <= BO ? rest : std::min<int64_t>(rest, std::max<int64_t>(BO, round_up(rest / 2, 64)));
+            const bool comp = d.kA > 0;
+            if (part == 1) {
+                d.nT = n1;
+                if (!comp) {  // a deferring or plain panel: its own operand buffers
+                    d.planesL = d.planesL1;
+                    d.strideL = d.strideL1;
+                    d.planesB = d.planesB1;
+                    d.strideB = STRIDE_W;
+                }
+            } else {
+                const int64_t off = part == 2 ? n1 : n1 + h;
+                d.T0 += off;
+                d.nT = !comp ? 0 : (part == 2 ? h : rest - h);
+                d.planesB += off * d.Kq;
+            }
+            const int64_t K = d.kA + d.q0;
+            if (K > 0 && d.nT > 0) {
+                out->macs += (double)(d.N - K) * K * d.nT;
+                out->bytes += 2.0 * sizeof(typename LT::T) * d.N * d.nT;
+                out->cols = std::max(out->cols, d.nT);
+            }
         }
         hd[m] = d;
     }
@@ -1307,19 +1400,20 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
         const BM& d = hd[m];
         ma[m] = ceil_div(d.N, G_M);
         pa[m + 1] = pa[m] + (part == 0 ? ma[m] * (XLD / TN) : 0);
-        const bool live = part != 0 && d.q0 > 0 && d.nT > 0;
-        mb[m] = std::max<int64_t>(1, ceil_div(d.q0, G_M));
-        pb[m + 1] = pb[m] + (live ? ceil_div(d.q0, G_M) * ceil_div(d.nT, TN) : 0);
+        const bool live = part != 0 && d.Kq > 0 && d.nT > 0;
+        const bool wlive = part == 1 && d.kA > 0 && d.q0 > 0 && d.nT > 0;
+        mb[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
+        pb[m + 1] = pb[m] + (wlive ? ceil_div(d.N, G_M) * ceil_div(d.kA, TN) : 0);
         mc[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
         pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, TN) : 0);
     }
     out->desc = (BM*)dv;
     const int grp = (int)tuning().gemm_group;
     out->inner = TileSched{ds, ds + (B + 1), (int)B, grp};
-    out->gq = TileSched{ds + (B + 1) + B, ds + 2 * (B + 1) + B, (int)B, grp};
+    out->wq = TileSched{ds + (B + 1) + B, ds + 2 * (B + 1) + B, (int)B, grp};
     out->main = TileSched{ds + 2 * (B + 1) + 2 * B, ds + 3 * (B + 1) + 2 * B, (int)B, grp};
     out->inner_tiles = pa[B];
-    out->gq_tiles = pb[B];
+    out->wq_tiles = pb[B];
     out->main_tiles = pc[B];
     out->status = (ItemStatus*)(dv + desc_b + sched_b);
     LINAC_HIP(hipMemcpyAsync(dv, h, desc_b + sched_b, hipMemcpyHostToDevice, s));
@@ -1397,69 +1491,69 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         for (int64_t i = 0; i < behind; ++i) LINAC_TRY(group());
     }
     // --- trailing update on so: first the next panel's columns (then the
-    // panel goes back to A and the next inner phase may start), then the rest
-    int64_t maxnT = 1;
+    // panel goes back to A and the next inner phase may start), then the rest.
+    // The columns beyond the next panel are either updated together with the
+    // previous panel's deferred part (kA > 0: one composite GEMM) or deferred
+    // to the next step (save_w).
     for (unsigned m = 0; m < B; ++m) {
         BM& d = act[m]->d;
         d.q0 = st[m].q0;
         d.rank_start = st[m].rank - st[m].q0;
         d.T0 = d.J + d.bcur;
         d.nT = d.M - d.T0;
-        d.Kq = (int32_t)round_up(std::max(1, d.q0), 64);
-        if (d.q0 > 0 && d.nT > 0) maxnT = std::max(maxnT, d.nT);
+        d.KqA = (int32_t)round_up(d.kA, 64);
+        d.Kq = d.KqA + (int32_t)round_up(d.q0, 64);
+        d.save_w = (d.kA == 0 && d.q0 > 0 && d.nT > BO) ? 1 : 0;
     }
-    const int p1 = 1 + 2 * parity, p2 = 2 + 2 * parity;
+    const int p1 = 1 + 3 * parity;
     parity ^= 1;
-    Staged t1, t2;
+    Staged t1, t2, t3;
     LINAC_TRY(upload(act, so, p1, 1, &t1));
-    LINAC_TRY(upload(act, so, p2, 2, &t2));
-    // algorithmic work of a part's main GEMM: (N - q0) q0 |T| modular MACs, C read + written
-    auto part_work = [&](int part, double* pmacs, double* pbytes) {
-        *pmacs = *pbytes = 0;
-        for (unsigned m = 0; m < B; ++m) {
-            const BM& d = act[m]->d;
-            if (d.q0 == 0 || d.nT == 0) continue;
-            const int64_t n1 = std::min<int64_t>(BO, d.nT), n = part == 1 ? n1 : d.nT - n1;
-            *pmacs += (double)(d.N - d.q0) * d.q0 * n;
-            *pbytes += 2.0 * sizeof(typename LT::T) * d.N * n;
+    LINAC_TRY(upload(act, so, p1 + 1, 2, &t2));
+    LINAC_TRY(upload(act, so, p1 + 2, 3, &t3));
+    bool operands = t1.main_tiles > 0;
+    for (auto* it : act) operands = operands || it->d.save_w;
+    if (operands) {
+        // -W of the current panel (kept in W when deferred), then for composite
+        // items W <- W + W' W[Q'] and -W of the composite
+        k_mark_slots<LT><<<dim3(1, B), 256, 0, so>>>(t1.desc);
+        k_make_lhs<LT><<<dim3(gs_grid(maxN * BO, 16384), B), 256, 0, so>>>(t1.desc, 0, epoch);
+        if (t1.wq_tiles > 0) {
+            k_make_rhs<LT><<<dim3(BO / 64, BO / 64, B), 256, 0, so>>>(t1.desc, 1);
+            const unsigned grid = (unsigned)(ceil_div(t1.wq_tiles, NUM_XCD) * NUM_XCD);
+            if constexpr (wide) k_gemm_outer64<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
+            else k_gemm_outer<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
         }
-    };
-    auto trailing = [&](const Staged& t, int part, int64_t cols) -> int {
-        if (t.main_tiles == 0 && t.gq_tiles == 0) return LINAC_OK;
-        k_make_rhs<LT><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(cols, 64)), 4, B), 256, 0, so>>>(t.desc);
-        if (t.gq_tiles > 0) {
-            const unsigned grid = (unsigned)(ceil_div(t.gq_tiles, NUM_XCD) * NUM_XCD);
-            if constexpr (wide) k_gemm_outer64<0><<<grid, G_T, 0, so>>>(t.desc, t.gq);
-            else k_gemm_outer<0><<<grid, G_T, 0, so>>>(t.desc, t.gq);
-        }
-        if (t.main_tiles > 0) {
-            double pm, pbts;
-            part_work(part, &pm, &pbts);
-            if (prof) LINAC_TRY(prof->sample_begin(so, act[0]->d.J, pbts, pm));
-            const unsigned grid = (unsigned)(ceil_div(t.main_tiles, NUM_XCD) * NUM_XCD);
-            if constexpr (wide) k_gemm_outer64<1><<<grid, G_T, 0, so>>>(t.desc, t.main);
-            else k_gemm_outer<1><<<grid, G_T, 0, so>>>(t.desc, t.main);
-            if (prof) LINAC_TRY(prof->sample_end(so));
-        }
-        k_scatter_q<LT><<<dim3((unsigned)std::min<int64_t>(64, ceil_div(cols, 256)), 256, B), 256, 0, so>>>(t.desc);
+        k_make_lhs<LT><<<dim3(gs_grid(maxN * BO, 16384), B), 256, 0, so>>>(t1.desc, 1, epoch);
         LINAC_HIP(hipGetLastError());
-        return LINAC_OK;
-    };
-    if (t1.main_tiles > 0 || t1.gq_tiles > 0) {
-        k_make_lhs<LT><<<dim3(gs_grid(maxN * 256, 16384), B), 256, 0, so>>>(t1.desc, 0, epoch);
-        k_make_lhs<LT><<<dim3(gs_grid(256 * 256), B), 256, 0, so>>>(t1.desc, 1, epoch);
     }
-    LINAC_TRY(trailing(t1, 1, std::min<int64_t>(maxnT, BO)));
+    const int64_t tag = act[0]->d.J;
+    LINAC_TRY(trailing(t1, prof, tag));
     int64_t maxpanel = 1;
     for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
     k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, so>>>(t1.desc);
     LINAC_HIP(hipGetLastError());
+    // The next inner phase waits only for the above.  Behind it on `so`: the
+    // second half of the previous composite update (its columns lie beyond the
+    // next panel's), then this step's first half; this step's second half is
+    // queued at the next step, behind that step's own next-panel update.
     const bool lookahead = tuning().lookahead != 0;
     if (lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));
-    LINAC_TRY(trailing(t2, 2, std::max<int64_t>(1, maxnT - BO)));
+    if (has_pend) {
+        has_pend = false;
+        LINAC_TRY(trailing(pend, prof, tag));
+    }
+    LINAC_TRY(trailing(t2, prof, tag));
+    if (lookahead && t3.main_tiles > 0) {
+        pend = t3;
+        has_pend = true;
+    } else {
+        LINAC_TRY(trailing(t3, prof, tag));
+    }
     if (!lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));  // the next inner phase waits for all of it
     ev_pending = true;
     for (auto* it : act) {
+        it->d.kA = it->d.save_w ? it->d.q0 : 0;  // deferred to the next step
         it->d.J += it->d.bcur;
         if (it->d.J >= it->d.M) it->finished = true;
     }
@@ -1476,6 +1570,7 @@ static void describe(BMat<LT>& d, typename LT::T* A, int64_t ld, int64_t N, int6
     d.pd = (double)f.p;
     d.pinv = 1.0 / (double)f.p;
     d.J = 0;
+    d.kA = 0;
 }
 
 template <class LT>
