@@ -241,6 +241,55 @@ def _float_case(kind, n, seed=0):
     return {"kind": kind, "n": n, "cond_2": cond, **out}
 
 
+def _coo_case(n=4000, density=0.05, seed=0):
+    """A sparse system from disk: a plain-text COO file of an n x n matrix
+    (density 5%, 10% of the values fractions a/b) loaded into HBM over
+    GF(2^31-1) by device_matrix_from_coo (host parse and reduction, device
+    scatter), against the host route of the reference's format (dense object
+    matrix from the file, reduced entrywise, copied to the device)."""
+    import tempfile
+    import numpy
+    import torch
+    from linac_amd import sparse_matrix_tools as smt
+    p = 2 ** 31 - 1
+    rs = numpy.random.RandomState(seed)
+    nnz = int(n * n * density)
+    flat = rs.choice(n * n, size=nnz, replace=False)
+    flat.sort()
+    ri, ci = flat // n, flat % n
+    vals = rs.randint(-10 ** 6, 10 ** 6, size=nnz)
+    vals[vals == 0] = 1
+    frac = rs.random_sample(nnz) < 0.1
+    den = rs.randint(2, 1000, size=nnz)
+    with tempfile.TemporaryDirectory() as d:
+        path = f"{d}/system.txt"
+        with open(path, "w") as f:
+            f.write("".join(f"{r} {c} {v}/{q}\n" if fr else f"{r} {c} {v}\n"
+                            for r, c, v, q, fr in zip(ri.tolist(), ci.tolist(), vals.tolist(), den.tolist(),
+                                                      frac.tolist())))
+        size_mb = os.path.getsize(path) / 1e6
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        dev = smt.device_matrix_from_coo(path, p, shape=(n, n))
+        torch.cuda.synchronize()
+        t_dev = time.perf_counter() - t
+        # host route: dense object matrix, entrywise residues, upload
+        t = time.perf_counter()
+        M = smt.matrix_from_plain_txt_coo(path)
+        nzr, nzc = numpy.nonzero(M != 0)
+        H = numpy.zeros(M.shape, dtype=numpy.uint32)
+        H[nzr, nzc] = smt.residues_mod_p(list(M[nzr, nzc]), p).astype(numpy.uint32)
+        hd = torch.from_numpy(H.view(numpy.int32)).cuda()
+        torch.cuda.synchronize()
+        t_host = time.perf_counter() - t
+        same = bool(torch.equal(hd, dev))
+    del dev, hd
+    torch.cuda.empty_cache()
+    return {"workload": f"{n}x{n} GF(2^31-1), {nnz} entries ({size_mb:.1f} MB plain-text COO, 10% fractions)",
+            "device_route_s": round(t_dev, 3), "entries_per_s": float(f"{nnz / t_dev:.3g}"),
+            "host_dense_route_s": round(t_host, 3), "speedup": round(t_host / t_dev, 2), "identical": same}
+
+
 def configs_main(args):
     """Every configuration of the GF(p) and float forms, device-resident."""
     import torch
@@ -257,14 +306,17 @@ def configs_main(args):
     if args.with_100k:
         gf_cases.append(("u32-100000-blocked", (100000, p31, 1, "blocked", 0, "u32")))
     fl_cases = [("f64-8000", ("f64", 8000)), ("c128-6000", ("c128", 6000)), ("illcond-4000", ("illcond", 4000))]
-    unknown = only - {k for k, _ in gf_cases + fl_cases}
+    unknown = only - {k for k, _ in gf_cases + fl_cases} - {"coo"}
     if unknown:
         print(f"unknown cases: {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
     _gf_device_case(2000, p31, 1, "auto")  # warm-up
     gf = [_gf_device_case(*a) for k, a in gf_cases if not only or k in only]
     fl = [_float_case(*a) for k, a in fl_cases if not only or k in only]
-    print(json.dumps({"configs": {"gf": gf, "float": fl}, "host": platform.node()}), flush=True)
+    out = {"gf": gf, "float": fl}
+    if not only or "coo" in only:
+        out["coo"] = _coo_case()
+    print(json.dumps({"configs": out, "host": platform.node()}), flush=True)
 
 
 def main():

@@ -182,6 +182,21 @@ int linac_load_matrix_f64(double* out, const double* bases, const uint32_t* lind
 int linac_load_matrix_c128(double* out, const double* bases, const uint32_t* lindices, int64_t rows, int64_t cols,
                            int64_t basis_len, int64_t degree, uint32_t flags, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Sparse systems from disk.  Replaces the dense host construction of
+ * linac/sparse_matrix_tools.py:37-44 (matrix_from_coo) for the GPU path:
+ * out (a device matrix, rows x cols, row stride ld elements) is zeroed and
+ * out[row_idx[i], col_idx[i]] = values[i] for i < nnz.  Values must already
+ * be reduced modulo the field's prime.  The index and value arrays are host
+ * memory, or device memory with LINAC_DEVICE_PTR.  An index outside the
+ * matrix is LINAC_ERR_INVALID (checked on the host for host arrays, by the
+ * kernel otherwise).  Synchronous.
+ * ------------------------------------------------------------------------ */
+int linac_coo_scatter_u32(uint32_t* out, int64_t ld, int64_t rows, int64_t cols, const int64_t* row_idx,
+                          const int64_t* col_idx, const uint32_t* values, int64_t nnz, uint32_t flags);
+int linac_coo_scatter_u64(uint64_t* out, int64_t ld, int64_t rows, int64_t cols, const int64_t* row_idx,
+                          const int64_t* col_idx, const uint64_t* values, int64_t nnz, uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

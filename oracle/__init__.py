@@ -15,5 +15,6 @@ modp      exact mod-p helpers used to build and certify test inputs
           (limb-split matmul, Freivalds check, planted solutions, sympy check).
 loader    restatement of ``linac/pycuda_matrix_loader.py:127-230`` and of the
           ``LoadMatrix`` kernel's arithmetic (``linac/matrix_loader.cu:88-112``).
-pivots    restatement of ``linac/linear_algebra_tools.py:10-16,48-51``.
+pivots    restatement of ``linac/linear_algebra_tools.py:10-36,48-78`` (pivots, canonical kernels).
+coo       restatement of ``linac/sparse_matrix_tools.py:20-42`` (COO dictionaries).
 """
