@@ -290,6 +290,65 @@ def _coo_case(n=4000, density=0.05, seed=0):
             "host_dense_route_s": round(t_host, 3), "speedup": round(t_host / t_dev, 2), "identical": same}
 
 
+def _growing_case(dim=1000, cols=2000, start=20, step=20, max_iteration=50, seed=0):
+    """The row-growing loop of a function-space basis search
+    (VectorSpaceOfFunctions.__get_pivots__): sample rows of a hidden
+    dim-dimensional space over GF(2^31-1) are added `step` at a time until
+    one is dependent.  The reference's pattern reduces the whole stacked
+    matrix again after every addition (here with this library's device RREF);
+    the resident path (linac_amd.incremental) keeps the RREF in HBM and
+    eliminates only the new rows.  Sample rows are generated on the device."""
+    import numpy
+    import torch
+    from linac_amd import incremental, rref_device
+    p = 2 ** 31 - 1
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    n_rows = start + step * max_iteration
+    V = torch.randint(0, p, (dim, cols), generator=g, device="cuda", dtype=torch.int64).to(torch.int32)
+    Cf = torch.randint(0, p, (n_rows, dim), generator=g, device="cuda", dtype=torch.int64).to(torch.int32)
+    S = torch.zeros((n_rows, cols), dtype=torch.int32, device="cuda")
+    incremental.gemm_sub(S, Cf, V, p, k=dim)  # S = -Cf V: rows spanning the hidden space
+    torch.cuda.synchronize()
+    ld = (cols + 3) // 4 * 4
+    work = torch.zeros((n_rows, ld), dtype=torch.int32, device="cuda")
+
+    def reference_pattern():
+        n, it = start, 0
+        while True:
+            work[:n, :cols] = S[:n]
+            torch.cuda.synchronize()
+            piv, rank = rref_device(work.data_ptr(), n, cols, ld, p, numpy.uint32)
+            if rank < n:
+                return piv, work[:rank, :cols].clone(), it
+            if it >= max_iteration:
+                raise RuntimeError("dimension exceeded")
+            n += step
+            it += 1
+
+    def resident():
+        return incremental.pivots_of_growing_matrix(lambda lo, hi: S[lo:hi], cols, p, start, step, max_iteration)
+
+    reference_pattern()  # warm-up (kernels, scratch)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    piv_r, R_r, iters = reference_pattern()
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t
+    resident()
+    t = time.perf_counter()
+    R_i, piv_i = resident()
+    torch.cuda.synchronize()
+    t_inc = time.perf_counter() - t
+    same = piv_i == piv_r and numpy.array_equal(R_i, R_r.cpu().numpy().view(numpy.uint32))
+    del V, Cf, S, work
+    torch.cuda.empty_cache()
+    return {"workload": f"hidden space of dimension {dim} in {cols} columns over GF(2^31-1), rows added {step} at a "
+                        f"time from {start} ({iters + 1} additions) until one is dependent",
+            "reference_pattern_s": round(t_ref, 4), "resident_s": round(t_inc, 4),
+            "speedup": round(t_ref / t_inc, 2), "rank": len(piv_i), "identical": bool(same)}
+
+
 def configs_main(args):
     """Every configuration of the GF(p) and float forms, device-resident."""
     import torch
@@ -306,7 +365,7 @@ def configs_main(args):
     if args.with_100k:
         gf_cases.append(("u32-100000-blocked", (100000, p31, 1, "blocked", 0, "u32")))
     fl_cases = [("f64-8000", ("f64", 8000)), ("c128-6000", ("c128", 6000)), ("illcond-4000", ("illcond", 4000))]
-    unknown = only - {k for k, _ in gf_cases + fl_cases} - {"coo"}
+    unknown = only - {k for k, _ in gf_cases + fl_cases} - {"coo", "growing"}
     if unknown:
         print(f"unknown cases: {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
@@ -316,6 +375,8 @@ def configs_main(args):
     out = {"gf": gf, "float": fl}
     if not only or "coo" in only:
         out["coo"] = _coo_case()
+    if not only or "growing" in only:
+        out["growing"] = _growing_case()
     print(json.dumps({"configs": out, "host": platform.node()}), flush=True)
 
 

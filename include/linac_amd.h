@@ -183,6 +183,23 @@ int linac_load_matrix_c128(double* out, const double* bases, const uint32_t* lin
                            int64_t basis_len, int64_t degree, uint32_t flags, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Exact modular GEMM with index lists, on the int8-limb matrix cores:
+ *   C[i, j] <- (C[i, j] - sum_{t<k} A[i, a_cols[t]] * B[b_rows[t], j]) mod p
+ * for i < m, j < n.  C (row stride ldc), A (m x a_ncols, stride lda) and B
+ * (b_nrows x n, stride ldb) are device matrices with canonical entries in
+ * [0, p) (flags must include LINAC_DEVICE_PTR); a_cols / b_rows are host
+ * arrays of k indices, or NULL for 0..k-1.  C may be the same matrix as A or
+ * B: all operands are captured before C is written.  2 < p <= 4278124287.
+ * Synchronous.  No counterpart in the reference, which re-reduces a whole
+ * stacked matrix instead (linac/vector_spaces/over_function_fields.py:74-101);
+ * this is the update step of the resident incremental RREF
+ * (linac_amd/incremental.py).
+ * ------------------------------------------------------------------------ */
+int linac_gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda, int64_t a_ncols, const int64_t* a_cols,
+                       const uint32_t* B, int64_t ldb, int64_t b_nrows, const int64_t* b_rows, int64_t m, int64_t n,
+                       int64_t k, uint32_t p, uint32_t flags);
+
+/* ------------------------------------------------------------------------
  * Sparse systems from disk.  Replaces the dense host construction of
  * linac/sparse_matrix_tools.py:37-44 (matrix_from_coo) for the GPU path:
  * out (a device matrix, rows x cols, row stride ld elements) is zeroed and

@@ -72,6 +72,8 @@ SIGNATURES = {
     "linac_load_matrix_u64": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _u64, _u32, _vp]),
     "linac_load_matrix_f64": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _u32, _vp]),
     "linac_load_matrix_c128": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _u32, _vp]),
+    "linac_gemm_sub_u32": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _i64, _i64, _i64,
+                                          _u32, _u32]),
     "linac_coo_scatter_u32": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _u32]),
     "linac_coo_scatter_u64": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _u32]),
 }

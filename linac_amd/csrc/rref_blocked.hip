@@ -137,7 +137,7 @@ struct BMat {
     int32_t Kq;         // KqA + q0 rounded up to 64 (GEMM depth, row stride of planesL / planesB)
     int32_t save_w;     // this panel's update beyond the next panel is deferred: keep W
     int32_t e0;         // first epoch of this outer panel
-    int32_t pad1;
+    int32_t full;       // every row is a pivot row: update every trailing column now, then stop
     int64_t rank_start; // index of its first pivot in piv_rows (the deferred panel's kA precede it)
     int64_t T0, nT;     // trailing columns [T0, T0 + nT)
 };
@@ -1365,18 +1365,20 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
             const int64_t n1 = std::min<int64_t>(BO, d.nT), rest = d.nT - n1;
             const int64_t h = rest <= BO ? rest : std::min<int64_t>(rest, std::max<int64_t>(BO, round_up(rest / 2, 64)));
             const bool comp = d.kA > 0;
+            if (!comp) {  // a deferring, plain or last panel: its own left operand
+                d.planesL = d.planesL1;
+                d.strideL = d.strideL1;
+            }
             if (part == 1) {
                 d.nT = n1;
-                if (!comp) {  // a deferring or plain panel: its own operand buffers
-                    d.planesL = d.planesL1;
-                    d.strideL = d.strideL1;
+                if (!comp) {
                     d.planesB = d.planesB1;
                     d.strideB = STRIDE_W;
                 }
             } else {
                 const int64_t off = part == 2 ? n1 : n1 + h;
                 d.T0 += off;
-                d.nT = !comp ? 0 : (part == 2 ? h : rest - h);
+                d.nT = !(comp || d.full) ? 0 : (part == 2 ? h : rest - h);
                 d.planesB += off * d.Kq;
             }
             const int64_t K = d.kA + d.q0;
@@ -1503,7 +1505,8 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         d.nT = d.M - d.T0;
         d.KqA = (int32_t)round_up(d.kA, 64);
         d.Kq = d.KqA + (int32_t)round_up(d.q0, 64);
-        d.save_w = (d.kA == 0 && d.q0 > 0 && d.nT > BO) ? 1 : 0;
+        d.full = st[m].rank >= d.N ? 1 : 0;  // no row left to pivot: the remaining columns are final after this step
+        d.save_w = (!d.full && d.kA == 0 && d.q0 > 0 && d.nT > BO) ? 1 : 0;
     }
     const int p1 = 1 + 3 * parity;
     parity ^= 1;
@@ -1555,7 +1558,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     for (auto* it : act) {
         it->d.kA = it->d.save_w ? it->d.q0 : 0;  // deferred to the next step
         it->d.J += it->d.bcur;
-        if (it->d.J >= it->d.M) it->finished = true;
+        if (it->d.J >= it->d.M || it->d.full) it->finished = true;
     }
     return LINAC_OK;
 }
@@ -1636,6 +1639,163 @@ template int blocked_dispatch<FieldGF2>(uint32_t*, int64_t, int64_t, int64_t, co
                                         const ElimWorkspace&, DeviceContext*, bool, Profiler*);
 
 // ---------------------------------------------------------------------------
+// Exact modular GEMM with index lists (linac_gemm_sub_u32):
+//   C[i, j] <- C[i, j] - sum_t A[i, a_cols[t]] B[b_rows[t], j]   (mod p)
+// on the trailing-update kernel, in depth chunks of at most 2 BO = 512 (the
+// limb fold's bound).  Every chunk's limb planes are built before the first
+// chunk's GEMM, so C may be the same matrix as A or B.
+// ---------------------------------------------------------------------------
+
+constexpr int GS_KC = 2 * BO;
+using BMat32 = BMat<LimbsU32>;
+
+// planes[l][i][t] = digit l of A[i, cols[t0 + t]] for t < kc (cols NULL:
+// column t0 + t), zero up to Kpad; rows i < m.  (The GEMM kernel computes
+// C - L B, so L holds A itself here, where the trailing update holds -W.)
+__global__ __launch_bounds__(256) void k_planes_cols(const uint32_t* __restrict__ A, int64_t lda,
+                                                     const int64_t* __restrict__ cols, int64_t t0, int kc, int Kpad,
+                                                     int64_t m, uint32_t p, int8_t* __restrict__ planes, int64_t stride) {
+    const int64_t total = m * Kpad;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = x / Kpad;
+        const int t = (int)(x - i * Kpad);
+        int8_t dg[4] = {0, 0, 0, 0};
+        if (t < kc) {
+            const int64_t c = cols ? cols[t0 + t] : t0 + t;
+            balanced_digits(centred(A[i * lda + c], p), dg);
+        }
+#pragma unroll
+        for (int l = 0; l < 4; ++l) planes[l * stride + i * Kpad + t] = dg[l];
+    }
+}
+
+// planes[l][j][t] = digit l of B[rows[t0 + t], j] for t < kc (rows NULL: row
+// t0 + t), zero up to Kpad; columns j < n.  64 x 64 tiles through LDS
+// (blockIdx.x strides over column tiles, blockIdx.y = 64-deep tile).
+__global__ __launch_bounds__(256) void k_planes_rows(const uint32_t* __restrict__ B, int64_t ldb,
+                                                     const int64_t* __restrict__ rows, int64_t t0, int kc, int Kpad,
+                                                     int64_t n, uint32_t p, int8_t* __restrict__ planes, int64_t stride) {
+    __shared__ int32_t tile[64][65];
+    const int tb = blockIdx.y * 64;
+    for (int64_t nb = (int64_t)blockIdx.x * 64; nb < n; nb += (int64_t)gridDim.x * 64) {
+        for (int x = threadIdx.x; x < 64 * 64; x += 256) {
+            const int tt = x >> 6, nn = x & 63;
+            const int t = tb + tt;
+            const int64_t j = nb + nn;
+            int32_t v = 0;
+            if (t < kc && j < n) v = centred(B[(rows ? rows[t0 + t] : t0 + t) * ldb + j], p);
+            tile[tt][nn] = v;
+        }
+        __syncthreads();
+        const int nn = threadIdx.x >> 2, tq = (threadIdx.x & 3) * 16;
+        const int64_t j = nb + nn;
+        if (j < n) {
+            int8_t buf[4][16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                int8_t dg[4];
+                balanced_digits(tile[tq + i][nn], dg);
+#pragma unroll
+                for (int l = 0; l < 4; ++l) buf[l][i] = dg[l];
+            }
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                v4i w;
+                __builtin_memcpy(&w, buf[l], 16);
+                *reinterpret_cast<v4i*>(planes + l * stride + j * Kpad + tb + tq) = w;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+static int gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda, int64_t a_ncols,
+                        const int64_t* a_cols, const uint32_t* B, int64_t ldb, int64_t b_nrows, const int64_t* b_rows,
+                        int64_t m, int64_t n, int64_t k, uint32_t p, uint32_t flags) {
+    if (m < 0 || n < 0 || k < 0 || a_ncols < 0 || b_nrows < 0) return fail(LINAC_ERR_INVALID, "negative size");
+    if (m == 0 || n == 0 || k == 0) return LINAC_OK;
+    if (!(flags & LINAC_DEVICE_PTR)) return fail(LINAC_ERR_INVALID, "C, A and B must be device matrices (LINAC_DEVICE_PTR)");
+    if (!C || !A || !B) return fail(LINAC_ERR_INVALID, "NULL matrix");
+    if (ldc < n || ldb < n || lda < a_ncols) return fail(LINAC_ERR_INVALID, "a row stride is below its row length");
+    if (!is_prime_u64(p) || p == 2 || p > LIMB_P_MAX)
+        return fail(LINAC_ERR_INVALID, "p = %u is not an odd prime <= %llu", p, (unsigned long long)LIMB_P_MAX);
+    for (int64_t t = 0; t < k; ++t) {
+        const int64_t c = a_cols ? a_cols[t] : t, r = b_rows ? b_rows[t] : t;
+        if (c < 0 || c >= a_ncols) return fail(LINAC_ERR_INVALID, "A column %lld outside [0, %lld)", (long long)c, (long long)a_ncols);
+        if (r < 0 || r >= b_nrows) return fail(LINAC_ERR_INVALID, "B row %lld outside [0, %lld)", (long long)r, (long long)b_nrows);
+    }
+    DeviceContext* ctx = nullptr;
+    LINAC_TRY(current_context(&ctx));
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    const FieldU32 f = make_field_u32(p);
+    const int64_t nch = ceil_div(k, GS_KC);
+    // plane regions per chunk (L: m x Kpad, B: n x Kpad, four limbs each)
+    std::vector<int64_t> offL(nch), offB(nch), sL(nch), sB(nch), kp(nch);
+    int64_t totL = 0, totB = 0;
+    for (int64_t c = 0; c < nch; ++c) {
+        kp[c] = round_up(std::min<int64_t>(GS_KC, k - c * GS_KC), 64);
+        sL[c] = round_up(m * kp[c], 256);
+        sB[c] = round_up(n * kp[c], 256);
+        offL[c] = totL;
+        offB[c] = totB;
+        totL += 4 * sL[c];
+        totB += 4 * sB[c];
+    }
+    void* pp = nullptr;
+    LINAC_TRY(ctx->get(SLOT_PANEL_B, (size_t)(totL + totB), &pp));
+    int8_t* planesL = (int8_t*)pp;
+    int8_t* planesB = planesL + totL;
+    const int64_t ntiles = ceil_div(m, G_M) * ceil_div(n, G_N);
+    const size_t desc_b = (size_t)round_up(nch * (int64_t)sizeof(BMat32), 256);
+    const size_t idx_b = (size_t)round_up(k * 8, 256);
+    void* pc = nullptr;
+    LINAC_TRY(ctx->get(SLOT_PANEL_C, desc_b + 256 + 2 * idx_b, &pc));
+    BMat32* d_desc = (BMat32*)pc;
+    int64_t* d_sched = (int64_t*)((char*)pc + desc_b);  // pre[0..1], mts[0]
+    int64_t* d_acols = (int64_t*)((char*)pc + desc_b + 256);
+    int64_t* d_brows = d_acols + idx_b / 8;
+    std::vector<BMat32> hd(nch);
+    for (int64_t c = 0; c < nch; ++c) {
+        BMat32 d{};
+        d.A = C;
+        d.ld = ldc;
+        d.N = m;
+        d.M = n;
+        d.f = f;
+        d.pd = (double)p;
+        d.pinv = 1.0 / (double)p;
+        d.planesL = planesL + offL[c];
+        d.strideL = sL[c];
+        d.planesB = planesB + offB[c];
+        d.strideB = sB[c];
+        d.T0 = 0;
+        d.nT = n;
+        d.Kq = (int32_t)kp[c];
+        hd[c] = d;
+    }
+    const int64_t hs[3] = {0, ntiles, ceil_div(m, G_M)};
+    LINAC_HIP(hipMemcpyAsync(d_desc, hd.data(), nch * sizeof(BMat32), hipMemcpyHostToDevice, s));
+    LINAC_HIP(hipMemcpyAsync(d_sched, hs, sizeof(hs), hipMemcpyHostToDevice, s));
+    if (a_cols) LINAC_HIP(hipMemcpyAsync(d_acols, a_cols, k * 8, hipMemcpyHostToDevice, s));
+    if (b_rows) LINAC_HIP(hipMemcpyAsync(d_brows, b_rows, k * 8, hipMemcpyHostToDevice, s));
+    for (int64_t c = 0; c < nch; ++c) {
+        const int64_t t0 = c * GS_KC;
+        const int kc = (int)std::min<int64_t>(GS_KC, k - t0);
+        k_planes_cols<<<gs_grid(m * kp[c], 16384), 256, 0, s>>>(A, lda, a_cols ? d_acols : nullptr, t0, kc, (int)kp[c],
+                                                                 m, p, planesL + offL[c], sL[c]);
+        k_planes_rows<<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(n, 64)), (unsigned)(kp[c] / 64)), 256, 0, s>>>(
+            B, ldb, b_rows ? d_brows : nullptr, t0, kc, (int)kp[c], n, p, planesB + offB[c], sB[c]);
+    }
+    const TileSched ts{d_sched, d_sched + 2, 1, (int)tuning().gemm_group};
+    for (int64_t c = 0; c < nch; ++c)
+        k_gemm_outer<0><<<(unsigned)(ceil_div(ntiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc + c, ts);
+    LINAC_HIP(hipGetLastError());
+    LINAC_HIP(hipStreamSynchronize(s));
+    return LINAC_OK;
+}
+
+// ---------------------------------------------------------------------------
 // batch objects: matrices join and leave between steps (continuous batching)
 // ---------------------------------------------------------------------------
 
@@ -1653,7 +1813,6 @@ struct linac_batch {
 namespace linac {
 
 using Item32 = Item<LimbsU32>;
-using BMat32 = BMat<LimbsU32>;
 
 static size_t matrix_bytes(int64_t rows, int64_t cols) { return (size_t)round_up(rows * round_up(cols, 64) * 4, 256); }
 
@@ -1889,6 +2048,12 @@ int linac_batch_take_many_u32(linac_batch* b, const int64_t* ids, int64_t count,
         delete it;
     }
     return LINAC_OK;
+}
+
+int linac_gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda, int64_t a_ncols, const int64_t* a_cols,
+                       const uint32_t* B, int64_t ldb, int64_t b_nrows, const int64_t* b_rows, int64_t m, int64_t n,
+                       int64_t k, uint32_t p, uint32_t flags) {
+    return gemm_sub_u32(C, ldc, A, lda, a_ncols, a_cols, B, ldb, b_nrows, b_rows, m, n, k, p, flags);
 }
 
 int linac_rref_batch_u32(uint32_t* const* mats, const int64_t* rows, const int64_t* cols, const int64_t* lds,

@@ -228,8 +228,10 @@ int plain_run(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hi
         k_elim<F><<<(unsigned)ceil_div(N, rows_per_block), 256, 0, s>>>(A, ld, N, M, j, f, ws.st, log2G);
         if (sampled) LINAC_TRY(prof->sample_end(s));
         // Once every row is a pivot row the remaining columns cannot change;
-        // poll the rank without blocking the launch stream.
-        if (M > N && (j & 255) == 255) {
+        // poll the rank without blocking the launch stream (every 32 columns
+        // for at most 64 rows, else every 256).
+        const int64_t poll = N <= 64 ? 31 : 255;
+        if (M > N && (j & poll) == poll) {
             if (probe_pending && hipEventQuery(probe) == hipSuccess) {
                 probe_pending = false;
                 if (ctx->pinned[0] >= N) break;
