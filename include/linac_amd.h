@@ -83,7 +83,14 @@ int linac_profile_last(double* out, int n);
  *   "gemm_group" 8: tile rows walked together in the GEMM tile order (a right
  *       tile is reused that many times while the left tiles stay in L2), 1..64;
  *   "lookahead" 1: the trailing update of a panel overlaps the next panel's
- *       inner steps (0: one after the other).
+ *       inner steps (0: one after the other);
+ *   "timeslice" 0: with look-ahead, each part of the trailing update beyond
+ *       the next panel runs as one launch beside the next inner steps (1: in
+ *       slices beside each inner step's factorisation, the inner step's GEMM
+ *       waiting for its slice);
+ *   "reserve_cus" 0: CUs in every 32 that the trailing-update stream leaves
+ *       to the inner steps (a CU mask on that stream), 0..3; at most 3 in 32
+ *       keeps every XCD populated whatever the mask's bit order.
  * Unknown keys and out-of-range values return LINAC_ERR_INVALID. */
 int linac_tune_set(const char* key, int64_t value);
 int linac_tune_get(const char* key, int64_t* value);

@@ -618,13 +618,23 @@ constexpr int NUM_XCD = 8;
 
 // blockIdx.x -> (matrix, tile row, tile column) through the prefix sums of
 // per-matrix tile counts `pre[0..B]` (row-tile counts in `mts`).
-__device__ __forceinline__ bool batch_tile(const int64_t* __restrict__ pre, const int64_t* __restrict__ mts, int B,
-                                           int group, int* m_out, int64_t* tm, int64_t* tn) {
-    const int64_t total = pre[B];
-    const int64_t per_xcd = (total + NUM_XCD - 1) / NUM_XCD;
+struct TileSched {
+    const int64_t* pre;  // [B + 1]
+    const int64_t* mts;  // [B]
+    int B;
+    int group;           // tile rows walked together (tuning knob "gemm_group")
+    int64_t t0, t1;      // the launch covers tiles [t0, t1) of the flattened list (t1 < 0: to its end)
+};
+
+__device__ __forceinline__ bool batch_tile(const TileSched& ts, int* m_out, int64_t* tm, int64_t* tn) {
+    const int64_t* __restrict__ pre = ts.pre;
+    const int64_t* __restrict__ mts = ts.mts;
+    const int B = ts.B, group = ts.group;
+    const int64_t end = ts.t1 < 0 ? pre[B] : ts.t1;
+    const int64_t per_xcd = (end - ts.t0 + NUM_XCD - 1) / NUM_XCD;
     const int64_t bid = blockIdx.x;
-    const int64_t t = (bid % NUM_XCD) * per_xcd + bid / NUM_XCD;
-    if (t >= total) return false;
+    const int64_t t = ts.t0 + (bid % NUM_XCD) * per_xcd + bid / NUM_XCD;
+    if (t >= end) return false;
     int lo = 0, hi = B;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -643,13 +653,6 @@ __device__ __forceinline__ bool batch_tile(const int64_t* __restrict__ pre, cons
     return true;
 }
 
-struct TileSched {
-    const int64_t* pre;  // [B + 1]
-    const int64_t* mts;  // [B]
-    int B;
-    int group;           // tile rows walked together (tuning knob "gemm_group")
-};
-
 // Trailing-update GEMMs, C <- C - L B with L = limbs of -W (see k_make_lhs):
 //   JOB 0: A[:, T] <- A[:, T] + W A[Qc, T]   (the whole depth Kq, every row);
 //   JOB 1: W <- W + W' W[Q']                 (the composite transformation of a
@@ -660,7 +663,7 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
     __shared__ __attribute__((aligned(16))) int8_t sB[2][4][G_N * G_ROW];
     int mi_;
     int64_t tm, tn;
-    if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
+    if (!batch_tile(ts, &mi_, &tm, &tn)) return;
     const BMat<LimbsU32>& d = D[mi_];
     uint32_t* C = JOB == 0 ? d.A + d.T0 : d.W;
     const int64_t ldc = JOB == 0 ? d.ld : BO;
@@ -790,7 +793,7 @@ __global__ __launch_bounds__(G_T, 4) void k_gemm_inner(const BMat<LimbsU32>* __r
     __shared__ __attribute__((aligned(16))) int8_t sB[4][G_N * G_ROW];
     int mi_;
     int64_t tm, tn;
-    if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
+    if (!batch_tile(ts, &mi_, &tm, &tn)) return;
     const BMat<LimbsU32>& d = D[mi_];
     const BlockState<uint32_t>* bs = d.bs;
     const int k = bs->k;
@@ -922,7 +925,7 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
     __shared__ __attribute__((aligned(16))) int8_t sB[NL64][TN64 * G_ROW];
     int mi_;
     int64_t tm, tn;
-    if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
+    if (!batch_tile(ts, &mi_, &tm, &tn)) return;
     const BMat<LimbsU64>& d = D[mi_];
     uint64_t* C = JOB == 0 ? d.A + d.T0 : d.W;
     const int64_t ldc = JOB == 0 ? d.ld : BO;
@@ -1018,7 +1021,7 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_inner64(const BMat<LimbsU64>
     __shared__ __attribute__((aligned(16))) int8_t sB[NL64][TN64 * G_ROW];
     int mi_;
     int64_t tm, tn;
-    if (!batch_tile(ts.pre, ts.mts, ts.B, ts.group, &mi_, &tm, &tn)) return;
+    if (!batch_tile(ts, &mi_, &tm, &tn)) return;
     const BMat<LimbsU64>& d = D[mi_];
     const BlockState<uint64_t>* bs = d.bs;
     const int k = bs->k;
@@ -1247,6 +1250,10 @@ class Engine {
             if (h_buf[k]) (void)hipHostFree(h_buf[k]);
         }
         if (ev_ready) (void)hipEventDestroy(ev_ready);
+        for (int i = 0; i < NEV; ++i) {
+            if (ev_f[i]) (void)hipEventDestroy(ev_f[i]);
+            if (ev_c[i]) (void)hipEventDestroy(ev_c[i]);
+        }
         if (so) (void)hipStreamDestroy(so);
     }
     // One outer panel for every unfinished item in `items`.  The inner phase
@@ -1281,12 +1288,24 @@ class Engine {
     // of the columns beyond it (composite updates only)
     int upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out);
     int trailing(const Staged& t, Profiler* prof, int64_t tag);
+    // A trailing part beyond the next panel, queued to run beside a later
+    // step's inner phase (`due`), launched in tile slices.
+    struct Bulk {
+        Staged t;
+        int64_t next;  // tiles launched so far
+        int64_t due;
+    };
+    int launch_slice(Bulk& b, int64_t ntiles, Profiler* prof, int64_t tag);  // ntiles < 0: the rest
+    int flush_due(int64_t upto, Profiler* prof, int64_t tag);                  // every part due by `upto`
     int ensure_stream();
     hipStream_t so = nullptr;
+    int64_t so_reserve = -1;        // CUs in 32 masked off `so` when it was created
     hipEvent_t ev_ready = nullptr;  // on `so`: next panel's columns updated, panel buffer free
     bool ev_pending = false;
-    Staged pend{};                  // second half of the last composite update, not yet queued
-    bool has_pend = false;
+    std::vector<Bulk> bulk;         // queued trailing parts, in due order
+    int64_t step_no = 0;
+    static constexpr int NEV = 16;  // slices per step (inner groups beyond this run unsliced)
+    hipEvent_t ev_f[NEV] = {}, ev_c[NEV] = {};  // inner step started (on s) / its slice done (on so)
     int parity = 0;
     void* d_buf[NSLOT] = {};
     void* h_buf[NSLOT] = {};
@@ -1295,19 +1314,77 @@ class Engine {
 
 template <class LT>
 int Engine<LT>::ensure_stream() {
-    if (so) return LINAC_OK;
-    LINAC_HIP(hipStreamCreateWithFlags(&so, hipStreamNonBlocking));
-    LINAC_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    int64_t r = tuning().reserve_cus;
+    if (so && so_reserve == r) return LINAC_OK;
+    if (so) {  // the knob changed: `so` is idle between solves (drain())
+        LINAC_HIP(hipStreamSynchronize(so));
+        LINAC_HIP(hipStreamDestroy(so));
+        so = nullptr;
+    }
+    int dev = 0, ncu = 0;
+    LINAC_HIP(hipGetDevice(&dev));
+    LINAC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (ncu % 32) r = 0;
+    if (r == 0) {
+        LINAC_HIP(hipStreamCreateWithFlags(&so, hipStreamNonBlocking));
+    } else {
+        // Mask bits i with i % 32 < r off: r * ncu / 32 CUs, fewer than any
+        // XCD holds, so no XCD is left without CUs under any bit order.
+        std::vector<uint32_t> mask((size_t)ncu / 32, ~0u << r);
+        LINAC_HIP(hipExtStreamCreateWithCUMask(&so, (uint32_t)mask.size(), mask.data()));
+    }
+    so_reserve = tuning().reserve_cus;
+    if (!ev_ready) {
+        LINAC_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+        for (int i = 0; i < NEV; ++i) {
+            LINAC_HIP(hipEventCreateWithFlags(&ev_f[i], hipEventDisableTiming));
+            LINAC_HIP(hipEventCreateWithFlags(&ev_c[i], hipEventDisableTiming));
+        }
+    }
     return LINAC_OK;
 }
 
 template <class LT>
 int Engine<LT>::drain() {
-    if (has_pend) {
-        has_pend = false;
-        LINAC_TRY(trailing(pend, nullptr, 0));
-    }
+    LINAC_TRY(flush_due(INT64_MAX, nullptr, 0));
     if (so) LINAC_HIP(hipStreamSynchronize(so));
+    return LINAC_OK;
+}
+
+// Tiles [next, next + ntiles) of a queued part's trailing GEMM on `so` (its
+// B operand planes before the first slice).
+template <class LT>
+int Engine<LT>::launch_slice(Bulk& b, int64_t ntiles, Profiler* prof, int64_t tag) {
+    const Staged& t = b.t;
+    const int64_t rest = t.main_tiles - b.next;
+    if (rest <= 0) return LINAC_OK;
+    const int64_t n = ntiles < 0 ? rest : std::min(rest, ntiles);
+    if (b.next == 0)
+        k_make_rhs<LT><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(t.cols, 64)), 2 * BO / 64,
+                              (unsigned)t.inner.B), 256, 0, so>>>(t.desc, 0);
+    TileSched ts = t.main;
+    ts.t0 = b.next;
+    ts.t1 = b.next + n;
+    const double frac = (double)n / (double)t.main_tiles;
+    if (prof) LINAC_TRY(prof->sample_begin(so, tag, t.bytes * frac, t.macs * frac));
+    const unsigned grid = (unsigned)(ceil_div(n, NUM_XCD) * NUM_XCD);
+    if constexpr (std::is_same<LT, LimbsU64>::value) k_gemm_outer64<0><<<grid, G_T, 0, so>>>(t.desc, ts);
+    else k_gemm_outer<0><<<grid, G_T, 0, so>>>(t.desc, ts);
+    if (prof) LINAC_TRY(prof->sample_end(so));
+    LINAC_HIP(hipGetLastError());
+    b.next += n;
+    return LINAC_OK;
+}
+
+template <class LT>
+int Engine<LT>::flush_due(int64_t upto, Profiler* prof, int64_t tag) {
+    size_t done = 0;
+    for (auto& b : bulk) {
+        if (b.due > upto) break;
+        LINAC_TRY(launch_slice(b, -1, prof, tag));
+        ++done;
+    }
+    bulk.erase(bulk.begin(), bulk.begin() + done);
     return LINAC_OK;
 }
 
@@ -1411,9 +1488,9 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
     }
     out->desc = (BM*)dv;
     const int grp = (int)tuning().gemm_group;
-    out->inner = TileSched{ds, ds + (B + 1), (int)B, grp};
-    out->wq = TileSched{ds + (B + 1) + B, ds + 2 * (B + 1) + B, (int)B, grp};
-    out->main = TileSched{ds + 2 * (B + 1) + 2 * B, ds + 3 * (B + 1) + 2 * B, (int)B, grp};
+    out->inner = TileSched{ds, ds + (B + 1), (int)B, grp, 0, -1};
+    out->wq = TileSched{ds + (B + 1) + B, ds + 2 * (B + 1) + B, (int)B, grp, 0, -1};
+    out->main = TileSched{ds + 2 * (B + 1) + 2 * B, ds + 3 * (B + 1) + 2 * B, (int)B, grp, 0, -1};
     out->inner_tiles = pa[B];
     out->wq_tiles = pb[B];
     out->main_tiles = pc[B];
@@ -1468,9 +1545,32 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     k_block_state_init<LT><<<B, 1, 0, s>>>(in.desc, first_step ? 1 : 0);
     first_step = false;
     const unsigned mu_grid = (unsigned)(MU_BLOCKS + std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)));
-    auto group = [&]() -> int {
+    // Trailing parts due now run in slices, one beside each inner step's
+    // factorisation and U; the step's inner GEMM, which needs the whole GPU,
+    // waits for its slice.
+    const int64_t tag = act[0]->d.J;
+    int64_t due_tiles = 0;
+    for (auto& b : bulk)
+        if (b.due <= step_no) due_tiles += b.t.main_tiles - b.next;
+    const bool slicing = due_tiles > 0 && tuning().lookahead != 0 && tuning().timeslice != 0;
+    const int64_t per_slice = slicing ? ceil_div(due_tiles, std::min<int64_t>(ninner, NEV)) : 0;
+    auto group = [&](int64_t i) -> int {
+        const bool sl = slicing && i < NEV;
+        if (sl) {
+            LINAC_HIP(hipEventRecord(ev_f[i], s));
+            LINAC_HIP(hipStreamWaitEvent(so, ev_f[i], 0));
+            int64_t want = per_slice;
+            for (auto& b : bulk) {
+                if (want <= 0 || b.due > step_no) break;
+                const int64_t before = b.next;
+                LINAC_TRY(launch_slice(b, want, prof, tag));
+                want -= b.next - before;
+            }
+            LINAC_HIP(hipEventRecord(ev_c[i], so));
+        }
         k_inner_factor<LT><<<B, IF_T, 0, s>>>(in.desc, epoch);
         k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(in.desc);
+        if (sl) LINAC_HIP(hipStreamWaitEvent(s, ev_c[i], 0));
         const unsigned grid = (unsigned)(ceil_div(in.inner_tiles, NUM_XCD) * NUM_XCD);
         if constexpr (wide) k_gemm_inner64<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch);
         else k_gemm_inner<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch);
@@ -1478,7 +1578,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         LINAC_HIP(hipGetLastError());
         return LINAC_OK;
     };
-    for (int64_t i = 0; i < ninner; ++i) LINAC_TRY(group());
+    for (int64_t i = 0; i < ninner; ++i) LINAC_TRY(group(i));
     // one synchronisation per step; extra groups only for rank-deficient panels
     std::vector<ItemStatus> st(B);
     for (int64_t guard = 0;; ++guard) {
@@ -1490,8 +1590,11 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
             if (st[m].c0_next < st[m].bcur) behind = std::max<int64_t>(behind, ceil_div(st[m].bcur - st[m].c0_next, BI));
         if (behind == 0) break;
         if (guard > (int64_t)BO * (BI + 1)) return fail(LINAC_ERR_DEVICE, "blocked elimination did not converge");
-        for (int64_t i = 0; i < behind; ++i) LINAC_TRY(group());
+        for (int64_t i = 0; i < behind; ++i) LINAC_TRY(group(NEV));
     }
+    // whatever of the parts due by now is left (rank-deficient panels add
+    // groups; slices round), before this step's own trailing update
+    LINAC_TRY(flush_due(step_no, prof, tag));
     // --- trailing update on so: first the next panel's columns (then the
     // panel goes back to A and the next inner phase may start), then the rest.
     // The columns beyond the next panel are either updated together with the
@@ -1530,30 +1633,25 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         k_make_lhs<LT><<<dim3(gs_grid(maxN * BO, 16384), B), 256, 0, so>>>(t1.desc, 1, epoch);
         LINAC_HIP(hipGetLastError());
     }
-    const int64_t tag = act[0]->d.J;
     LINAC_TRY(trailing(t1, prof, tag));
     int64_t maxpanel = 1;
     for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
     k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, so>>>(t1.desc);
     LINAC_HIP(hipGetLastError());
-    // The next inner phase waits only for the above.  Behind it on `so`: the
-    // second half of the previous composite update (its columns lie beyond the
-    // next panel's), then this step's first half; this step's second half is
-    // queued at the next step, behind that step's own next-panel update.
+    // The next inner phase waits only for the above.  The composite update's
+    // first half is due beside the next step's inner phase, its second half
+    // beside the one after (its columns lie beyond the next panel's, which
+    // that step's own update needs first).  Without time slices each part
+    // runs whole, queued behind the next-panel update of the step before
+    // its due step; without look-ahead everything runs now.
     const bool lookahead = tuning().lookahead != 0;
     if (lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));
-    if (has_pend) {
-        has_pend = false;
-        LINAC_TRY(trailing(pend, prof, tag));
-    }
-    LINAC_TRY(trailing(t2, prof, tag));
-    if (lookahead && t3.main_tiles > 0) {
-        pend = t3;
-        has_pend = true;
-    } else {
-        LINAC_TRY(trailing(t3, prof, tag));
-    }
+    if (t2.main_tiles > 0) bulk.push_back(Bulk{t2, 0, step_no + 1});
+    if (t3.main_tiles > 0) bulk.push_back(Bulk{t3, 0, step_no + 2});
+    if (!lookahead) LINAC_TRY(flush_due(INT64_MAX, prof, tag));
+    else if (tuning().timeslice == 0) LINAC_TRY(flush_due(step_no + 1, prof, tag));
     if (!lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));  // the next inner phase waits for all of it
+    ++step_no;
     ev_pending = true;
     for (auto* it : act) {
         it->d.kA = it->d.save_w ? it->d.q0 : 0;  // deferred to the next step
@@ -1787,7 +1885,7 @@ static int gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda
         k_planes_rows<<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(n, 64)), (unsigned)(kp[c] / 64)), 256, 0, s>>>(
             B, ldb, b_rows ? d_brows : nullptr, t0, kc, (int)kp[c], n, p, planesB + offB[c], sB[c]);
     }
-    const TileSched ts{d_sched, d_sched + 2, 1, (int)tuning().gemm_group};
+    const TileSched ts{d_sched, d_sched + 2, 1, (int)tuning().gemm_group, 0, -1};
     for (int64_t c = 0; c < nch; ++c)
         k_gemm_outer<0><<<(unsigned)(ceil_div(ntiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc + c, ts);
     LINAC_HIP(hipGetLastError());

@@ -109,6 +109,8 @@ struct Tuning {
     int64_t float_blocked_min_rows = 256, float_blocked_min_cols = 128;
     int64_t gemm_group = 8;
     int64_t lookahead = 1;
+    int64_t timeslice = 0;
+    int64_t reserve_cus = 0;
 };
 Tuning& tuning();
 

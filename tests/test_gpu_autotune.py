@@ -23,8 +23,8 @@ def test_quick_tune_sets_and_saves(gpu, tmp_path):
         autotune.apply(before)
 
 
-@pytest.mark.parametrize("group, lookahead", [(1, 0), (4, 1), (16, 0), (64, 1)])
-def test_knobs_keep_gf_results_exact(gpu, group, lookahead):
+@pytest.mark.parametrize("group, lookahead, reserve", [(1, 0, 0), (4, 1, 3), (16, 0, 1), (64, 1, 2), (8, 1, 0)])
+def test_knobs_keep_gf_results_exact(gpu, group, lookahead, reserve):
     from linac_amd import autotune
     before = autotune.current()
     p = 2 ** 31 - 1
@@ -32,8 +32,34 @@ def test_knobs_keep_gf_results_exact(gpu, group, lookahead):
     try:
         autotune.set_knob("gemm_group", group)
         autotune.set_knob("lookahead", lookahead)
+        autotune.set_knob("reserve_cus", reserve)
         R, piv, rank = gpu.rref_with_pivots(A, p, method="blocked")
     finally:
         autotune.apply(before)
     Ro, po = crref.rref_modp(A, p)
     assert piv == po and numpy.array_equal(R, Ro)
+
+
+@pytest.mark.parametrize("n, rank, p", [(1200, 1000, 2 ** 31 - 1), (1200, None, 4278124237),
+                                        (1100, None, 2 ** 61 - 1)])
+def test_time_sliced_trailing_parts_stay_exact(gpu, n, rank, p):
+    """With time slices the trailing update beyond the next panel runs in tile
+    windows beside later inner steps; every window is applied once, in order."""
+    from linac_amd import autotune
+    before = autotune.current()
+    rs = numpy.random.RandomState(23)
+    if rank is not None:
+        A = modp.low_rank(n, n + 64, rank, p, 23)
+    else:  # uniform rows, full rank with overwhelming probability
+        A = rs.randint(0, 2 ** 62, size=(n, n + 64), dtype=numpy.int64).astype(numpy.uint64) % numpy.uint64(p)
+    try:
+        autotune.set_knob("timeslice", 1)
+        autotune.set_knob("lookahead", 1)
+        R, piv, r = gpu.rref_with_pivots(A, p, method="blocked")
+        big = rs.randint(0, p, size=(3000, 3000), dtype=numpy.int64)
+        Rb = gpu.cuda_row_reduce(big, p, method="blocked")
+    finally:
+        autotune.apply(before)
+    Ro, po = crref.rref_modp(A, p)
+    assert piv == po and r == len(po) and numpy.array_equal(R, Ro)
+    assert numpy.array_equal(Rb, numpy.identity(3000, dtype=Rb.dtype))
