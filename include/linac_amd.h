@@ -87,10 +87,7 @@ int linac_profile_last(double* out, int n);
  *   "timeslice" 0: with look-ahead, each part of the trailing update beyond
  *       the next panel runs as one launch beside the next inner steps (1: in
  *       slices beside each inner step's factorisation, the inner step's GEMM
- *       waiting for its slice);
- *   "reserve_cus" 0: CUs in every 32 that the trailing-update stream leaves
- *       to the inner steps (a CU mask on that stream), 0..3; at most 3 in 32
- *       keeps every XCD populated whatever the mask's bit order.
+ *       waiting for its slice).
  * Unknown keys and out-of-range values return LINAC_ERR_INVALID. */
 int linac_tune_set(const char* key, int64_t value);
 int linac_tune_get(const char* key, int64_t* value);

@@ -23,7 +23,7 @@ import numpy
 from . import _lib
 
 KNOBS = ("gf_blocked_min_rows", "gf_blocked_min_cols", "float_blocked_min_rows", "float_blocked_min_cols",
-         "gemm_group", "lookahead", "timeslice", "reserve_cus")
+         "gemm_group", "lookahead", "timeslice")
 
 
 def get(key):

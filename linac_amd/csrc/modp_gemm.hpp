@@ -42,6 +42,12 @@ template <class V>
 __device__ __forceinline__ V gld(const void* p) {
     return *(const __attribute__((address_space(1))) V*)p;
 }
+// Stores through the global address space (a generic store would make the
+// next wait for a scalar load wait for it as well).
+template <class V>
+__device__ __forceinline__ void gst(void* p, V v) {
+    *(__attribute__((address_space(1))) V*)p = v;
+}
 
 __device__ __forceinline__ int32_t centred(uint32_t v, uint32_t p) {
     return v > (p - 1) / 2 ? (int32_t)((int64_t)v - (int64_t)p) : (int32_t)v;

@@ -195,6 +195,7 @@ __global__ void k_block_state_init(const BMat<LT>* __restrict__ D, int reset_tim
 constexpr int IF_T = 1024;  // threads of the inner factorisation: 16 waves, two candidate rows each
 static_assert(IF_T / 64 * 2 == CI, "each wave owns two candidate rows");
 static_assert(BI + CI == 64, "one augmented column per lane");
+static_assert((CI & (CI - 1)) == 0 && CI <= 32, "pivot slots index with a mask; the inner GEMM holds k <= 32 in its first two lane groups");
 
 // Candidate rows live in LDS as [panel columns | identity]; wave w owns rows
 // 2w and 2w+1.  Per column every wave ballots the column itself (all waves
@@ -405,16 +406,17 @@ __global__ __launch_bounds__(IF_T, LT::IF_WAVES) void k_inner_factor(const BMat<
 // U = Apiv^{-1} X[R, :] (k x XLD, Montgomery inverse times plain rows gives plain
 // values) and the limb planes of both operands of the inner GEMM:
 //   planesU[l][n][t] = digit l of U[t, n]                (t < k, zero up to KPI)
-//   planesA[l][r][t] = digit l of X[r, c0 + Cp[t]]       (every row r)
-// The GEMM overwrites the pivot columns X[:, c0 + Cp] in place, so its left
-// operand must be captured here, before any of its workgroups runs.
-// Workgroups [0, MU_BLOCKS) build U (4 columns each), the rest planesA
-// (MU_ROWS rows each, grid-strided).
+//   planesA[l][r][t] = digit l of X[r, c0 + Cp[t]]       (every row r; 64-bit only)
+// The 64-bit GEMM overwrites the pivot columns X[:, c0 + Cp] in place, so its
+// left operand must be captured here, before any of its workgroups runs (the
+// 32-bit GEMM captures its own, per row stripe, and for it the pivot rows
+// X[R, :] get U here).  Workgroups [0, MU_BLOCKS) build U (4 columns each),
+// the rest planesA (MU_ROWS rows each, grid-strided).
 constexpr int MU_BLOCKS = XLD / 4;
 constexpr int MU_ROWS = 16;
 
 template <class LT>
-__device__ static void make_u_body(const typename LT::T* __restrict__ X, int64_t N, typename LT::F f,
+__device__ static void make_u_body(typename LT::T* __restrict__ X, int64_t N, typename LT::F f,
                                    const BlockState<typename LT::T>* __restrict__ bs, typename LT::T* __restrict__ U,
                                    int8_t* __restrict__ planesU, int64_t strideU,
                                    int8_t* __restrict__ planesA, int64_t strideA) {
@@ -459,6 +461,13 @@ __device__ static void make_u_body(const typename LT::T* __restrict__ X, int64_t
     if (t < k) {
         for (int s = 0; s < k; ++s) u = f.add(u, f.mont(s_inv[t * CI + s], X[s_R[s] * XLD + n]));
         U[t * XLD + n] = u;
+    }
+    if constexpr (std::is_same<LT, LimbsU32>::value) {
+        // the pivot rows' new values go straight into the panel (the 32-bit
+        // inner GEMM leaves those rows alone); every read of column n above
+        // comes first
+        __syncthreads();
+        if (t < k) X[s_R[t] * XLD + n] = u;
     }
     int8_t d[NL] = {};
     if (t < k) balanced_digits(centred(u, f.p), d);
@@ -781,16 +790,21 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
 }
 
 // Inner-step GEMM on the panel buffer:  X[r, :] -= X[r, c0 + Cp] U  for rows
-// that are not pivots of this step, X[r, :] = U[slot(r), :] for those that
-// are.  K = 64 (k <= 32 pivots, zero padded), one LDS stage.  Both operands
-// come as limb planes from k_make_u (A: X[:, c0 + Cp] captured before this
-// launch, since its tiles overwrite those columns).  Tiles outside the live
-// column window [c0, BO + q0) only meet zeros and are skipped.  The kernel is
-// latency-bound (one K stage per tile): four workgroups per CU (a 20-byte
-// register spill in the epilogue) measured faster than three without.
-__global__ __launch_bounds__(G_T, 4) void k_gemm_inner(const BMat<LimbsU32>* __restrict__ D, TileSched ts, int epoch) {
-    __shared__ __attribute__((aligned(16))) int8_t sA[4][G_M * G_ROW];
-    __shared__ __attribute__((aligned(16))) int8_t sB[4][G_N * G_ROW];
+// that are not pivots of this step (k_make_u has set X[R[t], :] = U[t, :] for
+// those that are).  K = 64 (k <= 32 pivots, zero padded).  One workgroup owns a 64-row
+// stripe and walks the live column window [c0, BO + q0) in 64-column tiles
+// (columns outside it only meet zeros).  Owning the whole stripe lets it
+// capture its own left operand, the digits of X[r, c0 + Cp], before any of its
+// waves writes, and no other workgroup reads its rows.  Fragments come
+// straight from global memory (no LDS): the launch is one workgroup per
+// stripe, which matters because the trailing GEMM on the other stream holds
+// every register of a CU and each workgroup here waits for one of its slots
+// to free.  With few stripes (small matrices) `split` workgroups share a
+// stripe, taking every split-th tile, and read the left operand from planes.  A tile's panel entries are in flight during its MFMAs, the next
+// tile's right operand during its epilogue.
+__global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat<LimbsU32>* __restrict__ D, TileSched ts, int epoch,
+                                                     int split) {
+    __shared__ int s_cp[CI];
     int mi_;
     int64_t tm, tn;
     if (!batch_tile(ts, &mi_, &tm, &tn)) return;
@@ -798,77 +812,119 @@ __global__ __launch_bounds__(G_T, 4) void k_gemm_inner(const BMat<LimbsU32>* __r
     const BlockState<uint32_t>* bs = d.bs;
     const int k = bs->k;
     if (k == 0) return;
-    const int64_t m_base = tm * G_M, n_base = tn * G_N;
     const int c0 = bs->c0_cur;
-    if (n_base + G_N <= c0 || n_base >= BO + bs->q0) return;
-    uint32_t* X = d.X;
-    const int64_t N = d.N;
+    const int t_lo = c0 / G_N + (int)tn, t_hi = (BO + bs->q0 + G_N - 1) / G_N;
+    if (t_lo >= t_hi) return;
+    // addresses: a uniform stripe base plus 32-bit lane offsets
+    const int64_t m_base = tm * G_M;
+    const int rows = (int)min((int64_t)G_M, d.N - m_base);
+    uint32_t* Xs = d.X + m_base * XLD;
+    const int8_t* Ub = d.planesUi;
+    const uint32_t p = d.f.p;
+    const double pd = d.pd, pinv = d.pinv;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = tid + q * G_T;
-        const int l = c >> 8, rem = c & 255, col = rem >> 2, qu = rem & 3;
-        *reinterpret_cast<v4i*>(&sB[l][lds_at(col, qu)]) =
-            *reinterpret_cast<const v4i*>(d.planesUi + l * (int64_t)XLD * KPI + (n_base + col) * KPI + qu * 16);
-        const int64_t row = m_base + col;
-        v4i w = {0, 0, 0, 0};
-        if (row < N) w = *reinterpret_cast<const v4i*>(d.planesAi + l * d.strideAi + row * KPI + qu * 16);
-        *reinterpret_cast<v4i*>(&sA[l][lds_at(col, qu)]) = w;
-    }
+    if (tid < CI) s_cp[tid] = tid < k ? bs->Cp[tid] : 0;
     __syncthreads();
     const v4i zero = {0, 0, 0, 0};
-    v4i acc[7][2][2];
+    // left operand: byte j of af[l][mi] = digit l of X[row, c0 + Cp[t]], t = 16 g + j
+    // (zero for t >= k; those lanes read column c0, a valid address).  With a
+    // split stripe another workgroup may already be writing those columns, so
+    // the digits come from the planes k_make_u captured before the launch.
+    v4i af[4][2];
+    if (split > 1) {
 #pragma unroll
-    for (int s = 0; s < 7; ++s)
+        for (int mi = 0; mi < 2; ++mi) {
+            const int64_t row = m_base + min(wm * 32 + mi * 16 + r16, rows - 1);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int l = 0; l < 4; ++l) af[l][mi] = gld<v4i>(d.planesAi + l * d.strideAi + row * KPI + g * 16);
+        }
+    } else {
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
-    v4i bf[4][2];
+        for (int mi = 0; mi < 2; ++mi) {
+            const uint32_t ra = (uint32_t)min(wm * 32 + mi * 16 + r16, rows - 1) * XLD + c0;
+            uint32_t v[16];
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
+            for (int j = 0; j < 16; ++j) v[j] = gld<uint32_t>(Xs + (ra + s_cp[(16 * g + j) & (CI - 1)]));
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][lds_at(wn * 32 + ni * 16 + r16, g)]);
+            for (int l = 0; l < 4; ++l) af[l][mi] = zero;
 #pragma unroll
-    for (int l1 = 0; l1 < 4; ++l1) {
-        v4i af[2];
+            for (int j = 0; j < 16; ++j) {
+                int8_t a[4];
+                balanced_digits(16 * g + j < k ? centred(v[j], p) : 0, a);
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
-            af[mi] = *reinterpret_cast<const v4i*>(&sA[l1][lds_at(wm * 32 + mi * 16 + r16, g)]);
-#pragma unroll
-        for (int l2 = 0; l2 < 4; ++l2)
-#pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < 2; ++ni)
-                    acc[l1 + l2][mi][ni] =
-                        __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
+                for (int l = 0; l < 4; ++l) af[l][mi][j >> 2] |= (int32_t)(uint8_t)a[l] << (8 * (j & 3));
+            }
+        }
     }
+    // per output row of this lane: its offset in the stripe, and whether it is
+    // left alone (beyond the matrix, or a pivot row of this step: k_make_u has
+    // written U into those)
+    uint32_t ro[2][4];
+    unsigned skip = 0;
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-            if (row >= N) continue;
-            const bool pivot_row = d.epoch_of[row] == epoch;
-            const int slot = pivot_row ? d.slot_of[row] : 0;
+            const int rl = wm * 32 + mi * 16 + 4 * g + i, rc = min(rl, rows - 1);
+            ro[mi][i] = (uint32_t)rc * XLD + wn * 32 + r16;
+            const bool piv = gld<int32_t>(d.epoch_of + (m_base + rc)) == epoch;
+            skip |= (unsigned)(rl >= rows || piv) << (4 * mi + i);
+        }
+    const uint32_t bo = (uint32_t)(wn * 32 + r16) * KPI + g * 16;
+    v4i bf[4][2];
+    uint32_t cv[2][2][4];
+    auto bload = [&](int t) {
 #pragma unroll
-            for (int ni = 0; ni < 2; ++ni) {
-                const int64_t col = n_base + wn * 32 + ni * 16 + r16;
-                uint32_t* cp = X + row * XLD + col;
-                if (pivot_row) {
-                    *cp = d.U[slot * XLD + col];
-                } else {
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+                bf[l][ni] = gld<v4i>(Ub + (bo + (uint32_t)(l * XLD * KPI + (t * G_N + ni * 16) * KPI)));
+    };
+    bload(t_lo);
+    __syncthreads();  // every wave has its left operand before any wave writes the stripe
+    for (int t = t_lo; t < t_hi; t += split) {
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)  // in flight during the MFMAs
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cv[mi][ni][i] = gld<uint32_t>(Xs + (ro[mi][i] + t * G_N + ni * 16));
+        __builtin_amdgcn_sched_barrier(0);  // keep those loads ahead of the MFMAs
+        v4i acc[7][2][2];
+#pragma unroll
+        for (int s = 0; s < 7; ++s)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[s][i][j] = zero;
+#pragma unroll
+        for (int l1 = 0; l1 < 4; ++l1)
+#pragma unroll
+            for (int l2 = 0; l2 < 4; ++l2)
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < 2; ++ni)
+                        acc[l1 + l2][mi][ni] =
+                            __builtin_amdgcn_mfma_i32_16x16x64_i8(af[l1][mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        bload(min(t + split, t_hi - 1));  // the next tile's right operand, in flight during the epilogue
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if ((skip >> (4 * mi + i)) & 1u) continue;
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni) {
                     int32_t sv[7];
 #pragma unroll
                     for (int q = 0; q < 7; ++q) sv[q] = acc[q][mi][ni][i];
-                    *cp = fold_sub_k32(*cp, sv, d.pd, d.pinv);
+                    gst<uint32_t>(Xs + (ro[mi][i] + t * G_N + ni * 16), fold_sub_k32(cv[mi][ni][i], sv, pd, pinv));
                 }
             }
-        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1279,6 +1335,7 @@ class Engine {
         BM* desc;
         TileSched inner, wq, main;  // inner GEMM, composite W GEMM, trailing update
         int64_t inner_tiles, wq_tiles, main_tiles;
+        int inner_split;            // 32-bit inner GEMM: workgroups per row stripe
         ItemStatus* status;
         double macs, bytes;         // algorithmic work of the trailing update (profiler)
         int64_t cols;               // widest column range of the part
@@ -1299,7 +1356,6 @@ class Engine {
     int flush_due(int64_t upto, Profiler* prof, int64_t tag);                  // every part due by `upto`
     int ensure_stream();
     hipStream_t so = nullptr;
-    int64_t so_reserve = -1;        // CUs in 32 masked off `so` when it was created
     hipEvent_t ev_ready = nullptr;  // on `so`: next panel's columns updated, panel buffer free
     bool ev_pending = false;
     std::vector<Bulk> bulk;         // queued trailing parts, in due order
@@ -1314,32 +1370,12 @@ class Engine {
 
 template <class LT>
 int Engine<LT>::ensure_stream() {
-    int64_t r = tuning().reserve_cus;
-    if (so && so_reserve == r) return LINAC_OK;
-    if (so) {  // the knob changed: `so` is idle between solves (drain())
-        LINAC_HIP(hipStreamSynchronize(so));
-        LINAC_HIP(hipStreamDestroy(so));
-        so = nullptr;
-    }
-    int dev = 0, ncu = 0;
-    LINAC_HIP(hipGetDevice(&dev));
-    LINAC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    if (ncu % 32) r = 0;
-    if (r == 0) {
-        LINAC_HIP(hipStreamCreateWithFlags(&so, hipStreamNonBlocking));
-    } else {
-        // Mask bits i with i % 32 < r off: r * ncu / 32 CUs, fewer than any
-        // XCD holds, so no XCD is left without CUs under any bit order.
-        std::vector<uint32_t> mask((size_t)ncu / 32, ~0u << r);
-        LINAC_HIP(hipExtStreamCreateWithCUMask(&so, (uint32_t)mask.size(), mask.data()));
-    }
-    so_reserve = tuning().reserve_cus;
-    if (!ev_ready) {
-        LINAC_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
-        for (int i = 0; i < NEV; ++i) {
-            LINAC_HIP(hipEventCreateWithFlags(&ev_f[i], hipEventDisableTiming));
-            LINAC_HIP(hipEventCreateWithFlags(&ev_c[i], hipEventDisableTiming));
-        }
+    if (so) return LINAC_OK;
+    LINAC_HIP(hipStreamCreateWithFlags(&so, hipStreamNonBlocking));
+    LINAC_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    for (int i = 0; i < NEV; ++i) {
+        LINAC_HIP(hipEventCreateWithFlags(&ev_f[i], hipEventDisableTiming));
+        LINAC_HIP(hipEventCreateWithFlags(&ev_c[i], hipEventDisableTiming));
     }
     return LINAC_OK;
 }
@@ -1468,6 +1504,12 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
         hd[m] = d;
     }
     constexpr int64_t TN = std::is_same<LT, LimbsU64>::value ? TN64 : G_N;  // GEMM tile width
+    // 32-bit inner GEMM: one workgroup per 64-row stripe, or (few stripes)
+    // up to four that share a stripe's column tiles
+    int64_t stripes = 0;
+    for (size_t m = 0; m < B; ++m) stripes += ceil_div(hd[m].N, G_M);
+    const int split = (int)std::max<int64_t>(1, std::min<int64_t>(4, 256 / std::max<int64_t>(1, stripes)));
+    out->inner_split = split;
     int64_t* pa = hs;
     int64_t* ma = pa + (B + 1);
     int64_t* pb = ma + B;
@@ -1478,7 +1520,8 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
     for (size_t m = 0; m < B; ++m) {
         const BM& d = hd[m];
         ma[m] = ceil_div(d.N, G_M);
-        pa[m + 1] = pa[m] + (part == 0 ? ma[m] * (XLD / TN) : 0);
+        // inner GEMM: 64-bit storage tiles the window, 32-bit walks it per row stripe
+        pa[m + 1] = pa[m] + (part == 0 ? ma[m] * (std::is_same<LT, LimbsU64>::value ? XLD / TN : split) : 0);
         const bool live = part != 0 && d.Kq > 0 && d.nT > 0;
         const bool wlive = part == 1 && d.kA > 0 && d.q0 > 0 && d.nT > 0;
         mb[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
@@ -1544,7 +1587,9 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     k_panel_load<LT><<<dim3(gs_grid(maxNX, 8192), B), 256, 0, s>>>(in.desc);
     k_block_state_init<LT><<<B, 1, 0, s>>>(in.desc, first_step ? 1 : 0);
     first_step = false;
-    const unsigned mu_grid = (unsigned)(MU_BLOCKS + std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)));
+    // (the 32-bit inner GEMM captures its left operand itself)
+    const bool planes_a = wide || in.inner_split > 1;
+    const unsigned mu_grid = (unsigned)(MU_BLOCKS + (planes_a ? std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)) : 0));
     // Trailing parts due now run in slices, one beside each inner step's
     // factorisation and U; the step's inner GEMM, which needs the whole GPU,
     // waits for its slice.
@@ -1573,7 +1618,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         if (sl) LINAC_HIP(hipStreamWaitEvent(s, ev_c[i], 0));
         const unsigned grid = (unsigned)(ceil_div(in.inner_tiles, NUM_XCD) * NUM_XCD);
         if constexpr (wide) k_gemm_inner64<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch);
-        else k_gemm_inner<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch);
+        else k_gemm_inner<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch, in.inner_split);
         ++epoch;
         LINAC_HIP(hipGetLastError());
         return LINAC_OK;

@@ -275,8 +275,7 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"float_blocked_min_cols", &t.float_blocked_min_cols, 1, INT64_MAX},
                  {"gemm_group", &t.gemm_group, 1, 64},
                  {"lookahead", &t.lookahead, 0, 1},
-                 {"timeslice", &t.timeslice, 0, 1},
-                 {"reserve_cus", &t.reserve_cus, 0, 3}};
+                 {"timeslice", &t.timeslice, 0, 1}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             if (value < k.lo || value > k.hi)
@@ -297,7 +296,7 @@ int linac_tune_get(const char* key, int64_t* value) {
     } knobs[] = {{"gf_blocked_min_rows", t.gf_blocked_min_rows}, {"gf_blocked_min_cols", t.gf_blocked_min_cols},
                  {"float_blocked_min_rows", t.float_blocked_min_rows},
                  {"float_blocked_min_cols", t.float_blocked_min_cols}, {"gemm_group", t.gemm_group},
-                 {"lookahead", t.lookahead}, {"timeslice", t.timeslice}, {"reserve_cus", t.reserve_cus}};
+                 {"lookahead", t.lookahead}, {"timeslice", t.timeslice}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             *value = k.v;

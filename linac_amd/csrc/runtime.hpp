@@ -110,7 +110,6 @@ struct Tuning {
     int64_t gemm_group = 8;
     int64_t lookahead = 1;
     int64_t timeslice = 0;
-    int64_t reserve_cus = 0;
 };
 Tuning& tuning();
 

@@ -145,10 +145,6 @@ def test_tuning_knobs_set_get_and_validation():
         assert autotune.get("timeslice") == 1
         with pytest.raises(ValueError):
             autotune.set_knob("timeslice", 2)
-        autotune.set_knob("reserve_cus", 3)
-        assert autotune.get("reserve_cus") == 3
-        with pytest.raises(ValueError):
-            autotune.set_knob("reserve_cus", 4)
         with pytest.raises(ValueError):
             autotune.set_knob("no_such_knob", 1)
         with pytest.raises(ValueError):

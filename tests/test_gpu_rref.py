@@ -119,6 +119,31 @@ def test_planted_solution_large(gpu, method):
     assert numpy.array_equal(R[:, :5000], numpy.eye(5000, dtype=numpy.uint32))
 
 
+@pytest.mark.parametrize("rows, cols, rank, dup, seed", [(8400, 520, 300, False, 41), (9000, 600, 600, True, 42)])
+def test_blocked_many_row_stripes(gpu, rows, cols, rank, dup, seed):
+    """Over 128 stripes of 64 rows: each inner GEMM workgroup owns a whole
+    stripe and captures its own left operand (fewer stripes share them)."""
+    p = 2 ** 31 - 1
+    rs = numpy.random.RandomState(seed)
+    if dup:  # every row twice, shuffled: half of each panel's candidates reduce to zero
+        half = modp.low_rank(rows // 2, cols, min(rank, rows // 2), p, seed)
+        A = numpy.concatenate([half, half], axis=0)[rs.permutation(rows)]
+    else:
+        A = modp.low_rank(rows, cols, rank, p, seed)
+    Rg, pg, rk = gpu.rref_with_pivots(A, p, method="blocked")
+    Ro, po = crref.rref_modp(A, p)
+    assert pg == po and rk == len(po)
+    assert numpy.array_equal(Rg, Ro)
+
+
+def test_blocked_planted_many_stripes(gpu):
+    p = 2 ** 31 - 1
+    Ab, X = modp.planted(9000, p, 7, extra=2)
+    R = _rref(gpu, Ab.astype(numpy.uint32), p, "blocked")
+    assert numpy.array_equal(R[:, 9000:].astype(numpy.int64), X)
+    assert numpy.array_equal(R[:, :9000], numpy.eye(9000, dtype=numpy.uint32))
+
+
 def test_u64_planted(gpu):
     p = 2 ** 61 - 1
     n = 700
