@@ -192,14 +192,18 @@ __global__ void k_block_state_init(const BMat<LT>* __restrict__ D, int reset_tim
 // inner factorisation: one workgroup per matrix
 // ---------------------------------------------------------------------------
 
-constexpr int IF_T = 1024;  // threads of the inner factorisation: 16 waves, two candidate rows each
-static_assert(IF_T / 64 * 2 == CI, "each wave owns two candidate rows");
+constexpr int IF_T = 1024;  // threads of the inner factorisation: 16 waves
+constexpr int IF_RPW = CI / (IF_T / 64);  // candidate rows per wave
+static_assert(IF_RPW * (IF_T / 64) == CI && IF_T >= CI, "waves split the candidate rows evenly");
 static_assert(BI + CI == 64, "one augmented column per lane");
 static_assert((CI & (CI - 1)) == 0 && CI <= 32, "pivot slots index with a mask; the inner GEMM holds k <= 32 in its first two lane groups");
 
 // Candidate rows live in LDS as [panel columns | identity]; wave w owns rows
-// 2w and 2w+1.  Per column every wave ballots the column itself (all waves
-// see the same data, so they agree on the pivot without a barrier), then
+// w and w + 16.  (Each column step is a chain of dependent LDS reads and
+// products: four waves with eight rows each took twice as long as sixteen
+// with two, which hide each other's latency.)  Per column every wave ballots
+// the column itself (all waves see the same data, so they agree on the pivot
+// without a barrier), then
 // updates its own rows with lane j handling column j.  Within a wave the
 // reads of a row precede its writes in program order, so one barrier per
 // column suffices.  No rows are swapped: pivots are recorded by their LDS
@@ -232,34 +236,45 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
 
     if (tid == 0) s_total = 0;
     __syncthreads();
-    // 1. the first CI (+1 to know whether more exist) candidate rows, in row order
-    for (int64_t base = st->first_unused; base < N; base += IF_T) {
-        const int64_t r = base + tid;
+    // 1. the first CI (+1 to know whether more exist) candidate rows, in row
+    //    order, probing SCAN rows at a time with the first SCAN / 64 waves
+    //    (larger probes measured slower: most steps need the first few dozen)
+    constexpr int SCAN = 256;
+    for (int64_t base = st->first_unused; base < N; base += SCAN) {
+        // unused row with a non-zero entry in the inner panel; the flag and the
+        // row are fetched together (rows past N read row N-1 and are dropped)
+        const int64_t r = base + tid, rc = r < N ? r : N - 1;
         bool fl = false;
-        if (r < N && !used[r]) {  // unused row with a non-zero entry in the inner panel
+        if (tid < SCAN) {
             constexpr int PER = 16 / (int)sizeof(T);  // elements per 16-byte vector
-            const Vec16<T>* row = reinterpret_cast<const Vec16<T>*>(X + r * XLD + c0);  // 128-byte aligned
+            const Vec16<T>* row = reinterpret_cast<const Vec16<T>*>(X + rc * XLD + c0);  // 128-byte aligned
+            const uint8_t u = gld<uint8_t>(used + rc);
             Vec16<T> v[BI / PER];
 #pragma unroll
-            for (int q = 0; q < BI / PER; ++q) v[q] = row[q];
+            for (int q = 0; q < BI / PER; ++q) {
+                const v4i raw = gld<v4i>(row + q);
+                __builtin_memcpy(&v[q], &raw, 16);
+            }
             T acc = 0;
 #pragma unroll
             for (int q = 0; q < BI / PER; ++q)
 #pragma unroll
                 for (int e = 0; e < PER; ++e) acc |= (PER * q + e < w ? v[q].v[e] : (T)0);
-            fl = acc != (T)0;
+            fl = r < N && !u && acc != (T)0;
         }
         const unsigned long long m = __ballot(fl);
-        if (lane == 0) s_wcnt[wid] = __popcll(m);
+        if (lane == 0 && tid < SCAN) s_wcnt[wid] = __popcll(m);
         __syncthreads();
-        int before = s_total;
-        for (int q = 0; q < wid; ++q) before += s_wcnt[q];
-        const int pos = before + __popcll(m & ((1ull << lane) - 1ull));
-        if (fl && pos <= CI) cand[pos] = r;
+        if (tid < SCAN) {
+            int before = s_total;
+            for (int q = 0; q < wid; ++q) before += s_wcnt[q];
+            const int pos = before + __popcll(m & ((1ull << lane) - 1ull));
+            if (fl && pos <= CI) cand[pos] = r;
+        }
         __syncthreads();
         if (tid == 0) {
             int sum = 0;
-            for (int q = 0; q < IF_T / 64; ++q) sum += s_wcnt[q];
+            for (int q = 0; q < SCAN / 64; ++q) sum += s_wcnt[q];
             s_total += sum;
         }
         __syncthreads();
@@ -280,10 +295,10 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
 
     // 2. [X[S, c0:c0+w] | I] in LDS; rows >= nC are zero and never pivot
     {
-        const int i = tid >> 6, j = lane;  // two passes of 16 rows
+        const int j = lane;
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-            const int ii = i + 16 * half;
+        for (int q = 0; q < IF_RPW; ++q) {
+            const int ii = wid + (IF_T / 64) * q;
             T v = 0;
             if (ii < nC) {
                 if (j < w) v = X[cand[ii] * XLD + c0 + j];
@@ -299,8 +314,17 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
     //    (rows stay non-zero multiples of the exact Gauss-Jordan rows)
     unsigned long long pmask = 0ull;  // LDS rows already used as pivots (same in every wave)
     int t = 0;
-    const int r0 = 2 * wid, r1 = 2 * wid + 1;
+    constexpr int NW = IF_T / 64;  // wave w owns rows w, w + NW, w + 2 NW, ...
     for (int c = 0; c < w && t < nC; ++c) {
+        // the wave's own rows are read beside the pivot search (all reads of
+        // them come before any write)
+        T av[IF_RPW], xv[IF_RPW];
+#pragma unroll
+        for (int q = 0; q < IF_RPW; ++q) {
+            const int r = wid + NW * q;
+            av[q] = aug[r][c];
+            xv[q] = aug[r][lane];
+        }
         const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
         const unsigned long long m = __ballot(nz);
         if (m == 0ull) continue;  // no pivot in this column (every wave sees the same)
@@ -308,13 +332,9 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
         const T h = aug[pv][c];
         const T y = aug[pv][lane];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int r = q ? r1 : r0;
-            if (r < nC && r != pv) {
-                const T a = aug[r][c];
-                const T x = aug[r][lane];
-                aug[r][lane] = f.ff_update(h, x, a, y);
-            }
+        for (int q = 0; q < IF_RPW; ++q) {
+            const int r = wid + NW * q;
+            if (r < nC && r != pv) aug[r][lane] = f.ff_update(h, xv[q], av[q], y);
         }
         if (tid == 0) {
             prow_of[t] = pv;
@@ -340,8 +360,8 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int r = q ? r1 : r0;
+    for (int q = 0; q < IF_RPW; ++q) {
+        const int r = wid + NW * q;
         if ((pmask >> r) & 1ull) aug[r][lane] = f.mont(s_invm[s_tof[r]], aug[r][lane]);
     }
     __syncthreads();
@@ -456,18 +476,20 @@ __device__ static void make_u_body(typename LT::T* __restrict__ X, int64_t N, ty
     for (int x = threadIdx.x + threadIdx.y * 64; x < k * CI; x += 256) s_inv[x] = bs->apinv[x];
     if (threadIdx.y == 0 && t < k) s_R[t] = bs->R[t];
     __syncthreads();
-    const int n = blockIdx.x * 4 + threadIdx.y;
+    // the block's four columns of the k pivot rows, fetched at once (a load per
+    // term of the dot product would expose the memory latency k times)
+    __shared__ T s_x[CI][4];
+    const int y = threadIdx.y, n = blockIdx.x * 4 + y;
+    if (t < k) s_x[t][y] = X[s_R[t] * XLD + n];
+    __syncthreads();
     T u = 0;
     if (t < k) {
-        for (int s = 0; s < k; ++s) u = f.add(u, f.mont(s_inv[t * CI + s], X[s_R[s] * XLD + n]));
+        for (int s = 0; s < k; ++s) u = f.add(u, f.mont(s_inv[t * CI + s], s_x[s][y]));
         U[t * XLD + n] = u;
-    }
-    if constexpr (std::is_same<LT, LimbsU32>::value) {
         // the pivot rows' new values go straight into the panel (the 32-bit
-        // inner GEMM leaves those rows alone); every read of column n above
-        // comes first
-        __syncthreads();
-        if (t < k) X[s_R[t] * XLD + n] = u;
+        // inner GEMM leaves those rows alone); every read of X is behind the
+        // barrier above
+        if constexpr (std::is_same<LT, LimbsU32>::value) X[s_R[t] * XLD + n] = u;
     }
     int8_t d[NL] = {};
     if (t < k) balanced_digits(centred(u, f.p), d);
