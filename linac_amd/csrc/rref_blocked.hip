@@ -848,12 +848,25 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat<LimbsU32>* __r
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
     if (tid < CI) s_cp[tid] = tid < k ? bs->Cp[tid] : 0;
+    // the stripe's inner panel columns [c0, c0 + BI), fetched row-contiguous
+    // (rows past the matrix repeat its last row; their results are not stored)
+    __shared__ uint32_t s_win[G_M][BI + 1];
+    if (split == 1) {
+        const int wr = tid >> 2, wq = tid & 3;  // row, eight-column quarter
+        const uint32_t off = (uint32_t)min(wr, rows - 1) * XLD + c0 + wq * 8;
+        const v4i w0 = gld<v4i>(Xs + off), w1 = gld<v4i>(Xs + off + 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            s_win[wr][wq * 8 + e] = (uint32_t)w0[e];
+            s_win[wr][wq * 8 + 4 + e] = (uint32_t)w1[e];
+        }
+    }
     __syncthreads();
     const v4i zero = {0, 0, 0, 0};
     // left operand: byte j of af[l][mi] = digit l of X[row, c0 + Cp[t]], t = 16 g + j
-    // (zero for t >= k; those lanes read column c0, a valid address).  With a
-    // split stripe another workgroup may already be writing those columns, so
-    // the digits come from the planes k_make_u captured before the launch.
+    // (zero for t >= k).  With a split stripe another workgroup may already be
+    // writing those columns, so the digits come from the planes k_make_u
+    // captured before the launch.
     v4i af[4][2];
     if (split > 1) {
 #pragma unroll
@@ -865,10 +878,10 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_inner(const BMat<LimbsU32>* __r
     } else {
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi) {
-            const uint32_t ra = (uint32_t)min(wm * 32 + mi * 16 + r16, rows - 1) * XLD + c0;
+            const int rl = wm * 32 + mi * 16 + r16;
             uint32_t v[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) v[j] = gld<uint32_t>(Xs + (ra + s_cp[(16 * g + j) & (CI - 1)]));
+            for (int j = 0; j < 16; ++j) v[j] = s_win[rl][s_cp[(16 * g + j) & (CI - 1)]];
 #pragma unroll
             for (int l = 0; l < 4; ++l) af[l][mi] = zero;
 #pragma unroll
