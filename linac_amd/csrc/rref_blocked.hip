@@ -148,25 +148,52 @@ constexpr int64_t STRIDE_W = (int64_t)BO * BO;
 // panel buffer management
 // ---------------------------------------------------------------------------
 
+// Panel copies in 16-byte vectors: rows of A are 16-byte aligned and J is a
+// multiple of the panel width, so a vector never straddles a row; a panel
+// narrower than a whole vector at the right edge of A is masked.
 template <class LT>
 __global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__ D) {
+    using T = typename LT::T;
+    constexpr int PER = 16 / (int)sizeof(T);
+    constexpr int VPR = XLD / PER;  // vectors per panel-buffer row
     const BMat<LT>& d = D[blockIdx.y];
-    const int64_t total = d.N * XLD;
+    const int64_t total = d.N * VPR;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = x / XLD;
-        const int j = (int)(x - r * XLD);
-        d.X[x] = j < d.bcur ? d.A[r * d.ld + d.J + j] : 0u;
+        const int64_t r = x / VPR;
+        const int j = (int)(x - r * VPR) * PER;
+        Vec16<T> v = {};
+        if (j < d.bcur) {
+            const v4i raw = gld<v4i>(d.A + r * d.ld + d.J + j);
+            __builtin_memcpy(&v, &raw, 16);
+#pragma unroll
+            for (int e = 0; e < PER; ++e)
+                if (j + e >= d.bcur) v.v[e] = 0;
+        }
+        v4i out;
+        __builtin_memcpy(&out, &v, 16);
+        gst<v4i>(d.X + r * XLD + j, out);
     }
 }
 
 template <class LT>
 __global__ __launch_bounds__(256) void k_panel_store(const BMat<LT>* __restrict__ D) {
+    using T = typename LT::T;
+    constexpr int PER = 16 / (int)sizeof(T);
     const BMat<LT>& d = D[blockIdx.y];
-    const int64_t total = d.N * d.bcur;
+    const int vpr = (d.bcur + PER - 1) / PER;
+    const int64_t total = d.N * vpr;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = x / d.bcur;
-        const int j = (int)(x - r * d.bcur);
-        d.A[r * d.ld + d.J + j] = d.X[r * XLD + j];
+        const int64_t r = x / vpr;
+        const int j = (int)(x - r * vpr) * PER;
+        const v4i v = gld<v4i>(d.X + r * XLD + j);
+        T* dst = d.A + r * d.ld + d.J + j;
+        if (j + PER <= d.bcur) {
+            gst<v4i>(dst, v);
+        } else {
+            Vec16<T> e;
+            __builtin_memcpy(&e, &v, 16);
+            for (int q = 0; j + q < d.bcur; ++q) dst[q] = e.v[q];
+        }
     }
 }
 
@@ -538,28 +565,47 @@ __global__ __launch_bounds__(256) void k_make_lhs(const BMat<LT>* __restrict__ D
     const BMat<LT>& d = D[blockIdx.y];
     if (d.nT == 0 && !(src == 0 && d.save_w)) return;
     const int width = src == 0 ? d.q0 : d.kA;
-    const int kw = src == 0 ? d.Kq - d.KqA : d.KqA;
+    const int kw = src == 0 ? d.Kq - d.KqA : d.KqA;  // a multiple of 64, at most BO
     const int koff = src == 0 ? d.KqA : 0;
     if (width == 0) return;
     const bool save = src == 0 && d.save_w;
-    const int64_t total = d.N * kw;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t m = x / kw;
-        const int t = (int)(x - m * kw);
-        int8_t dg[NL] = {};
+    // four consecutive columns t per thread, BO / 4 threads per row
+    const int t = (threadIdx.x & (BO / 4 - 1)) * 4;
+    if (t >= kw) return;
+    constexpr int RPB = 256 / (BO / 4);  // rows per block and pass
+    for (int64_t m = (int64_t)blockIdx.x * RPB + threadIdx.x / (BO / 4); m < d.N; m += (int64_t)gridDim.x * RPB) {
+        int8_t dg[NL][4] = {};
         if (t < width) {
-            T w;
+            T w[4];
             if (src == 0) {
-                w = d.X[m * XLD + BO + t];
-                if (d.epoch_of[m] >= d.e0 && d.epoch_of[m] < e1 && d.slot_of[m] == t) w = d.f.sub(w, (T)1);
-                if (save) d.W[m * BO + t] = w;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) w[e] = d.X[m * XLD + BO + t + e];
+                const int32_t ep = d.epoch_of[m];
+                if (ep >= d.e0 && ep < e1) {
+                    const int sl = d.slot_of[m] - t;
+                    if (sl >= 0 && sl < 4) w[sl] = d.f.sub(w[sl], (T)1);
+                }
+                if (save)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) d.W[m * BO + t + e] = w[e];
             } else {
-                w = d.W[m * BO + t];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) w[e] = d.W[m * BO + t + e];
             }
-            balanced_digits(-centred(w, d.f.p), dg);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int8_t g[NL];
+                balanced_digits(t + e < width ? -centred(w[e], d.f.p) : 0, g);
+#pragma unroll
+                for (int l = 0; l < NL; ++l) dg[l][e] = g[l];
+            }
         }
 #pragma unroll
-        for (int l = 0; l < NL; ++l) d.planesL[l * d.strideL + m * d.Kq + koff + t] = dg[l];
+        for (int l = 0; l < NL; ++l) {
+            int32_t pk;
+            __builtin_memcpy(&pk, dg[l], 4);
+            *reinterpret_cast<int32_t*>(d.planesL + l * d.strideL + m * d.Kq + koff + t) = pk;
+        }
     }
 }
 
@@ -1359,13 +1405,13 @@ class Engine {
     bool first_step = true;
 
   private:
-    // Staging slots (pinned host + device): 0 = inner phase; 1..6 = the three
+    // Staging slots (pinned host + device): 0 = inner phase; 1, 2 = the three
     // parts of the trailing update of even and odd steps.  A slot's device
     // copy is refilled on `so` after every kernel that read it, and its host
     // side two steps after its copy was queued (the inner phase of a step
     // waits for the previous step's first part, which follows the parts
     // before it on the same stream).
-    static constexpr int NSLOT = 7;
+    static constexpr int NSLOT = 3;
     struct Staged {
         BM* desc;
         TileSched inner, wq, main;  // inner GEMM, composite W GEMM, trailing update
@@ -1378,7 +1424,9 @@ class Engine {
     int stage(int slot, size_t bytes);
     // part 0: inner phase; 1: the next panel's columns; 2, 3: the two halves
     // of the columns beyond it (composite updates only)
-    int upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out);
+    // parts part .. part + nparts - 1 into one staging slot, one copy
+    int upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out, int nparts = 1);
+    void fill(const std::vector<It*>& act, int part, char* h, char* dv, size_t desc_b, size_t sched_b, Staged* out);
     int trailing(const Staged& t, Profiler* prof, int64_t tag);
     // A trailing part beyond the next panel, queued to run beside a later
     // step's inner phase (`due`), launched in tile slices.
@@ -1492,14 +1540,24 @@ int Engine<LT>::stage(int slot, size_t bytes) {
 // arrays are prefix sums of per-matrix tile counts: (pa, ma) inner GEMM,
 // (pb, mb) composite W of a deferred panel, (pc, mc) trailing update.
 template <class LT>
-int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out) {
+int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out, int nparts) {
     const size_t B = act.size();
     const size_t desc_b = (size_t)round_up((int64_t)(B * sizeof(BM)), 256);
     const size_t sched_b = (size_t)round_up((int64_t)((6 * B + 3) * 8), 256);
     const size_t stat_b = (size_t)round_up((int64_t)(B * sizeof(ItemStatus)), 256);
-    LINAC_TRY(stage(slot, desc_b + sched_b + stat_b));
+    const size_t one = desc_b + sched_b + stat_b;
+    LINAC_TRY(stage(slot, one * nparts));
     char* h = (char*)h_buf[slot];
     char* dv = (char*)d_buf[slot];
+    for (int i = 0; i < nparts; ++i) fill(act, part + i, h + i * one, dv + i * one, desc_b, sched_b, out + i);
+    LINAC_HIP(hipMemcpyAsync(dv, h, one * (nparts - 1) + desc_b + sched_b, hipMemcpyHostToDevice, s));
+    return LINAC_OK;
+}
+
+template <class LT>
+void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, size_t desc_b, size_t sched_b,
+                      Staged* out) {
+    const size_t B = act.size();
     BM* hd = (BM*)h;
     int64_t* hs = (int64_t*)(h + desc_b);
     int64_t* ds = (int64_t*)(dv + desc_b);
@@ -1573,8 +1631,6 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
     out->wq_tiles = pb[B];
     out->main_tiles = pc[B];
     out->status = (ItemStatus*)(dv + desc_b + sched_b);
-    LINAC_HIP(hipMemcpyAsync(dv, h, desc_b + sched_b, hipMemcpyHostToDevice, s));
-    return LINAC_OK;
 }
 
 static unsigned gs_grid(int64_t work, int64_t cap = 4096) {
@@ -1616,10 +1672,10 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         ninner = std::max<int64_t>(ninner, ceil_div(d.bcur, BI));
     }
     // --- inner phase on s, after the previous step's first trailing part
-    if (ev_pending) LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
     Staged in;
-    LINAC_TRY(upload(act, s, 0, 0, &in));
-    k_panel_load<LT><<<dim3(gs_grid(maxNX, 8192), B), 256, 0, s>>>(in.desc);
+    LINAC_TRY(upload(act, s, 0, 0, &in));  // (its copy need not wait for the event)
+    if (ev_pending) LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
+    k_panel_load<LT><<<dim3(gs_grid(maxNX / 4, 8192), B), 256, 0, s>>>(in.desc);
     k_block_state_init<LT><<<B, 1, 0, s>>>(in.desc, first_step ? 1 : 0);
     first_step = false;
     // (the 32-bit inner GEMM captures its left operand itself)
@@ -1691,32 +1747,31 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         d.full = st[m].rank >= d.N ? 1 : 0;  // no row left to pivot: the remaining columns are final after this step
         d.save_w = (!d.full && d.kA == 0 && d.q0 > 0 && d.nT > BO) ? 1 : 0;
     }
-    const int p1 = 1 + 3 * parity;
+    const int p1 = 1 + parity;
     parity ^= 1;
-    Staged t1, t2, t3;
-    LINAC_TRY(upload(act, so, p1, 1, &t1));
-    LINAC_TRY(upload(act, so, p1 + 1, 2, &t2));
-    LINAC_TRY(upload(act, so, p1 + 2, 3, &t3));
+    Staged tp[3];
+    LINAC_TRY(upload(act, so, p1, 1, tp, 3));
+    const Staged &t1 = tp[0], &t2 = tp[1], &t3 = tp[2];
     bool operands = t1.main_tiles > 0;
     for (auto* it : act) operands = operands || it->d.save_w;
     if (operands) {
         // -W of the current panel (kept in W when deferred), then for composite
         // items W <- W + W' W[Q'] and -W of the composite
         k_mark_slots<LT><<<dim3(1, B), 256, 0, so>>>(t1.desc);
-        k_make_lhs<LT><<<dim3(gs_grid(maxN * BO, 16384), B), 256, 0, so>>>(t1.desc, 0, epoch);
+        k_make_lhs<LT><<<dim3(gs_grid(maxN * (BO / 4), 16384), B), 256, 0, so>>>(t1.desc, 0, epoch);
         if (t1.wq_tiles > 0) {
             k_make_rhs<LT><<<dim3(BO / 64, BO / 64, B), 256, 0, so>>>(t1.desc, 1);
             const unsigned grid = (unsigned)(ceil_div(t1.wq_tiles, NUM_XCD) * NUM_XCD);
             if constexpr (wide) k_gemm_outer64<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
             else k_gemm_outer<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
         }
-        k_make_lhs<LT><<<dim3(gs_grid(maxN * BO, 16384), B), 256, 0, so>>>(t1.desc, 1, epoch);
+        k_make_lhs<LT><<<dim3(gs_grid(maxN * (BO / 4), 16384), B), 256, 0, so>>>(t1.desc, 1, epoch);
         LINAC_HIP(hipGetLastError());
     }
     LINAC_TRY(trailing(t1, prof, tag));
     int64_t maxpanel = 1;
     for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
-    k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, so>>>(t1.desc);
+    k_panel_store<LT><<<dim3(gs_grid(maxpanel / 4, 8192), B), 256, 0, so>>>(t1.desc);
     LINAC_HIP(hipGetLastError());
     // The next inner phase waits only for the above.  The composite update's
     // first half is due beside the next step's inner phase, its second half
