@@ -179,12 +179,12 @@ template <class LT>
 __global__ __launch_bounds__(256) void k_panel_store(const BMat<LT>* __restrict__ D) {
     using T = typename LT::T;
     constexpr int PER = 16 / (int)sizeof(T);
+    constexpr int TPR = BO / PER;         // threads per panel row
+    constexpr int RPB = 256 / TPR;        // rows per block and pass
     const BMat<LT>& d = D[blockIdx.y];
-    const int vpr = (d.bcur + PER - 1) / PER;
-    const int64_t total = d.N * vpr;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = x / vpr;
-        const int j = (int)(x - r * vpr) * PER;
+    const int j = (threadIdx.x % TPR) * PER;
+    if (j >= d.bcur) return;
+    for (int64_t r = (int64_t)blockIdx.x * RPB + threadIdx.x / TPR; r < d.N; r += (int64_t)gridDim.x * RPB) {
         const v4i v = gld<v4i>(d.X + r * XLD + j);
         T* dst = d.A + r * d.ld + d.J + j;
         if (j + PER <= d.bcur) {
@@ -407,7 +407,7 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
         bs->Cp[tid] = pcol[tid];
         used[r] = 1;
         epoch_of[r] = epoch;
-        slot_of[r] = tid;
+        slot_of[r] = q0 + tid;  // its slot in the outer panel (the column of Z holding its 1)
         piv_rows[rank0 + tid] = r;
         piv_cols[rank0 + tid] = colbase + c0 + pcol[tid];
         X[r * XLD + BO + q0 + tid] = 1;  // Z[r, slot] = 1
@@ -545,24 +545,15 @@ __global__ __launch_bounds__(256) void k_make_u(const BMat<LT>* __restrict__ D) 
 // K layout of both operands: [deferred part, KqA = kA rounded up to 64 |
 // current panel, q0 rounded up to 64], zero padded.
 
-// slot_of[Q[i]] = i for the current panel's pivot rows (blockIdx.y = matrix),
-// which identifies the 1 of E in each pivot row.
-template <class LT>
-__global__ __launch_bounds__(256) void k_mark_slots(const BMat<LT>* __restrict__ D) {
-    const BMat<LT>& d = D[blockIdx.y];
-    const int64_t* Q = d.piv_rows + d.rank_start;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.q0; i += gridDim.x * blockDim.x) d.slot_of[Q[i]] = i;
-}
-
-// Limb planes of -W, the left operand (blockIdx.y = matrix):
+// Limb planes of -W, the left operand, for one matrix (workgroup bx of nbx
+// striding over rows; the 1 of E in pivot row m sits in column slot_of[m]):
 //   src 0: the current panel, W[m, t] = Z[m, t] - [m = Q[t]] (Z = X[:, BO + t], t < q0),
 //          written at K offset KqA; kept in d.W when the panel's update is deferred;
 //   src 1: the deferred panel's composite transformation d.W[:, t] (t < kA), at K offset 0.
 template <class LT>
-__global__ __launch_bounds__(256) void k_make_lhs(const BMat<LT>* __restrict__ D, int src, int e1) {
+__device__ static void make_lhs_body(const BMat<LT>& d, int src, int e1, int64_t bx, int64_t nbx) {
     using T = typename LT::T;
     constexpr int NL = LT::NL;
-    const BMat<LT>& d = D[blockIdx.y];
     if (d.nT == 0 && !(src == 0 && d.save_w)) return;
     const int width = src == 0 ? d.q0 : d.kA;
     const int kw = src == 0 ? d.Kq - d.KqA : d.KqA;  // a multiple of 64, at most BO
@@ -573,7 +564,7 @@ __global__ __launch_bounds__(256) void k_make_lhs(const BMat<LT>* __restrict__ D
     const int t = (threadIdx.x & (BO / 4 - 1)) * 4;
     if (t >= kw) return;
     constexpr int RPB = 256 / (BO / 4);  // rows per block and pass
-    for (int64_t m = (int64_t)blockIdx.x * RPB + threadIdx.x / (BO / 4); m < d.N; m += (int64_t)gridDim.x * RPB) {
+    for (int64_t m = bx * RPB + threadIdx.x / (BO / 4); m < d.N; m += nbx * RPB) {
         int8_t dg[NL][4] = {};
         if (t < width) {
             T w[4];
@@ -609,8 +600,8 @@ __global__ __launch_bounds__(256) void k_make_lhs(const BMat<LT>* __restrict__ D
     }
 }
 
-// Limb planes of the right operand, transposed (blockIdx.x strides over column
-// tiles, blockIdx.y = 64-deep K tile, blockIdx.z = matrix), 64 x 64 tiles
+// Limb planes of the right operand, transposed, for one matrix (workgroup bx
+// of nbx striding over column tiles, by = 64-deep K tile), 64 x 64 tiles
 // through LDS so that both the row reads and the plane writes are coalesced.
 //   job 0: planesB[l][n][t] = digit l of A[row(t), T0 + n], row(t) = the
 //          deferred panel's pivot Q_A[t] (t < kA) or the current panel's
@@ -618,21 +609,20 @@ __global__ __launch_bounds__(256) void k_make_lhs(const BMat<LT>* __restrict__ D
 //   job 1: planesW[l][n][t] = digit l of W[Q[t], n] (n < kA, t < q0): the
 //          factor W[Q'] of the composite transformation.  Row stride Kq - KqA.
 template <class LT>
-__global__ __launch_bounds__(256) void k_make_rhs(const BMat<LT>* __restrict__ D, int job) {
+__device__ static void make_rhs_body(const BMat<LT>& d, int job, int64_t bx, int64_t nbx, int by) {
     using S = typename LT::S;
     constexpr int NL = LT::NL;
     __shared__ S tile[64][65];
-    const BMat<LT>& d = D[blockIdx.z];
     const int64_t ncols = job == 0 ? d.nT : (d.q0 > 0 ? d.kA : 0);
     const int depth = job == 0 ? d.Kq : d.Kq - d.KqA;
     if (ncols == 0) return;
-    const int tb = blockIdx.y * 64;
+    const int tb = by * 64;
     if (tb >= depth) return;
     const int64_t* Q = d.piv_rows + d.rank_start;
     const int64_t* QA = Q - d.kA;
     int8_t* planes = job == 0 ? d.planesB : d.planesW;
     const int64_t pstride = job == 0 ? d.strideB : STRIDE_W;
-    for (int64_t nb = (int64_t)blockIdx.x * 64; nb < ncols; nb += (int64_t)gridDim.x * 64) {
+    for (int64_t nb = bx * 64; nb < ncols; nb += nbx * 64) {
         for (int x = threadIdx.x; x < 64 * 64; x += 256) {
             const int tt = x >> 6, nn = x & 63;
             const int t = tb + tt;
@@ -669,6 +659,27 @@ __global__ __launch_bounds__(256) void k_make_rhs(const BMat<LT>* __restrict__ D
             }
         }
         __syncthreads();
+    }
+}
+
+template <class LT>
+__global__ __launch_bounds__(256) void k_make_rhs(const BMat<LT>* __restrict__ D, int job) {
+    make_rhs_body<LT>(D[blockIdx.z], job, blockIdx.x, gridDim.x, blockIdx.y);
+}
+
+// Both operands' planes in one launch (blockIdx.y = matrix): workgroups
+// [0, nl) build the left operand (source lsrc, make_lhs_body; none if
+// nl = 0), the rest the right operand of job rjob in rx x ry tiles
+// (make_rhs_body; none if rjob < 0).  The two read and write disjoint data.
+template <class LT>
+__global__ __launch_bounds__(256) void k_make_ops(const BMat<LT>* __restrict__ D, int lsrc, int e1, int nl, int rjob,
+                                                  int rx) {
+    const BMat<LT>& d = D[blockIdx.y];
+    if ((int)blockIdx.x < nl) {
+        make_lhs_body<LT>(d, lsrc, e1, blockIdx.x, nl);
+    } else if (rjob >= 0) {
+        const int r = (int)blockIdx.x - nl;
+        make_rhs_body<LT>(d, rjob, r % rx, rx, r / rx);
     }
 }
 
@@ -730,7 +741,7 @@ __device__ __forceinline__ bool batch_tile(const TileSched& ts, int* m_out, int6
     return true;
 }
 
-// Trailing-update GEMMs, C <- C - L B with L = limbs of -W (see k_make_lhs):
+// Trailing-update GEMMs, C <- C - L B with L = limbs of -W (see make_lhs_body):
 //   JOB 0: A[:, T] <- A[:, T] + W A[Qc, T]   (the whole depth Kq, every row);
 //   JOB 1: W <- W + W' W[Q']                 (the composite transformation of a
 //          deferred panel: L = the current panel's part, B = planesW).
@@ -1196,7 +1207,7 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_inner64(const BMat<LimbsU64>
         for (int i = 0; i < 4; ++i) {
             const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, N - 1);
             prow[mi][i] = gld<int32_t>(d.epoch_of + row) == epoch;
-            slot[mi][i] = gld<int32_t>(d.slot_of + row);
+            slot[mi][i] = gld<int32_t>(d.slot_of + row) - (bs->q0 - k);  // slot within this step
 #pragma unroll
             for (int ni = 0; ni < NI64; ++ni)
                 cv[mi][ni][i] = gld<uint64_t>(X + row * XLD + n_base + wn * 16 * NI64 + ni * 16 + r16);
@@ -1427,7 +1438,7 @@ class Engine {
     // parts part .. part + nparts - 1 into one staging slot, one copy
     int upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out, int nparts = 1);
     void fill(const std::vector<It*>& act, int part, char* h, char* dv, size_t desc_b, size_t sched_b, Staged* out);
-    int trailing(const Staged& t, Profiler* prof, int64_t tag);
+    int trailing(const Staged& t, Profiler* prof, int64_t tag, bool rhs_ready = false);
     // A trailing part beyond the next panel, queued to run beside a later
     // step's inner phase (`due`), launched in tile slices.
     struct Bulk {
@@ -1509,11 +1520,12 @@ int Engine<LT>::flush_due(int64_t upto, Profiler* prof, int64_t tag) {
 
 // The trailing GEMM of one staged part on `so`: B operand planes, then the update.
 template <class LT>
-int Engine<LT>::trailing(const Staged& t, Profiler* prof, int64_t tag) {
+int Engine<LT>::trailing(const Staged& t, Profiler* prof, int64_t tag, bool rhs_ready) {
     if (t.main_tiles == 0) return LINAC_OK;
     const unsigned B = (unsigned)t.inner.B;
-    k_make_rhs<LT><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(t.cols, 64)), 2 * BO / 64, B), 256, 0, so>>>(
-        t.desc, 0);
+    if (!rhs_ready)
+        k_make_rhs<LT><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(t.cols, 64)), 2 * BO / 64, B), 256, 0, so>>>(
+            t.desc, 0);
     if (prof) LINAC_TRY(prof->sample_begin(so, tag, t.bytes, t.macs));
     const unsigned grid = (unsigned)(ceil_div(t.main_tiles, NUM_XCD) * NUM_XCD);
     if constexpr (std::is_same<LT, LimbsU64>::value) k_gemm_outer64<0><<<grid, G_T, 0, so>>>(t.desc, t.main);
@@ -1755,20 +1767,30 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     bool operands = t1.main_tiles > 0;
     for (auto* it : act) operands = operands || it->d.save_w;
     if (operands) {
-        // -W of the current panel (kept in W when deferred), then for composite
-        // items W <- W + W' W[Q'] and -W of the composite
-        k_mark_slots<LT><<<dim3(1, B), 256, 0, so>>>(t1.desc);
-        k_make_lhs<LT><<<dim3(gs_grid(maxN * (BO / 4), 16384), B), 256, 0, so>>>(t1.desc, 0, epoch);
-        if (t1.wq_tiles > 0) {
-            k_make_rhs<LT><<<dim3(BO / 64, BO / 64, B), 256, 0, so>>>(t1.desc, 1);
+        // -W of the current panel (kept in W when deferred) beside, for
+        // composite items, the planes of W[Q']; then W <- W + W' W[Q']; then -W
+        // of the composite beside the right operand of the next panel's update
+        const int nl = (int)gs_grid(maxN * (BO / 4), 16384);
+        const bool comp = t1.wq_tiles > 0, main1 = t1.main_tiles > 0;
+        constexpr int RW = BO / 64;  // right-operand tiles of W[Q'] per dimension
+        k_make_ops<LT><<<dim3(nl + (comp ? RW * RW : 0), B), 256, 0, so>>>(t1.desc, 0, epoch, nl, comp ? 1 : -1, RW);
+        if (comp) {
             const unsigned grid = (unsigned)(ceil_div(t1.wq_tiles, NUM_XCD) * NUM_XCD);
             if constexpr (wide) k_gemm_outer64<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
             else k_gemm_outer<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
         }
-        k_make_lhs<LT><<<dim3(gs_grid(maxN * (BO / 4), 16384), B), 256, 0, so>>>(t1.desc, 1, epoch);
+        const int rx = (int)std::min<int64_t>(2048, ceil_div(t1.cols, 64));
+        // (a deferred panel followed by one without pivots still needs its
+        // composite's planes, though there is no W' W[Q'] to add)
+        bool deferred = false;
+        for (auto* it : act) deferred = deferred || it->d.kA > 0;
+        const int nl1 = deferred ? nl : 0;
+        if (nl1 > 0 || main1)
+            k_make_ops<LT><<<dim3(nl1 + (main1 ? rx * (2 * BO / 64) : 0), B), 256, 0, so>>>(t1.desc, 1, epoch, nl1,
+                                                                                          main1 ? 0 : -1, rx);
         LINAC_HIP(hipGetLastError());
     }
-    LINAC_TRY(trailing(t1, prof, tag));
+    LINAC_TRY(trailing(t1, prof, tag, operands));
     int64_t maxpanel = 1;
     for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
     k_panel_store<LT><<<dim3(gs_grid(maxpanel / 4, 8192), B), 256, 0, so>>>(t1.desc);
