@@ -152,11 +152,16 @@ constexpr int64_t STRIDE_W = (int64_t)BO * BO;
 // multiple of the panel width, so a vector never straddles a row; a panel
 // narrower than a whole vector at the right edge of A is masked.
 template <class LT>
-__global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__ D) {
+__device__ static void block_state_init(const BMat<LT>& d, int reset_timers);
+
+// (workgroup 0 of each matrix also resets its inner-step state)
+template <class LT>
+__global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__ D, int reset_timers) {
     using T = typename LT::T;
     constexpr int PER = 16 / (int)sizeof(T);
     constexpr int VPR = XLD / PER;  // vectors per panel-buffer row
     const BMat<LT>& d = D[blockIdx.y];
+    if (blockIdx.x == 0 && threadIdx.x == 0) block_state_init<LT>(d, reset_timers);
     const int64_t total = d.N * VPR;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = x / VPR;
@@ -203,8 +208,7 @@ __global__ __launch_bounds__(256) void k_epoch_init(int32_t* epoch, int64_t N) {
 }
 
 template <class LT>
-__global__ void k_block_state_init(const BMat<LT>* __restrict__ D, int reset_timers) {
-    const BMat<LT>& d = D[blockIdx.x];
+__device__ static void block_state_init(const BMat<LT>& d, int reset_timers) {
     auto* bs = d.bs;
     if (reset_timers)
         for (int i = 0; i < 8; ++i) bs->tphase[i] = 0;
@@ -1687,8 +1691,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     Staged in;
     LINAC_TRY(upload(act, s, 0, 0, &in));  // (its copy need not wait for the event)
     if (ev_pending) LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
-    k_panel_load<LT><<<dim3(gs_grid(maxNX / 4, 8192), B), 256, 0, s>>>(in.desc);
-    k_block_state_init<LT><<<B, 1, 0, s>>>(in.desc, first_step ? 1 : 0);
+    k_panel_load<LT><<<dim3(gs_grid(maxNX / 4, 8192), B), 256, 0, s>>>(in.desc, first_step ? 1 : 0);
     first_step = false;
     // (the 32-bit inner GEMM captures its left operand itself)
     const bool planes_a = wide || in.inner_split > 1;
