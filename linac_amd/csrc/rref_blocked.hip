@@ -879,7 +879,7 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
 // (columns outside it only meet zeros).  Owning the whole stripe lets it
 // capture its own left operand, the digits of X[r, c0 + Cp], before any of its
 // waves writes, and no other workgroup reads its rows.  Fragments come
-// straight from global memory (no LDS): the launch is one workgroup per
+// straight from global memory (LDS holds only the pivot columns): the launch is one workgroup per
 // stripe, which matters because the trailing GEMM on the other stream holds
 // every register of a CU and each workgroup here waits for one of its slots
 // to free.  With few stripes (small matrices) `split` workgroups share a
