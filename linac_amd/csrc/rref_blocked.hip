@@ -450,9 +450,11 @@ __global__ __launch_bounds__(IF_T, LT::IF_WAVES) void k_inner_factor(const BMat<
 // The 64-bit GEMM overwrites the pivot columns X[:, c0 + Cp] in place, so its
 // left operand must be captured here, before any of its workgroups runs (the
 // 32-bit GEMM captures its own, per row stripe, and for it the pivot rows
-// X[R, :] get U here).  Workgroups [0, MU_BLOCKS) build U (4 columns each),
-// the rest planesA (MU_ROWS rows each, grid-strided).
-constexpr int MU_BLOCKS = XLD / 4;
+// X[R, :] get U here).  Workgroups [0, MU_BLOCKS) build U (8 columns each,
+// 32 threads per column), the rest planesA (MU_ROWS rows each, grid-strided).
+constexpr int MU_COLS = 8;
+constexpr int MU_BLOCKS = XLD / MU_COLS;
+static_assert(MU_COLS * CI == 256 && 2 * CI == KPI, "one thread per (column, pivot slot); slots pad to KPI");
 constexpr int MU_ROWS = 16;
 
 template <class LT>
@@ -493,28 +495,32 @@ __device__ static void make_u_body(typename LT::T* __restrict__ X, int64_t N, ty
         }
         return;
     }
-    for (int x = threadIdx.x + threadIdx.y * 64; x < k * CI; x += 256) s_inv[x] = bs->apinv[x];
-    if (threadIdx.y == 0 && t < k) s_R[t] = bs->R[t];
+    const int fl = threadIdx.x + threadIdx.y * 64;
+    for (int x = fl; x < k * CI; x += 256) s_inv[x] = bs->apinv[x];
+    if (fl < k) s_R[fl] = bs->R[fl];
     __syncthreads();
-    // the block's four columns of the k pivot rows, fetched at once (a load per
+    // the block's columns of the k pivot rows, fetched at once (a load per
     // term of the dot product would expose the memory latency k times)
-    __shared__ T s_x[CI][4];
-    const int y = threadIdx.y, n = blockIdx.x * 4 + y;
-    if (t < k) s_x[t][y] = X[s_R[t] * XLD + n];
+    __shared__ T s_x[CI][MU_COLS];
+    const int ts = fl % CI, y = fl / CI, n = blockIdx.x * MU_COLS + y;  // pivot slot, column
+    if (ts < k) s_x[ts][y] = X[s_R[ts] * XLD + n];
     __syncthreads();
     T u = 0;
-    if (t < k) {
-        for (int s = 0; s < k; ++s) u = f.add(u, f.mont(s_inv[t * CI + s], s_x[s][y]));
-        U[t * XLD + n] = u;
+    if (ts < k) {
+        for (int s = 0; s < k; ++s) u = f.add(u, f.mont(s_inv[ts * CI + s], s_x[s][y]));
+        U[ts * XLD + n] = u;
         // the pivot rows' new values go straight into the panel (the 32-bit
         // inner GEMM leaves those rows alone); every read of X is behind the
         // barrier above
-        if constexpr (std::is_same<LT, LimbsU32>::value) X[s_R[t] * XLD + n] = u;
+        if constexpr (std::is_same<LT, LimbsU32>::value) X[s_R[ts] * XLD + n] = u;
     }
     int8_t d[NL] = {};
-    if (t < k) balanced_digits(centred(u, f.p), d);
+    if (ts < k) balanced_digits(centred(u, f.p), d);
 #pragma unroll
-    for (int l = 0; l < NL; ++l) planesU[l * strideU + (int64_t)n * KPI + t] = d[l];
+    for (int l = 0; l < NL; ++l) {  // slots ts and ts + CI (the zero padding up to KPI)
+        planesU[l * strideU + (int64_t)n * KPI + ts] = d[l];
+        planesU[l * strideU + (int64_t)n * KPI + CI + ts] = 0;
+    }
 }
 
 
