@@ -447,11 +447,12 @@ __global__ __launch_bounds__(IF_T, LT::IF_WAVES) void k_inner_factor(const BMat<
 // values) and the limb planes of both operands of the inner GEMM:
 //   planesU[l][n][t] = digit l of U[t, n]                (t < k, zero up to KPI)
 //   planesA[l][r][t] = digit l of X[r, c0 + Cp[t]]       (every row r; 64-bit only)
-// The 64-bit GEMM overwrites the pivot columns X[:, c0 + Cp] in place, so its
-// left operand must be captured here, before any of its workgroups runs (the
-// 32-bit GEMM captures its own, per row stripe, and for it the pivot rows
-// X[R, :] get U here).  Workgroups [0, MU_BLOCKS) build U (8 columns each,
-// 32 threads per column), the rest planesA (MU_ROWS rows each, grid-strided).
+// The inner GEMM overwrites the pivot columns X[:, c0 + Cp] in place, so when
+// several of its workgroups share a row stripe (always for 64-bit storage) its
+// left operand is captured here, before any of them runs; a 32-bit workgroup
+// that owns its stripe captures its own.  The pivot rows X[R, :] get U here.
+// Workgroups [0, MU_BLOCKS) build U (8 columns each, 32 threads per column),
+// the rest planesA (MU_ROWS rows each, grid-strided).
 constexpr int MU_COLS = 8;
 constexpr int MU_BLOCKS = XLD / MU_COLS;
 static_assert(MU_COLS * CI == 256 && 2 * CI == KPI, "one thread per (column, pivot slot); slots pad to KPI");
@@ -509,10 +510,11 @@ __device__ static void make_u_body(typename LT::T* __restrict__ X, int64_t N, ty
     if (ts < k) {
         for (int s = 0; s < k; ++s) u = f.add(u, f.mont(s_inv[ts * CI + s], s_x[s][y]));
         U[ts * XLD + n] = u;
-        // the pivot rows' new values go straight into the panel (the 32-bit
-        // inner GEMM leaves those rows alone); every read of X is behind the
-        // barrier above
-        if constexpr (std::is_same<LT, LimbsU32>::value) X[s_R[ts] * XLD + n] = u;
+        // the pivot rows' new values go straight into the panel (the inner
+        // GEMM leaves those rows alone); every read of X by this workgroup is
+        // behind the barrier above, and the planes workgroups' reads of pivot
+        // rows feed only results that are never stored
+        X[s_R[ts] * XLD + n] = u;
     }
     int8_t d[NL] = {};
     if (ts < k) balanced_digits(centred(u, f.p), d);
@@ -1159,11 +1161,14 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
             }
 }
 
-// Inner-step GEMM for 64-bit storage (as k_gemm_inner, 64 x TN64 tiles).  The
-// panel entries, row epochs and slots the epilogue needs are fetched before
-// the MFMAs, so their latency hides behind the matrix cores.
+// Inner-step GEMM for 64-bit storage: as k_gemm_inner, a 64-row stripe per
+// workgroup (or `split` workgroups per stripe, each taking every split-th
+// tile) walking the live window in 64 x TN64 tiles.  The left operand comes
+// from the planes k_make_u captured and is staged in LDS once; each tile
+// stages its right operand.  Pivot rows of the step already hold U (k_make_u)
+// and are skipped; so are columns past the window.
 __global__ __launch_bounds__(G_T, GW64) void k_gemm_inner64(const BMat<LimbsU64>* __restrict__ D, TileSched ts,
-                                                         int epoch) {
+                                                         int epoch, int split) {
     __shared__ __attribute__((aligned(16))) int8_t sA[NL64][G_M * G_ROW];
     __shared__ __attribute__((aligned(16))) int8_t sB[NL64][TN64 * G_ROW];
     int mi_;
@@ -1173,9 +1178,10 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_inner64(const BMat<LimbsU64>
     const BlockState<uint64_t>* bs = d.bs;
     const int k = bs->k;
     if (k == 0) return;
-    const int64_t m_base = tm * G_M, n_base = tn * TN64;
-    const int c0 = bs->c0_cur;
-    if (n_base + TN64 <= c0 || n_base >= BO + bs->q0) return;
+    const int c0 = bs->c0_cur, colmax = BO + bs->q0;
+    const int t_lo = c0 / TN64, t_hi = (colmax + TN64 - 1) / TN64;
+    if (t_lo + (int)tn >= t_hi) return;
+    const int64_t m_base = tm * G_M;
     uint64_t* X = d.X;
     const int64_t N = d.N;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -1187,62 +1193,68 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_inner64(const BMat<LimbsU64>
         const int64_t row = min(m_base + row_l, N - 1);  // rows beyond N: clamped, results not stored
         *reinterpret_cast<v4i*>(&sA[q][lds_at(row_l, qu)]) = gld<v4i>(d.planesAi + q * d.strideAi + row * KPI + qu * 16);
     }
-    {
-        const int bl0 = tid / (TN64 * 4), bcol = (tid % (TN64 * 4)) >> 2, bqu = tid & 3;
-#pragma unroll
-        for (int q = 0; q < BQ64; ++q) {
-            const int l = bl0 + q * BLS64;
-            *reinterpret_cast<v4i*>(&sB[l][lds_at(bcol, bqu)]) =
-                gld<v4i>(d.planesUi + l * (int64_t)XLD * KPI + (n_base + bcol) * KPI + bqu * 16);
-        }
-    }
-    // epilogue operands, in flight during the MFMAs
-    uint64_t cv[2][NI64][4];
-    int slot[2][4];
-    bool prow[2][4];
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, N - 1);
-            prow[mi][i] = gld<int32_t>(d.epoch_of + row) == epoch;
-            slot[mi][i] = gld<int32_t>(d.slot_of + row) - (bs->q0 - k);  // slot within this step
-#pragma unroll
-            for (int ni = 0; ni < NI64; ++ni)
-                cv[mi][ni][i] = gld<uint64_t>(X + row * XLD + n_base + wn * 16 * NI64 + ni * 16 + r16);
-        }
-    __syncthreads();
-    const v4i zero = {0, 0, 0, 0};
-    v4i acc[ND64][2][NI64];
-#pragma unroll
-    for (int s = 0; s < ND64; ++s)
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < NI64; ++j) acc[s][i][j] = zero;
-    mfma_stage64(sA, sB, wm, wn, r16, g, acc);
-    const uint64_t p = d.f.p;
-    const double pinv = d.pinv;
+    // this lane's output rows that are left alone (beyond the matrix or pivot rows of the step)
+    unsigned skip = 0;
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-            if (row >= N) continue;
+            const bool piv = gld<int32_t>(d.epoch_of + min(row, N - 1)) == epoch;
+            skip |= (unsigned)(row >= N || piv) << (4 * mi + i);
+        }
+    const uint64_t p = d.f.p;
+    const double pinv = d.pinv;
+    const v4i zero = {0, 0, 0, 0};
+    for (int t = t_lo + (int)tn; t < t_hi; t += split) {
+        const int64_t n_base = (int64_t)t * TN64;
+        __syncthreads();  // every wave is done with the previous tile's right operand
+        {
+            const int bl0 = tid / (TN64 * 4), bcol = (tid % (TN64 * 4)) >> 2, bqu = tid & 3;
 #pragma unroll
-            for (int ni = 0; ni < NI64; ++ni) {
-                const int64_t col = n_base + wn * 16 * NI64 + ni * 16 + r16;
-                uint64_t* cp = X + row * XLD + col;
-                if (prow[mi][i]) {
-                    *cp = d.U[slot[mi][i] * XLD + col];
-                } else {
+            for (int q = 0; q < BQ64; ++q) {
+                const int l = bl0 + q * BLS64;
+                *reinterpret_cast<v4i*>(&sB[l][lds_at(bcol, bqu)]) =
+                    gld<v4i>(d.planesUi + l * (int64_t)XLD * KPI + (n_base + bcol) * KPI + bqu * 16);
+            }
+        }
+        // panel entries, in flight during the MFMAs
+        uint64_t cv[2][NI64][4];
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, N - 1);
+#pragma unroll
+                for (int ni = 0; ni < NI64; ++ni)
+                    cv[mi][ni][i] = gld<uint64_t>(X + row * XLD + n_base + wn * 16 * NI64 + ni * 16 + r16);
+            }
+        __syncthreads();
+        v4i acc[ND64][2][NI64];
+#pragma unroll
+        for (int s = 0; s < ND64; ++s)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < NI64; ++j) acc[s][i][j] = zero;
+        mfma_stage64(sA, sB, wm, wn, r16, g, acc);
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if ((skip >> (4 * mi + i)) & 1u) continue;
+                const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
+#pragma unroll
+                for (int ni = 0; ni < NI64; ++ni) {
+                    const int64_t col = n_base + wn * 16 * NI64 + ni * 16 + r16;
+                    if (col >= colmax) continue;
                     int32_t sv[ND64];
 #pragma unroll
                     for (int q = 0; q < ND64; ++q) sv[q] = acc[q][mi][ni][i];
-                    *cp = fold_sub64(cv[mi][ni][i], sv, p, pinv);
+                    gst<uint64_t>(X + row * XLD + col, fold_sub64(cv[mi][ni][i], sv, p, pinv));
                 }
             }
-        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1612,7 +1624,10 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
     // up to four that share a stripe's column tiles
     int64_t stripes = 0;
     for (size_t m = 0; m < B; ++m) stripes += ceil_div(hd[m].N, G_M);
-    const int split = (int)std::max<int64_t>(1, std::min<int64_t>(4, 256 / std::max<int64_t>(1, stripes)));
+    // (64-bit tiles are half as wide and their fold is heavier: up to eight per stripe)
+    const int split = std::is_same<LT, LimbsU64>::value
+                          ? (int)std::max<int64_t>(1, std::min<int64_t>(8, 512 / std::max<int64_t>(1, stripes)))
+                          : (int)std::max<int64_t>(1, std::min<int64_t>(4, 256 / std::max<int64_t>(1, stripes)));
     out->inner_split = split;
     int64_t* pa = hs;
     int64_t* ma = pa + (B + 1);
@@ -1624,8 +1639,8 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
     for (size_t m = 0; m < B; ++m) {
         const BM& d = hd[m];
         ma[m] = ceil_div(d.N, G_M);
-        // inner GEMM: 64-bit storage tiles the window, 32-bit walks it per row stripe
-        pa[m + 1] = pa[m] + (part == 0 ? ma[m] * (std::is_same<LT, LimbsU64>::value ? XLD / TN : split) : 0);
+        // inner GEMM: split workgroups per 64-row stripe
+        pa[m + 1] = pa[m] + (part == 0 ? ma[m] * split : 0);
         const bool live = part != 0 && d.Kq > 0 && d.nT > 0;
         const bool wlive = part == 1 && d.kA > 0 && d.q0 > 0 && d.nT > 0;
         mb[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
@@ -1718,7 +1733,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(in.desc);
         if (sl) LINAC_HIP(hipStreamWaitEvent(s, ev_c[i], 0));
         const unsigned grid = (unsigned)(ceil_div(in.inner_tiles, NUM_XCD) * NUM_XCD);
-        if constexpr (wide) k_gemm_inner64<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch);
+        if constexpr (wide) k_gemm_inner64<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch, in.inner_split);
         else k_gemm_inner<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch, in.inner_split);
         ++epoch;
         LINAC_HIP(hipGetLastError());
