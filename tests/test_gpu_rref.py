@@ -203,6 +203,19 @@ def test_u64_blocked_planted(gpu, p):
     assert numpy.array_equal(Rp, Rb)
 
 
+@pytest.mark.parametrize("n", [5000, 9000])
+def test_u64_blocked_matches_plain_many_stripes(gpu, n):
+    """64-bit storage with many 64-row stripes (6 and 3 inner-GEMM workgroups
+    per stripe), rank-deficient rows and columns: blocked and plain agree bit
+    for bit (the plain kernel is pinned against the oracle above)."""
+    p = 2 ** 61 - 1
+    A = _u64_structured(n, n // 2, p, seed=n)
+    Rb, pb, rb = gpu.rref_with_pivots(A, p, method="blocked")
+    Rp, pp, rp = gpu.rref_with_pivots(A, p, method="plain")
+    assert pb == pp and rb == rp < n // 2
+    assert numpy.array_equal(Rb, Rp)
+
+
 def test_u64_blocked_thresholds(gpu):
     """64-bit storage of a prime below 2**32 (forced by the caller) is outside
     the 64-bit limb range: "blocked" refuses it, "auto" takes the plain kernel."""
