@@ -168,22 +168,30 @@ def _gf_device_case(n, p, rank_frac, method, seed=0, dtype="u32"):
     import numpy
     import torch
     import linac_amd
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
     tdt = torch.int32 if dtype == "u32" else torch.int64
-    A = torch.empty((n, n), dtype=tdt, device="cuda")
-    chunk = max(1, (1 << 28) // n)
-    for r0 in range(0, n, chunk):  # int64 draws in slices (a whole int64 copy would double the footprint)
-        A[r0:r0 + chunk] = torch.randint(0, p, (min(chunk, n - r0), n), generator=g, device="cuda",
-                                         dtype=torch.int64).to(tdt)
     r = n if rank_frac >= 1 else int(round(rank_frac * n))
-    if r < n:
-        c = torch.randint(1, 1 << 20, (n - r, 1), generator=g, device="cuda", dtype=torch.int64)
-        A[r:] = ((A[:n - r].to(torch.int64) * c) % p).to(tdt)
+    npdt = numpy.uint32 if dtype == "u32" else numpy.uint64
+
+    def make():
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed)
+        A = torch.empty((n, n), dtype=tdt, device="cuda")
+        chunk = max(1, (1 << 28) // n)
+        for r0 in range(0, n, chunk):  # int64 draws in slices (a whole int64 copy would double the footprint)
+            A[r0:r0 + chunk] = torch.randint(0, p, (min(chunk, n - r0), n), generator=g, device="cuda",
+                                             dtype=torch.int64).to(tdt)
+        if r < n:
+            c = torch.randint(1, 1 << 20, (n - r, 1), generator=g, device="cuda", dtype=torch.int64)
+            A[r:] = ((A[:n - r].to(torch.int64) * c) % p).to(tdt)
+        return A
+
+    A = make()
+    if method != "plain":  # an untimed solve first: the blocked workspace grows to this size
+        linac_amd.rref_device(A.data_ptr(), n, n, n, p, npdt, method=method)
+        A = make()
     torch.cuda.synchronize()
     t = time.perf_counter()
-    piv, rank = linac_amd.rref_device(A.data_ptr(), n, n, n, p, numpy.uint32 if dtype == "u32" else numpy.uint64,
-                                      method=method)
+    piv, rank = linac_amd.rref_device(A.data_ptr(), n, n, n, p, npdt, method=method)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t
     ok = rank == r and piv == list(range(r))
@@ -222,6 +230,10 @@ def _float_case(kind, n, seed=0):
     out = {}
     for method in ("blocked", "plain"):
         w = d.clone()
+        if method == "blocked":  # an untimed solve first: the blocked workspace grows to this size
+            linac_amd.rref_device(w.data_ptr(), n, n + 1, n + 1, 0,
+                                  numpy.complex128 if kind == "c128" else numpy.float64, method=method)
+            w.copy_(d)
         torch.cuda.synchronize()
         t = time.perf_counter()
         piv, rank = linac_amd.rref_device(w.data_ptr(), n, n + 1, n + 1, 0,
