@@ -730,15 +730,17 @@ __device__ __forceinline__ bool batch_tile(const TileSched& ts, int* m_out, int6
         if (pre[mid] <= t) lo = mid;
         else hi = mid;
     }
-    const int64_t loc = t - pre[lo];
-    const int64_t mt = mts[lo];
-    const int64_t nt = (pre[lo + 1] - pre[lo]) / mt;
-    const int64_t gsz = (int64_t)group * nt;
-    const int64_t g = loc / gsz, tt = loc - g * gsz;
-    const int64_t gm = min((int64_t)group, mt - g * group);
+    // tile counts stay far below 2^32: 32-bit divisions (a 64-bit one costs
+    // several times as many instructions, once per workgroup)
+    const uint32_t loc = (uint32_t)(t - pre[lo]);
+    const uint32_t mt = (uint32_t)mts[lo];
+    const uint32_t nt = (uint32_t)(pre[lo + 1] - pre[lo]) / mt;
+    const uint32_t gsz = (uint32_t)group * nt;
+    const uint32_t g = loc / gsz, tt = loc - g * gsz;
+    const uint32_t gm = min((uint32_t)group, mt - g * (uint32_t)group);
     *m_out = lo;
-    *tm = g * group + tt % gm;
-    *tn = tt / gm;
+    *tm = (int64_t)(g * (uint32_t)group + tt % gm);
+    *tn = (int64_t)(tt / gm);
     return true;
 }
 
