@@ -644,18 +644,23 @@ __device__ static void make_rhs_body(const BMat<LT>& d, int job, int64_t bx, int
         const int nn = threadIdx.x >> 2, tq = (threadIdx.x & 3) * 16;
         const int64_t n = nb + nn;
         if (n < ncols) {
-            int8_t buf[NL][16];
+            // 16 digits of each limb packed into four words with shifts (a
+            // byte array written element-wise ends up in scratch memory)
+            uint32_t wd[NL][4];
+#pragma unroll
+            for (int l = 0; l < NL; ++l)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) wd[l][q] = 0u;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 int8_t dg[NL];
                 balanced_digits(tile[tq + i][nn], dg);
 #pragma unroll
-                for (int l = 0; l < NL; ++l) buf[l][i] = dg[l];
+                for (int l = 0; l < NL; ++l) wd[l][i >> 2] |= (uint32_t)(uint8_t)dg[l] << (8 * (i & 3));
             }
 #pragma unroll
             for (int l = 0; l < NL; ++l) {
-                v4i w;
-                __builtin_memcpy(&w, buf[l], 16);
+                const v4i w = {(int)wd[l][0], (int)wd[l][1], (int)wd[l][2], (int)wd[l][3]};
                 *reinterpret_cast<v4i*>(planes + l * pstride + n * depth + tb + tq) = w;
             }
         }
@@ -1788,27 +1793,37 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         const int nl = (int)gs_grid(maxN * (BO / 4), 16384);
         const bool comp = t1.wq_tiles > 0, main1 = t1.main_tiles > 0;
         constexpr int RW = BO / 64;  // right-operand tiles of W[Q'] per dimension
-        k_make_ops<LT><<<dim3(nl + (comp ? RW * RW : 0), B), 256, 0, so>>>(t1.desc, 0, epoch, nl, comp ? 1 : -1, RW);
-        if (comp) {
-            const unsigned grid = (unsigned)(ceil_div(t1.wq_tiles, NUM_XCD) * NUM_XCD);
-            if constexpr (wide) k_gemm_outer64<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
-            else k_gemm_outer<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
-        }
         const int rx = (int)std::min<int64_t>(2048, ceil_div(t1.cols, 64));
+        const int nr = main1 ? rx * (2 * BO / 64) : 0;
         // (a deferred panel followed by one without pivots still needs its
         // composite's planes, though there is no W' W[Q'] to add)
         bool deferred = false;
         for (auto* it : act) deferred = deferred || it->d.kA > 0;
-        const int nl1 = deferred ? nl : 0;
-        if (nl1 > 0 || main1)
-            k_make_ops<LT><<<dim3(nl1 + (main1 ? rx * (2 * BO / 64) : 0), B), 256, 0, so>>>(t1.desc, 1, epoch, nl1,
-                                                                                          main1 ? 0 : -1, rx);
+        if (!comp && !deferred) {
+            // no composite: -W beside the right operand, one launch
+            k_make_ops<LT><<<dim3(nl + nr, B), 256, 0, so>>>(t1.desc, 0, epoch, nl, main1 ? 0 : -1, rx);
+        } else {
+            k_make_ops<LT><<<dim3(nl + (comp ? RW * RW : 0), B), 256, 0, so>>>(t1.desc, 0, epoch, nl, comp ? 1 : -1,
+                                                                              RW);
+            if (comp) {
+                const unsigned grid = (unsigned)(ceil_div(t1.wq_tiles, NUM_XCD) * NUM_XCD);
+                if constexpr (wide) k_gemm_outer64<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
+                else k_gemm_outer<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
+            }
+            k_make_ops<LT><<<dim3(nl + nr, B), 256, 0, so>>>(t1.desc, 1, epoch, nl, main1 ? 0 : -1, rx);
+        }
         LINAC_HIP(hipGetLastError());
     }
     LINAC_TRY(trailing(t1, prof, tag, operands));
+    // The panel goes back to A on the inner stream, beside the rest of the
+    // update: it writes only the panel's own columns, which no later step
+    // changes, and reads the panel buffer, which only the next inner phase
+    // (after it on that stream) overwrites.
+    LINAC_HIP(hipEventRecord(ev_ready, so));
+    LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
     int64_t maxpanel = 1;
     for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
-    k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, so>>>(t1.desc);
+    k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, s>>>(t1.desc);
     LINAC_HIP(hipGetLastError());
     // The next inner phase waits only for the above.  The composite update's
     // first half is due beside the next step's inner phase, its second half
@@ -1817,14 +1832,13 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     // runs whole, queued behind the next-panel update of the step before
     // its due step; without look-ahead everything runs now.
     const bool lookahead = tuning().lookahead != 0;
-    if (lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));
     if (t2.main_tiles > 0) bulk.push_back(Bulk{t2, 0, step_no + 1});
     if (t3.main_tiles > 0) bulk.push_back(Bulk{t3, 0, step_no + 2});
     if (!lookahead) LINAC_TRY(flush_due(INT64_MAX, prof, tag));
     else if (tuning().timeslice == 0) LINAC_TRY(flush_due(step_no + 1, prof, tag));
     if (!lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));  // the next inner phase waits for all of it
     ++step_no;
-    ev_pending = true;
+    ev_pending = !lookahead;  // with look-ahead the inner stream has waited already
     for (auto* it : act) {
         it->d.kA = it->d.save_w ? it->d.q0 : 0;  // deferred to the next step
         it->d.J += it->d.bcur;
