@@ -87,7 +87,10 @@ int linac_profile_last(double* out, int n);
  *   "timeslice" 0: with look-ahead, each part of the trailing update beyond
  *       the next panel runs as one launch beside the next inner steps (1: in
  *       slices beside each inner step's factorisation, the inner step's GEMM
- *       waiting for its slice).
+ *       waiting for its slice);
+ *   "gemm_lds" 1: the 32-bit trailing GEMM brings its operand stages and its
+ *       C tile into LDS with direct-to-LDS loads where every matrix's rows are
+ *       16-byte aligned (0: register-staged copies everywhere).
  * Unknown keys and out-of-range values return LINAC_ERR_INVALID. */
 int linac_tune_set(const char* key, int64_t value);
 int linac_tune_get(const char* key, int64_t* value);

@@ -23,7 +23,7 @@ import numpy
 from . import _lib
 
 KNOBS = ("gf_blocked_min_rows", "gf_blocked_min_cols", "float_blocked_min_rows", "float_blocked_min_cols",
-         "gemm_group", "lookahead", "timeslice")
+         "gemm_group", "lookahead", "timeslice", "gemm_lds")
 
 
 def get(key):

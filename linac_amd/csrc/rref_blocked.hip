@@ -876,6 +876,160 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
             }
 }
 
+// ---- direct-to-LDS staging (gfx950 global_load_lds_dwordx4) ----
+// s_waitcnt vmcnt(N) with the other counters left alone.
+template <int N>
+__device__ __forceinline__ void vm_wait() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+// 16 bytes per lane from global memory straight into LDS: lane i writes
+// bytes [dst + 16 i, dst + 16 i + 16) (dst uniform across the wave).  Written
+// as asm so that the compiler does not track it (its own tracking would wait
+// for every such load before each LDS read); callers wait with
+// lds_dma_barrier.  The asm has no register outputs.
+__device__ __forceinline__ void dma16(const void* src, void* dst) {
+    const uint32_t lds =
+        __builtin_amdgcn_readfirstlane((uint32_t)(size_t)(__attribute__((address_space(3))) int8_t*)dst);
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds) : "memory", "m0");
+}
+// every wave's direct-to-LDS loads have landed and every wave is past its
+// reads of the buffers they refill
+__device__ __forceinline__ void lds_dma_barrier() {
+    vm_wait<0>();
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+typedef int8_t StageBuf[4][G_M * G_ROW];
+
+// k_gemm_outer with its stages (and, during the last stage, the C tile)
+// brought into LDS by direct-to-LDS loads: no staging registers, no stores
+// to LDS, and one wait per stage.  Wave w stages limb w of both operands, four
+// 16-row chunks each: lane i covers row i / 4 of a chunk and LDS column i % 4,
+// which holds source column (i % 4) ^ ((i / 16) % 4) (the swizzle of lds_at).
+// The C tile (64 rows x 256 bytes) goes into the stage buffer the last stage
+// does not read.  Requires 16-byte aligned C rows (the host checks; rows and
+// columns past the matrix read clamped addresses and are never stored) and
+// operand offsets below 2^32 bytes.
+template <int JOB>
+__global__ __launch_bounds__(G_T, 2) void k_gemm_outer_lds(const BMat<LimbsU32>* __restrict__ D, TileSched ts) {
+    __shared__ __attribute__((aligned(16))) StageBuf sA0, sA1, sB0, sB1;
+    int mi_;
+    int64_t tm, tn;
+    if (!batch_tile(ts, &mi_, &tm, &tn)) return;
+    const BMat<LimbsU32>& d = D[mi_];
+    uint32_t* C = JOB == 0 ? d.A + d.T0 : d.W;
+    const int64_t ldc = JOB == 0 ? d.ld : BO;
+    const int64_t Mrows = d.N;
+    const int64_t Ncols = JOB == 0 ? d.nT : d.kA;
+    const int64_t ldL = d.Kq;
+    const int8_t* L = d.planesL + (JOB == 0 ? 0 : d.KqA);
+    const int8_t* B = JOB == 0 ? d.planesB : d.planesW;
+    const int64_t planeB = JOB == 0 ? d.strideB : STRIDE_W;
+    const int Kpad = JOB == 0 ? d.Kq : d.Kq - d.KqA;
+    const double pd = d.pd, pinv = d.pinv;
+    const int64_t m_base = tm * G_M, n_base = tn * G_N;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int r16 = lane & 15, g = lane >> 4;
+    const v4i zero = {0, 0, 0, 0};
+    const int lrow = lane >> 2, lqu = (lane & 3) ^ ((lane >> 4) & 3);
+    const int8_t* la = L + wid * d.strideL;
+    const int8_t* lb = B + wid * planeB;
+    uint32_t oa[4], ob[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        oa[c] = (uint32_t)(min(m_base + 16 * c + lrow, Mrows - 1) * ldL + lqu * 16);
+        ob[c] = (uint32_t)(min(n_base + 16 * c + lrow, Ncols - 1) * Kpad + lqu * 16);
+    }
+    auto issue = [&](int kk, StageBuf& dA, StageBuf& dB) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            dma16(la + (oa[c] + (uint32_t)(kk * G_K)), &dA[wid][c * 1024]);
+            dma16(lb + (ob[c] + (uint32_t)(kk * G_K)), &dB[wid][c * 1024]);
+        }
+    };
+    // wave w, chunk c: rows 16 w + 4 c .. + 3; lane i: row + i / 16, columns 4 (i % 16) .. + 3
+    auto issue_c = [&](StageBuf& dst) {
+        int8_t* base = &dst[0][0];
+        const int64_t cmax = (Ncols - 1) / 4 * 4;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int rl = 16 * wid + 4 * c + (lane >> 4);
+            const int64_t row = min(m_base + rl, Mrows - 1);
+            const int64_t col = min(n_base + (lane & 15) * 4, cmax);
+            dma16(C + row * ldc + col, base + (16 * wid + 4 * c) * 256);
+        }
+    };
+    v4i acc[7][2][2];
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[q][i][j] = zero;
+    auto mfmas = [&](StageBuf& cA, StageBuf& cB) {
+        v4i bf[4][2];
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+                bf[l][ni] = *reinterpret_cast<const v4i*>(&cB[l][lds_at(wn * 32 + ni * 16 + r16, g)]);
+#pragma unroll
+        for (int l1 = 0; l1 < 4; ++l1) {
+            v4i af[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+                af[mi] = *reinterpret_cast<const v4i*>(&cA[l1][lds_at(wm * 32 + mi * 16 + r16, g)]);
+#pragma unroll
+            for (int l2 = 0; l2 < 4; ++l2)
+#pragma unroll
+                for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < 2; ++ni)
+                        acc[l1 + l2][mi][ni] =
+                            __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
+        }
+    };
+    const int nk = Kpad / G_K;
+    auto stage = [&](int kk, StageBuf& cA, StageBuf& cB, StageBuf& nA, StageBuf& nB) {
+        lds_dma_barrier();
+        if (kk + 1 < nk) issue(kk + 1, nA, nB);
+        else issue_c(nA);
+        mfmas(cA, cB);
+    };
+    issue(0, sA0, sB0);
+    for (int kk = 0; kk < nk; kk += 2) {
+        stage(kk, sA0, sB0, sA1, sB1);
+        if (kk + 1 < nk) stage(kk + 1, sA1, sB1, sA0, sB0);
+    }
+    lds_dma_barrier();
+    const uint32_t* sc = reinterpret_cast<const uint32_t*>(nk & 1 ? &sA1[0][0] : &sA0[0][0]);
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int rl = wm * 32 + mi * 16 + 4 * g + i, cl = wn * 32 + ni * 16 + r16;
+                const int64_t row = m_base + rl, col = n_base + cl;
+                if (row < Mrows && col < Ncols) {
+                    int32_t sv[7];
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) sv[q] = acc[q][mi][ni][i];
+                    gst<uint32_t>(C + row * ldc + col, fold_sub(sc[rl * 64 + cl], sv, pd, pinv));
+                }
+            }
+}
+
+// The 32-bit trailing GEMM launch: direct-to-LDS staging when allowed (tuning
+// knob "gemm_lds" and the part's alignment), else register-staged copies.
+template <int JOB>
+static void launch_outer32(unsigned grid, hipStream_t st, const BMat<LimbsU32>* desc, const TileSched& ts, bool lds_ok) {
+    if (lds_ok && tuning().gemm_lds != 0) k_gemm_outer_lds<JOB><<<grid, G_T, 0, st>>>(desc, ts);
+    else k_gemm_outer<JOB><<<grid, G_T, 0, st>>>(desc, ts);
+}
+
 // Inner-step GEMM on the panel buffer:  X[r, :] -= X[r, c0 + Cp] U  for rows
 // that are not pivots of this step (k_make_u has set X[R[t], :] = U[t, :] for
 // those that are).  K = 64 (k <= 32 pivots, zero padded).  One workgroup owns a 64-row
@@ -1449,6 +1603,7 @@ class Engine {
         ItemStatus* status;
         double macs, bytes;         // algorithmic work of the trailing update (profiler)
         int64_t cols;               // widest column range of the part
+        bool lds_main, lds_wq;      // 32-bit trailing / composite GEMMs may stage through direct-to-LDS loads
     };
     int stage(int slot, size_t bytes);
     // part 0: inner phase; 1: the next panel's columns; 2, 3: the two halves
@@ -1517,7 +1672,7 @@ int Engine<LT>::launch_slice(Bulk& b, int64_t ntiles, Profiler* prof, int64_t ta
     if (prof) LINAC_TRY(prof->sample_begin(so, tag, t.bytes * frac, t.macs * frac));
     const unsigned grid = (unsigned)(ceil_div(n, NUM_XCD) * NUM_XCD);
     if constexpr (std::is_same<LT, LimbsU64>::value) k_gemm_outer64<0><<<grid, G_T, 0, so>>>(t.desc, ts);
-    else k_gemm_outer<0><<<grid, G_T, 0, so>>>(t.desc, ts);
+    else launch_outer32<0>(grid, so, t.desc, ts, t.lds_main);
     if (prof) LINAC_TRY(prof->sample_end(so));
     LINAC_HIP(hipGetLastError());
     b.next += n;
@@ -1547,7 +1702,7 @@ int Engine<LT>::trailing(const Staged& t, Profiler* prof, int64_t tag, bool rhs_
     if (prof) LINAC_TRY(prof->sample_begin(so, tag, t.bytes, t.macs));
     const unsigned grid = (unsigned)(ceil_div(t.main_tiles, NUM_XCD) * NUM_XCD);
     if constexpr (std::is_same<LT, LimbsU64>::value) k_gemm_outer64<0><<<grid, G_T, 0, so>>>(t.desc, t.main);
-    else k_gemm_outer<0><<<grid, G_T, 0, so>>>(t.desc, t.main);
+    else launch_outer32<0>(grid, so, t.desc, t.main, t.lds_main);
     if (prof) LINAC_TRY(prof->sample_end(so));
     LINAC_HIP(hipGetLastError());
     return LINAC_OK;
@@ -1655,6 +1810,15 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
         mc[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
         pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, TN) : 0);
     }
+    // direct-to-LDS staging: 16-byte aligned C rows, operand offsets below 2^31
+    bool offs = true, aligned = true;
+    for (size_t m = 0; m < B; ++m) {
+        const BM& d = hd[m];
+        offs = offs && d.N * (int64_t)d.Kq < (1ll << 31) && d.nT * (int64_t)d.Kq < (1ll << 31);
+        aligned = aligned && ((((uintptr_t)(d.A + d.T0)) | (uintptr_t)(d.ld * (int64_t)sizeof(typename LT::T))) & 15) == 0;
+    }
+    out->lds_main = offs && aligned;
+    out->lds_wq = offs;
     out->desc = (BM*)dv;
     const int grp = (int)tuning().gemm_group;
     out->inner = TileSched{ds, ds + (B + 1), (int)B, grp, 0, -1};
@@ -1808,7 +1972,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
             if (comp) {
                 const unsigned grid = (unsigned)(ceil_div(t1.wq_tiles, NUM_XCD) * NUM_XCD);
                 if constexpr (wide) k_gemm_outer64<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
-                else k_gemm_outer<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
+                else launch_outer32<1>(grid, so, t1.desc, t1.wq, t1.lds_wq);
             }
             k_make_ops<LT><<<dim3(nl + nr, B), 256, 0, so>>>(t1.desc, 1, epoch, nl, main1 ? 0 : -1, rx);
         }
@@ -2072,8 +2236,10 @@ static int gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda
             B, ldb, b_rows ? d_brows : nullptr, t0, kc, (int)kp[c], n, p, planesB + offB[c], sB[c]);
     }
     const TileSched ts{d_sched, d_sched + 2, 1, (int)tuning().gemm_group, 0, -1};
+    bool lds_ok = (((uintptr_t)C | (uintptr_t)(ldc * 4)) & 15) == 0;
+    for (int64_t c = 0; c < nch; ++c) lds_ok = lds_ok && m * kp[c] < (1ll << 31) && n * kp[c] < (1ll << 31);
     for (int64_t c = 0; c < nch; ++c)
-        k_gemm_outer<0><<<(unsigned)(ceil_div(ntiles, NUM_XCD) * NUM_XCD), G_T, 0, s>>>(d_desc + c, ts);
+        launch_outer32<0>((unsigned)(ceil_div(ntiles, NUM_XCD) * NUM_XCD), s, d_desc + c, ts, lds_ok);
     LINAC_HIP(hipGetLastError());
     LINAC_HIP(hipStreamSynchronize(s));
     return LINAC_OK;

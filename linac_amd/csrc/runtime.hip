@@ -275,7 +275,8 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"float_blocked_min_cols", &t.float_blocked_min_cols, 1, INT64_MAX},
                  {"gemm_group", &t.gemm_group, 1, 64},
                  {"lookahead", &t.lookahead, 0, 1},
-                 {"timeslice", &t.timeslice, 0, 1}};
+                 {"timeslice", &t.timeslice, 0, 1},
+                 {"gemm_lds", &t.gemm_lds, 0, 1}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             if (value < k.lo || value > k.hi)
@@ -296,7 +297,7 @@ int linac_tune_get(const char* key, int64_t* value) {
     } knobs[] = {{"gf_blocked_min_rows", t.gf_blocked_min_rows}, {"gf_blocked_min_cols", t.gf_blocked_min_cols},
                  {"float_blocked_min_rows", t.float_blocked_min_rows},
                  {"float_blocked_min_cols", t.float_blocked_min_cols}, {"gemm_group", t.gemm_group},
-                 {"lookahead", t.lookahead}, {"timeslice", t.timeslice}};
+                 {"lookahead", t.lookahead}, {"timeslice", t.timeslice}, {"gemm_lds", t.gemm_lds}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             *value = k.v;

@@ -110,6 +110,7 @@ struct Tuning {
     int64_t gemm_group = 8;
     int64_t lookahead = 1;
     int64_t timeslice = 0;
+    int64_t gemm_lds = 1;
 };
 Tuning& tuning();
 

@@ -145,6 +145,10 @@ def test_tuning_knobs_set_get_and_validation():
         assert autotune.get("timeslice") == 1
         with pytest.raises(ValueError):
             autotune.set_knob("timeslice", 2)
+        autotune.set_knob("gemm_lds", 0)
+        assert autotune.get("gemm_lds") == 0
+        with pytest.raises(ValueError):
+            autotune.set_knob("gemm_lds", 2)
         with pytest.raises(ValueError):
             autotune.set_knob("no_such_knob", 1)
         with pytest.raises(ValueError):

@@ -62,3 +62,40 @@ def test_time_sliced_trailing_parts_stay_exact(gpu, n, rank, p):
     Ro, po = crref.rref_modp(A, p)
     assert piv == po and r == len(po) and numpy.array_equal(R, Ro)
     assert numpy.array_equal(Rb, numpy.identity(3000, dtype=Rb.dtype))
+
+
+@pytest.mark.parametrize("n, rank, p", [(1500, 1300, 2 ** 31 - 1), (1300, None, 4278124237)])
+def test_direct_to_lds_trailing_gemm_matches_register_staged(gpu, n, rank, p):
+    """The trailing GEMM staged by direct-to-LDS loads (knob gemm_lds = 1) and
+    the register-staged one (0) give the oracle's RREF, including the
+    composite updates of a rank-deficient input, and the same standalone
+    modular GEMM on 16-byte aligned rows."""
+    import torch
+    from linac_amd import autotune
+    from linac_amd.incremental import gemm_sub
+    before = autotune.current()
+    rs = numpy.random.RandomState(31)
+    if rank is not None:
+        A = modp.low_rank(n, n + 200, rank, p, 31)
+    else:
+        A = rs.randint(0, 2 ** 62, size=(n, n + 200), dtype=numpy.int64).astype(numpy.uint64) % numpy.uint64(p)
+    C = rs.randint(0, p, size=(700, 384), dtype=numpy.int64)
+    L = rs.randint(0, p, size=(700, 450), dtype=numpy.int64)
+    Bm = rs.randint(0, p, size=(450, 384), dtype=numpy.int64)
+    dev = lambda a: torch.from_numpy(a.astype(numpy.uint32).view(numpy.int32)).cuda()
+    out = {}
+    try:
+        for v in (0, 1):
+            autotune.set_knob("gemm_lds", v)
+            R, piv, r = gpu.rref_with_pivots(A, p, method="blocked")
+            dC = dev(C)
+            gemm_sub(dC, dev(L), dev(Bm), p, k=450)
+            out[v] = (R, piv, r, dC.cpu().numpy().view(numpy.uint32).copy())
+    finally:
+        autotune.apply(before)
+    Ro, po = crref.rref_modp(A, p)
+    want = (C.astype(object) - (L.astype(object) @ Bm.astype(object)) % p) % p
+    for v in (0, 1):
+        R, piv, r, G = out[v]
+        assert piv == po and r == len(po) and numpy.array_equal(R, Ro), v
+        assert numpy.array_equal(G.astype(object), want), v
