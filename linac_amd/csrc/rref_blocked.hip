@@ -62,16 +62,18 @@ constexpr int KPI = 64;        // padded depth of the inner GEMM
 // and the number of balanced base-256 digits (int8 limb planes) per entry.
 // 32-bit: p <= 4278124287, 4 digits, 16 limb products on 7 diagonals;
 // 64-bit: 2^32 <= p < 2^63, 8 digits, 64 limb products on 15 diagonals.
-// IF_WAVES: minimum waves per SIMD of the inner factorisation, i.e. its
-// register cap (512 / IF_WAVES).  The 64-bit one is held to 64 registers so
-// that its 16 waves fit in the space one GEMM workgroup frees on a CU; the
-// trailing update of the previous panel runs beside it and otherwise refills
-// every freed slot first.
+// IF_T: threads of the inner factorisation; IF_WAVES: its minimum waves per
+// SIMD, i.e. its register cap (512 / IF_WAVES).  The 64-bit one runs 8 waves
+// held to 128 registers so that they fit in the space one GEMM workgroup frees
+// on a CU (the trailing update of the previous panel runs beside it and
+// otherwise refills every freed slot first); 16 waves would have to live in
+// 64 registers, which spills.
 struct LimbsU32 {
     using T = uint32_t;
     using F = FieldU32;
     using S = int32_t;
     static constexpr int NL = 4;
+    static constexpr int IF_T = 1024;
     static constexpr int IF_WAVES = 1;
 };
 struct LimbsU64 {
@@ -79,7 +81,8 @@ struct LimbsU64 {
     using F = FieldU64;
     using S = int64_t;
     static constexpr int NL = 8;
-    static constexpr int IF_WAVES = 8;
+    static constexpr int IF_T = 512;
+    static constexpr int IF_WAVES = 4;
 };
 
 template <class T>
@@ -212,9 +215,6 @@ __device__ static void block_state_init(const BMat<LT>& d, int reset_timers) {
 // inner factorisation: one workgroup per matrix
 // ---------------------------------------------------------------------------
 
-constexpr int IF_T = 1024;  // threads of the inner factorisation: 16 waves
-constexpr int IF_RPW = CI / (IF_T / 64);  // candidate rows per wave
-static_assert(IF_RPW * (IF_T / 64) == CI && IF_T >= CI, "waves split the candidate rows evenly");
 static_assert(BI + CI == 64, "one augmented column per lane");
 static_assert((CI & (CI - 1)) == 0 && CI <= 32, "pivot slots index with a mask; the inner GEMM holds k <= 32 in its first two lane groups");
 
@@ -237,6 +237,9 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
                                          int64_t* __restrict__ piv_rows, int64_t* __restrict__ piv_cols,
                                          BlockState<typename LT::T>* bs) {
     using T = typename LT::T;
+    constexpr int IF_T = LT::IF_T;
+    constexpr int IF_RPW = CI / (IF_T / 64);  // candidate rows per wave
+    static_assert(IF_RPW * (IF_T / 64) == CI && IF_T >= 256, "waves split the candidate rows evenly; SCAN rows per probe");
     constexpr int W2 = BI + CI;
     __shared__ T aug[CI][W2 + 1];
     __shared__ int64_t cand[CI + 1];
@@ -438,7 +441,7 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
 
 
 template <class LT>
-__global__ __launch_bounds__(IF_T, LT::IF_WAVES) void k_inner_factor(const BMat<LT>* __restrict__ D, int epoch) {
+__global__ __launch_bounds__(LT::IF_T, LT::IF_WAVES) void k_inner_factor(const BMat<LT>* __restrict__ D, int epoch) {
     const BMat<LT>& d = D[blockIdx.x];
     inner_factor_body<LT>(d.X, d.N, d.used, d.slot_of, d.epoch_of, epoch, d.J, d.f, d.st, d.piv_rows, d.piv_cols, d.bs);
 }
@@ -1900,7 +1903,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
             }
             LINAC_HIP(hipEventRecord(ev_c[i], so));
         }
-        k_inner_factor<LT><<<B, IF_T, 0, s>>>(in.desc, epoch);
+        k_inner_factor<LT><<<B, LT::IF_T, 0, s>>>(in.desc, epoch);
         k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(in.desc);
         if (sl) LINAC_HIP(hipStreamWaitEvent(s, ev_c[i], 0));
         const unsigned grid = (unsigned)(ceil_div(in.inner_tiles, NUM_XCD) * NUM_XCD);
