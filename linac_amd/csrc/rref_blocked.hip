@@ -1573,6 +1573,7 @@ class Engine {
             if (h_buf[k]) (void)hipHostFree(h_buf[k]);
         }
         if (ev_ready) (void)hipEventDestroy(ev_ready);
+        if (ev_status) (void)hipEventDestroy(ev_status);
         for (int i = 0; i < NEV; ++i) {
             if (ev_f[i]) (void)hipEventDestroy(ev_f[i]);
             if (ev_c[i]) (void)hipEventDestroy(ev_c[i]);
@@ -1603,7 +1604,8 @@ class Engine {
         TileSched inner, wq, main;  // inner GEMM, composite W GEMM, trailing update
         int64_t inner_tiles, wq_tiles, main_tiles;
         int inner_split;            // 32-bit inner GEMM: workgroups per row stripe
-        ItemStatus* status;
+        ItemStatus* status;         // device copy; status_h: its pinned host mirror
+        ItemStatus* status_h;
         double macs, bytes;         // algorithmic work of the trailing update (profiler)
         int64_t cols;               // widest column range of the part
         bool lds_main, lds_wq;      // 32-bit trailing / composite GEMMs may stage through direct-to-LDS loads
@@ -1627,6 +1629,7 @@ class Engine {
     int ensure_stream();
     hipStream_t so = nullptr;
     hipEvent_t ev_ready = nullptr;  // on `so`: next panel's columns updated, panel buffer free
+    hipEvent_t ev_status = nullptr;  // on `s`: the inner phase's status has reached host memory
     bool ev_pending = false;
     std::vector<Bulk> bulk;         // queued trailing parts, in due order
     int64_t step_no = 0;
@@ -1643,6 +1646,7 @@ int Engine<LT>::ensure_stream() {
     if (so) return LINAC_OK;
     LINAC_HIP(hipStreamCreateWithFlags(&so, hipStreamNonBlocking));
     LINAC_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    LINAC_HIP(hipEventCreateWithFlags(&ev_status, hipEventDisableTiming));
     for (int i = 0; i < NEV; ++i) {
         LINAC_HIP(hipEventCreateWithFlags(&ev_f[i], hipEventDisableTiming));
         LINAC_HIP(hipEventCreateWithFlags(&ev_c[i], hipEventDisableTiming));
@@ -1831,6 +1835,7 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
     out->wq_tiles = pb[B];
     out->main_tiles = pc[B];
     out->status = (ItemStatus*)(dv + desc_b + sched_b);
+    out->status_h = (ItemStatus*)(h + desc_b + sched_b);  // pinned; never part of an upload
 }
 
 static unsigned gs_grid(int64_t work, int64_t cap = 4096) {
@@ -1917,9 +1922,20 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     // one synchronisation per step; extra groups only for rank-deficient panels
     std::vector<ItemStatus> st(B);
     for (int64_t guard = 0;; ++guard) {
+        // into pinned memory, then a spin on the event: the next launches are
+        // on the critical path whenever the inner phase outlasts the trailing
+        // update, and a blocking synchronisation wakes up late
         k_gather_status<LT><<<B, 1, 0, s>>>(in.desc, in.status);
-        LINAC_HIP(hipMemcpyAsync(st.data(), in.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
-        LINAC_HIP(hipStreamSynchronize(s));
+        LINAC_HIP(hipMemcpyAsync(in.status_h, in.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipEventRecord(ev_status, s));
+        for (;;) {
+            const hipError_t q = hipEventQuery(ev_status);
+            if (q == hipSuccess) break;
+            if (q != hipErrorNotReady)
+                return fail(LINAC_ERR_DEVICE, "hipEventQuery: %s", hipGetErrorString(q));
+            __builtin_ia32_pause();
+        }
+        std::copy(in.status_h, in.status_h + B, st.begin());
         int64_t behind = 0;
         for (unsigned m = 0; m < B; ++m)
             if (st[m].c0_next < st[m].bcur) behind = std::max<int64_t>(behind, ceil_div(st[m].bcur - st[m].c0_next, BI));
