@@ -756,6 +756,8 @@ __device__ __forceinline__ bool batch_tile(const TileSched& ts, int* m_out, int6
 //   JOB 0: A[:, T] <- A[:, T] + W A[Qc, T]   (the whole depth Kq, every row);
 //   JOB 1: W <- W + W' W[Q']                 (the composite transformation of a
 //          deferred panel: L = the current panel's part, B = planesW).
+// This form stages through registers; k_gemm_outer_lds (below) does the same
+// work with direct-to-LDS loads and runs wherever C rows are 16-byte aligned.
 template <int JOB>
 __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __restrict__ D, TileSched ts) {
     __shared__ __attribute__((aligned(16))) int8_t sA[2][4][G_M * G_ROW];
