@@ -402,7 +402,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(independent solves shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+              "(independent solves shard across GPUs without exchange; one solve's row stripes would "
+              "need a broadcast of each panel's pivot rows)", file=sys.stderr)
         sys.exit(2)
     import numpy
     import torch
