@@ -61,6 +61,14 @@ int linac_available(void);
 int linac_init(int device);
 /* Synchronise and release cached device memory of the current device. */
 int linac_release_cache(void);
+/* Synchronise every device this process used and release everything the
+ * library holds there: cached engines, open batches (their handles become
+ * invalid), scratch, the pinned status page, events and streams.  Later calls
+ * start afresh.  Also runs by itself at process exit (registered with atexit
+ * once the HIP runtime is up, so it runs before the runtime's own teardown).
+ * No counterpart in the reference, which leaves its context to PyCUDA's
+ * autoinit (linac/_optional.py:15-17). */
+int linac_shutdown(void);
 /* 1 if n is prime (deterministic Miller-Rabin), else 0.  Host only. */
 int linac_is_prime(uint64_t n);
 /* HIP-event timings of this thread's last call made with LINAC_PROFILE, all

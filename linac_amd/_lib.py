@@ -4,6 +4,7 @@ The shared library is built in-tree by ``linac_amd.build`` (hipcc, gfx950).
 There is no JIT and no CPU fallback: if the library is missing, every compute
 call raises ``RuntimeError`` naming the file it looked for.
 """
+import atexit
 import ctypes
 import importlib.util
 import os
@@ -32,6 +33,7 @@ METHODS = {"auto": METHOD_AUTO, "plain": METHOD_PLAIN, "blocked": METHOD_BLOCKED
 _lock = threading.Lock()
 _lib = None
 _load_error = None
+_load_pid = os.getpid()
 
 _i64 = ctypes.c_int64
 _u32 = ctypes.c_uint32
@@ -50,6 +52,7 @@ SIGNATURES = {
     "linac_available": (ctypes.c_int, []),
     "linac_init": (ctypes.c_int, [ctypes.c_int]),
     "linac_release_cache": (ctypes.c_int, []),
+    "linac_shutdown": (ctypes.c_int, []),
     "linac_is_prime": (ctypes.c_int, [_u64]),
     "linac_profile_last": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "linac_rref_u32": (ctypes.c_int, [_vp, _i64, _i64, _i64, _u32, _u32, _vp, _i64p, _i64p]),
@@ -116,6 +119,10 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = lib
+        # release streams, pinned memory and device buffers while torch and
+        # the HIP runtime are still up (the library's own exit hook would run
+        # after the interpreter is gone)
+        atexit.register(_shutdown_at_exit)
     # a saved tuning for this device (linac_amd.autotune) applies from the start
     try:
         from . import autotune
@@ -124,6 +131,12 @@ def load():
     except (OSError, ValueError, RuntimeError):
         pass
     return lib
+
+
+def _shutdown_at_exit():
+    lib = _lib
+    if lib is not None and os.getpid() == _load_pid:
+        lib.linac_shutdown()
 
 
 def last_error():

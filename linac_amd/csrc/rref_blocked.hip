@@ -1592,6 +1592,15 @@ class Engine {
     int drain();
     int epoch = 0;
     bool first_step = true;
+    // Ready for a new solve after drain(): the engine (its stream, events and
+    // staging buffers) is cached per device and reused across calls.
+    void reset_for_solve() {
+        epoch = 0;
+        first_step = true;
+        step_no = 0;
+        bulk.clear();
+        ev_pending = false;
+    }
 
   private:
     // Staging slots (pinned host + device): 0 = inner phase; 1, 2 = the three
@@ -2057,6 +2066,29 @@ static void print_timers(const BMat<LT>& d, DeviceContext* ctx) {
             tp[4] * us / tp[5]);
 }
 
+// The context's engine for storage width LT (created on first use; freed by
+// the shutdown hook with the context).
+template <class LT>
+static void free_engine(void* e) {
+    delete static_cast<Engine<LT>*>(e);
+}
+
+static void free_cached_engines(DeviceContext* ctx) {
+    if (ctx->engines[0]) free_engine<LimbsU32>(ctx->engines[0]);
+    if (ctx->engines[1]) free_engine<LimbsU64>(ctx->engines[1]);
+    ctx->engines[0] = ctx->engines[1] = nullptr;
+}
+
+template <class LT>
+static Engine<LT>* cached_engine(DeviceContext* ctx) {
+    void*& slot = ctx->engines[std::is_same<LT, LimbsU64>::value ? 1 : 0];
+    if (!slot) {
+        add_shutdown_hook(free_cached_engines);
+        slot = new Engine<LT>();
+    }
+    return static_cast<Engine<LT>*>(slot);
+}
+
 // Single matrix through the engine (a batch of one); workspace from the
 // context's cached scratch, elimination state from `ws`.
 template <class LT>
@@ -2072,12 +2104,16 @@ static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, cons
     it.d.piv_rows = ws.piv_rows;
     it.d.piv_cols = ws.piv_cols;
     it.d.used = ws.used;
-    Engine<LT> eng;
+    Engine<LT>& eng = *cached_engine<LT>(ctx);
+    eng.reset_for_solve();
     std::vector<Item<LT>*> items{&it};
     // state and row flags were initialised by the caller; epochs here
     k_epoch_init<<<gs_grid(N), 256, 0, s>>>(it.d.epoch_of, N);
-    while (!it.finished) LINAC_TRY(eng.step(items, s, prof));
-    LINAC_TRY(eng.drain());
+    int rc = LINAC_OK;
+    while (!it.finished && rc == LINAC_OK) rc = eng.step(items, s, prof);
+    const int rd = eng.drain();
+    LINAC_TRY(rc);
+    LINAC_TRY(rd);
     LINAC_HIP(hipStreamSynchronize(s));
     if (getenv("LINAC_DEBUG_TIMING")) print_timers(it.d, ctx);
     return LINAC_OK;
@@ -2297,6 +2333,44 @@ static Item32* find_item(linac_batch* b, int64_t id) {
     return nullptr;
 }
 
+// Open batches, so that linac_shutdown (and process exit) can release the
+// ones a caller never destroyed, and a handle released that way is refused
+// rather than used.
+static std::mutex g_batches_mu;
+static std::vector<linac_batch*> g_batches;
+
+static bool batch_live(linac_batch* b) {
+    std::lock_guard<std::mutex> lk(g_batches_mu);
+    return std::find(g_batches.begin(), g_batches.end(), b) != g_batches.end();
+}
+
+static void batch_free(linac_batch* b) {
+    (void)b->eng->drain();  // no trailing update may still run on an item's memory
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (auto* it : b->items) {
+        if (it->arena) (void)hipFree(it->arena);
+        delete it;
+    }
+    delete b->eng;
+    delete b;
+}
+
+static void free_context_batches(DeviceContext* ctx) {
+    std::vector<linac_batch*> mine;
+    {
+        std::lock_guard<std::mutex> lk(g_batches_mu);
+        for (auto it = g_batches.begin(); it != g_batches.end();) {
+            if ((*it)->ctx == ctx) {
+                mine.push_back(*it);
+                it = g_batches.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    for (linac_batch* b : mine) batch_free(b);
+}
+
 }  // namespace linac
 
 using namespace linac;
@@ -2317,21 +2391,25 @@ int linac_batch_create(linac_batch** out, uint64_t mem_cap_bytes) {
     b->ctx = ctx;
     b->eng = new Engine<LimbsU32>();
     b->cap = mem_cap_bytes;
+    add_shutdown_hook(free_context_batches);
+    {
+        std::lock_guard<std::mutex> lk(g_batches_mu);
+        g_batches.push_back(b);
+    }
     *out = b;
     return LINAC_OK;
 }
 
 int linac_batch_destroy(linac_batch* b) {
     if (!b) return LINAC_OK;
-    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
-    (void)b->eng->drain();  // no trailing update may still run on an item's memory
-    (void)hipStreamSynchronize(b->ctx->stream);
-    for (auto* it : b->items) {
-        if (it->arena) (void)hipFree(it->arena);
-        delete it;
+    {
+        std::lock_guard<std::mutex> lk(g_batches_mu);
+        auto it = std::find(g_batches.begin(), g_batches.end(), b);
+        if (it == g_batches.end()) return LINAC_OK;  // already released by linac_shutdown
+        g_batches.erase(it);
     }
-    delete b->eng;
-    delete b;
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    batch_free(b);
     return LINAC_OK;
 }
 
@@ -2345,6 +2423,7 @@ int linac_batch_memory(linac_batch* b, uint64_t* in_use_out, uint64_t* cap_out) 
 int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p,
                         uint32_t flags, int64_t* id_out) {
     if (!b || !id_out) return fail(LINAC_ERR_INVALID, "batch or id_out is NULL");
+    if (!batch_live(b)) return fail(LINAC_ERR_INVALID, "batch handle was released (linac_shutdown)");
     if (rows <= 0 || cols <= 0) return fail(LINAC_ERR_INVALID, "empty matrix");
     if (ld < cols) return fail(LINAC_ERR_INVALID, "ld < cols");
     if (!a) return fail(LINAC_ERR_INVALID, "matrix pointer is NULL");
@@ -2394,6 +2473,7 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
 
 int linac_batch_step(linac_batch* b, int64_t* unfinished_out) {
     if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
+    if (!batch_live(b)) return fail(LINAC_ERR_INVALID, "batch handle was released (linac_shutdown)");
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
     hipStream_t s = b->ctx->stream;
     LINAC_TRY(b->eng->init_items(b->fresh, s));

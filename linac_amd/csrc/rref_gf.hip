@@ -208,8 +208,7 @@ int plain_run(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hi
     constexpr int V = 16 / sizeof(T);
     // sample ~256 row-update launches spread over the sweep
     const int64_t stride = std::max<int64_t>(1, M / 256);
-    hipEvent_t probe = nullptr;
-    LINAC_HIP(hipEventCreateWithFlags(&probe, hipEventDisableTiming));
+    hipEvent_t probe = ctx->probe;
     bool probe_pending = false;
     ctx->pinned[0] = 0;
     for (int64_t j = 0; j < M; ++j) {
@@ -245,7 +244,6 @@ int plain_run(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hi
     }
     LINAC_HIP(hipGetLastError());
     LINAC_HIP(hipStreamSynchronize(s));
-    LINAC_HIP(hipEventDestroy(probe));
     return LINAC_OK;
 }
 

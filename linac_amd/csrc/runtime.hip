@@ -1,10 +1,12 @@
 // Per-device context, error reporting, scratch memory and host utilities.
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 #include <unistd.h>
 
 #include <array>
+#include <vector>
 
 #include "runtime.hpp"
 
@@ -76,6 +78,67 @@ int DeviceContext::release() {
     return LINAC_OK;
 }
 
+static std::vector<ShutdownHook>& shutdown_hooks() {
+    static std::vector<ShutdownHook> hooks;
+    return hooks;
+}
+
+void add_shutdown_hook(ShutdownHook fn) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    for (ShutdownHook h : shutdown_hooks())
+        if (h == fn) return;
+    shutdown_hooks().push_back(fn);
+}
+
+// Synchronises and frees everything the library holds on every device:
+// cached engines and batches (hooks), scratch, the pinned page, the probe
+// event and the stream.  Contexts are created again on the next call.
+static int release_all(bool from_exit) {
+    if (g_owner_pid != getpid()) return LINAC_OK;  // a forked child owns nothing
+    std::vector<ShutdownHook> hooks;
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        hooks = shutdown_hooks();
+    }
+    bool any = false;
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        for (DeviceContext* c : g_ctx) any = any || c != nullptr;
+    }
+    if (!any) {
+        if (!from_exit) thread_profiler().clear();
+        return LINAC_OK;  // nothing to release (and no need to touch the runtime)
+    }
+    int rc = LINAC_OK;
+    int cur = -1;
+    const bool have_cur = hipGetDevice(&cur) == hipSuccess;
+    for (size_t d = 0; d < g_ctx.size(); ++d) {
+        DeviceContext* c = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(g_ctx_mu);
+            c = g_ctx[d];
+            g_ctx[d] = nullptr;
+        }
+        if (!c) continue;
+        {
+            std::lock_guard<std::recursive_mutex> lk(c->mu);
+            if (hipSetDevice(c->device) != hipSuccess) rc = LINAC_ERR_DEVICE;
+            for (ShutdownHook h : hooks) h(c);
+            if (c->release() != LINAC_OK) rc = LINAC_ERR_DEVICE;
+            if (c->probe) (void)hipEventDestroy(c->probe);
+            if (c->pinned) (void)hipHostFree(c->pinned);
+            if (c->stream) (void)hipStreamDestroy(c->stream);
+        }
+        delete c;
+    }
+    // (at exit the calling thread's thread-local profiler is already gone)
+    if (!from_exit) thread_profiler().clear();
+    if (have_cur) (void)hipSetDevice(cur);
+    return rc;
+}
+
+static void release_all_at_exit() { (void)release_all(true); }
+
 int current_context(DeviceContext** out) {
     LINAC_TRY(check_process());
     int dev = 0;
@@ -102,7 +165,21 @@ int current_context(DeviceContext** out) {
             return fail(LINAC_ERR_DEVICE, "hipHostMalloc: %s", hipGetErrorString(se));
         }
         c->pinned = (int64_t*)pin;
+        se = hipEventCreateWithFlags(&c->probe, hipEventDisableTiming);
+        if (se != hipSuccess) {
+            (void)hipHostFree(pin);
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return fail(LINAC_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(se));
+        }
         g_ctx[dev] = c;
+        // Registered after the HIP runtime is up, so it runs before the
+        // runtime's own teardown (and before a profiler's) at process exit.
+        static bool hooked = false;
+        if (!hooked) {
+            hooked = true;
+            atexit(release_all_at_exit);
+        }
     }
     *out = g_ctx[dev];
     return LINAC_OK;
@@ -166,6 +243,21 @@ bool is_prime_u64(uint64_t n) {
 Profiler& thread_profiler() {
     static thread_local Profiler prof;
     return prof;
+}
+
+void Profiler::clear() {
+    for (auto& x : samples) {
+        (void)hipEventDestroy(x.a);
+        (void)hipEventDestroy(x.b);
+    }
+    samples.clear();
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    t0 = t1 = nullptr;
+}
+
+Profiler::~Profiler() {
+    if (g_owner_pid == getpid()) clear();
 }
 
 int Profiler::begin(hipStream_t s) {
@@ -258,6 +350,12 @@ int linac_release_cache(void) {
     LINAC_TRY(current_context(&ctx));
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return ctx->release();
+}
+
+int linac_shutdown(void) {
+    LINAC_TRY(check_process());
+    const int rc = release_all(false);
+    return rc == LINAC_OK ? LINAC_OK : fail(rc, "releasing a device context failed");
 }
 
 int linac_is_prime(uint64_t n) { return is_prime_u64(n) ? 1 : 0; }

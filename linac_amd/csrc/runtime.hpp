@@ -52,10 +52,18 @@ struct DeviceContext {
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
     Scratch scratch[SLOT_COUNT];
-    int64_t* pinned = nullptr;  // small pinned host buffer for status reads
+    int64_t* pinned = nullptr;    // small pinned host buffer for status reads
+    hipEvent_t probe = nullptr;   // reusable timing-free event (rank polls of the plain sweep)
+    void* engines[2] = {nullptr, nullptr};  // cached blocked-elimination engines (32- and 64-bit storage)
     int get(ScratchSlot slot, size_t bytes, void** out);
     int release();
 };
+
+// Called with the context's lock held, before its stream is destroyed, by
+// linac_shutdown and at process exit: frees what other parts of the library
+// cache in a context (engines, batches).
+typedef void (*ShutdownHook)(DeviceContext* ctx);
+void add_shutdown_hook(ShutdownHook fn);
 
 // Context of the current device (created on first use).  Fails with
 // LINAC_ERR_FORKED in a child process forked after initialisation and with
@@ -96,6 +104,8 @@ struct Profiler {
     };
     std::vector<Sample> samples;
     double result[6] = {0, 0, 0, 0, 0, 0};  // see linac_profile_last
+    ~Profiler();
+    void clear();                                    // destroys every event
     int begin(hipStream_t s);                        // resets and records t0
     int sample_begin(hipStream_t s, int64_t tag, double bytes, double macs);
     int sample_end(hipStream_t s);

@@ -1,24 +1,30 @@
-"""Wall-clock wrapper with linac's printing rule (``linac/timeit_decorator.py:8-22``).
+"""Wall-clock reporting for the drop-in calls (``linac/timeit_decorator.py:8-22``).
 
-The elapsed time is printed only when the call passes ``verbose=True`` as a
-keyword argument, in the reference's h:mm:ss:mmm format.
+Callers of linac see a report only when they pass ``verbose=True`` as a
+keyword argument (a positional ``True`` stays silent, as in linac), printed
+as ``Time elapsed in <name>: h:mm:ss:mmm.`` after a carriage return and
+padded with blanks so that it overwrites a progress line.
 """
 import functools
 import time
 
+_PAD = " " * 40
+
+
+def _report(name, seconds):
+    whole = int(seconds)
+    millis = int((seconds * 1000.0) % 1000.0)
+    return "\rTime elapsed in %s: %d:%02d:%02d:%03d.%s" % (
+        name, whole // 3600, (whole // 60) % 60, whole % 60, millis, _PAD)
+
 
 def timeit(func):
-    """Print the runtime of the decorated function as hours:minutes:seconds:milliseconds"""
+    """Time ``func``; report the wall time when called with ``verbose=True``."""
     @functools.wraps(func)
-    def wrapper_timeit(*args, **kwargs):
-        start = time.time()
-        value = func(*args, **kwargs)
-        seconds = time.time() - start
-        _, ms = divmod(seconds * 1000, 1000)
-        m, s = divmod(seconds, 60)
-        h, m = divmod(m, 60)
+    def timed(*args, **kwargs):
+        t0 = time.time()
+        result = func(*args, **kwargs)
         if kwargs.get("verbose") is True:
-            print("\rTime elapsed in %s: %d:%02d:%02d:%03d.                                        "
-                  % (func.__name__, h, m, s, ms))
-        return value
-    return wrapper_timeit
+            print(_report(func.__name__, time.time() - t0))
+        return result
+    return timed
