@@ -120,6 +120,21 @@ int linac_rref_u32(uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t
 int linac_rref_u64(uint64_t* a, int64_t rows, int64_t cols, int64_t ld, uint64_t p,
                    uint32_t flags, void* stream, int64_t* pivots_out, int64_t* rank_out);
 
+/* The plain method of linac_rref_u32 on a device matrix with NO allocation:
+ * every buffer beyond the caller's matrix (state, pivot lists, row flags, the
+ * gather buffer of the final permutation) comes from a static device array of
+ * 32 MiB that each GPU holds until the process ends.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * Requires linac_init(device) first (which resolves the array's address on the
+ * device), LINAC_DEVICE_PTR, LINAC_METHOD_PLAIN or _AUTO (plain runs), no
+ * LINAC_PROFILE, and rows * ld * 4 + 17 * min(rows, cols) + rows + ~1 KiB
+ * <= 32 MiB (LINAC_ERR_INVALID otherwise).  Results equal linac_rref_u32 with
+ * LINAC_METHOD_PLAIN.  Calls share the device's array: calls that overlap on
+ * one GPU race (the library serialises its own calls per device). */
+int linac_rref_u32_noalloc(uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p,
+                           uint32_t flags, void* stream, int64_t* pivots_out, int64_t* rank_out);
+
 /* ---- batches of GF(p) matrices (blocked method, 2 < p <= 4278124287) ----
  * A batch advances all its unfinished matrices together, one 256-column panel
  * per step, in shared kernel launches (one launch per kernel for the whole

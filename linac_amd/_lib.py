@@ -57,6 +57,7 @@ SIGNATURES = {
     "linac_profile_last": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "linac_rref_u32": (ctypes.c_int, [_vp, _i64, _i64, _i64, _u32, _u32, _vp, _i64p, _i64p]),
     "linac_rref_u64": (ctypes.c_int, [_vp, _i64, _i64, _i64, _u64, _u32, _vp, _i64p, _i64p]),
+    "linac_rref_u32_noalloc": (ctypes.c_int, [_vp, _i64, _i64, _i64, _u32, _u32, _vp, _i64p, _i64p]),
     "linac_rref_f64": (ctypes.c_int, [_vp, _i64, _i64, _i64, ctypes.c_double, _u32, _vp, _i64p, _i64p]),
     "linac_rref_c128": (ctypes.c_int, [_vp, _i64, _i64, _i64, ctypes.c_double, _u32, _vp, _i64p, _i64p]),
     "linac_batch_footprint_u32": (ctypes.c_int, [_i64, _i64, ctypes.POINTER(_u64)]),

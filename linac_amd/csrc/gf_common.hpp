@@ -41,4 +41,10 @@ int gf_finalize_apply(typename F::T* A, int64_t ld, int64_t N, int64_t M, const 
                       const std::vector<int64_t>& cols, bool resorted);
 bool sort_pivots_by_column(std::vector<int64_t>& rows, std::vector<int64_t>& cols);
 
+// The plain method on a device matrix with all of its buffers carved from
+// `scratch` (rref_gf.hip); noalloc_scratch_bytes is what it needs.
+size_t noalloc_scratch_bytes(int64_t rows, int64_t cols, int64_t ld, size_t elem);
+int rref_u32_noalloc(uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p, uint32_t flags, hipStream_t s,
+                     DeviceContext* ctx, uint8_t* scratch, int64_t* pivots_out, int64_t* rank_out);
+
 }  // namespace linac

@@ -144,11 +144,12 @@ __global__ __launch_bounds__(256) void k_finalize(const typename F::T* A, int64_
                                                   const ElimState* __restrict__ st,
                                                   const int64_t* __restrict__ piv_rows,
                                                   const int64_t* __restrict__ piv_cols, int normalize,
-                                                  int vec) {
+                                                  int vec, const int* gate = nullptr, int want = 0) {
     using T = typename F::T;
     constexpr int V = 16 / sizeof(T);
     using VT = Vec16<T>;
     __shared__ T s_scale;
+    if (gate && *gate != want) return;  // the other form of the gather applies
     const int64_t k = blockIdx.x;
     const int64_t rank = st->rank;
     // vec: rows are 16-byte aligned and padded to whole vectors (ld % V == 0),
@@ -183,6 +184,34 @@ __global__ __launch_bounds__(256) void k_finalize(const typename F::T* A, int64_
         }
     } else {
         for (int64_t c = threadIdx.x; c < M; c += blockDim.x) orow[c] = normalize ? f.mont(sc, s[c]) : s[c];
+    }
+}
+
+// *ident = 1 when the k-th pivot row is row k for every k < rank (the gather
+// can run in place), else 0.  One workgroup.
+__global__ __launch_bounds__(256) void k_identity_flag(const ElimState* __restrict__ st,
+                                                       const int64_t* __restrict__ piv_rows, int* ident) {
+    __shared__ int s_ok;
+    if (threadIdx.x == 0) s_ok = 1;
+    __syncthreads();
+    const int64_t rank = st->rank;
+    bool ok = true;
+    for (int64_t k = threadIdx.x; k < rank; k += blockDim.x) ok = ok && piv_rows[k] == k;
+    if (!ok) s_ok = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) *ident = s_ok;
+}
+
+// dst[r, :cols] = src[r, :cols] unless *gate != want.
+template <class T>
+__global__ __launch_bounds__(256) void k_copy_rows_gated(T* __restrict__ dst, int64_t ld_dst, const T* __restrict__ src,
+                                                         int64_t ld_src, int64_t rows, int64_t cols,
+                                                         const int* __restrict__ gate, int want) {
+    if (*gate != want) return;
+    const int64_t total = rows * cols;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = x / cols, c = x - r * cols;
+        dst[r * ld_dst + c] = src[r * ld_src + c];
     }
 }
 
@@ -410,6 +439,73 @@ static int rref_gf_entry(typename F::T* a, int64_t rows, int64_t cols, int64_t l
         for (int64_t k = 0; k < rank; ++k) pivots_out[k] = pc[k];
     if (rank_out) *rank_out = rank;
     return LINAC_OK;
+}
+
+// Bytes of caller-provided scratch the plain method needs without any other
+// allocation: elimination state, pivot lists, row flags, a gate word and a
+// gather buffer of the whole matrix (used when pivot rows are not rows
+// 0..rank-1).
+size_t noalloc_scratch_bytes(int64_t rows, int64_t cols, int64_t ld, size_t elem) {
+    const int64_t cap = std::max<int64_t>(1, std::min(rows, cols));
+    return 256 + 2 * (size_t)round_up(cap * 8, 256) + (size_t)round_up(rows, 256) + 256 +
+           (size_t)round_up(rows * ld * (int64_t)elem, 256);
+}
+
+// The plain method on a device matrix with every buffer carved from `scratch`
+// (device memory of at least noalloc_scratch_bytes): no allocation, no
+// context scratch.  Uses the context's stream, pinned status page and probe
+// event, which exist once the context does.
+template <class F>
+static int rref_gf_noalloc(typename F::T* A, int64_t rows, int64_t cols, int64_t ld, const F& f, uint32_t flags,
+                           hipStream_t s, DeviceContext* ctx, uint8_t* scratch, int64_t* pivots_out,
+                           int64_t* rank_out) {
+    using T = typename F::T;
+    const int64_t cap = std::max<int64_t>(1, std::min(rows, cols));
+    char* b = (char*)scratch;
+    ElimWorkspace ws;
+    ws.st = (ElimState*)b;
+    b += 256;
+    ws.piv_rows = (int64_t*)b;
+    b += round_up(cap * 8, 256);
+    ws.piv_cols = (int64_t*)b;
+    b += round_up(cap * 8, 256);
+    ws.used = (uint8_t*)b;
+    b += round_up(rows, 256);
+    ws.cap = cap;
+    int* ident = (int*)b;
+    b += 256;
+    T* gather = (T*)b;
+    LINAC_TRY(plain_launch_init(A, ld, rows, cols, f, s, ws));
+    LINAC_TRY(plain_run(A, ld, rows, cols, f, s, ws, ctx, nullptr));
+    if (!(flags & LINAC_NO_FINALIZE)) {
+        constexpr int V = 16 / sizeof(T);
+        const int vec = (ld % V == 0 && ((uintptr_t)A & 15) == 0) ? 1 : 0;
+        k_identity_flag<<<1, 256, 0, s>>>(ws.st, ws.piv_rows, ident);
+        // in place when the pivot rows are 0..rank-1, else through the gather buffer
+        k_finalize<F><<<(unsigned)rows, 256, 0, s>>>(A, ld, A, ld, rows, cols, f, ws.st, ws.piv_rows, ws.piv_cols, 1,
+                                                      vec, ident, 1);
+        k_finalize<F><<<(unsigned)rows, 256, 0, s>>>(A, ld, gather, ld, rows, cols, f, ws.st, ws.piv_rows,
+                                                      ws.piv_cols, 1, vec, ident, 0);
+        const unsigned grid = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, ceil_div(rows * cols, 256)));
+        k_copy_rows_gated<T><<<grid, 256, 0, s>>>(A, ld, gather, ld, rows, cols, ident, 0);
+        LINAC_HIP(hipGetLastError());
+    }
+    LINAC_HIP(hipMemcpyAsync(ctx->pinned, &ws.st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    LINAC_HIP(hipStreamSynchronize(s));
+    const int64_t rank = ctx->pinned[0];
+    if (pivots_out && rank > 0) {
+        LINAC_HIP(hipMemcpyAsync(pivots_out, ws.piv_cols, rank * 8, hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipStreamSynchronize(s));
+    }
+    if (rank_out) *rank_out = rank;
+    return LINAC_OK;
+}
+
+int rref_u32_noalloc(uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p, uint32_t flags, hipStream_t s,
+                     DeviceContext* ctx, uint8_t* scratch, int64_t* pivots_out, int64_t* rank_out) {
+    if (p == 2)
+        return rref_gf_noalloc(a, rows, cols, ld, FieldGF2{2u}, flags, s, ctx, scratch, pivots_out, rank_out);
+    return rref_gf_noalloc(a, rows, cols, ld, make_field_u32(p), flags, s, ctx, scratch, pivots_out, rank_out);
 }
 
 }  // namespace linac

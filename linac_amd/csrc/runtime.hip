@@ -185,6 +185,17 @@ int current_context(DeviceContext** out) {
     return LINAC_OK;
 }
 
+int existing_context(DeviceContext** out) {
+    LINAC_TRY(check_process());
+    int dev = 0;
+    LINAC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= (int)g_ctx.size()) return fail(LINAC_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (!g_ctx[dev]) return fail(LINAC_ERR_INVALID, "device %d is not initialised: call linac_init(%d) first", dev, dev);
+    *out = g_ctx[dev];
+    return LINAC_OK;
+}
+
 int make_workspace(DeviceContext* ctx, int64_t N, int64_t M, ElimWorkspace* ws) {
     const int64_t cap = std::max<int64_t>(1, std::min(N, M));
     const size_t st_bytes = 256;
@@ -342,7 +353,8 @@ int linac_init(int device) {
     LINAC_TRY(check_process());
     LINAC_HIP(hipSetDevice(device));
     DeviceContext* ctx = nullptr;
-    return current_context(&ctx);
+    LINAC_TRY(current_context(&ctx));
+    return noalloc_setup(device);
 }
 
 int linac_release_cache(void) {

@@ -69,6 +69,13 @@ void add_shutdown_hook(ShutdownHook fn);
 // LINAC_ERR_FORKED in a child process forked after initialisation and with
 // LINAC_ERR_DEVICE when no GPU is usable.
 int current_context(DeviceContext** out);
+// The current device's context if it has been created (by linac_init or an
+// earlier call), without creating one; LINAC_ERR_INVALID otherwise.
+int existing_context(DeviceContext** out);
+
+// No-allocation entry point (noalloc.hip): resolves the current device's
+// address of the library's static scratch array.  Called by linac_init.
+int noalloc_setup(int device);
 
 // Shared state of one elimination (lives in device memory).
 struct ElimState {
