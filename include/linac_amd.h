@@ -47,6 +47,7 @@ extern "C" {
 #define LINAC_NO_FINALIZE     0x100u /* leave rows unpermuted (pivots_out says where)*/
 #define LINAC_PROFILE         0x200u /* time the call with HIP events (linac_profile_last) */
 #define LINAC_IN_PLACE        0x400u /* batch: work on the caller's device matrix, no copy */
+#define LINAC_REFINE          0x800u /* float: refine the solution columns (blocked method) */
 
 /* ---- library ---- */
 const char* linac_last_error(void);
@@ -190,6 +191,21 @@ int linac_rref_f64(double* a, int64_t rows, int64_t cols, int64_t ld, double tol
                    int64_t* pivots_out, int64_t* rank_out);
 int linac_rref_c128(double* a, int64_t rows, int64_t cols, int64_t ld, double tol, uint32_t flags, void* stream,
                     int64_t* pivots_out, int64_t* rank_out);
+/* With LINAC_REFINE (blocked method only; AUTO then runs blocked, PLAIN is
+ * LINAC_ERR_INVALID) the call also keeps every panel's factors and runs two
+ * passes of residual-based refinement on the non-pivot columns: the residual
+ * A[:, NP] - A[:, P] X of the original matrix in double-double arithmetic,
+ * replayed through the factors, added to X.  The pivot decisions (and so the
+ * rank and pivot columns) are those of the unrefined elimination.  No
+ * counterpart in the reference (its float path has no refinement,
+ * linac/row_reduce.cu:138-196).  Extra device memory: about 2 x the matrix.
+ * linac_refine_last reports this thread's last refined call:
+ *   out[0] passes run, out[1] max|correction| / max|X| of the first pass (an
+ *   estimate of the unrefined result's relative error), out[2] the same for
+ *   the last pass, out[3] the number of non-pivot columns.
+ * Writes min(n, 4) values and returns how many were written (zeros after a
+ * call without LINAC_REFINE). */
+int linac_refine_last(double* out, int n);
 
 /* ---- monomial-evaluation matrix loader ----
  * Replaces cuda_load_matrix(bases, lindices, shape, field_characteristic)

@@ -27,6 +27,7 @@ METHOD_PLAIN = 0x10
 METHOD_BLOCKED = 0x20
 NO_FINALIZE = 0x100
 PROFILE = 0x200
+REFINE = 0x800
 
 METHODS = {"auto": METHOD_AUTO, "plain": METHOD_PLAIN, "blocked": METHOD_BLOCKED}
 
@@ -55,6 +56,7 @@ SIGNATURES = {
     "linac_shutdown": (ctypes.c_int, []),
     "linac_is_prime": (ctypes.c_int, [_u64]),
     "linac_profile_last": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "linac_refine_last": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "linac_rref_u32": (ctypes.c_int, [_vp, _i64, _i64, _i64, _u32, _u32, _vp, _i64p, _i64p]),
     "linac_rref_u64": (ctypes.c_int, [_vp, _i64, _i64, _i64, _u64, _u32, _vp, _i64p, _i64p]),
     "linac_rref_u32_noalloc": (ctypes.c_int, [_vp, _i64, _i64, _i64, _u32, _u32, _vp, _i64p, _i64p]),
@@ -164,6 +166,14 @@ def profile_last():
     buf = (ctypes.c_double * 6)()
     n = load().linac_profile_last(buf, 6)
     keys = ["device_ms", "samples", "sampled_ms", "sampled_bytes", "sampled_macs", "method"]
+    return dict(zip(keys[:n], list(buf)[:n]))
+
+
+def refine_last():
+    """Statistics of this thread's last LINAC_REFINE call (see linac_refine_last)."""
+    buf = (ctypes.c_double * 4)()
+    n = load().linac_refine_last(buf, 4)
+    keys = ["passes", "first_correction", "last_correction", "solution_columns"]
     return dict(zip(keys[:n], list(buf)[:n]))
 
 

@@ -16,13 +16,18 @@
 #include <vector>
 
 #include "float_ops.hpp"
+#include "float_refine.hpp"
 #include "runtime.hpp"
 
 namespace linac {
 
 template <class T>
 int float_blocked_run(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStream_t s, int64_t* rank,
-                      uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, Profiler* prof);
+                      uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, Profiler* prof,
+                      FloatFactors* keep);
+
+// Refinement passes run with LINAC_REFINE.
+constexpr int REFINE_PASSES = 2;
 
 struct FloatState {
     int64_t prow;
@@ -175,23 +180,35 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
     int64_t* piv_rows = (int64_t*)((char*)p + 256);
     int64_t* piv_cols = (int64_t*)((char*)p + 256 + round_up(cap * 8, 256));
     uint8_t* used = (uint8_t*)((char*)p + 256 + 2 * round_up(cap * 8, 256));
-    if (prof) LINAC_TRY(prof->begin(s));
-    k_row_scale<T><<<(unsigned)rows, 256, 0, s>>>(A, lda, rows, cols);
-    k_float_init<<<(unsigned)std::min<int64_t>(1024, ceil_div(rows, 256)), 256, 0, s>>>(st, used, rows);
     const uint32_t method = flags & LINAC_METHOD_MASK;
     if (method != LINAC_METHOD_PLAIN && method != LINAC_METHOD_AUTO && method != LINAC_METHOD_BLOCKED)
         return fail(LINAC_ERR_INVALID, "unknown method flag 0x%x", method);
+    const bool refine = (flags & LINAC_REFINE) != 0;
+    if (refine && method == LINAC_METHOD_PLAIN)
+        return fail(LINAC_ERR_INVALID, "LINAC_REFINE needs the blocked float method (its panel factors)");
+    T* Aorig = nullptr;
+    if (refine) {  // the unscaled input, for the residuals
+        LINAC_TRY(ctx->get(SLOT_AUX, (size_t)rows * lda * sizeof(T), &p));
+        Aorig = (T*)p;
+        LINAC_HIP(hipMemcpy2DAsync(Aorig, lda * sizeof(T), A, lda * sizeof(T), cols * sizeof(T), rows,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    if (prof) LINAC_TRY(prof->begin(s));
+    k_row_scale<T><<<(unsigned)rows, 256, 0, s>>>(A, lda, rows, cols);
+    k_float_init<<<(unsigned)std::min<int64_t>(1024, ceil_div(rows, 256)), 256, 0, s>>>(st, used, rows);
     // blocked: panels of 64 columns with a cooperative pivot kernel and an
     // FP64 matrix-core trailing update; the plain sweep stays the method for
     // small matrices, where per-column launches cost less than the panels
-    bool blocked = method == LINAC_METHOD_BLOCKED ||
+    bool blocked = method == LINAC_METHOD_BLOCKED || refine ||
                    (method == LINAC_METHOD_AUTO && rows >= tuning().float_blocked_min_rows &&
                     cols >= tuning().float_blocked_min_cols);
+    FloatFactors fac;
     if (blocked) {
-        const int rc = float_blocked_run<T>(A, lda, rows, cols, tol, s, &st->rank, used, piv_rows, piv_cols, ctx, prof);
+        const int rc = float_blocked_run<T>(A, lda, rows, cols, tol, s, &st->rank, used, piv_rows, piv_cols, ctx, prof,
+                                            refine ? &fac : nullptr);
         if (rc < 0) return rc;
         if (rc == 1) {
-            if (method == LINAC_METHOD_BLOCKED)
+            if (method == LINAC_METHOD_BLOCKED || refine)
                 return fail(LINAC_ERR_INVALID, "blocked float method does not fit this device for %lld rows",
                             (long long)rows);
             blocked = false;
@@ -213,6 +230,16 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
     if (prof) {
         LINAC_TRY(prof->finish(s));
         prof->result[5] = blocked ? 2 : 1;
+    }
+    if (refine) {
+        int64_t rk = 0;
+        LINAC_HIP(hipMemcpyAsync(&rk, &st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipStreamSynchronize(s));
+        std::vector<int64_t> pcs(rk);
+        if (rk) LINAC_HIP(hipMemcpy(pcs.data(), piv_cols, rk * 8, hipMemcpyDeviceToHost));
+        LINAC_TRY(float_refine<T>(A, lda, Aorig, lda, rows, cols, piv_rows, piv_cols, pcs, fac, REFINE_PASSES, s, ctx));
+    } else {
+        thread_refine_stats() = RefineStats{};
     }
     void* outp = nullptr;
     LINAC_TRY(ctx->get(SLOT_OUT, (size_t)rows * lda * sizeof(T), &outp));

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "float_ops.hpp"
+#include "float_refine.hpp"
 #include "runtime.hpp"
 
 namespace linac {
@@ -584,13 +585,33 @@ static int cooperative_fits(int64_t G, DeviceContext* ctx, bool* fits, unsigned*
 template <class T, int RW>
 static int float_blocked_sweep(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStream_t s,
                                int64_t* rank, uint8_t* used, int64_t* piv_rows, int64_t* piv_cols,
-                               DeviceContext* ctx, Profiler* prof, unsigned lds) {
+                               DeviceContext* ctx, Profiler* prof, unsigned lds, FloatFactors* keep) {
     constexpr int R = FloatOps<T>::kReal;
     const int64_t G = ceil_div(N, (int64_t)FWAVES * RW);
     const FloatBlockedLayout l = float_layout<T>(N, M, G);
     void* base = nullptr;
     LINAC_TRY(ctx->get(SLOT_PANEL_A, l.total, &base));
     char* b = (char*)base;
+    const int64_t npanels = ceil_div(M, FB);
+    // with `keep`, every panel's factors (ZL, ZQ, pivot rows) get their own
+    // slice, kept after the sweep for the refinement's replay
+    T* keepZL = nullptr;
+    T* keepZQ = nullptr;
+    int64_t* keepQ = nullptr;
+    if (keep) {
+        const size_t zl = (size_t)round_up(N * FB * (int64_t)sizeof(T), 256);
+        const size_t zq = (size_t)FB * FB * sizeof(T), q = (size_t)FB * 8;
+        void* kb = nullptr;
+        LINAC_TRY(ctx->get(SLOT_AUX2, (size_t)npanels * (zl + zq + q), &kb));
+        keepZL = (T*)kb;
+        keepZQ = (T*)((char*)kb + npanels * zl);
+        keepQ = (int64_t*)((char*)kb + npanels * (zl + zq));
+        keep->ZL = keepZL;
+        keep->ZQ = keepZQ;
+        keep->qrows = keepQ;
+        keep->npanels = npanels;
+        keep->pepoch = (const int32_t*)(b + l.pepoch);
+    }
     FPanelArgs<T> P{};
     P.A = A;
     P.ld = ld;
@@ -615,12 +636,16 @@ static int float_blocked_sweep(T* A, int64_t ld, int64_t N, int64_t M, double to
     k_fpanel_init<<<(unsigned)std::min<int64_t>(1024, ceil_div(std::max<int64_t>(N, 4 * G), 256)), 256, 0, s>>>(
         (uint64_t*)P.slots, 4 * G, P.err, P.pepoch, N, P.tphase, 1);
     LINAC_HIP(hipGetLastError());
-    const int64_t npanels = ceil_div(M, FB);
     for (int64_t pi = 0; pi < npanels; ++pi) {
         P.J = pi * FB;
         P.w = (int)std::min<int64_t>(FB, M - P.J);
         P.panel = (int)pi;
         P.kpanel = kpanel + pi;
+        if (keep) {
+            P.ZL = keepZL + pi * (round_up(N * FB * (int64_t)sizeof(T), 256) / (int64_t)sizeof(T));
+            P.ZQ = keepZQ + pi * FB * FB;
+            P.qrows = keepQ + pi * FB;
+        }
         void* args[] = {&P};
         LINAC_HIP(hipLaunchCooperativeKernel((const void*)k_fpanel<T, RW>, dim3((unsigned)G), dim3(256), args, lds, s));
         const int64_t T0 = P.J + P.w, nT = M - T0;
@@ -679,25 +704,29 @@ static int float_blocked_sweep(T* A, int64_t ld, int64_t N, int64_t M, double to
     }
     if (prof)
         for (auto& x : prof->samples) x.macs *= (double)R * kp[(size_t)x.tag];
+    if (keep) keep->kp = kp;
     return LINAC_OK;
 }
 
 // Returns 1 (not applicable) when the cooperative panel grid cannot be resident.
 template <class T>
 int float_blocked_run(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStream_t s, int64_t* rank,
-                      uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, Profiler* prof) {
+                      uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, Profiler* prof,
+                      FloatFactors* keep) {
     bool fits = false;
     unsigned lds = 0;
     LINAC_TRY((cooperative_fits<T, 16>(ceil_div(N, FWAVES * 16), ctx, &fits, &lds)));
-    if (fits) return float_blocked_sweep<T, 16>(A, ld, N, M, tol, s, rank, used, piv_rows, piv_cols, ctx, prof, lds);
+    if (fits)
+        return float_blocked_sweep<T, 16>(A, ld, N, M, tol, s, rank, used, piv_rows, piv_cols, ctx, prof, lds, keep);
     LINAC_TRY((cooperative_fits<T, 32>(ceil_div(N, FWAVES * 32), ctx, &fits, &lds)));
-    if (fits) return float_blocked_sweep<T, 32>(A, ld, N, M, tol, s, rank, used, piv_rows, piv_cols, ctx, prof, lds);
+    if (fits)
+        return float_blocked_sweep<T, 32>(A, ld, N, M, tol, s, rank, used, piv_rows, piv_cols, ctx, prof, lds, keep);
     return 1;
 }
 
 template int float_blocked_run<double>(double*, int64_t, int64_t, int64_t, double, hipStream_t, int64_t*, uint8_t*,
-                                       int64_t*, int64_t*, DeviceContext*, Profiler*);
+                                       int64_t*, int64_t*, DeviceContext*, Profiler*, FloatFactors*);
 template int float_blocked_run<CplxF>(CplxF*, int64_t, int64_t, int64_t, double, hipStream_t, int64_t*, uint8_t*,
-                                      int64_t*, int64_t*, DeviceContext*, Profiler*);
+                                      int64_t*, int64_t*, DeviceContext*, Profiler*, FloatFactors*);
 
 }  // namespace linac

@@ -363,6 +363,7 @@ static int rref_gf_entry(typename F::T* a, int64_t rows, int64_t cols, int64_t l
     if (rows < 0 || cols < 0) return fail(LINAC_ERR_INVALID, "negative shape (%lld, %lld)", (long long)rows, (long long)cols);
     if (ld < cols) return fail(LINAC_ERR_INVALID, "ld (%lld) < cols (%lld)", (long long)ld, (long long)cols);
     if (rows > 0 && cols > 0 && !a) return fail(LINAC_ERR_INVALID, "matrix pointer is NULL");
+    if (flags & LINAC_REFINE) return fail(LINAC_ERR_INVALID, "LINAC_REFINE applies to float64 / complex128 only");
     if (rank_out) *rank_out = 0;
     if (rows == 0 || cols == 0) return LINAC_OK;
     const bool on_device = (flags & LINAC_DEVICE_PTR) != 0;

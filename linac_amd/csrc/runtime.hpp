@@ -44,6 +44,7 @@ enum ScratchSlot {
     SLOT_PANEL_B,
     SLOT_PANEL_C,
     SLOT_AUX,
+    SLOT_AUX2,        // float refinement: kept panel factors
     SLOT_COUNT
 };
 

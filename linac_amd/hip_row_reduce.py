@@ -87,7 +87,8 @@ def _prepare(matrix, field_characteristic, _real, _mod64):
 
 
 @timeit
-def cuda_row_reduce(matrix, field_characteristic=0, verbose=False, _real=None, _mod64=None, method="auto"):
+def cuda_row_reduce(matrix, field_characteristic=0, verbose=False, _real=None, _mod64=None, method="auto",
+                    refine=False):
     r"""
     Args:
         matrix (2D numpy.ndarray):
@@ -98,33 +99,40 @@ def cuda_row_reduce(matrix, field_characteristic=0, verbose=False, _real=None, _
             If True, prints additional information.
         method ("auto" | "plain" | "blocked"):
             Elimination method on the GPU (extension; default picks).
+        refine (bool):
+            Real and complex matrices only: residual-based refinement of the
+            non-pivot columns (extension; blocked method).
     """
     t0 = time.time()
     p, a = _prepare(matrix, field_characteristic, _real, _mod64)
     t1 = time.time()
     if p > 0:
+        if refine:
+            raise ValueError("refine applies to real and complex matrices")
         _rref_modp_host(a, p, method, False)
     else:
         from .hip_float import rref_float_host
-        rref_float_host(a, method)
+        rref_float_host(a, method, refine=refine)
     if verbose:
         print(f"\rconversion {t1 - t0:.3f} s, device call {time.time() - t1:.3f} s", end="\n")
     return a
 
 
-def rref_with_pivots(matrix, field_characteristic=0, _real=None, _mod64=None, method="auto"):
+def rref_with_pivots(matrix, field_characteristic=0, _real=None, _mod64=None, method="auto", refine=False):
     """(R, pivot_columns, rank): ``cuda_row_reduce`` plus the device's pivot list."""
     p, a = _prepare(matrix, field_characteristic, _real, _mod64)
     if p > 0:
+        if refine:
+            raise ValueError("refine applies to real and complex matrices")
         piv, rank = _rref_modp_host(a, p, method, True)
     else:
         from .hip_float import rref_float_host
-        piv, rank = rref_float_host(a, method)
+        piv, rank = rref_float_host(a, method, refine=refine)
     return a, piv, rank
 
 
 def rref_device(ptr, rows, cols, ld, field_characteristic, dtype, method="auto", stream=None, finalize=True,
-                profile=False):
+                profile=False, refine=False):
     """In-place RREF of a matrix already in HBM (``ptr`` = device address).
 
     ``dtype``: numpy.uint32 / numpy.uint64 (field_characteristic = p, rows
@@ -137,7 +145,7 @@ def rref_device(ptr, rows, cols, ld, field_characteristic, dtype, method="auto",
     piv = numpy.zeros(max(1, min(rows, cols)), dtype=numpy.int64)
     rank = ctypes.c_int64(0)
     flags = _lib.DEVICE_PTR | _lib.METHODS[method] | (0 if finalize else _lib.NO_FINALIZE) | \
-        (_lib.PROFILE if profile else 0)
+        (_lib.PROFILE if profile else 0) | (_lib.REFINE if refine else 0)
     st = ctypes.c_void_p(stream) if stream else None
     out = (piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(rank))
     if dt.kind in 'fc':

@@ -140,3 +140,16 @@ def float_system(kind, n, seed=0):
     A[:, :n] = (haar(n) * numpy.logspace(0, -8, n)) @ haar(n).T
     A[:, n] = rs.random_sample(n)
     return A
+
+
+def refine_solution(A, B, X, iters=4):
+    """X (solutions of A X = B, square A) refined with residuals in 80-bit long
+    double and LAPACK LU corrections: accurate to about cond(A) 2^-64 + 2^-53."""
+    import scipy.linalg
+    lu = scipy.linalg.lu_factor(A)
+    ext = numpy.clongdouble if numpy.iscomplexobj(A) or numpy.iscomplexobj(B) else numpy.longdouble
+    Al, Bl, Xl = A.astype(ext), B.astype(ext), X.astype(ext)
+    for _ in range(iters):
+        R = Bl - Al @ Xl
+        Xl = Xl + scipy.linalg.lu_solve(lu, R.astype(A.dtype)).astype(ext)
+    return Xl.astype(A.dtype)

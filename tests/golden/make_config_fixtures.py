@@ -4,16 +4,17 @@ on the CPU; the outputs are committed and the GPU tests only read them).
 * ``lu5000.json`` -- the N = 5000 GF(2^31-1) input of rank 4500, the seeded
   mod-p product modp.low_rank(5000, 5000, 4500, p, 11): sha256 of the input,
   the C oracle's pivot columns and the sha256 of its RREF (uint32, C order).
-* ``float_configs.npz`` -- for each float configuration [A | b]
-  (tests/certificates.float_system: f64 8000, c128 6000, ill-conditioned f64
-  4000, seed 0): sha256 of the input, the C oracle's FP64 RREF solution column
+* ``float_configs.npz`` -- for the float configurations whose inputs are pure
+  RandomState draws, [A | b] (tests/certificates.float_system: f64 8000, c128
+  6000, seed 0; the ill-conditioned one is built with LAPACK and its oracle runs
+  in the test): sha256 of the input, the C oracle's FP64 RREF solution column
   ``x_oracle`` (linac's float path: row scaling, partial pivoting, 1e-9
   threshold), its max |R[:, :n] - I|, and ``x_star``: the same solution
   refined to extended accuracy (LAPACK LU corrections, residuals b - A x in
   80-bit long double) -- the reference point for errors, since the FP64 oracle
   itself carries an error of order cond(A) * 2^-53.  Also cond_2(A).
 
-Usage: python tests/golden/make_config_fixtures.py [lu5000] [f64] [c128] [illcond]
+Usage: python tests/golden/make_config_fixtures.py [lu5000] [f64] [c128]
 """
 import hashlib
 import json
@@ -32,7 +33,7 @@ from oracle import crref, modp  # noqa: E402
 import certificates  # noqa: E402
 
 P31 = 2 ** 31 - 1
-FLOAT_CASES = {"f64": 8000, "c128": 6000, "illcond": 4000}
+FLOAT_CASES = {"f64": 8000, "c128": 6000}
 
 
 def sha(a):
@@ -50,35 +51,18 @@ def lu5000():
         json.dump(out, f)
 
 
-def refine(A, b, x, iters=4):
-    """x refined with long-double residuals and LAPACK LU corrections."""
-    import scipy.linalg
-    lu = scipy.linalg.lu_factor(A)
-    ext = numpy.clongdouble if numpy.iscomplexobj(A) else numpy.longdouble
-    Al, bl = A.astype(ext), b.astype(ext)
-    xl = x.astype(ext)
-    for _ in range(iters):
-        r = bl - Al @ xl
-        d = scipy.linalg.lu_solve(lu, r.astype(A.dtype))
-        xl = xl + d.astype(ext)
-    return xl.astype(A.dtype)
-
-
 def float_case(kind):
     n = FLOAT_CASES[kind]
     Ab = certificates.float_system(kind, n, seed=0)
     t = time.time()
-    if kind == "c128":
-        R, piv = crref.rref_c128(Ab)
-    else:
-        R, piv = crref.rref_f64(Ab)
+    R, piv = crref.rref_c128(Ab) if kind == "c128" else crref.rref_f64(Ab)
     t_or = time.time() - t
     assert piv == list(range(n)), (kind, len(piv))
     x_or = R[:, n].copy()
     eye_err = float(numpy.max(numpy.abs(R[:, :n] - numpy.eye(n))))
     del R
     A, b = Ab[:, :n], Ab[:, n]
-    x_star = refine(A, b, numpy.linalg.solve(A, b))
+    x_star = certificates.refine_solution(A, b, numpy.linalg.solve(A, b))
     cond = float(numpy.linalg.cond(A))
     err = float(numpy.max(numpy.abs(x_or - x_star)) / numpy.max(numpy.abs(x_star)))
     print(f"{kind} {n}: oracle {t_or:.1f} s, cond {cond:.3g}, oracle vs refined {err:.3g}, |R-I| {eye_err:.3g}",
@@ -101,4 +85,4 @@ def main(which):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["lu5000", "f64", "c128", "illcond"])
+    main(sys.argv[1:] or ["lu5000", "f64", "c128"])
