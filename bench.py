@@ -34,7 +34,8 @@ def parse():
     ap.add_argument("--method", default="auto", choices=["auto", "plain", "blocked"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-kernel comparator")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-n", type=int, default=1200)
+    ap.add_argument("--cpu-sizes", default="1000,2000", help="uniform sizes of the CPU baseline protocol")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-inclusive drop-in call")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched loader sweep (64 x 3000^2)")
     ap.add_argument("--sweep-systems", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run (2000 mixed requests)")
@@ -48,29 +49,79 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(n_target, n_sample):
-    import numpy
-    from oracle import rref
-    p = 2 ** 31 - 1
-    A = numpy.random.RandomState(0).randint(p, size=(n_sample, n_sample)).astype(numpy.int64)
-    t0 = time.perf_counter()
-    R = rref.rref_modp(A, p)
-    t = time.perf_counter() - t0
-    assert R[0, 0] == 1
-    scale = (n_target / n_sample) ** 3
-    cpu = "unknown"
+def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": t * scale, "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"numpy restatement of linac.row_reduce (prime path, pivoting=1, scaling=False) on a "
-                      f"{n_sample}x{n_sample} uniform GF(2^31-1) matrix: {t:.2f} s, scaled by "
-                      f"({n_target}/{n_sample})^3; host {cpu}, os.cpu_count()={os.cpu_count()}, "
-                      f"numpy {numpy.__version__}"}
+    return "unknown"
+
+
+def _stats(times):
+    import numpy
+    t = numpy.array(times)
+    return {"runs": len(times), "median_s": round(float(numpy.median(t)), 4), "mean_s": round(float(t.mean()), 4),
+            "std_s": round(float(t.std()), 4)}
+
+
+def cpu_baseline(n_target, sizes=(1000, 2000)):
+    """linac's CPU row_reduce as the numpy restatement (prime path,
+    pivoting=1, scaling=False, int64 mod p), on this host, one core.
+    Protocol: config C0 (500x500 of rank 400, seed 0) and uniform N x N
+    matrices mod 2^31-1 (RandomState(N).randint); one warm-up, then three
+    timed runs each (the largest size without its own warm-up).  `value` is
+    the largest size's median scaled by (n_target / N)^3: an extrapolation,
+    labelled as such; N = 5000 was measured once separately."""
+    import numpy
+    from oracle import modp, rref
+    p = 2 ** 31 - 1
+    rs = numpy.random.RandomState(0)
+    L = rs.randint(p, size=(500, 400), dtype=numpy.uint32)
+    U = rs.randint(p, size=(400, 500), dtype=numpy.uint32)
+    C0 = modp.matmul_modp(L, U, p)
+    cases = [("C0 500x500 rank 400", C0, True)]
+    for n in sizes:
+        cases.append((f"{n}x{n}", numpy.random.RandomState(n).randint(p, size=(n, n)).astype(numpy.int64),
+                      n != sizes[-1]))
+    out = {}
+    for name, A, warm in cases:
+        if warm:
+            rref.rref_modp(A, p)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            R = rref.rref_modp(A, p)
+            times.append(time.perf_counter() - t0)
+        assert R[0, 0] == 1
+        out[name] = _stats(times)
+    n = sizes[-1]
+    med = out[f"{n}x{n}"]["median_s"]
+    return {"value": round(med * (n_target / n) ** 3, 1), "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"numpy restatement of linac.row_reduce (prime path, pivoting=1, scaling=False), 1 core; value = "
+                      f"median of the {n}x{n} runs ({med:.2f} s) x ({n_target}/{n})^3, an N^3 extrapolation to "
+                      f"{n_target} (not a measurement at that size)",
+            "protocol": out, "host_cpu": _cpu_model(), "os_cpu_count": os.cpu_count(),
+            "numpy": numpy.__version__}
+
+
+def host_inclusive(n, p=2 ** 31 - 1, seed=0):
+    """The drop-in call as linac's users make it: cuda_row_reduce(numpy array,
+    p) at N x N, host array in and out (conversion, H2D, RREF, D2H); the
+    reference's published N = 20000 time (README.md:133) is of this kind."""
+    import numpy
+    import linac_amd
+    A = numpy.random.RandomState(seed).randint(p, size=(n, n), dtype=numpy.uint32)
+    linac_amd.cuda_row_reduce(A[:2048, :2048], p)  # warm-up (library, small workspace)
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        R = linac_amd.cuda_row_reduce(A, p)
+        times.append(time.perf_counter() - t0)
+    ok = bool(numpy.array_equal(R.diagonal(), numpy.ones(n, dtype=numpy.uint32))) and int(numpy.count_nonzero(R)) == n
+    del R, A
+    return {"n": n, "seconds": round(min(times), 4), "runs": [round(t, 4) for t in times], "correct": ok}
 
 
 def sweep_bench(n_systems, p=2 ** 31 - 1):
@@ -158,13 +209,27 @@ def serve_bench(n, rate):
     }
 
 
+def _limb_matmul_mod(A, B, p, chunk=4096):
+    """(A @ B) mod p for int64 device tensors in [0, p), p < 2^32: float64
+    products of 16-bit limbs (exact: partial sums below 2^53), reduced in int64."""
+    import torch
+    b0, b1 = (B & 0xFFFF).double(), (B >> 16).double()
+    out = torch.empty((A.shape[0], B.shape[1]), dtype=torch.int64, device=A.device)
+    for r0 in range(0, A.shape[0], chunk):
+        a = A[r0:r0 + chunk]
+        a0, a1 = (a & 0xFFFF).double(), (a >> 16).double()
+        q = [(x @ y).long() % p for x, y in ((a0, b0), (a0, b1), (a1, b0), (a1, b1))]
+        hi = (((q[3] << 16) % p) << 16) % p
+        out[r0:r0 + chunk] = (q[0] + (((q[1] + q[2]) % p) << 16) % p + hi) % p
+    return out
+
+
 def _gf_device_case(n, p, rank_frac, method, seed=0, dtype="u32"):
-    """Seconds of an in-place device RREF of an n x n matrix uniform in [0, p)
-    (rank n w.h.p.), or of rank round(rank_frac * n): its last rows are
-    multiples c_i * row_i of the first ones (a low-rank product L U with
-    L = [I; diag(c)] selecting rows).  Checks the rank and that the result is
-    [I | *] over the pivot rows (the unique RREF when the first r columns are
-    independent, as they are w.h.p.)."""
+    """Seconds of an in-place device RREF of an n x n matrix: uniform in
+    [0, p) (rank n w.h.p.), or of rank r = round(rank_frac * n) as the seeded
+    product L U mod p (L n x r, U r x n uniform).  Checked: rank and pivots,
+    the [I | X; 0] shape, and for L U Freivalds' U v == U[:, :r] (R v) mod p
+    (which pins R, since RREF(L U) = RREF(U))."""
     import numpy
     import torch
     import linac_amd
@@ -172,23 +237,29 @@ def _gf_device_case(n, p, rank_frac, method, seed=0, dtype="u32"):
     r = n if rank_frac >= 1 else int(round(rank_frac * n))
     npdt = numpy.uint32 if dtype == "u32" else numpy.uint64
 
+    def rand(rows, cols, g):
+        out = torch.empty((rows, cols), dtype=tdt, device="cuda")
+        chunk = max(1, (1 << 28) // cols)
+        for r0 in range(0, rows, chunk):  # int64 draws in slices
+            out[r0:r0 + chunk] = torch.randint(0, p, (min(chunk, rows - r0), cols), generator=g, device="cuda",
+                                               dtype=torch.int64).to(tdt)
+        return out
+
     def make():
         g = torch.Generator(device="cuda")
         g.manual_seed(seed)
-        A = torch.empty((n, n), dtype=tdt, device="cuda")
-        chunk = max(1, (1 << 28) // n)
-        for r0 in range(0, n, chunk):  # int64 draws in slices (a whole int64 copy would double the footprint)
-            A[r0:r0 + chunk] = torch.randint(0, p, (min(chunk, n - r0), n), generator=g, device="cuda",
-                                             dtype=torch.int64).to(tdt)
-        if r < n:
-            c = torch.randint(1, 1 << 20, (n - r, 1), generator=g, device="cuda", dtype=torch.int64)
-            A[r:] = ((A[:n - r].to(torch.int64) * c) % p).to(tdt)
-        return A
+        if r == n:
+            return rand(n, n, g), None
+        L = rand(n, r, g).long() & 0xFFFFFFFF
+        U = rand(r, n, g).long() & 0xFFFFFFFF
+        A = _limb_matmul_mod(L, U, p).to(tdt)
+        del L
+        return A, U
 
-    A = make()
+    A, U = make()
     if method != "plain":  # an untimed solve first: the blocked workspace grows to this size
         linac_amd.rref_device(A.data_ptr(), n, n, n, p, npdt, method=method)
-        A = make()
+        A, U = make()
     torch.cuda.synchronize()
     t = time.perf_counter()
     piv, rank = linac_amd.rref_device(A.data_ptr(), n, n, n, p, npdt, method=method)
@@ -199,18 +270,28 @@ def _gf_device_case(n, p, rank_frac, method, seed=0, dtype="u32"):
         top = A[:r, :r]
         ok = bool(top.diagonal().eq(1).all().item()) and int(top.count_nonzero().item()) == r and \
             int(A[r:].count_nonzero().item()) == 0
-    del A
+    if ok and U is not None:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed + 99)
+        v = torch.randint(0, p, (n, 1), generator=g, device="cuda", dtype=torch.int64)
+        w = _limb_matmul_mod(A[:r].long() & 0xFFFFFFFF, v, p)
+        ok = bool(torch.equal(_limb_matmul_mod(U, v, p), _limb_matmul_mod(U[:, :r], w, p)))
+    del A, U
     torch.cuda.empty_cache()
-    return {"n": n, "p": p, "rank": rank, "method": method, "seconds": round(sec, 4), "correct": ok}
+    return {"n": n, "p": p, "rank": rank, "method": method, "seconds": round(sec, 4), "correct": ok,
+            "input": "uniform" if r == n else f"L U mod p, L {n}x{r}, U {r}x{n} uniform (seed {seed})"}
 
 
 def _float_case(kind, n, seed=0):
-    """[A | b] on the device through the blocked float path; error of the
-    solution column against numpy's FP64 LAPACK solve (the FP64 RREF of
-    [A | b] for square non-singular A), plus max |R[:, :n] - I|."""
+    """[A | b] on the device through the blocked float path (unrefined and
+    refined) and the plain one; error of the solution column against the
+    exact solution (numpy's LAPACK solve refined with long-double residuals),
+    plus max |R[:, :n] - I|."""
     import numpy
+    import scipy.linalg
     import torch
     import linac_amd
+    from linac_amd import _lib
     rs = numpy.random.RandomState(seed)
     if kind == "f64":
         A = rs.random_sample((n, n + 1))
@@ -223,34 +304,40 @@ def _float_case(kind, n, seed=0):
         A = numpy.empty((n, n + 1))
         A[:, :n] = (haar(n) * numpy.logspace(0, -8, n)) @ haar(n).T
         A[:, n] = rs.random_sample(n)
-    x = numpy.linalg.solve(A[:, :n], A[:, n])
-    dt = torch.complex128 if kind == "c128" else torch.float64
-    d = torch.from_numpy(A).to("cuda", dtype=dt)
+    lu = scipy.linalg.lu_factor(A[:, :n])
+    x = scipy.linalg.lu_solve(lu, A[:, n])
+    ext = numpy.clongdouble if kind == "c128" else numpy.longdouble
+    Al, bl = A[:, :n].astype(ext), A[:, n].astype(ext)
+    for _ in range(3):
+        x = x + scipy.linalg.lu_solve(lu, (bl - Al @ x.astype(ext)).astype(A.dtype))
+    del Al, bl
+    dtn = numpy.complex128 if kind == "c128" else numpy.float64
+    d = torch.from_numpy(A).to("cuda", dtype=torch.complex128 if kind == "c128" else torch.float64)
     torch.cuda.synchronize()
     out = {}
-    for method in ("blocked", "plain"):
+    for method, refine in (("blocked", False), ("blocked", True), ("plain", False)):
         w = d.clone()
         if method == "blocked":  # an untimed solve first: the blocked workspace grows to this size
-            linac_amd.rref_device(w.data_ptr(), n, n + 1, n + 1, 0,
-                                  numpy.complex128 if kind == "c128" else numpy.float64, method=method)
+            linac_amd.rref_device(w.data_ptr(), n, n + 1, n + 1, 0, dtn, method=method, refine=refine)
             w.copy_(d)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        piv, rank = linac_amd.rref_device(w.data_ptr(), n, n + 1, n + 1, 0,
-                                          numpy.complex128 if kind == "c128" else numpy.float64, method=method)
+        piv, rank = linac_amd.rref_device(w.data_ptr(), n, n + 1, n + 1, 0, dtn, method=method, refine=refine)
         torch.cuda.synchronize()
         sec = time.perf_counter() - t
         R = w.cpu().numpy()
         err = float(numpy.max(numpy.abs(R[:, n] - x)) / numpy.max(numpy.abs(x)))
-        out[method] = {"seconds": round(sec, 4), "rank": rank, "rel_err_solution_vs_lapack": float(f"{err:.3g}"),
-                       "max_abs_R_minus_I": float(f"{float(numpy.max(numpy.abs(R[:, :n] - numpy.eye(n)))):.3g}")}
+        key = method + ("_refined" if refine else "")
+        out[key] = {"seconds": round(sec, 4), "rank": rank, "rel_err_vs_exact": float(f"{err:.3g}"),
+                    "max_abs_R_minus_I": float(f"{float(numpy.max(numpy.abs(R[:, :n] - numpy.eye(n)))):.3g}")}
+        if refine:
+            st = _lib.refine_last()
+            out[key]["correction_first_last"] = [float(f"{st['first_correction']:.3g}"),
+                                                 float(f"{st['last_correction']:.3g}")]
         del w
     del d
     torch.cuda.empty_cache()
-    cond = None
-    if kind == "illcond":
-        cond = 1e8
-    return {"kind": kind, "n": n, "cond_2": cond, **out}
+    return {"kind": kind, "n": n, "cond_2": 1e8 if kind == "illcond" else None, **out}
 
 
 def _coo_case(n=4000, density=0.05, seed=0):
@@ -477,6 +564,10 @@ def main():
         roofline["traffic_algorithmic"] = round(t["algorithmic_bytes_per_launch"])
         roofline["traffic_source"] = os.path.relpath(tfile, ROOT)
     comparators = {"device_ms_profiled_run": round(prof["device_ms"], 2)}
+    host = None
+    if not args.no_host:
+        host = host_inclusive(N)
+        comparators["host_inclusive_drop_in"] = host
     if method != "plain" and not args.no_plain and used != "plain":
         t1 = time.perf_counter()
         step("plain")
@@ -497,8 +588,10 @@ def main():
         "ms_per_step": round(sec * 1e3, 3),
         "higher_is_better": False,
         "scaling": "weak",
-        "vs_baseline": round(sec / A100_SECONDS_N20000, 5) if N == 20000 else None,
-        "vs_baseline_note": "seconds / 23.40 s (A100, README.md:133); below 1 is faster",
+        "vs_baseline": round(host["seconds"] / A100_SECONDS_N20000, 5) if (N == 20000 and host) else None,
+        "vs_baseline_note": "host-inclusive cuda_row_reduce(numpy, p) seconds at N=20000 (comparators) / 23.40 s, "
+                            "the reference's A100 time of the same call (README.md:133); below 1 is faster. "
+                            "`value` itself is device-resident.",
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": f"rref_gf_u32_N{N}", "rows": N, "cols": N, "p": p, "method": used,
@@ -513,7 +606,7 @@ def main():
     if not args.no_serve:
         out["serving"] = serve_bench(args.serve_requests, args.serve_rate)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(N, args.cpu_n)
+        out["cpu_baseline"] = cpu_baseline(N, tuple(int(x) for x in args.cpu_sizes.split(",")))
     print(json.dumps(out), flush=True)
 
 

@@ -143,6 +143,7 @@ struct BMat {
     int32_t full;       // every row is a pivot row: update every trailing column now, then stop
     int64_t rank_start; // index of its first pivot in piv_rows (the deferred panel's kA precede it)
     int64_t T0, nT;     // trailing columns [T0, T0 + nT)
+    int64_t* cand;      // [BO] rows of the compact candidate block (full-rank panel fast path)
 };
 
 constexpr int64_t STRIDE_W = (int64_t)BO * BO;
@@ -1486,11 +1487,519 @@ __global__ void k_gather_status(const BMat<LT>* __restrict__ D, ItemStatus* out)
 }
 
 // ---------------------------------------------------------------------------
+// Full-rank panel fast path.  If the first bcur unused rows (the candidate
+// block) have full rank on the panel's bcur columns, every panel column is a
+// pivot column, those rows can be the panel's pivot rows, and the panel's
+// transformation restricted to them, Z[Q] = M^-1 with M = P[Q, pivot
+// columns in slot order], follows from the candidate block alone.  Every other
+// row r then has Z[r] = -P[r, :] B with B[c, s] = M^-1[slot of column c, s]
+// (one GEMM) and a zero panel; the pivot rows get the block's rows.  The block
+// is factorised by the inner-step kernels themselves, run on a compact copy
+// (descriptors Dc, N = bcur); the regular inner steps over all N rows run only
+// for matrices whose block is rank-deficient.  Over a field the RREF does not
+// depend on which rows serve as pivots, so the result is the same.
+// ---------------------------------------------------------------------------
+
+// Candidate block of each matrix with Dc[m].N > 0: the first Dc[m].N unused
+// rows (row order) into Dc[m].cand, their panel-buffer rows into the compact
+// buffer; the compact view's state reset (a matrix with Dc[m].N == 0 gets a
+// finished block state, so that the compact inner steps skip it).
+template <class LT>
+__global__ __launch_bounds__(1024) void k_cand_gather(const BMat<LT>* __restrict__ D, const BMat<LT>* __restrict__ Dc) {
+    using T = typename LT::T;
+    const BMat<LT>& d = D[blockIdx.x];
+    const BMat<LT>& c = Dc[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) {
+        c.st->prow = -1;
+        c.st->rank = 0;
+        c.st->first_unused = 0;
+        c.st->factor = 0;
+        block_state_init<LT>(c, 0);
+        if (c.N == 0) c.bs->c0_next = c.bcur;
+    }
+    if (c.N == 0) return;
+    const int C = (int)c.N;
+    __shared__ int s_total;
+    __shared__ int s_wcnt[16];
+    if (tid == 0) s_total = 0;
+    __syncthreads();
+    for (int64_t base = d.st->first_unused; base < d.N; base += 1024) {
+        const int64_t r = base + tid;
+        const bool fl = r < d.N && !d.used[r];
+        const unsigned long long m = __ballot(fl);
+        if (lane == 0) s_wcnt[wid] = __popcll(m);
+        __syncthreads();
+        int before = s_total;
+        for (int q = 0; q < wid; ++q) before += s_wcnt[q];
+        const int pos = before + __popcll(m & ((1ull << lane) - 1ull));
+        if (fl && pos < C) c.cand[pos] = r;
+        __syncthreads();
+        if (tid == 0) {
+            int sum = 0;
+            for (int q = 0; q < 16; ++q) sum += s_wcnt[q];
+            s_total += sum;
+        }
+        __syncthreads();
+        if (s_total >= C) break;
+    }
+    // (the host starts the fast path only where at least C rows are unused)
+}
+
+// The candidate rows' panel-buffer rows into the compact buffer, two rows
+// per workgroup (blockIdx.x), and their compact row state reset.
+template <class LT>
+__global__ __launch_bounds__(256) void k_cand_copy(const BMat<LT>* __restrict__ D, const BMat<LT>* __restrict__ Dc) {
+    using T = typename LT::T;
+    const BMat<LT>& d = D[blockIdx.y];
+    const BMat<LT>& c = Dc[blockIdx.y];
+    constexpr int PER = 16 / (int)sizeof(T), VPR = XLD / PER;
+    static_assert(2 * VPR <= 256 * 2, "two rows per workgroup");
+    const int i = (int)blockIdx.x * 2 + (int)(threadIdx.x / 128);
+    if (i >= c.N) return;
+    const int64_t src = c.cand[i];
+    for (int v = threadIdx.x % 128; v < VPR; v += 128)
+        gst<v4i>(c.X + (int64_t)i * XLD + v * PER, gld<v4i>(d.X + src * XLD + v * PER));
+    if (threadIdx.x % 128 == 0) {
+        c.used[i] = 0;
+        c.epoch_of[i] = -1;
+        c.slot_of[i] = 0;
+    }
+}
+
+// The candidate block's inner steps in one workgroup (32-bit storage), in
+// place of k_inner_factor + k_make_u + k_gemm_inner per inner panel.  With a
+// block of full rank the candidates of inner panel i are the block's rows
+// [q0, q0 + BI) (every earlier group became pivots), so per inner panel:
+//   1. fraction-free Gauss-Jordan on [X[cand, c0:c0+w] | I] in LDS (as in
+//      k_inner_factor), normalisation, the pivots' bookkeeping and Z slots;
+//   2. U = Apiv^-1 X[R, win] on the int8 matrix cores (limb planes of -Apiv^-1
+//      and of X[R, win]^T in LDS, K = 32 slots), written to the pivot rows;
+//   3. X[r, win] -= X[r, c0 + Cp] U for every other row of the block, a wave
+//      per 32 rows (the left operand read before the wave writes its rows).
+// win = the live columns [c0, BO + q0 + k) widened to multiples of 64.  A
+// group that yields fewer pivots than its width stops the kernel; the block
+// state then shows q0 < bcur and the host takes the regular path.
+// Sized to share a CU with one workgroup of the trailing GEMM, which holds the
+// rest of the device meanwhile: 8 waves of at most 128 registers, < 80 KB LDS.
+// (Global writes of one wave are read by others only after a barrier: all
+// waves of a workgroup share the CU's vector cache.)
+constexpr int FB_WMAX = 320;  // widest window: BO + BI + 63 rounded to 64
+constexpr int FB_WAVES = 8;
+
+// Byte offset of the 8-byte K group g (slots 8 g .. 8 g + 7) of row `row` in a
+// limb plane of 32-byte rows; the XOR keeps 16 consecutive rows of one group
+// on distinct banks.
+__device__ __forceinline__ int lds8(int row, int g) { return row * 32 + ((g ^ ((row >> 3) & 3)) << 3); }
+
+typedef long v1l;  // the 8 int8 operand bytes of a 16x16x32 MFMA lane
+
+__global__ __launch_bounds__(64 * FB_WAVES, 4) void k_block_factor(const BMat<LimbsU32>* __restrict__ Dc, int epoch0) {
+    const BMat<LimbsU32>& c = Dc[blockIdx.x];
+    if (c.N == 0) return;
+    const FieldU32 f = c.f;
+    const double pd = c.pd, pinv = c.pinv;
+    uint32_t* __restrict__ X = c.X;
+    const int C = (int)c.N, bcur = c.bcur;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int r16 = lane & 15, g = lane >> 4;
+    constexpr int W2 = BI + CI, NW = FB_WAVES, RPW = CI / NW, NT = 64 * NW;
+    static_assert(W2 == 64 && RPW * NW == CI, "one augmented column per lane, whole rows per wave");
+    __shared__ uint32_t aug[CI][W2 + 1];
+    __shared__ int prow_of[CI], pcol[CI], s_tof[CI];
+    __shared__ uint32_t s_invm[CI];
+    __shared__ __attribute__((aligned(16))) int8_t sA[4][CI * 32];       // -Apiv^-1: row t, K = slot
+    __shared__ __attribute__((aligned(16))) int8_t sB[4][FB_WMAX * 32];  // X[R, win]^T, then U^T: row n, K = slot
+    const v4i zero = {0, 0, 0, 0};
+    int q0 = 0;
+    int c0 = 0;
+    unsigned long long tacc[6] = {0, 0, 0, 0, 0, 0};
+    for (; c0 < bcur; c0 += BI) {
+        const unsigned long long t0 = wall_clock64();
+        const int w = min(BI, bcur - c0);
+        const int nC = min(CI, C - q0);
+        // 1. [X[q0 + i, c0:c0 + w] | I]; rows >= nC are zero and never pivot
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int ii = wid + NW * q, j = lane;
+            uint32_t v = 0;
+            if (ii < nC) {
+                if (j < w) v = X[(int64_t)(q0 + ii) * XLD + c0 + j];
+                else if (j >= BI) v = (j - BI == ii) ? 1u : 0u;
+            }
+            aug[ii][j] = v;
+        }
+        __syncthreads();
+        unsigned long long pmask = 0ull;
+        int t = 0;
+        for (int cc = 0; cc < w && t < nC; ++cc) {
+            uint32_t av[RPW], xv[RPW];
+#pragma unroll
+            for (int q = 0; q < RPW; ++q) {
+                const int r = wid + NW * q;
+                av[q] = aug[r][cc];
+                xv[q] = aug[r][lane];
+            }
+            const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][cc] != 0u;
+            const unsigned long long m = __ballot(nz);
+            if (m == 0ull) continue;
+            const int pv = __ffsll((long long)m) - 1;
+            const uint32_t h = aug[pv][cc];
+            const uint32_t y = aug[pv][lane];
+#pragma unroll
+            for (int q = 0; q < RPW; ++q) {
+                const int r = wid + NW * q;
+                if (r < nC && r != pv) aug[r][lane] = f.ff_update(h, xv[q], av[q], y);
+            }
+            if (tid == 0) {
+                prow_of[t] = pv;
+                pcol[t] = cc;
+            }
+            pmask |= 1ull << pv;
+            ++t;
+            __syncthreads();
+        }
+        const int k = t;
+        __syncthreads();
+        const unsigned long long t1 = wall_clock64();
+        if (k < w) break;  // rank-deficient group: the host falls back (uniform: every wave found the same k)
+        if (wid == 0 && lane < k) {
+            const int r = prow_of[lane];
+            s_invm[lane] = f.to_mont(f.inverse(aug[r][pcol[lane]]));
+            s_tof[r] = lane;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int r = wid + NW * q;
+            if ((pmask >> r) & 1ull) aug[r][lane] = f.mont(s_invm[s_tof[r]], aug[r][lane]);
+        }
+        __syncthreads();
+        // bookkeeping of the group's pivots; their Z slots become 1
+        if (tid < k) {
+            const int r = q0 + prow_of[tid];
+            c.used[r] = 1;
+            c.epoch_of[r] = epoch0 + c0 / BI;
+            c.slot_of[r] = q0 + tid;
+            c.piv_rows[q0 + tid] = r;
+            c.piv_cols[q0 + tid] = c.J + c0 + pcol[tid];
+            X[(int64_t)r * XLD + BO + q0 + tid] = 1u;
+        }
+        // limb planes of -Apiv^-1 (normal form): row t, K = slot s, 8 slots per store
+        if (tid < CI * 4) {
+            const int tt = tid >> 2, kg = tid & 3;
+            int32_t pk[4][2] = {};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int sk = 8 * kg + j;
+                const uint32_t v = (tt < k && sk < k) ? aug[prow_of[tt]][BI + prow_of[sk]] : 0u;
+                int8_t d[4];
+                balanced_digits(-centred(v, f.p), d);
+#pragma unroll
+                for (int l = 0; l < 4; ++l) pk[l][j >> 2] |= (int32_t)(uint8_t)d[l] << (8 * (j & 3));
+            }
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                v1l w8;
+                __builtin_memcpy(&w8, pk[l], 8);
+                *reinterpret_cast<v1l*>(&sA[l][lds8(tt, kg)]) = w8;
+            }
+        }
+        __syncthreads();  // (also orders the Z slot writes before the reads of X[R] below)
+        const unsigned long long t2 = wall_clock64();
+        const int qn = q0 + k;
+        const int wlo = (c0 / 64) * 64, whi = (int)round_up(BO + qn, 64);
+        const int W = whi - wlo;
+        // limb planes of X[R, win]^T: row n, K = slot s; a thread per (n, 8-slot group)
+        for (int x = tid; x < W * 4; x += NT) {
+            const int n = x % W, kg = x / W;
+            uint32_t v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int sk = 8 * kg + j;
+                v[j] = X[(int64_t)(q0 + prow_of[sk < k ? sk : 0]) * XLD + wlo + n];
+            }
+            int32_t pk[4][2] = {};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                int8_t d[4];
+                balanced_digits(8 * kg + j < k ? centred(v[j], f.p) : 0, d);
+#pragma unroll
+                for (int l = 0; l < 4; ++l) pk[l][j >> 2] |= (int32_t)(uint8_t)d[l] << (8 * (j & 3));
+            }
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                v1l w8;
+                __builtin_memcpy(&w8, pk[l], 8);
+                *reinterpret_cast<v1l*>(&sB[l][lds8(n, kg)]) = w8;
+            }
+        }
+        __syncthreads();
+        const unsigned long long t3 = wall_clock64();
+        // 2. U = Apiv^-1 X[R, win]: 16 x 16 output blocks (2 row halves x W / 16), spread over the waves
+        const int nblk = 2 * (W / 16);
+        for (int b = wid; b < nblk; b += NW) {
+            const int th = b & 1, nb = b >> 1;
+            v4i acc[7];
+#pragma unroll
+            for (int q = 0; q < 7; ++q) acc[q] = zero;
+            v1l af[4], bf[4];
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                af[l] = *reinterpret_cast<const v1l*>(&sA[l][lds8(th * 16 + r16, g)]);
+                bf[l] = *reinterpret_cast<const v1l*>(&sB[l][lds8(nb * 16 + r16, g)]);
+            }
+#pragma unroll
+            for (int l1 = 0; l1 < 4; ++l1)
+#pragma unroll
+                for (int l2 = 0; l2 < 4; ++l2)
+                    acc[l1 + l2] = __builtin_amdgcn_mfma_i32_16x16x32_i8(af[l1], bf[l2], acc[l1 + l2], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                int32_t sv[7];
+#pragma unroll
+                for (int q = 0; q < 7; ++q) sv[q] = acc[q][i];
+                const int tt = th * 16 + 4 * g + i;  // 0 - sum(-Apiv^-1 X) = U, straight into the pivot row
+                if (tt < k) X[(int64_t)(q0 + prow_of[tt]) * XLD + wlo + nb * 16 + r16] = fold_sub_k32(0u, sv, pd, pinv);
+            }
+        }
+        __syncthreads();  // every wave is done reading sB
+        const unsigned long long t4 = wall_clock64();
+        // limb planes of U (now in the pivot rows; row n, K = slot t)
+        for (int x = tid; x < W * 4; x += NT) {
+            const int n = x % W, kg = x / W;
+            uint32_t v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int tt = 8 * kg + j;
+                v[j] = X[(int64_t)(q0 + prow_of[tt < k ? tt : 0]) * XLD + wlo + n];
+            }
+            int32_t pk[4][2] = {};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                int8_t d[4];
+                balanced_digits(8 * kg + j < k ? centred(v[j], f.p) : 0, d);
+#pragma unroll
+                for (int l = 0; l < 4; ++l) pk[l][j >> 2] |= (int32_t)(uint8_t)d[l] << (8 * (j & 3));
+            }
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                v1l w8;
+                __builtin_memcpy(&w8, pk[l], 8);
+                *reinterpret_cast<v1l*>(&sB[l][lds8(n, kg)]) = w8;
+            }
+        }
+        __syncthreads();
+        const unsigned long long t5 = wall_clock64();
+        // 3. the other rows of the block: wave wid owns rows 32 wid .. 32 wid + 31, in two groups of 16
+#pragma unroll 1
+        for (int grp = 0; grp < 2; ++grp) {
+            const int rw = 32 * wid + 16 * grp;
+            if (rw >= C || (rw < qn && rw + 16 > q0)) continue;  // beyond the block, or this group's pivots
+            // left operand: byte j of af[l] = digit l of X[row, c0 + Cp[t]], t = 8 g + j
+            v1l af[4];
+            {
+                const int row = min(rw + r16, C - 1);
+                int32_t pk[4][2] = {};
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int tt = 8 * g + j;
+                    const uint32_t v = tt < k ? X[(int64_t)row * XLD + c0 + pcol[tt]] : 0u;
+                    int8_t d[4];
+                    balanced_digits(centred(v, f.p), d);
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) pk[l][j >> 2] |= (int32_t)(uint8_t)d[l] << (8 * (j & 3));
+                }
+#pragma unroll
+                for (int l = 0; l < 4; ++l) __builtin_memcpy(&af[l], pk[l], 8);
+            }
+            uint32_t* xb = X + (int64_t)rw * XLD + wlo + r16;  // the group's rows: 32-bit offsets from here
+            uint32_t xo[4], cn[4];  // cn: C values of the next block, in flight during this one
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                xo[i] = (uint32_t)(min(rw + 4 * g + i, C - 1) - rw) * XLD;
+                cn[i] = xb[xo[i]];
+            }
+            for (int nb = 0; nb < W / 16; ++nb) {
+                uint32_t cv[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    cv[i] = cn[i];
+                    cn[i] = xb[xo[i] + (uint32_t)min(nb + 1, W / 16 - 1) * 16];
+                }
+                v1l bf[4];
+#pragma unroll
+                for (int l = 0; l < 4; ++l) bf[l] = *reinterpret_cast<const v1l*>(&sB[l][lds8(nb * 16 + r16, g)]);
+                v4i acc[7];
+#pragma unroll
+                for (int q = 0; q < 7; ++q) acc[q] = zero;
+#pragma unroll
+                for (int l1 = 0; l1 < 4; ++l1)
+#pragma unroll
+                    for (int l2 = 0; l2 < 4; ++l2)
+                        acc[l1 + l2] = __builtin_amdgcn_mfma_i32_16x16x32_i8(af[l1], bf[l2], acc[l1 + l2], 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (rw + 4 * g + i >= C) continue;
+                    int32_t sv[7];
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) sv[q] = acc[q][i];
+                    xb[xo[i] + nb * 16] = fold_sub_k32(cv[i], sv, pd, pinv);
+                }
+            }
+        }
+        __syncthreads();
+        const unsigned long long t6 = wall_clock64();
+        tacc[0] += t1 - t0;
+        tacc[1] += t2 - t1;
+        tacc[2] += t3 - t2;
+        tacc[3] += t4 - t3;
+        tacc[4] += t5 - t4;
+        tacc[5] += t6 - t5;
+        q0 = qn;
+    }
+    if (tid == 0) {
+        for (int i = 0; i < 6; ++i) c.bs->tphase[i] += tacc[i];
+        c.bs->tphase[6] += 1;
+        c.st->rank = q0;
+        c.bs->q0 = q0;
+        c.bs->k = 0;
+        c.bs->c0_cur = c0 < bcur ? c0 : 0;
+        c.bs->c0_next = c0 < bcur ? c0 : bcur;
+    }
+}
+
+// Operands of Z = -P B for the matrices on the fast path (Dw[m].nT > 0, depth
+// Kq = bcur rounded up to 64): workgroups [0, nl) write the limb planes of
+// the panel P = X[:, 0:bcur] (every row, zero beyond bcur) with row stride Kq
+// into planesL1; the rest those of B^T, B[c, s] = M^-1[slot of column c, s]
+// taken from the factorised compact block, into planesB1 (row s, stride Kq).
+template <class LT>
+__global__ __launch_bounds__(256) void k_fast_operands(const BMat<LT>* __restrict__ Dw, const BMat<LT>* __restrict__ Dc,
+                                                       int nl) {
+    constexpr int NL = LT::NL;
+    const BMat<LT>& d = Dw[blockIdx.y];
+    if (d.nT == 0) return;
+    const int Kq = d.Kq, bcur = d.bcur;
+    const int64_t bx = blockIdx.x;
+    if (bx < nl) {
+        const int64_t total = d.N * (Kq / 4);
+        for (int64_t x = bx * 256 + threadIdx.x; x < total; x += (int64_t)nl * 256) {
+            const int64_t r = x / (Kq / 4);
+            const int c = (int)(x - r * (Kq / 4)) * 4;
+            int8_t dg[NL][4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int8_t g[NL];
+                balanced_digits(c + e < bcur ? centred(d.X[r * XLD + c + e], d.f.p) : (typename LT::S)0, g);
+#pragma unroll
+                for (int l = 0; l < NL; ++l) dg[l][e] = g[l];
+            }
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                int32_t pk;
+                __builtin_memcpy(&pk, dg[l], 4);
+                *reinterpret_cast<int32_t*>(d.planesL1 + l * d.strideL1 + r * Kq + c) = pk;
+            }
+        }
+        return;
+    }
+    const BMat<LT>& c = Dc[blockIdx.y];
+    const int64_t nb = gridDim.x - nl;
+    for (int64_t x = (bx - nl) * 256 + threadIdx.x; x < (int64_t)bcur * bcur; x += nb * 256) {
+        const int t = (int)(x / bcur), sidx = (int)(x - (int64_t)t * bcur);
+        const int col = (int)(c.piv_cols[t] - c.J);
+        int8_t g[NL];
+        balanced_digits(centred(c.X[c.piv_rows[t] * XLD + BO + sidx], d.f.p), g);
+#pragma unroll
+        for (int l = 0; l < NL; ++l) d.planesB1[l * STRIDE_W + (int64_t)sidx * Kq + col] = g[l];
+    }
+}
+
+// Panel columns of every row of a fast-path matrix set to zero (the pivot
+// rows get theirs back from the block in k_fast_commit).
+template <class LT>
+__global__ __launch_bounds__(256) void k_fast_zero(const BMat<LT>* __restrict__ Dw) {
+    using T = typename LT::T;
+    const BMat<LT>& d = Dw[blockIdx.y];
+    if (d.nT == 0) return;
+    constexpr int PER = 16 / (int)sizeof(T);
+    const int vpr = (int)ceil_div(d.bcur, PER);
+    const v4i z = {0, 0, 0, 0};
+    for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < d.N * vpr; x += (int64_t)gridDim.x * 256) {
+        const int64_t r = x / vpr;
+        gst<v4i>(d.X + r * XLD + (x - r * vpr) * PER, z);
+    }
+}
+
+// The block's pivot rows back into the panel buffer (normalised panel row
+// and M^-1 row) and their bookkeeping -- row flags, epochs, slots, pivot lists
+// -- two pivots per workgroup (blockIdx.x); k_fast_finish then advances the
+// first unused row and sets rank and block state.
+template <class LT>
+__global__ __launch_bounds__(256) void k_fast_commit(const BMat<LT>* __restrict__ D, const BMat<LT>* __restrict__ Dc,
+                                                     const BMat<LT>* __restrict__ Dw, int epoch) {
+    using T = typename LT::T;
+    const BMat<LT>& d = D[blockIdx.y];
+    const BMat<LT>& c = Dc[blockIdx.y];
+    if (Dw[blockIdx.y].nT == 0) return;
+    const int q = (int)c.st->rank;  // == bcur on the fast path
+    const int t = (int)blockIdx.x * 2 + (int)(threadIdx.x / 128);
+    if (t >= q) return;
+    constexpr int PER = 16 / (int)sizeof(T), VPR = XLD / PER;
+    const int64_t cr = c.piv_rows[t];
+    const int64_t gr = c.cand[cr];
+    for (int v = threadIdx.x % 128; v < VPR; v += 128)
+        gst<v4i>(d.X + gr * XLD + v * PER, gld<v4i>(c.X + cr * XLD + v * PER));
+    if (threadIdx.x % 128 == 0) {
+        const int64_t rank0 = d.st->rank;  // updated only by k_fast_finish, after this kernel
+        d.used[gr] = 1;
+        d.epoch_of[gr] = epoch;
+        d.slot_of[gr] = t;
+        d.piv_rows[rank0 + t] = gr;
+        d.piv_cols[rank0 + t] = c.piv_cols[t];
+    }
+}
+
+template <class LT>
+__global__ __launch_bounds__(1024) void k_fast_finish(const BMat<LT>* __restrict__ D, const BMat<LT>* __restrict__ Dc,
+                                                      const BMat<LT>* __restrict__ Dw) {
+    const BMat<LT>& d = D[blockIdx.x];
+    const BMat<LT>& c = Dc[blockIdx.x];
+    if (Dw[blockIdx.x].nT == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int q = (int)c.st->rank;
+    __shared__ unsigned long long s_first;
+    if (tid == 0) s_first = ~0ull;
+    __syncthreads();
+    for (int64_t base = d.st->first_unused; base < d.N; base += 1024) {
+        const int64_t r = base + tid;
+        const unsigned long long m = __ballot(r < d.N && !d.used[r]);
+        if (lane == 0 && m) atomicMin(&s_first, (unsigned long long)(base + wid * 64 + __ffsll((long long)m) - 1));
+        __syncthreads();
+        const bool found = s_first != ~0ull;
+        __syncthreads();
+        if (found) break;
+    }
+    if (tid == 0) {
+        d.st->rank += q;
+        d.st->first_unused = s_first != ~0ull ? (int64_t)s_first : d.N;
+        d.bs->k = 0;
+        d.bs->q0 = q;
+        d.bs->c0_cur = 0;
+        d.bs->c0_next = d.bcur;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host engine: a set of matrices advanced together, one outer panel per step
 // ---------------------------------------------------------------------------
 
 struct WorkspaceLayout {
-    size_t X, L, Wp, B, L1, B1, Ai, Ui, U, W, bs, st, piv, used, epoch, slot, total;
+    size_t X, L, Wp, B, L1, B1, Ai, Ui, U, W, bs, st, piv, used, epoch, slot;
+    // the compact candidate block (BO rows) of the full-rank panel fast path
+    size_t cX, cAi, cUi, cU, cbs, cst, cpiv, cused, cepoch, cslot, cand;
+    size_t total;
 };
 
 template <class LT>
@@ -1519,6 +2028,17 @@ static WorkspaceLayout workspace_layout(int64_t N, int64_t M, bool with_elim_sta
         w.piv = off; off += 2 * al((size_t)cap * 8);
         w.used = off; off += al((size_t)N);
     }
+    w.cX = off; off += al((size_t)BO * XLD * E);
+    w.cAi = off; off += al(NL * BO * KPI);
+    w.cUi = off; off += al(NL * XLD * KPI);
+    w.cU = off; off += al((size_t)CI * XLD * E);
+    w.cbs = off; off += al(sizeof(BlockState<T>));
+    w.cst = off; off += al(sizeof(ElimState));
+    w.cpiv = off; off += 2 * al((size_t)BO * 8);
+    w.cused = off; off += al(BO);
+    w.cepoch = off; off += al((size_t)4 * BO);
+    w.cslot = off; off += al((size_t)4 * BO);
+    w.cand = off; off += al((size_t)8 * BO);
     w.total = off;
     return w;
 }
@@ -1550,11 +2070,35 @@ static void carve(BMat<LT>& d, char* base, const WorkspaceLayout& w, int64_t N, 
         d.piv_cols = (int64_t*)(base + w.piv + round_up(cap * 8, 256));
         d.used = (uint8_t*)(base + w.used);
     }
+    d.cand = (int64_t*)(base + w.cand);
+}
+
+// The compact view of the candidate block: the inner-step kernels' fields
+// only (panel buffer, inner GEMM planes, U, state), BO rows.
+template <class LT>
+static void carve_compact(BMat<LT>& c, char* base, const WorkspaceLayout& w) {
+    using T = typename LT::T;
+    c = BMat<LT>{};
+    c.X = (T*)(base + w.cX);
+    c.planesAi = (int8_t*)(base + w.cAi);
+    c.strideAi = round_up((int64_t)BO * KPI, 256);
+    c.planesUi = (int8_t*)(base + w.cUi);
+    c.U = (T*)(base + w.cU);
+    c.bs = (BlockState<T>*)(base + w.cbs);
+    c.st = (ElimState*)(base + w.cst);
+    c.piv_rows = (int64_t*)(base + w.cpiv);
+    c.piv_cols = (int64_t*)(base + w.cpiv + round_up((int64_t)BO * 8, 256));
+    c.used = (uint8_t*)(base + w.cused);
+    c.epoch_of = (int32_t*)(base + w.cepoch);
+    c.slot_of = (int32_t*)(base + w.cslot);
+    c.cand = (int64_t*)(base + w.cand);
 }
 
 template <class LT>
 struct Item {
     BMat<LT> d{};
+    BMat<LT> dc{};             // compact view of the candidate block (fast path)
+    int64_t rank = 0;          // rank after the last step, as the host knows it
     void* arena = nullptr;     // device memory owned by the item (null when borrowed)
     size_t bytes = 0;
     bool finished = false;
@@ -1576,6 +2120,7 @@ class Engine {
         }
         if (ev_ready) (void)hipEventDestroy(ev_ready);
         if (ev_status) (void)hipEventDestroy(ev_status);
+        if (ev_inner) (void)hipEventDestroy(ev_inner);
         for (int i = 0; i < NEV; ++i) {
             if (ev_f[i]) (void)hipEventDestroy(ev_f[i]);
             if (ev_c[i]) (void)hipEventDestroy(ev_c[i]);
@@ -1609,7 +2154,7 @@ class Engine {
     // side two steps after its copy was queued (the inner phase of a step
     // waits for the previous step's first part, which follows the parts
     // before it on the same stream).
-    static constexpr int NSLOT = 3;
+    static constexpr int NSLOT = 5;  // 3: compact block views, 4: fast-path GEMM descriptors
     struct Staged {
         BM* desc;
         TileSched inner, wq, main;  // inner GEMM, composite W GEMM, trailing update
@@ -1626,6 +2171,10 @@ class Engine {
     // of the columns beyond it (composite updates only)
     // parts part .. part + nparts - 1 into one staging slot, one copy
     int upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out, int nparts = 1);
+    // parts 4 (compact block views) and 5 (fast-path GEMM Z = -P B): which items take part
+    std::vector<char> sel;
+    int fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, int64_t ninner, int64_t maxN,
+                   std::vector<char>& fast);
     void fill(const std::vector<It*>& act, int part, char* h, char* dv, size_t desc_b, size_t sched_b, Staged* out);
     int trailing(const Staged& t, Profiler* prof, int64_t tag, bool rhs_ready = false);
     // A trailing part beyond the next panel, queued to run beside a later
@@ -1641,6 +2190,7 @@ class Engine {
     hipStream_t so = nullptr;
     hipEvent_t ev_ready = nullptr;  // on `so`: next panel's columns updated, panel buffer free
     hipEvent_t ev_status = nullptr;  // on `s`: the inner phase's status has reached host memory
+    hipEvent_t ev_inner = nullptr;   // on `s`: the inner phase is complete (fast path without a status read)
     bool ev_pending = false;
     std::vector<Bulk> bulk;         // queued trailing parts, in due order
     int64_t step_no = 0;
@@ -1658,6 +2208,7 @@ int Engine<LT>::ensure_stream() {
     LINAC_HIP(hipStreamCreateWithFlags(&so, hipStreamNonBlocking));
     LINAC_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     LINAC_HIP(hipEventCreateWithFlags(&ev_status, hipEventDisableTiming));
+    LINAC_HIP(hipEventCreateWithFlags(&ev_inner, hipEventDisableTiming));
     for (int i = 0; i < NEV; ++i) {
         LINAC_HIP(hipEventCreateWithFlags(&ev_f[i], hipEventDisableTiming));
         LINAC_HIP(hipEventCreateWithFlags(&ev_c[i], hipEventDisableTiming));
@@ -1768,7 +2319,31 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
     out->cols = 1;
     for (size_t m = 0; m < B; ++m) {
         BM d = act[m]->d;
-        if (part != 0) {
+        if (part == 4) {  // compact view of the candidate block (N = 0: not on the fast path)
+            BM c = act[m]->dc;
+            c.ld = XLD;
+            c.N = sel[m] ? d.bcur : 0;
+            c.M = d.bcur;
+            c.f = d.f;
+            c.pd = d.pd;
+            c.pinv = d.pinv;
+            c.J = d.J;
+            c.bcur = d.bcur;
+            c.e0 = d.e0;
+            d = c;
+        } else if (part == 5) {  // Z = -P B into the panel buffer's Z columns (nT = 0: not fast)
+            d.A = d.X + BO;
+            d.T0 = 0;
+            d.ld = XLD;
+            d.nT = sel[m] ? d.bcur : 0;
+            d.kA = 0;
+            d.KqA = 0;
+            d.Kq = (int32_t)round_up(d.bcur, 64);
+            d.planesL = d.planesL1;
+            d.strideL = d.strideL1;
+            d.planesB = d.planesB1;
+            d.strideB = STRIDE_W;
+        } else if (part != 0) {
             // the next panel's n1 columns, then (composite updates only) the rest
             // in two halves; the first half covers at least the panel after next
             const int64_t n1 = std::min<int64_t>(BO, d.nT), rest = d.nT - n1;
@@ -1820,8 +2395,8 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
         const BM& d = hd[m];
         ma[m] = ceil_div(d.N, G_M);
         // inner GEMM: split workgroups per 64-row stripe
-        pa[m + 1] = pa[m] + (part == 0 ? ma[m] * split : 0);
-        const bool live = part != 0 && d.Kq > 0 && d.nT > 0;
+        pa[m + 1] = pa[m] + ((part == 0 || part == 4) ? ma[m] * split : 0);
+        const bool live = part != 0 && part != 4 && d.Kq > 0 && d.nT > 0;
         const bool wlive = part == 1 && d.kA > 0 && d.q0 > 0 && d.nT > 0;
         mb[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
         pb[m + 1] = pb[m] + (wlive ? ceil_div(d.N, G_M) * ceil_div(d.kA, TN) : 0);
@@ -1863,6 +2438,78 @@ int Engine<LT>::init_items(const std::vector<It*>& items, hipStream_t s) {
     k_item_init<LT><<<dim3(gs_grid(maxw), (unsigned)items.size()), 256, 0, s>>>(st.desc);
     LINAC_HIP(hipGetLastError());
     LINAC_HIP(hipStreamSynchronize(s));  // the next upload reuses this staging slot
+    return LINAC_OK;
+}
+
+// The full-rank panel fast path for the items whose panel can be full rank
+// (at least bcur unused rows): inner steps on the compact candidate blocks,
+// one status read, then for every block of full rank the GEMM Z = -P B and
+// the commit.  `fast[m]` tells which items are done with this panel's inner
+// phase.  Runs on s after the panel load.
+template <class LT>
+int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, int64_t ninner,
+                           int64_t maxN, std::vector<char>& fast) {
+    constexpr bool wide = std::is_same<LT, LimbsU64>::value;
+    const unsigned B = (unsigned)act.size();
+    fast.assign(B, 0);
+    sel.assign(B, 0);
+    bool any = false;
+    for (unsigned m = 0; m < B; ++m) {
+        const BM& d = act[m]->d;
+        sel[m] = d.N - act[m]->rank >= d.bcur ? 1 : 0;
+        any = any || sel[m];
+    }
+    if (!any) return LINAC_OK;
+    Staged cs;
+    LINAC_TRY(upload(act, s, 3, 4, &cs));
+    k_cand_gather<LT><<<B, 1024, 0, s>>>(in.desc, cs.desc);
+    k_cand_copy<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc);
+    const unsigned mu_grid =
+        (unsigned)(MU_BLOCKS + ((wide || cs.inner_split > 1) ? std::min<int64_t>(4096, ceil_div(BO, MU_ROWS)) : 0));
+    const unsigned igrid = (unsigned)(ceil_div(cs.inner_tiles, NUM_XCD) * NUM_XCD);
+    if constexpr (!wide) {
+        if (tuning().fast_panel == 1) {  // the block's inner steps in one workgroup per matrix
+            k_block_factor<<<B, 64 * FB_WAVES, 0, s>>>(cs.desc, epoch);
+            epoch += (int)ninner;
+        }
+    }
+    if (wide || tuning().fast_panel == 2)  // the inner-step kernels on the block
+        for (int64_t i = 0; i < ninner; ++i) {
+            k_inner_factor<LT><<<B, LT::IF_T, 0, s>>>(cs.desc, epoch);
+            k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(cs.desc);
+            if constexpr (wide) k_gemm_inner64<<<igrid, G_T, 0, s>>>(cs.desc, cs.inner, epoch, cs.inner_split);
+            else k_gemm_inner<<<igrid, G_T, 0, s>>>(cs.desc, cs.inner, epoch, cs.inner_split);
+            ++epoch;
+        }
+    k_gather_status<LT><<<B, 1, 0, s>>>(cs.desc, cs.status);
+    LINAC_HIP(hipMemcpyAsync(cs.status_h, cs.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
+    LINAC_HIP(hipEventRecord(ev_status, s));
+    for (;;) {
+        const hipError_t q = hipEventQuery(ev_status);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) return fail(LINAC_ERR_DEVICE, "hipEventQuery: %s", hipGetErrorString(q));
+        __builtin_ia32_pause();
+    }
+    int nfast = 0;
+    for (unsigned m = 0; m < B; ++m) {
+        const BM& d = act[m]->d;
+        fast[m] = sel[m] && cs.status_h[m].q0 == d.bcur && cs.status_h[m].rank == d.bcur ? 1 : 0;
+        nfast += fast[m];
+    }
+    if (nfast == 0) return LINAC_OK;
+    sel = fast;
+    Staged ws;
+    LINAC_TRY(upload(act, s, 4, 5, &ws));
+    const int nl = (int)gs_grid(maxN * (BO / 4), 16384);
+    const int nb = (int)ceil_div((int64_t)BO * BO, 256);
+    k_fast_operands<LT><<<dim3(nl + nb, B), 256, 0, s>>>(ws.desc, cs.desc, nl);
+    const unsigned grid = (unsigned)(ceil_div(ws.main_tiles, NUM_XCD) * NUM_XCD);
+    if constexpr (wide) k_gemm_outer64<0><<<grid, G_T, 0, s>>>(ws.desc, ws.main);
+    else launch_outer32<0>(grid, s, ws.desc, ws.main, ws.lds_main);
+    k_fast_zero<LT><<<dim3(gs_grid(maxN * (BO / 4), 8192), B), 256, 0, s>>>(ws.desc);
+    k_fast_commit<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc, ws.desc, act[0]->d.e0);
+    k_fast_finish<LT><<<B, 1024, 0, s>>>(in.desc, cs.desc, ws.desc);
+    LINAC_HIP(hipGetLastError());
     return LINAC_OK;
 }
 
@@ -1929,10 +2576,19 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         LINAC_HIP(hipGetLastError());
         return LINAC_OK;
     };
-    for (int64_t i = 0; i < ninner; ++i) LINAC_TRY(group(i));
-    // one synchronisation per step; extra groups only for rank-deficient panels
+    // full-rank panels through their candidate blocks; the regular inner steps
+    // over every row for the rest
+    std::vector<char> fast(B, 0);
+    int nfast = 0;
+    if (tuning().fast_panel != 0 && !slicing) {
+        LINAC_TRY(fast_panel(act, s, in, ninner, maxN, fast));
+        for (char f : fast) nfast += f;
+    }
     std::vector<ItemStatus> st(B);
-    for (int64_t guard = 0;; ++guard) {
+    if (nfast < (int)B)
+        for (int64_t i = 0; i < ninner; ++i) LINAC_TRY(group(i));
+    // one synchronisation per step; extra groups only for rank-deficient panels
+    for (int64_t guard = 0; nfast < (int)B; ++guard) {
         // into pinned memory, then a spin on the event: the next launches are
         // on the critical path whenever the inner phase outlasts the trailing
         // update, and a blocking synchronisation wakes up late
@@ -1954,6 +2610,17 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         if (guard > (int64_t)BO * (BI + 1)) return fail(LINAC_ERR_DEVICE, "blocked elimination did not converge");
         for (int64_t i = 0; i < behind; ++i) LINAC_TRY(group(NEV));
     }
+    if (nfast == (int)B) {
+        // every panel was full rank: the status is known without reading it,
+        // and the trailing update waits for the inner phase by an event
+        for (unsigned m = 0; m < B; ++m) {
+            const BM& d = act[m]->d;
+            st[m] = ItemStatus{d.bcur, d.bcur, d.bcur, 0, act[m]->rank + d.bcur};
+        }
+        LINAC_HIP(hipEventRecord(ev_inner, s));
+        LINAC_HIP(hipStreamWaitEvent(so, ev_inner, 0));
+    }
+    for (unsigned m = 0; m < B; ++m) act[m]->rank = st[m].rank;
     // whatever of the parts due by now is left (rank-deficient panels add
     // groups; slices round), before this step's own trailing update
     LINAC_TRY(flush_due(step_no, prof, tag));
@@ -2100,6 +2767,7 @@ static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, cons
     Item<LT> it;
     describe(it.d, A, ld, N, M, f);
     carve(it.d, (char*)p, w, N, M, false);
+    carve_compact(it.dc, (char*)p, w);
     it.d.st = ws.st;
     it.d.piv_rows = ws.piv_rows;
     it.d.piv_cols = ws.piv_cols;
@@ -2115,7 +2783,19 @@ static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, cons
     LINAC_TRY(rc);
     LINAC_TRY(rd);
     LINAC_HIP(hipStreamSynchronize(s));
-    if (getenv("LINAC_DEBUG_TIMING")) print_timers(it.d, ctx);
+    if (getenv("LINAC_DEBUG_TIMING")) {
+        print_timers(it.d, ctx);
+        unsigned long long tp[8];
+        int rate_khz = 0;
+        if (hipMemcpy(tp, it.dc.bs->tphase, sizeof(tp), hipMemcpyDeviceToHost) == hipSuccess && tp[6] &&
+            hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess) {
+            const double us = 1e3 / (double)rate_khz;
+            fprintf(stderr, "k_block_factor phases over %llu blocks (us/block): gj %.1f norm %.1f xr %.1f u %.1f uw %.1f upd %.1f\n",
+                    tp[6], tp[0] * us / tp[6], tp[1] * us / tp[6], tp[2] * us / tp[6], tp[3] * us / tp[6],
+                    tp[4] * us / tp[6], tp[5] * us / tp[6]);
+        }
+        (void)hipMemset(it.dc.bs->tphase, 0, sizeof(tp));
+    }
     return LINAC_OK;
 }
 
@@ -2450,6 +3130,7 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
     if (in_place) {
         describe(it->d, const_cast<uint32_t*>(a), ld, rows, cols, make_field_u32(p));
         carve(it->d, base, workspace_layout<LimbsU32>(rows, cols, true), rows, cols, true);
+        carve_compact(it->dc, base, workspace_layout<LimbsU32>(rows, cols, true));
     } else {
         const int64_t ldp = round_up(cols, 64);
         uint32_t* A = (uint32_t*)base;
@@ -2462,6 +3143,7 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
         }
         describe(it->d, A, ldp, rows, cols, make_field_u32(p));
         carve(it->d, base + matrix_bytes(rows, cols), workspace_layout<LimbsU32>(rows, cols, true), rows, cols, true);
+        carve_compact(it->dc, base + matrix_bytes(rows, cols), workspace_layout<LimbsU32>(rows, cols, true));
     }
     it->id = b->next_id++;
     b->in_use += need;

@@ -1,10 +1,11 @@
 // Residual-based refinement of the float RREF's solution columns (see
 // float_refine.hpp for the method).  Kernels:
+//   k_gather_x         Xc = X (the solution in the pivot rows), compact;
 //   k_refine_residual  E = (Aorig[:, NP] - Aorig[:, P] X) / scale, summed in
-//                      double-double (Dekker's exact product by FMA, Knuth's
-//                      two-sum), rounded to float64 once per entry;
-//   k_replay_y         Y = ZQ_i E[Q_i, :]          (one panel's pivot rows)
-//   k_replay_rows      E[r] += ZL_i[r] Y, E[Q_i[t]] = Y[t]
+//   (_narrow for few   double-double (Dekker's exact product by FMA, Knuth's
+//   solution columns)  two-sum), rounded to float64 once per entry;
+//   k_replay_panel     one panel: Y = ZQ_i E[Q_i, :], E[Q_i[t]] = Y[t],
+//                      E[r] += ZL_i[r] Y (ping-pong buffers);
 //   k_refine_apply     X += E[Q, :], and max |E[Q, :]|, max |X| for the stats.
 #include <algorithm>
 #include <vector>
@@ -54,14 +55,16 @@ template <class T>
 struct DDAcc;
 template <>
 struct DDAcc<double> {
-    DD v{0.0, 0.0};
+    DD v;  // no initialiser: the type also lives in LDS (init() sets it)
     __device__ void init(double c) { v = DD{c, 0.0}; }
     __device__ void sub_prod(double a, double x) { v = dd_add_prod(v, -a, x); }
+    __device__ void merge(const DDAcc& o) { v = dd_add(v, o.v); }
+    __device__ void add_value(double c) { v = dd_add(v, DD{c, 0.0}); }
     __device__ double result(double scale) const { return (v.hi + v.lo) / scale; }
 };
 template <>
 struct DDAcc<CplxF> {
-    DD re{0.0, 0.0}, im{0.0, 0.0};
+    DD re, im;
     __device__ void init(CplxF c) {
         re = DD{c.re, 0.0};
         im = DD{c.im, 0.0};
@@ -71,6 +74,14 @@ struct DDAcc<CplxF> {
         re = dd_add_prod(re, a.im, x.im);
         im = dd_add_prod(im, -a.re, x.im);
         im = dd_add_prod(im, -a.im, x.re);
+    }
+    __device__ void merge(const DDAcc& o) {
+        re = dd_add(re, o.re);
+        im = dd_add(im, o.im);
+    }
+    __device__ void add_value(CplxF c) {
+        re = dd_add(re, DD{c.re, 0.0});
+        im = dd_add(im, DD{c.im, 0.0});
     }
     __device__ CplxF result(double scale) const { return CplxF{(re.hi + re.lo) / scale, (im.hi + im.lo) / scale}; }
 };
@@ -96,17 +107,27 @@ __global__ __launch_bounds__(256) void k_refine_scales(const T* __restrict__ Ao,
     if (threadIdx.x == 0) scale[r] = s_max[0] == 0.0 ? 1.0 : s_max[0];
 }
 
+// Xc[t, j] = A[PR[t], NP[j]]: the current solution, compact (rank x nr).
+template <class T>
+__global__ __launch_bounds__(256) void k_gather_x(const T* __restrict__ A, int64_t ld, const int64_t* __restrict__ PR,
+                                                  int64_t rank, const int64_t* __restrict__ NP, int64_t nr,
+                                                  T* __restrict__ Xc) {
+    const int64_t total = rank * nr;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = x / nr, j = x - t * nr;
+        Xc[x] = A[PR[t] * ld + NP[j]];
+    }
+}
+
 // 64 rows x 64 solution columns per workgroup; pivots in chunks of 16 through
-// LDS; each thread owns 4 x 4 outputs.
+// LDS; each thread owns 4 x 4 outputs.  For many solution columns.
 constexpr int RR_M = 64, RR_N = 64, RR_K = 16;
 
 template <class T>
-__global__ __launch_bounds__(256) void k_refine_residual(const T* __restrict__ Ao, int64_t ldo, const T* __restrict__ A,
-                                                         int64_t ld, const int64_t* __restrict__ PR,
+__global__ __launch_bounds__(256) void k_refine_residual(const T* __restrict__ Ao, int64_t ldo, const T* __restrict__ Xc,
                                                          const int64_t* __restrict__ PC, int64_t rank,
                                                          const int64_t* __restrict__ NP, int64_t nr, int64_t N,
-                                                         const double* __restrict__ scale, T* __restrict__ E,
-                                                         int64_t lde) {
+                                                         const double* __restrict__ scale, T* __restrict__ E) {
     __shared__ T sA[RR_M][RR_K + 1];
     __shared__ T sX[RR_K][RR_N + 1];
     const int64_t r0 = (int64_t)blockIdx.y * RR_M, j0 = (int64_t)blockIdx.x * RR_N;
@@ -128,7 +149,7 @@ __global__ __launch_bounds__(256) void k_refine_residual(const T* __restrict__ A
         for (int x = threadIdx.x; x < RR_K * RR_N; x += 256) {
             const int k = x / RR_N, jj = x % RR_N;
             const int64_t t = t0 + k, j = j0 + jj;
-            sX[k][jj] = (t < rank && j < nr) ? A[PR[t] * ld + NP[j]] : FloatOps<T>::zero();
+            sX[k][jj] = (t < rank && j < nr) ? Xc[t * nr + j] : FloatOps<T>::zero();
         }
         __syncthreads();
 #pragma unroll 4
@@ -144,60 +165,101 @@ __global__ __launch_bounds__(256) void k_refine_residual(const T* __restrict__ A
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int64_t r = r0 + ty + 16 * a, j = j0 + tx + 16 * b;
-            if (r < N && j < nr) E[r * lde + j] = acc[a][b].result(scale[r]);
+            if (r < N && j < nr) E[r * nr + j] = acc[a][b].result(scale[r]);
         }
 }
 
-// Y[t, j] = sum_{s<k} ZQ[t][s] E[Q[s], j] for t < k; 64 columns per workgroup.
+// The same for at most RN_MAX solution columns: one workgroup per row, the
+// pivots split over its threads, partial double-double sums combined in LDS.
+constexpr int RN_MAX = 4;
+
 template <class T>
-__global__ __launch_bounds__(256) void k_replay_y(const T* __restrict__ ZQ, const int64_t* __restrict__ Q, int k,
-                                                  const T* __restrict__ E, int64_t lde, int64_t nr, T* __restrict__ Y) {
-    __shared__ T sE[RF_FB][64 + 1];
-    const int64_t j0 = (int64_t)blockIdx.x * 64;
-    for (int x = threadIdx.x; x < k * 64; x += 256) {
-        const int sidx = x / 64, jj = x % 64;
-        sE[sidx][jj] = j0 + jj < nr ? E[Q[sidx] * lde + j0 + jj] : FloatOps<T>::zero();
-    }
-    __syncthreads();
-    for (int x = threadIdx.x; x < k * 64; x += 256) {
-        const int t = x / 64, jj = x % 64;
-        if (j0 + jj >= nr) continue;
-        T y = FloatOps<T>::zero();
-        for (int sidx = 0; sidx < k; ++sidx) y = FloatOps<T>::fms(y, FloatOps<T>::neg(ZQ[t * RF_FB + sidx]), sE[sidx][jj]);
-        Y[(int64_t)t * nr + j0 + jj] = y;
+__global__ __launch_bounds__(256) void k_refine_residual_narrow(const T* __restrict__ Ao, int64_t ldo,
+                                                                const T* __restrict__ Xc,
+                                                                const int64_t* __restrict__ PC, int64_t rank,
+                                                                const int64_t* __restrict__ NP, int nr,
+                                                                const double* __restrict__ scale, T* __restrict__ E) {
+    __shared__ DDAcc<T> part[256];
+    const int64_t r = blockIdx.x;
+    const T* row = Ao + r * ldo;
+    for (int j0 = 0; j0 < nr; j0 += RN_MAX) {
+        DDAcc<T> acc[RN_MAX];
+#pragma unroll
+        for (int q = 0; q < RN_MAX; ++q) acc[q].init(FloatOps<T>::zero());
+        for (int64_t t = threadIdx.x; t < rank; t += 256) {
+            const T a = row[PC[t]];
+#pragma unroll
+            for (int q = 0; q < RN_MAX; ++q)
+                if (j0 + q < nr) acc[q].sub_prod(a, Xc[t * nr + j0 + q]);
+        }
+#pragma unroll
+        for (int q = 0; q < RN_MAX; ++q) {
+            if (j0 + q >= nr) break;
+            __syncthreads();
+            part[threadIdx.x] = acc[q];
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if (threadIdx.x < w) part[threadIdx.x].merge(part[threadIdx.x + w]);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                DDAcc<T> tot = part[0];
+                tot.add_value(row[NP[j0 + q]]);
+                E[r * nr + j0 + q] = tot.result(scale[r]);
+            }
+        }
     }
 }
 
-// E[r, j] += sum_{t<k} ZL[r][t] Y[t, j] for rows that are not pivots of this
-// panel (their ZL rows are zero), E[Q[t], j] = Y[t, j] for those that are.
-// Four rows x 64 columns per workgroup.
+// One panel of the replay, Ein -> Eout (so that every workgroup reads the
+// panel's pivot rows before any is overwritten):
+//   Y = ZQ Ein[Q, :],  Eout[Q[t]] = Y[t],  Eout[r] = Ein[r] + ZL[r] Y  (other rows).
+// A workgroup takes RP_ROWS rows and JW columns (JW = nr rounded up to a
+// power of two, at most 64) and forms its columns of Y itself.
+constexpr int RP_ROWS = 64;
+
 template <class T>
-__global__ __launch_bounds__(256) void k_replay_rows(const T* __restrict__ ZL, const int64_t* __restrict__ Q, int k,
-                                                     const int32_t* __restrict__ pepoch, int panel, const T* __restrict__ Y,
-                                                     T* __restrict__ E, int64_t lde, int64_t nr, int64_t N) {
+__global__ __launch_bounds__(256) void k_replay_panel(const T* __restrict__ ZL, const T* __restrict__ ZQ,
+                                                      const int64_t* __restrict__ Q, int k,
+                                                      const int32_t* __restrict__ pepoch, int panel,
+                                                      const T* __restrict__ Ein, T* __restrict__ Eout, int64_t nr,
+                                                      int64_t N, int jw) {
+    __shared__ T sE[RF_FB][64 + 1];
     __shared__ T sY[RF_FB][64 + 1];
     __shared__ int64_t sQ[RF_FB];
-    const int64_t j0 = (int64_t)blockIdx.y * 64;
-    for (int x = threadIdx.x; x < k * 64; x += 256) {
-        const int t = x / 64, jj = x % 64;
-        sY[t][jj] = j0 + jj < nr ? Y[(int64_t)t * nr + j0 + jj] : FloatOps<T>::zero();
-    }
+    const int64_t j0 = (int64_t)blockIdx.y * jw;
     if (threadIdx.x < k) sQ[threadIdx.x] = Q[threadIdx.x];
     __syncthreads();
-    const int jj = threadIdx.x & 63;
-    const int64_t j = j0 + jj;
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= N || j >= nr) return;
-    if (pepoch[r] == panel) {
-        int t = 0;
-        while (t < k && sQ[t] != r) ++t;
-        if (t < k) E[r * lde + j] = sY[t][jj];
-        return;
+    for (int x = threadIdx.x; x < k * jw; x += 256) {
+        const int sidx = x / jw, jj = x % jw;
+        sE[sidx][jj] = j0 + jj < nr ? Ein[sQ[sidx] * nr + j0 + jj] : FloatOps<T>::zero();
     }
-    T e = E[r * lde + j];
-    const T* zl = ZL + r * RF_FB;
-    for (int t = 0; t < k; ++t) e = FloatOps<T>::fms(e, FloatOps<T>::neg(zl[t]), sY[t][jj]);
-    E[r * lde + j] = e;
+    __syncthreads();
+    for (int x = threadIdx.x; x < k * jw; x += 256) {
+        const int t = x / jw, jj = x % jw;
+        T y = FloatOps<T>::zero();
+        const T* zq = ZQ + t * RF_FB;
+        for (int sidx = 0; sidx < k; ++sidx) y = FloatOps<T>::fms(y, FloatOps<T>::neg(zq[sidx]), sE[sidx][jj]);
+        sY[t][jj] = y;
+    }
+    __syncthreads();
+    const int rows_per_pass = 256 / jw;
+    const int jj = threadIdx.x % jw;
+    const int64_t j = j0 + jj;
+    for (int64_t r = (int64_t)blockIdx.x * RP_ROWS + threadIdx.x / jw; r < min(N, (int64_t)(blockIdx.x + 1) * RP_ROWS);
+         r += rows_per_pass) {
+        if (j >= nr) continue;
+        if (pepoch[r] == panel) {
+            int t = 0;
+            while (t < k && sQ[t] != r) ++t;
+            Eout[r * nr + j] = t < k ? sY[t][jj] : Ein[r * nr + j];
+            continue;
+        }
+        T e = Ein[r * nr + j];
+        const T* zl = ZL + r * RF_FB;
+        for (int t = 0; t < k; ++t) e = FloatOps<T>::fms(e, FloatOps<T>::neg(zl[t]), sY[t][jj]);
+        Eout[r * nr + j] = e;
+    }
 }
 
 // X[t, j] += E[Q[t], j] (X in place in A's pivot rows); stats[0] = max |E[Q]|,
@@ -244,37 +306,45 @@ int float_refine(T* A, int64_t ld, const T* Aorig, int64_t ldo, int64_t N, int64
     const int64_t nr = (int64_t)np.size();
     rs.solution_columns = (double)nr;
     if (nr == 0 || rank == 0 || passes <= 0) return LINAC_OK;
-    // scratch: NP list, row scales, stats, E (N x nr), Y (FB x nr)
+    // scratch: NP list, row scales, stats, Xc (rank x nr), E twice (N x nr)
     auto al = [](size_t x) { return (size_t)round_up((int64_t)x, 256); };
-    const size_t o_np = 0, o_sc = o_np + al(nr * 8), o_st = o_sc + al(N * 8), o_e = o_st + 256,
-                 o_y = o_e + al((size_t)N * nr * sizeof(T)), total = o_y + al((size_t)RF_FB * nr * sizeof(T));
+    const size_t o_np = 0, o_sc = o_np + al(nr * 8), o_st = o_sc + al(N * 8), o_x = o_st + 256,
+                 o_e0 = o_x + al((size_t)rank * nr * sizeof(T)), o_e1 = o_e0 + al((size_t)N * nr * sizeof(T)),
+                 total = o_e1 + al((size_t)N * nr * sizeof(T));
     void* base = nullptr;
     LINAC_TRY(ctx->get(SLOT_OUT, total, &base));
     char* b = (char*)base;
     int64_t* d_np = (int64_t*)(b + o_np);
     double* d_scale = (double*)(b + o_sc);
     unsigned long long* d_stats = (unsigned long long*)(b + o_st);
-    T* E = (T*)(b + o_e);
-    T* Y = (T*)(b + o_y);
+    T* Xc = (T*)(b + o_x);
+    T* Ebuf[2] = {(T*)(b + o_e0), (T*)(b + o_e1)};
     LINAC_HIP(hipMemcpyAsync(d_np, np.data(), nr * 8, hipMemcpyHostToDevice, s));
     k_refine_scales<T><<<(unsigned)N, 256, 0, s>>>(Aorig, ldo, M, d_scale);
-    const dim3 rgrid((unsigned)ceil_div(nr, RR_N), (unsigned)ceil_div(N, RR_M));
+    int jw = 1;  // columns per replay workgroup: nr rounded up to a power of two, at most 64
+    while (jw < nr && jw < 64) jw *= 2;
+    const unsigned gx = (unsigned)std::min<int64_t>(4096, ceil_div(rank * nr, 256));
     for (int pass = 0; pass < passes; ++pass) {
-        k_refine_residual<T><<<rgrid, 256, 0, s>>>(Aorig, ldo, A, ld, piv_rows, piv_cols, rank, d_np, nr, N, d_scale, E,
-                                                    nr);
+        k_gather_x<T><<<gx, 256, 0, s>>>(A, ld, piv_rows, rank, d_np, nr, Xc);
+        if (nr <= 4 * RN_MAX)
+            k_refine_residual_narrow<T><<<(unsigned)N, 256, 0, s>>>(Aorig, ldo, Xc, piv_cols, rank, d_np, (int)nr,
+                                                                     d_scale, Ebuf[0]);
+        else
+            k_refine_residual<T><<<dim3((unsigned)ceil_div(nr, RR_N), (unsigned)ceil_div(N, RR_M)), 256, 0, s>>>(
+                Aorig, ldo, Xc, piv_cols, rank, d_np, nr, N, d_scale, Ebuf[0]);
+        int cur = 0;
         for (int64_t i = 0; i < fac.npanels; ++i) {
             const int k = fac.kp[(size_t)i];
             if (k == 0) continue;
             const T* ZQ = (const T*)fac.ZQ + i * RF_FB * RF_FB;
             const T* ZL = (const T*)fac.ZL + i * N * RF_FB;
             const int64_t* Q = fac.qrows + i * RF_FB;
-            k_replay_y<T><<<(unsigned)ceil_div(nr, 64), 256, 0, s>>>(ZQ, Q, k, E, nr, nr, Y);
-            k_replay_rows<T><<<dim3((unsigned)ceil_div(N, 4), (unsigned)ceil_div(nr, 64)), 256, 0, s>>>(
-                ZL, Q, k, fac.pepoch, (int)i, Y, E, nr, nr, N);
+            k_replay_panel<T><<<dim3((unsigned)ceil_div(N, RP_ROWS), (unsigned)ceil_div(nr, jw)), 256, 0, s>>>(
+                ZL, ZQ, Q, k, fac.pepoch, (int)i, Ebuf[cur], Ebuf[cur ^ 1], nr, N, jw);
+            cur ^= 1;
         }
         LINAC_HIP(hipMemsetAsync(d_stats, 0, 16, s));
-        k_refine_apply<T><<<(unsigned)std::min<int64_t>(1024, ceil_div(rank * nr, 256)), 256, 0, s>>>(
-            A, ld, piv_rows, rank, d_np, nr, E, nr, d_stats);
+        k_refine_apply<T><<<gx, 256, 0, s>>>(A, ld, piv_rows, rank, d_np, nr, Ebuf[cur], nr, d_stats);
         LINAC_HIP(hipGetLastError());
         unsigned long long st[2] = {0, 0};
         LINAC_HIP(hipMemcpyAsync(st, d_stats, 16, hipMemcpyDeviceToHost, s));

@@ -386,7 +386,8 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"gemm_group", &t.gemm_group, 1, 64},
                  {"lookahead", &t.lookahead, 0, 1},
                  {"timeslice", &t.timeslice, 0, 1},
-                 {"gemm_lds", &t.gemm_lds, 0, 1}};
+                 {"gemm_lds", &t.gemm_lds, 0, 1},
+                 {"fast_panel", &t.fast_panel, 0, 2}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             if (value < k.lo || value > k.hi)
@@ -407,7 +408,8 @@ int linac_tune_get(const char* key, int64_t* value) {
     } knobs[] = {{"gf_blocked_min_rows", t.gf_blocked_min_rows}, {"gf_blocked_min_cols", t.gf_blocked_min_cols},
                  {"float_blocked_min_rows", t.float_blocked_min_rows},
                  {"float_blocked_min_cols", t.float_blocked_min_cols}, {"gemm_group", t.gemm_group},
-                 {"lookahead", t.lookahead}, {"timeslice", t.timeslice}, {"gemm_lds", t.gemm_lds}};
+                 {"lookahead", t.lookahead}, {"timeslice", t.timeslice}, {"gemm_lds", t.gemm_lds},
+                 {"fast_panel", t.fast_panel}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             *value = k.v;

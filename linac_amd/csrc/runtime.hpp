@@ -129,6 +129,7 @@ struct Tuning {
     int64_t lookahead = 1;
     int64_t timeslice = 0;
     int64_t gemm_lds = 1;
+    int64_t fast_panel = 2;
 };
 Tuning& tuning();
 
