@@ -187,10 +187,10 @@ def serve_bench(n, rate):
     serving.serve(serving.make_requests(8, 0, seed=99, n_min=200, n_max=2000), seed=99)  # warm-up
     reqs = serving.make_requests(n, rate, seed=0)
     res, wall, st = serving.serve(reqs, seed=0)
-    timed = serving.latency_summary(res, wall)
+    timed = serving.latency_summary(res, wall, reqs)
     sat_reqs = serving.make_requests(n, 0, seed=0)
     res_s, wall_s, _ = serving.serve(sat_reqs, seed=0)
-    sat = serving.latency_summary(res_s, wall_s)
+    sat = serving.latency_summary(res_s, wall_s, sat_reqs)
     step = max(1, n // 50)
     sample = [serving.Request(r.rid, r.rows, r.cols, r.p) for r in sat_reqs[::step]]
     t_plain = serving.sequential_plain(sample, seed=0, method="plain")

@@ -207,6 +207,28 @@ int linac_rref_c128(double* a, int64_t rows, int64_t cols, int64_t ld, double to
  * call without LINAC_REFINE). */
 int linac_refine_last(double* out, int n);
 
+/* ---- resumable float RREF (blocked method, device matrix) ----
+ * The same elimination as linac_rref_f64 / linac_rref_c128 with
+ * LINAC_DEVICE_PTR and LINAC_METHOD_BLOCKED, split into calls that each run
+ * at most max_panels of the 64-column panels (and wait for them), so that a
+ * long solve can be interleaved with other work -- the serving loop runs
+ * float requests a few panels at a time between batch steps.  finish runs
+ * any remaining panels, writes the RREF in place, the pivot columns and rank,
+ * and releases the job; destroy abandons it.  Workspace: a device buffer of
+ * linac_float_job_workspace bytes (256-byte aligned) that the caller keeps
+ * until finish / destroy; with workspace NULL the device context's is used
+ * and only one such job may be open per device.  flags must hold
+ * LINAC_DEVICE_PTR.  linac_shutdown releases open jobs (destroy on such a
+ * handle is a no-op).  No counterpart in the reference, whose calls solve one
+ * matrix at once (linac/pycuda_row_reduce.py:94-131). */
+typedef struct linac_float_job linac_float_job;
+int linac_float_job_workspace(int64_t rows, int64_t cols, int is_complex, int64_t* bytes_out);
+int linac_float_job_create(linac_float_job** out, double* a, int64_t rows, int64_t cols, int64_t ld, int is_complex,
+                           double tol, uint32_t flags, void* workspace, int64_t workspace_bytes);
+int linac_float_job_step(linac_float_job* job, int64_t max_panels, int64_t* remaining_out);
+int linac_float_job_finish(linac_float_job* job, int64_t* pivots_out, int64_t* rank_out);
+int linac_float_job_destroy(linac_float_job* job);
+
 /* ---- monomial-evaluation matrix loader ----
  * Replaces cuda_load_matrix(bases, lindices, shape, field_characteristic)
  * (linac/pycuda_matrix_loader.py:28-124; kernel linac/matrix_loader.cu:88-112):

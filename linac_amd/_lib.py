@@ -74,6 +74,12 @@ SIGNATURES = {
     "linac_batch_take_many_u32": (ctypes.c_int, [_vp, ctypes.POINTER(_i64), _i64, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
                                                   _u32, ctypes.POINTER(_vp), ctypes.POINTER(_i64)]),
     "linac_rref_batch_u32": (ctypes.c_int, [_vp, _i64p, _i64p, _i64p, ctypes.POINTER(_u32), _i64, _u32, _vp, _i64p]),
+    "linac_float_job_workspace": (ctypes.c_int, [_i64, _i64, ctypes.c_int, _i64p]),
+    "linac_float_job_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _i64, _i64, _i64, ctypes.c_int, ctypes.c_double,
+                                              _u32, _vp, _i64]),
+    "linac_float_job_step": (ctypes.c_int, [_vp, _i64, _i64p]),
+    "linac_float_job_finish": (ctypes.c_int, [_vp, _i64p, _i64p]),
+    "linac_float_job_destroy": (ctypes.c_int, [_vp]),
     "linac_load_matrix_u32": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _u32, _u32, _vp]),
     "linac_load_matrix_u64": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _u64, _u32, _vp]),
     "linac_load_matrix_f64": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _u32, _vp]),

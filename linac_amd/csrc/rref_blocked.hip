@@ -44,6 +44,9 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <deque>
+#include <iterator>
+#include <map>
 #include <type_traits>
 #include <vector>
 
@@ -2988,9 +2991,93 @@ static int gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda
 
 }  // namespace linac
 
+namespace linac {
+
+// Device memory of a batch's items: chunks from hipMalloc, sub-allocated
+// best-fit with coalescing free lists, returned to the runtime only when the
+// batch is destroyed.  A finished item's memory is reused without hipFree,
+// which would synchronise the whole device (and stall the trailing updates
+// still running for other items).
+class Arena {
+  public:
+    ~Arena() { release_all(); }
+    int alloc(size_t n, void** out) {
+        n = (size_t)round_up((int64_t)std::max<size_t>(n, 1), 256);
+        Chunk* best = nullptr;
+        std::map<size_t, size_t>::iterator bit;
+        for (auto& c : chunks)
+            for (auto it = c.free.begin(); it != c.free.end(); ++it)
+                if (it->second >= n && (!best || it->second < bit->second)) {
+                    best = &c;
+                    bit = it;
+                }
+        if (!best) {
+            const size_t sz = (size_t)round_up((int64_t)std::max(n, CHUNK), 2 << 20);
+            void* p = nullptr;
+            const hipError_t e = hipMalloc(&p, sz);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(LINAC_ERR_NOMEM, "device allocation of %zu bytes failed: %s", sz, hipGetErrorString(e));
+            }
+            chunks.push_back(Chunk{(char*)p, sz, {}});
+            chunks.back().free[0] = sz;
+            best = &chunks.back();
+            bit = best->free.begin();
+        }
+        const size_t off = bit->first, len = bit->second;
+        best->free.erase(bit);
+        if (len > n) best->free[off + n] = len - n;
+        *out = best->base + off;
+        return LINAC_OK;
+    }
+    void release(void* p, size_t n) {
+        n = (size_t)round_up((int64_t)std::max<size_t>(n, 1), 256);
+        for (auto& c : chunks) {
+            if ((char*)p < c.base || (char*)p >= c.base + c.size) continue;
+            size_t off = (size_t)((char*)p - c.base);
+            auto next = c.free.lower_bound(off);
+            if (next != c.free.end() && next->first == off + n) {  // merge with the following free block
+                n += next->second;
+                next = c.free.erase(next);
+            }
+            if (next != c.free.begin()) {  // and with the preceding one
+                auto prev = std::prev(next);
+                if (prev->first + prev->second == off) {
+                    off = prev->first;
+                    n += prev->second;
+                    c.free.erase(prev);
+                }
+            }
+            c.free[off] = n;
+            return;
+        }
+    }
+    void release_all() {
+        for (auto& c : chunks) (void)hipFree(c.base);
+        chunks.clear();
+    }
+    size_t reserved() const {
+        size_t t = 0;
+        for (auto& c : chunks) t += c.size;
+        return t;
+    }
+
+  private:
+    static constexpr size_t CHUNK = (size_t)1 << 30;
+    struct Chunk {
+        char* base;
+        size_t size;
+        std::map<size_t, size_t> free;  // offset -> length
+    };
+    std::deque<Chunk> chunks;  // (stable addresses)
+};
+
+}  // namespace linac
+
 struct linac_batch {
     linac::DeviceContext* ctx = nullptr;
     linac::Engine<linac::LimbsU32>* eng = nullptr;
+    linac::Arena arena;                                  // the items' device memory
     std::vector<linac::Item<linac::LimbsU32>*> items;   // not yet taken
     std::vector<linac::Item<linac::LimbsU32>*> fresh;   // added since the last step (need init)
     int64_t next_id = 0;
@@ -3027,10 +3114,8 @@ static bool batch_live(linac_batch* b) {
 static void batch_free(linac_batch* b) {
     (void)b->eng->drain();  // no trailing update may still run on an item's memory
     (void)hipStreamSynchronize(b->ctx->stream);
-    for (auto* it : b->items) {
-        if (it->arena) (void)hipFree(it->arena);
-        delete it;
-    }
+    for (auto* it : b->items) delete it;
+    b->arena.release_all();
     delete b->eng;
     delete b;
 }
@@ -3119,11 +3204,10 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
     hipStream_t s = b->ctx->stream;
     auto* it = new Item32();
-    hipError_t e = hipMalloc(&it->arena, need);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
+    const int ra = b->arena.alloc(need, &it->arena);
+    if (ra != LINAC_OK) {
         delete it;
-        return fail(LINAC_ERR_NOMEM, "device allocation of %zu bytes failed", need);
+        return ra;
     }
     it->bytes = need;
     char* base = (char*)it->arena;
@@ -3137,7 +3221,7 @@ int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t
         hipError_t ce = hipMemcpy2DAsync(A, ldp * 4, a, ld * 4, cols * 4, rows,
                                          (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
         if (ce != hipSuccess) {
-            (void)hipFree(it->arena);
+            b->arena.release(it->arena, it->bytes);
             delete it;
             return fail(LINAC_ERR_DEVICE, "copy into the batch failed: %s", hipGetErrorString(ce));
         }
@@ -3205,7 +3289,7 @@ int linac_batch_take_u32(linac_batch* b, int64_t id, uint32_t* out, int64_t ld_o
         for (int64_t k = 0; k < rank; ++k) pivots_out[k] = pc[k];
     if (rank_out) *rank_out = rank;
     b->in_use -= it->bytes;
-    (void)hipFree(it->arena);
+    b->arena.release(it->arena, it->bytes);
     b->items.erase(std::find(b->items.begin(), b->items.end(), it));
     delete it;
     return LINAC_OK;
@@ -3276,7 +3360,7 @@ int linac_batch_take_many_u32(linac_batch* b, const int64_t* ids, int64_t count,
     LINAC_HIP(hipStreamSynchronize(s));
     for (auto* it : its) {
         b->in_use -= it->bytes;
-        (void)hipFree(it->arena);
+        b->arena.release(it->arena, it->bytes);
         b->items.erase(std::find(b->items.begin(), b->items.end(), it));
         delete it;
     }

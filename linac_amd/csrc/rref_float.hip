@@ -13,8 +13,11 @@
 //      column order (no row swaps during the sweep, as in the GF(p) path).
 // The arithmetic is float64 throughout (complex: pairs of float64).
 #include <algorithm>
+#include <mutex>
+#include <type_traits>
 #include <vector>
 
+#include "float_blocked.hpp"
 #include "float_ops.hpp"
 #include "float_refine.hpp"
 #include "runtime.hpp"
@@ -275,9 +278,206 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
 
 }  // namespace linac
 
+// A blocked float RREF of a device matrix that runs a few panels per call, so
+// that a caller can interleave a long solve with other work (the serving
+// loop does, between batch steps).  Its workspace is the caller's, or, when
+// the caller gives none, the context's (slots SLOT_JOB_A/B: then one such
+// job per device at a time).
+struct linac_float_job {
+    linac::DeviceContext* ctx = nullptr;
+    bool own_ws = false;  // caller's workspace
+    bool cplx = false;
+    void* a = nullptr;
+    int64_t rows = 0, cols = 0, ld = 0;
+    linac::FloatState* st = nullptr;
+    int64_t* piv_rows = nullptr;
+    int64_t* piv_cols = nullptr;
+    uint8_t* used = nullptr;
+    linac::FloatSweep<double> sd;
+    linac::FloatSweep<linac::CplxF> sc;
+};
+
+namespace linac {
+
+// Job state (rank, pivot lists, used flags) and the sweep's workspace, in
+// that order, in one caller-supplied buffer.
+static size_t job_state_bytes(int64_t rows, int64_t cols) {
+    const int64_t cap = std::max<int64_t>(1, std::min(rows, cols));
+    return 256 + 2 * round_up(cap * 8, 256) + round_up(rows, 256);
+}
+
+template <class T>
+static int float_job_setup(linac_float_job* j, double tol, hipStream_t s, void* ws) {
+    T* A = (T*)j->a;
+    k_row_scale<T><<<(unsigned)j->rows, 256, 0, s>>>(A, j->ld, j->rows, j->cols);
+    k_float_init<<<(unsigned)std::min<int64_t>(1024, ceil_div(j->rows, 256)), 256, 0, s>>>(j->st, j->used, j->rows);
+    LINAC_HIP(hipGetLastError());
+    FloatSweep<T>& sw = *reinterpret_cast<FloatSweep<T>*>(std::is_same<T, double>::value ? (void*)&j->sd : (void*)&j->sc);
+    const int rc = sw.setup(A, j->ld, j->rows, j->cols, tol, s, &j->st->rank, j->used, j->piv_rows, j->piv_cols,
+                            j->ctx, SLOT_JOB_A, nullptr, ws);
+    if (rc == 1)
+        return fail(LINAC_ERR_INVALID, "a float job of %lld rows does not fit this device's cooperative grid",
+                    (long long)j->rows);
+    return rc;
+}
+
+template <class T>
+static int float_job_finish(linac_float_job* j, hipStream_t s, int64_t* pivots_out, int64_t* rank_out) {
+    FloatSweep<T>& sw = *reinterpret_cast<FloatSweep<T>*>(std::is_same<T, double>::value ? (void*)&j->sd : (void*)&j->sc);
+    LINAC_TRY(sw.finish(s, nullptr));
+    void* outp = nullptr;
+    LINAC_TRY(j->ctx->get(SLOT_OUT, (size_t)j->rows * j->ld * sizeof(T), &outp));
+    k_float_finalize<T><<<(unsigned)j->rows, 256, 0, s>>>((const T*)j->a, j->ld, (T*)outp, j->rows, j->cols, j->st,
+                                                          j->piv_rows, j->piv_cols);
+    LINAC_HIP(hipGetLastError());
+    LINAC_HIP(hipMemcpy2DAsync(j->a, j->ld * sizeof(T), outp, j->ld * sizeof(T), j->cols * sizeof(T), j->rows,
+                               hipMemcpyDeviceToDevice, s));
+    int64_t rank = 0;
+    LINAC_HIP(hipMemcpyAsync(j->ctx->pinned, &j->st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    LINAC_HIP(hipStreamSynchronize(s));
+    rank = j->ctx->pinned[0];
+    if (pivots_out && rank) LINAC_HIP(hipMemcpy(pivots_out, j->piv_cols, rank * 8, hipMemcpyDeviceToHost));
+    if (rank_out) *rank_out = rank;
+    return LINAC_OK;
+}
+
+// Open jobs: linac_shutdown releases them with the contexts; a handle used
+// after that is recognised as released (destroy is then a no-op).
+static std::mutex g_jobs_mu;
+static std::vector<linac_float_job*> g_jobs;
+
+static bool job_open(linac_float_job* j) {
+    std::lock_guard<std::mutex> lk(g_jobs_mu);
+    return std::find(g_jobs.begin(), g_jobs.end(), j) != g_jobs.end();
+}
+
+static bool job_forget(linac_float_job* j) {
+    std::lock_guard<std::mutex> lk(g_jobs_mu);
+    auto it = std::find(g_jobs.begin(), g_jobs.end(), j);
+    if (it == g_jobs.end()) return false;
+    g_jobs.erase(it);
+    return true;
+}
+
+static void free_float_jobs(DeviceContext* ctx) {
+    std::lock_guard<std::mutex> lk(g_jobs_mu);
+    for (auto it = g_jobs.begin(); it != g_jobs.end();) {
+        if ((*it)->ctx == ctx) {
+            delete *it;
+            it = g_jobs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    ctx->float_job = nullptr;
+}
+
+}  // namespace linac
+
 using namespace linac;
 
 extern "C" {
+
+int linac_float_job_workspace(int64_t rows, int64_t cols, int is_complex, int64_t* bytes_out) {
+    if (!bytes_out) return fail(LINAC_ERR_INVALID, "bytes_out is NULL");
+    if (rows <= 0 || cols <= 0) return fail(LINAC_ERR_INVALID, "empty matrix");
+    const size_t sweep = is_complex ? FloatSweep<CplxF>::workspace_bytes(rows, cols)
+                                    : FloatSweep<double>::workspace_bytes(rows, cols);
+    *bytes_out = (int64_t)(job_state_bytes(rows, cols) + sweep);
+    return LINAC_OK;
+}
+
+int linac_float_job_create(linac_float_job** out, double* a, int64_t rows, int64_t cols, int64_t ld, int is_complex,
+                           double tol, uint32_t flags, void* workspace, int64_t workspace_bytes) {
+    if (!out) return fail(LINAC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!(flags & LINAC_DEVICE_PTR)) return fail(LINAC_ERR_INVALID, "a float job works on a device matrix (LINAC_DEVICE_PTR)");
+    if (rows <= 0 || cols <= 0 || !a) return fail(LINAC_ERR_INVALID, "empty matrix or NULL pointer");
+    if (ld < cols) return fail(LINAC_ERR_INVALID, "ld (%lld) < cols (%lld)", (long long)ld, (long long)cols);
+    if (!(tol >= 0.0)) return fail(LINAC_ERR_INVALID, "tolerance must be >= 0");
+    DeviceContext* ctx = nullptr;
+    LINAC_TRY(current_context(&ctx));
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (workspace) {
+        int64_t need = 0;
+        LINAC_TRY(linac_float_job_workspace(rows, cols, is_complex, &need));
+        if (workspace_bytes < need)
+            return fail(LINAC_ERR_INVALID, "workspace of %lld bytes, %lld needed", (long long)workspace_bytes,
+                        (long long)need);
+        if ((uintptr_t)workspace % 256)
+            return fail(LINAC_ERR_INVALID, "workspace must be 256-byte aligned");
+    } else if (ctx->float_job) {
+        return fail(LINAC_ERR_INVALID, "a float job without its own workspace is already open on this device");
+    }
+    auto* j = new linac_float_job();
+    j->ctx = ctx;
+    j->own_ws = workspace != nullptr;
+    j->cplx = is_complex != 0;
+    j->a = a;
+    j->rows = rows;
+    j->cols = cols;
+    j->ld = ld;
+    const int64_t cap = std::max<int64_t>(1, std::min(rows, cols));
+    void* p = workspace;
+    void* sweep_ws = workspace ? (char*)workspace + job_state_bytes(rows, cols) : nullptr;
+    int rc = p ? LINAC_OK : ctx->get(SLOT_JOB_B, job_state_bytes(rows, cols), &p);
+    if (rc == LINAC_OK) {
+        j->st = (FloatState*)p;
+        j->piv_rows = (int64_t*)((char*)p + 256);
+        j->piv_cols = (int64_t*)((char*)p + 256 + round_up(cap * 8, 256));
+        j->used = (uint8_t*)((char*)p + 256 + 2 * round_up(cap * 8, 256));
+        rc = j->cplx ? float_job_setup<CplxF>(j, tol, ctx->stream, sweep_ws)
+                     : float_job_setup<double>(j, tol, ctx->stream, sweep_ws);
+    }
+    if (rc != LINAC_OK) {
+        delete j;
+        return rc;
+    }
+    add_shutdown_hook(free_float_jobs);
+    if (!j->own_ws) ctx->float_job = j;
+    {
+        std::lock_guard<std::mutex> g(g_jobs_mu);
+        g_jobs.push_back(j);
+    }
+    *out = j;
+    return LINAC_OK;
+}
+
+int linac_float_job_step(linac_float_job* j, int64_t max_panels, int64_t* remaining_out) {
+    if (!j || !job_open(j)) return fail(LINAC_ERR_INVALID, "job is NULL or already released");
+    if (max_panels <= 0) return fail(LINAC_ERR_INVALID, "max_panels must be positive");
+    std::lock_guard<std::recursive_mutex> lk(j->ctx->mu);
+    hipStream_t s = j->ctx->stream;
+    if (j->cplx) LINAC_TRY(j->sc.run(s, max_panels, nullptr));
+    else LINAC_TRY(j->sd.run(s, max_panels, nullptr));
+    LINAC_HIP(hipStreamSynchronize(s));
+    if (remaining_out) *remaining_out = j->cplx ? j->sc.npanels - j->sc.next : j->sd.npanels - j->sd.next;
+    return LINAC_OK;
+}
+
+int linac_float_job_finish(linac_float_job* j, int64_t* pivots_out, int64_t* rank_out) {
+    if (!j || !job_forget(j)) return fail(LINAC_ERR_INVALID, "job is NULL or already released");
+    std::lock_guard<std::recursive_mutex> lk(j->ctx->mu);
+    hipStream_t s = j->ctx->stream;
+    if (!(j->cplx ? j->sc.done() : j->sd.done())) {
+        if (j->cplx) LINAC_TRY(j->sc.run(s, -1, nullptr));
+        else LINAC_TRY(j->sd.run(s, -1, nullptr));
+    }
+    const int rc = j->cplx ? float_job_finish<CplxF>(j, s, pivots_out, rank_out)
+                           : float_job_finish<double>(j, s, pivots_out, rank_out);
+    if (j->ctx->float_job == j) j->ctx->float_job = nullptr;
+    delete j;
+    return rc;
+}
+
+int linac_float_job_destroy(linac_float_job* j) {
+    if (!j || !job_forget(j)) return LINAC_OK;  // NULL, finished, or released by linac_shutdown
+    std::lock_guard<std::recursive_mutex> lk(j->ctx->mu);
+    (void)hipStreamSynchronize(j->ctx->stream);
+    if (j->ctx->float_job == j) j->ctx->float_job = nullptr;
+    delete j;
+    return LINAC_OK;
+}
 
 int linac_rref_f64(double* a, int64_t rows, int64_t cols, int64_t ld, double tol, uint32_t flags, void* stream,
                    int64_t* pivots_out, int64_t* rank_out) {

@@ -36,6 +36,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "float_blocked.hpp"
 #include "float_ops.hpp"
 #include "float_refine.hpp"
 #include "runtime.hpp"
@@ -45,40 +46,11 @@ namespace linac {
 constexpr int FB = 64;      // panel width: one column per lane
 constexpr int FWAVES = 4;   // waves per panel workgroup
 
-// One workgroup's candidate for a column.  Both words carry the column's
-// stamp in bit 63 (|x| >= 0 leaves the sign bit free; rows are < 2^62), so a
-// reader that sees the stamp in both words has a consistent pair without any
-// ordering between the two stores.
-struct FSlot {
-    uint64_t val;  // bits of |x|
-    uint64_t row;  // row index, or ROW_NONE
-};
+
 constexpr uint64_t STAMP = 1ull << 63;
 constexpr uint64_t ROW_NONE = (1ull << 62) - 1;
 
-template <class T>
-struct FPanelArgs {
-    T* A;
-    int64_t ld, N, M;
-    int64_t J;        // first column of the panel
-    int w;            // its width (<= FB)
-    int panel;        // panel index
-    double tol;
-    T* ZL;            // [N][FB]  -P[r, pc] (multipliers), zero rows for this panel's pivots
-    T* ZQ;            // [FB][FB] Z rows of this panel's pivots, by slot
-    int64_t* qrows;   // [FB]     pivot rows of this panel, by slot
-    int32_t* kpanel;  // pivots of this panel
-    int32_t* pepoch;  // [N] panel in which the row became a pivot (-1: none)
-    uint8_t* used;    // [N]
-    int64_t* piv_rows;
-    int64_t* piv_cols;
-    int64_t* rank;
-    FSlot* slots;     // [2][G], alternating by column
-    T* crow;          // [2][G][2 * FB] each workgroup's candidate row (panel | Z)
-    int* err;
-    int64_t spin_limit;
-    unsigned long long* tphase;  // phase timers (workgroup 0, thread 0)
-};
+
 
 __device__ __forceinline__ double bcast(double x, int src) {
     const uint64_t u = __double_as_longlong(x);
@@ -582,37 +554,44 @@ static int cooperative_fits(int64_t G, DeviceContext* ctx, bool* fits, unsigned*
     return LINAC_OK;
 }
 
-template <class T, int RW>
-static int float_blocked_sweep(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStream_t s,
-                               int64_t* rank, uint8_t* used, int64_t* piv_rows, int64_t* piv_cols,
-                               DeviceContext* ctx, Profiler* prof, unsigned lds, FloatFactors* keep) {
-    constexpr int R = FloatOps<T>::kReal;
-    const int64_t G = ceil_div(N, (int64_t)FWAVES * RW);
+// One blocked float sweep, resumable panel by panel (FloatSweep, declared in
+// float_blocked.hpp): set-up, any number of run() calls, finish().
+template <class T>
+int FloatSweep<T>::setup(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStream_t s, int64_t* rank,
+                         uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, ScratchSlot slot,
+                         FloatFactors* keep_, void* ws) {
+    bool fits = false;
+    LINAC_TRY((cooperative_fits<T, 16>(ceil_div(N, FWAVES * 16), ctx, &fits, &lds)));
+    int rw = 16;
+    if (!fits) {
+        LINAC_TRY((cooperative_fits<T, 32>(ceil_div(N, FWAVES * 32), ctx, &fits, &lds)));
+        rw = 32;
+    }
+    if (!fits) return 1;
+    kfn = rw == 16 ? (const void*)k_fpanel<T, 16> : (const void*)k_fpanel<T, 32>;
+    G = ceil_div(N, (int64_t)FWAVES * rw);
+    this->ctx = ctx;
+    keep = keep_;
     const FloatBlockedLayout l = float_layout<T>(N, M, G);
-    void* base = nullptr;
-    LINAC_TRY(ctx->get(SLOT_PANEL_A, l.total, &base));
+    void* base = ws;
+    if (!base) LINAC_TRY(ctx->get(slot, l.total, &base));
     char* b = (char*)base;
-    const int64_t npanels = ceil_div(M, FB);
+    npanels = ceil_div(M, FB);
+    next = 0;
     // with `keep`, every panel's factors (ZL, ZQ, pivot rows) get their own
     // slice, kept after the sweep for the refinement's replay
-    T* keepZL = nullptr;
-    T* keepZQ = nullptr;
-    int64_t* keepQ = nullptr;
     if (keep) {
         const size_t zl = (size_t)round_up(N * FB * (int64_t)sizeof(T), 256);
         const size_t zq = (size_t)FB * FB * sizeof(T), q = (size_t)FB * 8;
         void* kb = nullptr;
         LINAC_TRY(ctx->get(SLOT_AUX2, (size_t)npanels * (zl + zq + q), &kb));
-        keepZL = (T*)kb;
-        keepZQ = (T*)((char*)kb + npanels * zl);
-        keepQ = (int64_t*)((char*)kb + npanels * (zl + zq));
-        keep->ZL = keepZL;
-        keep->ZQ = keepZQ;
-        keep->qrows = keepQ;
+        keep->ZL = kb;
+        keep->ZQ = (char*)kb + npanels * zl;
+        keep->qrows = (int64_t*)((char*)kb + npanels * (zl + zq));
         keep->npanels = npanels;
         keep->pepoch = (const int32_t*)(b + l.pepoch);
     }
-    FPanelArgs<T> P{};
+    P = FPanelArgs<T>{};
     P.A = A;
     P.ld = ld;
     P.N = N;
@@ -621,7 +600,7 @@ static int float_blocked_sweep(T* A, int64_t ld, int64_t N, int64_t M, double to
     P.ZL = (T*)(b + l.ZL);
     P.ZQ = (T*)(b + l.ZQ);
     P.qrows = (int64_t*)(b + l.qrows);
-    int32_t* kpanel = (int32_t*)(b + l.kpanel);
+    kpanel = (int32_t*)(b + l.kpanel);
     P.pepoch = (int32_t*)(b + l.pepoch);
     P.used = used;
     P.piv_rows = piv_rows;
@@ -632,22 +611,38 @@ static int float_blocked_sweep(T* A, int64_t ld, int64_t N, int64_t M, double to
     P.err = (int*)(b + l.err);
     P.tphase = (unsigned long long*)(b + l.tphase);
     P.spin_limit = (int64_t)1 << 24;  // ~ seconds of polling: far beyond any healthy barrier
-    double* BQ = (double*)(b + l.BQ);
+    BQ = (double*)(b + l.BQ);
     k_fpanel_init<<<(unsigned)std::min<int64_t>(1024, ceil_div(std::max<int64_t>(N, 4 * G), 256)), 256, 0, s>>>(
         (uint64_t*)P.slots, 4 * G, P.err, P.pepoch, N, P.tphase, 1);
     LINAC_HIP(hipGetLastError());
-    for (int64_t pi = 0; pi < npanels; ++pi) {
+    return LINAC_OK;
+}
+
+template <class T>
+size_t FloatSweep<T>::workspace_bytes(int64_t N, int64_t M) {
+    // the grid is largest with 16 rows per wave
+    return float_layout<T>(N, M, ceil_div(N, (int64_t)FWAVES * 16)).total;
+}
+
+template <class T>
+int FloatSweep<T>::run(hipStream_t s, int64_t max_panels, Profiler* prof) {
+    constexpr int R = FloatOps<T>::kReal;
+    const int64_t N = P.N, M = P.M, ld = P.ld;
+    T* A = P.A;
+    const int64_t stop = max_panels < 0 ? npanels : std::min(npanels, next + max_panels);
+    for (; next < stop; ++next) {
+        const int64_t pi = next;
         P.J = pi * FB;
         P.w = (int)std::min<int64_t>(FB, M - P.J);
         P.panel = (int)pi;
         P.kpanel = kpanel + pi;
         if (keep) {
-            P.ZL = keepZL + pi * (round_up(N * FB * (int64_t)sizeof(T), 256) / (int64_t)sizeof(T));
-            P.ZQ = keepZQ + pi * FB * FB;
-            P.qrows = keepQ + pi * FB;
+            P.ZL = (T*)keep->ZL + pi * (round_up(N * FB * (int64_t)sizeof(T), 256) / (int64_t)sizeof(T));
+            P.ZQ = (T*)keep->ZQ + pi * FB * FB;
+            P.qrows = keep->qrows + pi * FB;
         }
         void* args[] = {&P};
-        LINAC_HIP(hipLaunchCooperativeKernel((const void*)k_fpanel<T, RW>, dim3((unsigned)G), dim3(256), args, lds, s));
+        LINAC_HIP(hipLaunchCooperativeKernel(kfn, dim3((unsigned)G), dim3(256), args, lds, s));
         const int64_t T0 = P.J + P.w, nT = M - T0;
         if (nT <= 0) continue;
         const int64_t nTr = R * nT;
@@ -683,6 +678,12 @@ static int float_blocked_sweep(T* A, int64_t ld, int64_t N, int64_t M, double to
         if (prof) LINAC_TRY(prof->sample_end(s));
         LINAC_HIP(hipGetLastError());
     }
+    return LINAC_OK;
+}
+
+template <class T>
+int FloatSweep<T>::finish(hipStream_t s, Profiler* prof) {
+    constexpr int R = FloatOps<T>::kReal;
     int err = 0;
     std::vector<int32_t> kp((size_t)npanels);
     LINAC_HIP(hipMemcpyAsync(&err, P.err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -713,17 +714,15 @@ template <class T>
 int float_blocked_run(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStream_t s, int64_t* rank,
                       uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, Profiler* prof,
                       FloatFactors* keep) {
-    bool fits = false;
-    unsigned lds = 0;
-    LINAC_TRY((cooperative_fits<T, 16>(ceil_div(N, FWAVES * 16), ctx, &fits, &lds)));
-    if (fits)
-        return float_blocked_sweep<T, 16>(A, ld, N, M, tol, s, rank, used, piv_rows, piv_cols, ctx, prof, lds, keep);
-    LINAC_TRY((cooperative_fits<T, 32>(ceil_div(N, FWAVES * 32), ctx, &fits, &lds)));
-    if (fits)
-        return float_blocked_sweep<T, 32>(A, ld, N, M, tol, s, rank, used, piv_rows, piv_cols, ctx, prof, lds, keep);
-    return 1;
+    FloatSweep<T> sw;
+    const int rc = sw.setup(A, ld, N, M, tol, s, rank, used, piv_rows, piv_cols, ctx, SLOT_PANEL_A, keep);
+    if (rc != LINAC_OK) return rc;
+    LINAC_TRY(sw.run(s, -1, prof));
+    return sw.finish(s, prof);
 }
 
+template struct FloatSweep<double>;
+template struct FloatSweep<CplxF>;
 template int float_blocked_run<double>(double*, int64_t, int64_t, int64_t, double, hipStream_t, int64_t*, uint8_t*,
                                        int64_t*, int64_t*, DeviceContext*, Profiler*, FloatFactors*);
 template int float_blocked_run<CplxF>(CplxF*, int64_t, int64_t, int64_t, double, hipStream_t, int64_t*, uint8_t*,

@@ -45,6 +45,8 @@ enum ScratchSlot {
     SLOT_PANEL_C,
     SLOT_AUX,
     SLOT_AUX2,        // float refinement: kept panel factors
+    SLOT_JOB_A,       // resumable float job: sweep workspace
+    SLOT_JOB_B,       // resumable float job: state, pivot lists, row flags
     SLOT_COUNT
 };
 
@@ -56,6 +58,7 @@ struct DeviceContext {
     int64_t* pinned = nullptr;    // small pinned host buffer for status reads
     hipEvent_t probe = nullptr;   // reusable timing-free event (rank polls of the plain sweep)
     void* engines[2] = {nullptr, nullptr};  // cached blocked-elimination engines (32- and 64-bit storage)
+    void* float_job = nullptr;              // the device's resumable float job, if one is open
     int get(ScratchSlot slot, size_t bytes, void** out);
     int release();
 };

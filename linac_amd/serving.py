@@ -10,14 +10,17 @@ matrix loader) with its field: a prime p for GF(p) or 0 for float64.
   shared launches, so requests that arrive while others are on the GPU start
   at the next step and their work overlaps on the device.  They are reduced
   in place (the request's buffer receives the RREF).
-* float64 requests run between batch steps through the blocked float path,
-  in place.
+* float64 requests run through the blocked float path as resumable jobs
+  (``FloatJob``, each with its own workspace): between batch steps, panel by
+  panel for up to a time budget per round, the job with the least remaining
+  work first, so that a long float solve holds back neither the GF(p)
+  requests nor shorter float requests.  In place.
 * Admission is by HBM footprint: a request is admitted only while the bytes
   of every admitted request (its matrix plus the solver's workspace for it)
   stay under the cap; otherwise it waits, first come first served.
 
-``Server.pump`` performs one round (admit, at most one float request, one
-batch step, collect); ``serve`` drives a list of timed requests through it
+``Server.pump`` performs one round (admit, float panels for up to
+``float_budget_s``, one batch step, collect); ``serve`` drives a list of timed requests through it
 against the wall clock and reports per-request latency.
 """
 import collections
@@ -27,6 +30,7 @@ from dataclasses import dataclass, field
 import numpy
 
 from .batch import Batch, footprint
+from .hip_float import FloatJob, float_job_workspace
 from .hip_row_reduce import rref_device
 
 # Eight 31-bit primes: numpy.random.RandomState(2024), x = randint(2**30, 2**31),
@@ -59,24 +63,34 @@ class Result:
 
 
 def float_footprint(rows, cols):
-    """Device bytes of a float64 request beyond its matrix: state, pivot
-    lists and the blocked path's panel workspace (upper bound)."""
-    return 64 * rows + 1024 * rows + 2048 * max(cols, 1) + (1 << 20)
+    """Device bytes of a float64 request beyond its matrix: its job's
+    workspace and the finishing copy (the RREF is gathered into a scratch
+    matrix before it is written back)."""
+    return float_job_workspace(rows, cols) + rows * cols * 8
 
 
 class Server:
-    """GF(p) continuous batch plus in-line float64 solves under one HBM cap."""
+    """GF(p) continuous batch plus interleaved float64 solves under one HBM
+    cap.  ``float_budget_s``: wall time per round given to the float job while
+    GF(p) requests are resident (at least one panel per round); with none
+    resident the float job runs to its end."""
 
-    def __init__(self, mem_cap_bytes=200 * 10 ** 9):
+    def __init__(self, mem_cap_bytes=200 * 10 ** 9, float_budget_s=0.002, float_ratio=None):
         self.cap = int(mem_cap_bytes)
+        self.float_budget_s = float(float_budget_s)
+        self.float_ratio = float_ratio       # float time per round at most this times the last batch step's
+        self.last_step_s = 0.0
+        self.fjobs = []                      # [request, bytes, FloatJob] of the open float solves
         self.batch = Batch(self.cap)
         self.in_use = 0
         self.waiting = collections.deque()   # arrived, not admitted (FIFO)
-        self.floats = collections.deque()    # admitted float64 requests
         self.active = {}                     # batch id -> (request, bytes)
         self.max_in_use = 0
 
     def close(self):
+        for _, _, job in self.fjobs:
+            job.close()
+        self.fjobs = []
         self.batch.close()
 
     def submit(self, req):
@@ -86,7 +100,7 @@ class Server:
         self.waiting.append(req)
 
     def idle(self):
-        return not (self.waiting or self.floats or self.active)
+        return not (self.waiting or self.active or self.fjobs)
 
     def _bytes(self, req):
         if req.p == 0:
@@ -101,7 +115,10 @@ class Server:
             if self.in_use + need > self.cap:
                 break  # wait for memory to free up (first come, first served)
             if req.p == 0:
-                self.floats.append((req, need))
+                import torch
+                ws = torch.empty(float_job_workspace(req.rows, req.cols), dtype=torch.uint8, device=req.tensor.device)
+                job = FloatJob(req.tensor.data_ptr(), req.rows, req.cols, req.cols, numpy.float64, workspace=ws)
+                self.fjobs.append([req, need, job])
             else:
                 bid = self.batch.add_device(req.tensor.data_ptr(), req.rows, req.cols, req.cols, req.p,
                                             in_place=True)
@@ -114,13 +131,25 @@ class Server:
         """One round; returns the requests finished in it."""
         done = []
         self._admit()
-        if self.floats:
-            req, need = self.floats.popleft()
-            piv, rank = rref_device(req.tensor.data_ptr(), req.rows, req.cols, req.cols, 0, numpy.float64)
-            done.append(Result(req.rid, piv, rank, req.arrival, clock(), req.tensor))
-            self.in_use -= need
+        if self.fjobs:
+            budget = self.float_budget_s
+            if self.active and self.float_ratio is not None:
+                budget = min(budget, self.float_ratio * self.last_step_s)
+            t_end = time.perf_counter() + budget
+            while self.fjobs:
+                ent = min(self.fjobs, key=_remaining_work)
+                req, need, job = ent
+                if job.step(1) == 0:
+                    piv, rank = job.finish()
+                    done.append(Result(req.rid, piv, rank, req.arrival, clock(), req.tensor))
+                    self.in_use -= need
+                    self.fjobs.remove(ent)
+                if time.perf_counter() >= t_end:
+                    break
         if self.active:
+            t = time.perf_counter()
             self.batch.step()
+            self.last_step_s = time.perf_counter() - t
             fin = [b for b in self.active if self.batch.done(b)]
             taken = self.batch.take_many(fin)
             now = clock()
@@ -129,6 +158,14 @@ class Server:
                 done.append(Result(req.rid, piv, rank, req.arrival, now, req.tensor))
                 self.in_use -= need
         return done
+
+
+def _remaining_work(ent):
+    """Work left in a float job, in units of rows x columns^2 (the trailing
+    updates of its remaining panels)."""
+    req, _, job = ent
+    panels = (req.cols + 63) // 64 if job.remaining is None else job.remaining
+    return req.rows * min(req.cols, 64 * panels) ** 2
 
 
 def make_requests(n, rate, seed=0, n_min=200, n_max=8000, float_share=0.1, primes=SERVING_PRIMES):
@@ -161,14 +198,14 @@ def materialise(req, seed=0):
     return req
 
 
-def serve(requests, mem_cap_bytes=200 * 10 ** 9, seed=0, keep=False):
+def serve(requests, mem_cap_bytes=200 * 10 ** 9, seed=0, keep=False, float_budget_s=0.002, float_ratio=None):
     """Run timed requests through a Server against the wall clock.
 
     Each request is materialised in HBM when it arrives (not before), then
     submitted.  Returns (results sorted by rid, wall seconds, server stats)."""
     import torch
     reqs = sorted(requests, key=lambda r: r.arrival)
-    srv = Server(mem_cap_bytes)
+    srv = Server(mem_cap_bytes, float_budget_s, float_ratio)
     results = []
     t0 = time.perf_counter()
 
@@ -239,8 +276,26 @@ def _rref_u32_unaligned(req, method):
     req.tensor.copy_(buf[:, :req.cols])
 
 
-def latency_summary(results, wall):
+SIZE_CLASSES = [(200, 500), (500, 1000), (1000, 2000), (2000, 4000), (4000, 8001)]
+
+
+def latency_summary(results, wall, requests=None):
+    """Throughput and p50/p99 latency; with the requests, also per size class
+    (rows) for GF(p) and for float64 requests."""
     lat = numpy.array([r.latency for r in results])
-    return {"requests": len(results), "wall_s": round(wall, 3), "solves_per_s": round(len(results) / wall, 2),
-            "p50_latency_s": round(float(numpy.percentile(lat, 50)), 4),
-            "p99_latency_s": round(float(numpy.percentile(lat, 99)), 4)}
+    out = {"requests": len(results), "wall_s": round(wall, 3), "solves_per_s": round(len(results) / wall, 2),
+           "p50_latency_s": round(float(numpy.percentile(lat, 50)), 4),
+           "p99_latency_s": round(float(numpy.percentile(lat, 99)), 4)}
+    if requests is not None:
+        by_id = {r.rid: r for r in requests}
+        classes = {}
+        for lo, hi in SIZE_CLASSES:
+            for kind in ("gf", "f64"):
+                sel = [res.latency for res in results
+                       if lo <= by_id[res.rid].rows < hi and (by_id[res.rid].p == 0) == (kind == "f64")]
+                if sel:
+                    classes[f"{kind} N in [{lo}, {hi})"] = {
+                        "n": len(sel), "p50_s": round(float(numpy.percentile(sel, 50)), 4),
+                        "p99_s": round(float(numpy.percentile(sel, 99)), 4)}
+        out["by_class"] = classes
+    return out

@@ -102,3 +102,85 @@ def test_blocked_equals_plain_on_rank_deficient_sweep(gpu):
     Rb, pb, rb = gpu.rref_with_pivots(A, 0, method="blocked")
     assert pp == pb and rp == rb == 42
     assert numpy.max(numpy.abs(Rp - Rb)) < 1e-9 * max(1.0, numpy.max(numpy.abs(Rp)))
+
+
+@pytest.mark.parametrize("shape, rank, cplx, per_step", [
+    ((700, 900), 400, False, 1), ((333, 517), 333, True, 3), ((1500, 1501), 1500, False, 0)])
+def test_float_job_in_steps_equals_single_call(gpu, shape, rank, cplx, per_step):
+    """A float job advanced a few panels per call (per_step 0: finish at once)
+    gives exactly the single-call blocked result: the same kernels on the same
+    data in the same order."""
+    import torch
+    from linac_amd.hip_float import FloatJob
+    from linac_amd.hip_row_reduce import rref_device
+    rs = numpy.random.RandomState(rank)
+    A = rs.standard_normal((shape[0], rank)) @ rs.standard_normal((rank, shape[1]))
+    if cplx:
+        A = A + 1j * (rs.standard_normal((shape[0], rank)) @ rs.standard_normal((rank, shape[1])))
+    dt = numpy.complex128 if cplx else numpy.float64
+    one = torch.from_numpy(A.astype(dt)).cuda()
+    steps = torch.from_numpy(A.astype(dt)).cuda()
+    torch.cuda.synchronize()
+    p1, r1 = rref_device(one.data_ptr(), shape[0], shape[1], shape[1], 0, dt, method="blocked")
+    job = FloatJob(steps.data_ptr(), shape[0], shape[1], shape[1], dt)
+    calls = 0
+    if per_step:
+        while job.step(per_step) > 0:
+            calls += 1
+        assert calls >= (shape[1] + 63) // 64 // per_step - 1
+    p2, r2 = job.finish()
+    torch.cuda.synchronize()
+    assert (p1, r1) == (p2, r2) and r1 == min(rank, *shape)
+    assert torch.equal(one, steps)
+
+
+def test_float_job_one_per_device(gpu):
+    """A second job on the same device is refused while the first is open;
+    after destroy (or finish) a new one opens."""
+    import torch
+    from linac_amd.hip_float import FloatJob
+    a = torch.rand((200, 201), dtype=torch.float64, device="cuda")
+    b = a.clone()
+    torch.cuda.synchronize()
+    j1 = FloatJob(a.data_ptr(), 200, 201, 201, numpy.float64)
+    with pytest.raises(ValueError, match="already open"):
+        FloatJob(b.data_ptr(), 200, 201, 201, numpy.float64)
+    j1.step(1)
+    j1.close()
+    j2 = FloatJob(b.data_ptr(), 200, 201, 201, numpy.float64)
+    piv, rank = j2.finish()
+    assert rank == 200 and piv == list(range(200))
+
+
+def test_float_jobs_with_own_workspace_interleave(gpu):
+    """Jobs with their own workspaces can be open together and advanced in any
+    interleaving; each ends with exactly its single-call blocked result.  A
+    workspace below the stated size is refused."""
+    import torch
+    from linac_amd.hip_float import FloatJob, float_job_workspace
+    from linac_amd.hip_row_reduce import rref_device
+    rs = numpy.random.RandomState(12)
+    shapes = [(500, 700, numpy.float64), (900, 400, numpy.complex128), (256, 1300, numpy.float64)]
+    mats, refs, jobs = [], [], []
+    for rows, cols, dt in shapes:
+        A = rs.standard_normal((rows, cols))
+        if dt == numpy.complex128:
+            A = A + 1j * rs.standard_normal((rows, cols))
+        one = torch.from_numpy(A.astype(dt)).cuda()
+        mats.append(torch.from_numpy(A.astype(dt)).cuda())
+        torch.cuda.synchronize()
+        refs.append((one, rref_device(one.data_ptr(), rows, cols, cols, 0, dt, method="blocked")))
+    with pytest.raises(ValueError, match="needed"):
+        small = torch.empty(float_job_workspace(500, 700) - 256, dtype=torch.uint8, device="cuda")
+        FloatJob(mats[0].data_ptr(), 500, 700, 700, numpy.float64, workspace=small)
+    for (rows, cols, dt), m in zip(shapes, mats):
+        ws = torch.empty(float_job_workspace(rows, cols, dt), dtype=torch.uint8, device="cuda")
+        jobs.append(FloatJob(m.data_ptr(), rows, cols, cols, dt, workspace=ws))
+    order = rs.randint(0, 3, size=60)
+    for k in order:
+        if jobs[k].remaining != 0:
+            jobs[k].step(int(rs.randint(1, 3)))
+    for job, m, (one, ref) in zip(jobs, mats, refs):
+        assert job.finish() == ref
+        torch.cuda.synchronize()
+        assert torch.equal(m, one)

@@ -1,6 +1,6 @@
 """Continuous-batching serving loop: every request's result is the same alone
-and in the mix (bit-identical for GF(p), float64 too: the float path runs
-each request by itself), admission holds the memory cap, and requests that
+and in the mix (bit-identical for GF(p), float64 too: a float request runs
+as a job of its own, a few panels between batch steps), admission holds the memory cap, and requests that
 arrive while others run join the batch."""
 import numpy
 import pytest
@@ -17,13 +17,10 @@ def _alone(req, seed):
     else:
         A = req.tensor.cpu().numpy().view(numpy.uint32)
     req.tensor = None
-    from linac_amd import rref_with_pivots, autotune
-    if req.p:
-        method = "blocked"  # GF(p): any method gives the unique RREF
-    else:  # float: the method the serving loop's "auto" picks for this shape
-        big = req.rows >= autotune.get("float_blocked_min_rows") and req.cols >= autotune.get("float_blocked_min_cols")
-        method = "blocked" if big else "plain"
-    return rref_with_pivots(A, req.p, method=method)
+    from linac_amd import rref_with_pivots
+    # GF(p): any method gives the unique RREF; float: the serving loop's
+    # float jobs run the blocked method
+    return rref_with_pivots(A, req.p, method="blocked")
 
 
 def test_results_same_alone_and_in_the_mix(gpu):
@@ -67,3 +64,24 @@ def test_request_larger_than_the_cap_is_rejected(gpu):
     with pytest.raises(MemoryError):
         srv.submit(req)
     srv.close()
+
+
+def test_served_results_equal_the_oracle(gpu):
+    """Requests served in the mix against the C oracle (linac's row_reduce
+    restated): GF(p) bit for bit with the same pivots, float64 within 1e-9
+    with the same pivots."""
+    from linac_amd import serving
+    from oracle import crref
+    reqs = serving.make_requests(24, rate=0, seed=11, n_min=40, n_max=300, float_share=0.25)
+    results, wall, stats = serving.serve(reqs, seed=11, keep=True)
+    assert any(r.p == 0 for r in reqs) and any(r.p for r in reqs)
+    for req, res in zip(sorted(reqs, key=lambda r: r.rid), results):
+        A = serving.materialise(serving.Request(req.rid, req.rows, req.cols, req.p), 11).tensor.cpu().numpy()
+        got = res.tensor.cpu().numpy()
+        if req.p:
+            Ro, po = crref.rref_modp(A.view(numpy.uint32), req.p)
+            assert numpy.array_equal(got.view(numpy.uint32), Ro), req
+        else:
+            Ro, po = crref.rref_f64(A)
+            assert float(numpy.max(numpy.abs(got - Ro))) <= 1e-9, req
+        assert res.pivots == po and res.rank == len(po), req
