@@ -136,10 +136,11 @@ int linac_rref_u64(uint64_t* a, int64_t rows, int64_t cols, int64_t ld, uint64_t
 int linac_rref_u32_noalloc(uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p,
                            uint32_t flags, void* stream, int64_t* pivots_out, int64_t* rank_out);
 
-/* ---- batches of GF(p) matrices (blocked method, 2 < p <= 4278124287) ----
+/* ---- batches of GF(p) matrices (blocked method, 2 < p < 2^63) ----
  * A batch advances all its unfinished matrices together, one 256-column panel
  * per step, in shared kernel launches (one launch per kernel for the whole
- * batch).  Matrices may be added between steps and taken out as soon as they
+ * batch and storage width: uint32 matrices for p <= 4278124287, uint64
+ * matrices for 2^32 <= p < 2^63, both in one batch; ids are shared).  Matrices may be added between steps and taken out as soon as they
  * are finished, so a serving loop can keep the device busy while requests
  * arrive and leave (continuous batching).  Each matrix lives in its own device
  * allocation of linac_batch_footprint_u32(rows, cols) bytes; with a non-zero
@@ -160,6 +161,11 @@ int linac_batch_memory(linac_batch* b, uint64_t* in_use_out, uint64_t* cap_out);
  * afterwards. */
 int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p,
                         uint32_t flags, int64_t* id_out);
+/* The same for 64-bit storage, 2^32 <= p < 2^63 (footprint: _u64; the matrix
+ * part is rows*round_up(cols,64)*8 rounded to 256). */
+int linac_batch_footprint_u64(int64_t rows, int64_t cols, uint64_t* bytes_out);
+int linac_batch_add_u64(linac_batch* b, const uint64_t* a, int64_t rows, int64_t cols, int64_t ld, uint64_t p,
+                        uint32_t flags, int64_t* id_out);
 /* One panel for every unfinished matrix; *unfinished_out counts those left. */
 int linac_batch_step(linac_batch* b, int64_t* unfinished_out);
 int linac_batch_run(linac_batch* b);
@@ -169,10 +175,15 @@ int linac_batch_status(linac_batch* b, int64_t id);
  * LINAC_DEVICE_PTR; may be NULL), its pivot columns and rank, and releases it. */
 int linac_batch_take_u32(linac_batch* b, int64_t id, uint32_t* out, int64_t ld_out, uint32_t flags,
                          int64_t* pivots_out, int64_t* rank_out);
+int linac_batch_take_u64(linac_batch* b, int64_t id, uint64_t* out, int64_t ld_out, uint32_t flags,
+                         int64_t* pivots_out, int64_t* rank_out);
 /* Many finished matrices at once: as linac_batch_take_u32 for each ids[i]
  * (outs[i] / pivots_out[i] may be NULL, as may the arrays themselves), with two
  * host round trips in all instead of several per matrix. */
 int linac_batch_take_many_u32(linac_batch* b, const int64_t* ids, int64_t count, uint32_t* const* outs,
+                              const int64_t* ld_outs, uint32_t flags, int64_t* const* pivots_out,
+                              int64_t* ranks_out);
+int linac_batch_take_many_u64(linac_batch* b, const int64_t* ids, int64_t count, uint64_t* const* outs,
                               const int64_t* ld_outs, uint32_t flags, int64_t* const* pivots_out,
                               int64_t* ranks_out);
 /* One-shot: RREF of `count` matrices in place (each with its own shape and

@@ -67,11 +67,16 @@ SIGNATURES = {
     "linac_batch_destroy": (ctypes.c_int, [_vp]),
     "linac_batch_memory": (ctypes.c_int, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
     "linac_batch_add_u32": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _u32, _u32, _i64p]),
+    "linac_batch_footprint_u64": (ctypes.c_int, [_i64, _i64, ctypes.POINTER(_u64)]),
+    "linac_batch_add_u64": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _u64, _u32, _i64p]),
+    "linac_batch_take_u64": (ctypes.c_int, [_vp, _i64, _vp, _i64, _u32, _i64p, _i64p]),
     "linac_batch_step": (ctypes.c_int, [_vp, _i64p]),
     "linac_batch_run": (ctypes.c_int, [_vp]),
     "linac_batch_status": (ctypes.c_int, [_vp, _i64]),
     "linac_batch_take_u32": (ctypes.c_int, [_vp, _i64, _vp, _i64, _u32, _i64p, _i64p]),
     "linac_batch_take_many_u32": (ctypes.c_int, [_vp, ctypes.POINTER(_i64), _i64, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
+                                                  _u32, ctypes.POINTER(_vp), ctypes.POINTER(_i64)]),
+    "linac_batch_take_many_u64": (ctypes.c_int, [_vp, ctypes.POINTER(_i64), _i64, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
                                                   _u32, ctypes.POINTER(_vp), ctypes.POINTER(_i64)]),
     "linac_rref_batch_u32": (ctypes.c_int, [_vp, _i64p, _i64p, _i64p, ctypes.POINTER(_u32), _i64, _u32, _vp, _i64p]),
     "linac_float_job_workspace": (ctypes.c_int, [_i64, _i64, ctypes.c_int, _i64p]),

@@ -20,34 +20,66 @@
 
 namespace linac {
 
+// High words of products, and leading zeros: the device instructions in
+// kernels, plain C++ in host code (the CPU exactness tests call the field and
+// fold routines on the host).
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+__host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+__host__ __device__ __forceinline__ int clz32(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __clz(x);
+#else
+    return __builtin_clz(x);
+#endif
+}
+__host__ __device__ __forceinline__ int clz64(uint64_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __clzll(x);
+#else
+    return __builtin_clzll(x);
+#endif
+}
+
 struct FieldU32 {
     using T = uint32_t;
     uint32_t p;
     uint32_t pneg;  // -p^{-1} mod 2^32
     uint32_t r2;    // 2^64 mod p
 
-    __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b) const {
+    __host__ __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b) const {
         const uint32_t t_lo = a * b;
-        const uint32_t t_hi = __umulhi(a, b);
+        const uint32_t t_hi = mulhi32(a, b);
         const uint32_t m = t_lo * pneg;
-        const uint64_t u = (uint64_t)t_hi + __umulhi(m, p) + (t_lo != 0u);
+        const uint64_t u = (uint64_t)t_hi + mulhi32(m, p) + (t_lo != 0u);
         return (uint32_t)(u >= p ? u - p : u);
     }
-    __device__ __forceinline__ uint32_t sub(uint32_t a, uint32_t b) const {
+    __host__ __device__ __forceinline__ uint32_t sub(uint32_t a, uint32_t b) const {
         return a >= b ? a - b : a + (p - b);
     }
     // a + b mod p without forming a + b, which wraps 32 bits for p > 2^31
-    __device__ __forceinline__ uint32_t add(uint32_t a, uint32_t b) const {
+    __host__ __device__ __forceinline__ uint32_t add(uint32_t a, uint32_t b) const {
         return a >= p - b ? a - (p - b) : a + b;
     }
-    __device__ __forceinline__ uint32_t to_mont(uint32_t a) const { return mont(a, r2); }
+    __host__ __device__ __forceinline__ uint32_t to_mont(uint32_t a) const { return mont(a, r2); }
     // h^{-1} = h^(p-2) (Fermat), square-and-multiply in the Montgomery domain:
     // a fixed, branch-uniform sequence of about 2 log2(p) products.
-    __device__ __forceinline__ uint32_t inverse(uint32_t h) const {
+    __host__ __device__ __forceinline__ uint32_t inverse(uint32_t h) const {
         const uint32_t e = p - 2u;
         const uint32_t hm = to_mont(h);
         uint32_t r = to_mont(1u);
-        for (int bit = 31 - __clz(e); bit >= 0; --bit) {
+        for (int bit = 31 - clz32(e); bit >= 0; --bit) {
             r = mont(r, r);
             if ((e >> bit) & 1u) r = mont(r, hm);
         }
@@ -55,25 +87,25 @@ struct FieldU32 {
     }
     // Normal-form multiplier X with mont(a, X) = a * h^{-1} * R (the
     // Montgomery form of the elimination factor a / h).
-    __device__ __forceinline__ uint32_t factor_of_head(uint32_t h) const {
+    __host__ __device__ __forceinline__ uint32_t factor_of_head(uint32_t h) const {
         return to_mont(to_mont(inverse(h)));
     }
     // (h x - a y) R^{-1} mod p for canonical h, x, a, y: a fraction-free row
     // update with a constant extra factor R^{-1}.  For p < 2^31 the two
     // products share one reduction: h x + (p - a) y < 2 p^2 < p 2^32.
-    __device__ __forceinline__ uint32_t ff_update(uint32_t h, uint32_t x, uint32_t a, uint32_t y) const {
+    __host__ __device__ __forceinline__ uint32_t ff_update(uint32_t h, uint32_t x, uint32_t a, uint32_t y) const {
         if (p < 0x80000000u) {
             const uint64_t t = (uint64_t)h * x + (uint64_t)(p - a) * y;
             const uint32_t t_lo = (uint32_t)t, t_hi = (uint32_t)(t >> 32);
             const uint32_t m = t_lo * pneg;
-            const uint32_t u = t_hi + __umulhi(m, p) + (t_lo != 0u);
+            const uint32_t u = t_hi + mulhi32(m, p) + (t_lo != 0u);
             return u >= p ? u - p : u;
         }
         return sub(mont(h, x), mont(a, y));
     }
     // a * b mod p for two normal-form operands.
-    __device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t b) const { return mont(to_mont(a), b); }
-    __device__ __forceinline__ uint32_t reduce(uint32_t a) const { return a >= p ? a % p : a; }
+    __host__ __device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t b) const { return mont(to_mont(a), b); }
+    __host__ __device__ __forceinline__ uint32_t reduce(uint32_t a) const { return a >= p ? a % p : a; }
 };
 
 struct FieldU64 {
@@ -82,45 +114,45 @@ struct FieldU64 {
     uint64_t pneg;  // -p^{-1} mod 2^64
     uint64_t r2;    // 2^128 mod p
 
-    __device__ __forceinline__ uint64_t mont(uint64_t a, uint64_t b) const {
+    __host__ __device__ __forceinline__ uint64_t mont(uint64_t a, uint64_t b) const {
         const uint64_t t_lo = a * b;
-        const uint64_t t_hi = __umul64hi(a, b);
+        const uint64_t t_hi = mulhi64(a, b);
         const uint64_t m = t_lo * pneg;
-        const uint64_t u = t_hi + __umul64hi(m, p) + (t_lo != 0ull);
+        const uint64_t u = t_hi + mulhi64(m, p) + (t_lo != 0ull);
         return u >= p ? u - p : u;
     }
-    __device__ __forceinline__ uint64_t sub(uint64_t a, uint64_t b) const {
+    __host__ __device__ __forceinline__ uint64_t sub(uint64_t a, uint64_t b) const {
         return a >= b ? a - b : a + (p - b);
     }
-    __device__ __forceinline__ uint64_t add(uint64_t a, uint64_t b) const {
+    __host__ __device__ __forceinline__ uint64_t add(uint64_t a, uint64_t b) const {
         return a >= p - b ? a - (p - b) : a + b;
     }
-    __device__ __forceinline__ uint64_t to_mont(uint64_t a) const { return mont(a, r2); }
+    __host__ __device__ __forceinline__ uint64_t to_mont(uint64_t a) const { return mont(a, r2); }
     // (h x - a y) R^{-1} mod p, the fraction-free row update of FieldU32 with
     // one shared reduction: t = h x + (p - a) y < 2 p^2 < p 2^64 (p < 2^63) is
     // a valid REDC input, summed in 128 bits.
-    __device__ __forceinline__ uint64_t ff_update(uint64_t h, uint64_t x, uint64_t a, uint64_t y) const {
+    __host__ __device__ __forceinline__ uint64_t ff_update(uint64_t h, uint64_t x, uint64_t a, uint64_t y) const {
         const uint64_t b = p - a;
         const uint64_t l1 = h * x, l2 = b * y;
         const uint64_t t_lo = l1 + l2;
-        const uint64_t t_hi = __umul64hi(h, x) + __umul64hi(b, y) + (t_lo < l1 ? 1ull : 0ull);
+        const uint64_t t_hi = mulhi64(h, x) + mulhi64(b, y) + (t_lo < l1 ? 1ull : 0ull);
         const uint64_t m = t_lo * pneg;
-        const uint64_t u = t_hi + __umul64hi(m, p) + (t_lo != 0ull);
+        const uint64_t u = t_hi + mulhi64(m, p) + (t_lo != 0ull);
         return u >= p ? u - p : u;
     }
-    __device__ __forceinline__ uint64_t inverse(uint64_t h) const {
+    __host__ __device__ __forceinline__ uint64_t inverse(uint64_t h) const {
         const uint64_t e = p - 2ull;
         const uint64_t hm = to_mont(h);
         uint64_t r = to_mont(1ull);
-        for (int bit = 63 - __clzll(e); bit >= 0; --bit) {
+        for (int bit = 63 - clz64(e); bit >= 0; --bit) {
             r = mont(r, r);
             if ((e >> bit) & 1ull) r = mont(r, hm);
         }
         return mont(r, 1ull);
     }
-    __device__ __forceinline__ uint64_t factor_of_head(uint64_t h) const { return to_mont(to_mont(inverse(h))); }
-    __device__ __forceinline__ uint64_t mul(uint64_t a, uint64_t b) const { return mont(to_mont(a), b); }
-    __device__ __forceinline__ uint64_t reduce(uint64_t a) const { return a >= p ? a % p : a; }
+    __host__ __device__ __forceinline__ uint64_t factor_of_head(uint64_t h) const { return to_mont(to_mont(inverse(h))); }
+    __host__ __device__ __forceinline__ uint64_t mul(uint64_t a, uint64_t b) const { return mont(to_mont(a), b); }
+    __host__ __device__ __forceinline__ uint64_t reduce(uint64_t a) const { return a >= p ? a % p : a; }
 };
 
 // GF(2): the only even prime; Montgomery needs an odd modulus.

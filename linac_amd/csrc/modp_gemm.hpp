@@ -49,12 +49,12 @@ __device__ __forceinline__ void gst(void* p, V v) {
     *(__attribute__((address_space(1))) V*)p = v;
 }
 
-__device__ __forceinline__ int32_t centred(uint32_t v, uint32_t p) {
+__host__ __device__ __forceinline__ int32_t centred(uint32_t v, uint32_t p) {
     return v > (p - 1) / 2 ? (int32_t)((int64_t)v - (int64_t)p) : (int32_t)v;
 }
 
 // Balanced base-256 digits of c (|c| <= 2139062143); d[3] is the top digit.
-__device__ __forceinline__ void balanced_digits(int32_t c, int8_t d[4]) {
+__host__ __device__ __forceinline__ void balanced_digits(int32_t c, int8_t d[4]) {
     int32_t x = c;
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
@@ -65,13 +65,13 @@ __device__ __forceinline__ void balanced_digits(int32_t c, int8_t d[4]) {
     d[3] = (int8_t)x;
 }
 
-__device__ __forceinline__ double red_rint(double x, double p, double pinv) {
+__host__ __device__ __forceinline__ double red_rint(double x, double p, double pinv) {
     const double q = rint(x * pinv);
     return fma(-q, p, x);
 }
 
 // (cval - sum_s 256^s S_s) mod p, or (- sum) mod p when cval is not used.
-__device__ __forceinline__ uint32_t fold_sub(uint32_t cval, const int32_t s[7], double p, double pinv) {
+__host__ __device__ __forceinline__ uint32_t fold_sub(uint32_t cval, const int32_t s[7], double p, double pinv) {
     const double P0 = (double)s[0] + 256.0 * (double)s[1];
     const double P1 = (double)s[2] + 256.0 * (double)s[3];
     const double P2 = (double)s[4] + 256.0 * (double)s[5];
@@ -91,7 +91,7 @@ __device__ __forceinline__ uint32_t fold_sub(uint32_t cval, const int32_t s[7], 
 // inner GEMM: k <= 32 pivots), where |S_s| <= 4 * 32 * 2^14 = 2^21, so the
 // pairs S_2i + 256 S_2i+1 are exact in int32 (|.| < 2^30) and only four values
 // go through double precision.
-__device__ __forceinline__ uint32_t fold_sub_k32(uint32_t cval, const int32_t s[7], double p, double pinv) {
+__host__ __device__ __forceinline__ uint32_t fold_sub_k32(uint32_t cval, const int32_t s[7], double p, double pinv) {
     const int32_t Q0 = s[0] + s[1] * 256, Q1 = s[2] + s[3] * 256, Q2 = s[4] + s[5] * 256;
     double r = (double)s[6];
     r = red_rint(fma(r, 65536.0, (double)Q2), p, pinv);
@@ -122,11 +122,11 @@ __device__ __forceinline__ uint32_t fold_sub_k32(uint32_t cval, const int32_t s[
 constexpr uint64_t LIMB64_P_MIN = 1ull << 32;   // smaller primes use 32-bit storage
 constexpr uint64_t LIMB64_P_MAX = 1ull << 63;   // exclusive; the field arithmetic needs p < 2^63
 
-__device__ __forceinline__ int64_t centred(uint64_t v, uint64_t p) {
+__host__ __device__ __forceinline__ int64_t centred(uint64_t v, uint64_t p) {
     return v > (p - 1) / 2 ? (int64_t)(v - p) : (int64_t)v;
 }
 
-__device__ __forceinline__ void balanced_digits(int64_t c, int8_t d[8]) {
+__host__ __device__ __forceinline__ void balanced_digits(int64_t c, int8_t d[8]) {
     int64_t x = c;
 #pragma unroll
     for (int l = 0; l < 7; ++l) {
@@ -138,7 +138,7 @@ __device__ __forceinline__ void balanced_digits(int64_t c, int8_t d[8]) {
 }
 
 // (cval - sum_{s<15} 256^s S_s) mod p for 2^32 <= p < 2^63.
-__device__ __forceinline__ uint64_t fold_sub64(uint64_t cval, const int32_t s[15], uint64_t p, double pinv) {
+__host__ __device__ __forceinline__ uint64_t fold_sub64(uint64_t cval, const int32_t s[15], uint64_t p, double pinv) {
     int64_t T[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i)

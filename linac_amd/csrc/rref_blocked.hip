@@ -155,9 +155,12 @@ constexpr int64_t STRIDE_W = (int64_t)BO * BO;
 // panel buffer management
 // ---------------------------------------------------------------------------
 
-// Panel copies in 16-byte vectors: rows of A are 16-byte aligned and J is a
-// multiple of the panel width, so a vector never straddles a row; a panel
-// narrower than a whole vector at the right edge of A is masked.
+// Panel copies in 16-byte vectors (J is a multiple of the panel width, so a
+// vector that ends within the panel stays within its row); at the right edge
+// of A the last, partial vector of a row is read element by element.  Rows
+// need not be 16-byte aligned (a batch works in place on the caller's
+// matrix, whatever its row stride): global loads accept any dword-aligned
+// address.
 template <class LT>
 __device__ static void block_state_init(const BMat<LT>& d, int reset_timers);
 
@@ -174,12 +177,15 @@ __global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__
         const int64_t r = x / VPR;
         const int j = (int)(x - r * VPR) * PER;
         Vec16<T> v = {};
-        if (j < d.bcur) {
+        if (j + PER <= d.bcur) {
             const v4i raw = gld<v4i>(d.A + r * d.ld + d.J + j);
             __builtin_memcpy(&v, &raw, 16);
+        } else if (j < d.bcur) {
+            // the vector would run past the panel's last column (at the right
+            // edge of A, past the end of a row, and of the matrix on its last row)
 #pragma unroll
             for (int e = 0; e < PER; ++e)
-                if (j + e >= d.bcur) v.v[e] = 0;
+                if (j + e < d.bcur) v.v[e] = d.A[r * d.ld + d.J + j + e];
         }
         v4i out;
         __builtin_memcpy(&out, &v, 16);
@@ -3076,26 +3082,43 @@ class Arena {
 
 struct linac_batch {
     linac::DeviceContext* ctx = nullptr;
-    linac::Engine<linac::LimbsU32>* eng = nullptr;
+    linac::Engine<linac::LimbsU32>* eng = nullptr;      // 32-bit storage (p <= 4278124287)
+    linac::Engine<linac::LimbsU64>* eng64 = nullptr;    // 64-bit storage (2^32 <= p < 2^63), made on first use
     linac::Arena arena;                                  // the items' device memory
     std::vector<linac::Item<linac::LimbsU32>*> items;   // not yet taken
     std::vector<linac::Item<linac::LimbsU32>*> fresh;   // added since the last step (need init)
-    int64_t next_id = 0;
+    std::vector<linac::Item<linac::LimbsU64>*> items64;
+    std::vector<linac::Item<linac::LimbsU64>*> fresh64;
+    int64_t next_id = 0;  // ids are shared by both widths
     uint64_t cap = 0, in_use = 0;
 };
 
 namespace linac {
 
-using Item32 = Item<LimbsU32>;
-
-static size_t matrix_bytes(int64_t rows, int64_t cols) { return (size_t)round_up(rows * round_up(cols, 64) * 4, 256); }
-
-static size_t item_bytes(int64_t rows, int64_t cols) {
-    return matrix_bytes(rows, cols) + workspace_layout<LimbsU32>(rows, cols, true).total;
+// The batch's per-width members, selected by storage type.
+static std::vector<Item<LimbsU32>*>& items_of(linac_batch* b, LimbsU32) { return b->items; }
+static std::vector<Item<LimbsU64>*>& items_of(linac_batch* b, LimbsU64) { return b->items64; }
+static std::vector<Item<LimbsU32>*>& fresh_of(linac_batch* b, LimbsU32) { return b->fresh; }
+static std::vector<Item<LimbsU64>*>& fresh_of(linac_batch* b, LimbsU64) { return b->fresh64; }
+static Engine<LimbsU32>* engine_of(linac_batch* b, LimbsU32) { return b->eng; }
+static Engine<LimbsU64>* engine_of(linac_batch* b, LimbsU64) {
+    if (!b->eng64) b->eng64 = new Engine<LimbsU64>();
+    return b->eng64;
 }
 
-static Item32* find_item(linac_batch* b, int64_t id) {
-    for (auto* it : b->items)
+template <class LT>
+static size_t matrix_bytes(int64_t rows, int64_t cols) {
+    return (size_t)round_up(rows * round_up(cols, 64) * (int64_t)sizeof(typename LT::T), 256);
+}
+
+template <class LT>
+static size_t item_bytes(int64_t rows, int64_t cols) {
+    return matrix_bytes<LT>(rows, cols) + workspace_layout<LT>(rows, cols, true).total;
+}
+
+template <class LT>
+static Item<LT>* find_item(linac_batch* b, int64_t id) {
+    for (auto* it : items_of(b, LT{}))
         if (it->id == id) return it;
     return nullptr;
 }
@@ -3113,10 +3136,13 @@ static bool batch_live(linac_batch* b) {
 
 static void batch_free(linac_batch* b) {
     (void)b->eng->drain();  // no trailing update may still run on an item's memory
+    if (b->eng64) (void)b->eng64->drain();
     (void)hipStreamSynchronize(b->ctx->stream);
     for (auto* it : b->items) delete it;
+    for (auto* it : b->items64) delete it;
     b->arena.release_all();
     delete b->eng;
+    delete b->eng64;
     delete b;
 }
 
@@ -3136,6 +3162,180 @@ static void free_context_batches(DeviceContext* ctx) {
     for (linac_batch* b : mine) batch_free(b);
 }
 
+template <class LT>
+static int batch_add(linac_batch* b, const typename LT::T* a, int64_t rows, int64_t cols, int64_t ld,
+                     const typename LT::F& f, uint32_t flags, int64_t* id_out) {
+    using T = typename LT::T;
+    const bool in_place = (flags & LINAC_IN_PLACE) != 0;
+    if (in_place && !(flags & LINAC_DEVICE_PTR)) return fail(LINAC_ERR_INVALID, "LINAC_IN_PLACE needs LINAC_DEVICE_PTR");
+    const size_t need = item_bytes<LT>(rows, cols) - (in_place ? matrix_bytes<LT>(rows, cols) : 0);
+    if (b->cap && b->in_use + need > b->cap)
+        return fail(LINAC_ERR_NOMEM, "admission: %zu bytes needed, %llu of %llu in use", need,
+                    (unsigned long long)b->in_use, (unsigned long long)b->cap);
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    hipStream_t s = b->ctx->stream;
+    auto* it = new Item<LT>();
+    const int ra = b->arena.alloc(need, &it->arena);
+    if (ra != LINAC_OK) {
+        delete it;
+        return ra;
+    }
+    it->bytes = need;
+    char* base = (char*)it->arena;
+    const WorkspaceLayout wl = workspace_layout<LT>(rows, cols, true);
+    if (in_place) {
+        describe(it->d, const_cast<T*>(a), ld, rows, cols, f);
+        carve(it->d, base, wl, rows, cols, true);
+        carve_compact(it->dc, base, wl);
+    } else {
+        const int64_t ldp = round_up(cols, 64);
+        T* A = (T*)base;
+        hipError_t ce = hipMemcpy2DAsync(A, ldp * sizeof(T), a, ld * sizeof(T), cols * sizeof(T), rows,
+                                         (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
+        if (ce != hipSuccess) {
+            b->arena.release(it->arena, it->bytes);
+            delete it;
+            return fail(LINAC_ERR_DEVICE, "copy into the batch failed: %s", hipGetErrorString(ce));
+        }
+        describe(it->d, A, ldp, rows, cols, f);
+        carve(it->d, base + matrix_bytes<LT>(rows, cols), wl, rows, cols, true);
+        carve_compact(it->dc, base + matrix_bytes<LT>(rows, cols), wl);
+    }
+    it->id = b->next_id++;
+    b->in_use += need;
+    items_of(b, LT{}).push_back(it);
+    fresh_of(b, LT{}).push_back(it);
+    *id_out = it->id;
+    return LINAC_OK;
+}
+
+template <class LT>
+static int batch_step(linac_batch* b, hipStream_t s) {
+    auto& items = items_of(b, LT{});
+    auto& fresh = fresh_of(b, LT{});
+    if (items.empty()) return LINAC_OK;
+    Engine<LT>* eng = engine_of(b, LT{});
+    LINAC_TRY(eng->init_items(fresh, s));
+    fresh.clear();
+    return eng->step(items, s, nullptr);
+}
+
+template <class LT>
+static void batch_release_item(linac_batch* b, Item<LT>* it) {
+    b->in_use -= it->bytes;
+    b->arena.release(it->arena, it->bytes);
+    auto& items = items_of(b, LT{});
+    items.erase(std::find(items.begin(), items.end(), it));
+    delete it;
+}
+
+template <class LT>
+static int batch_take(linac_batch* b, int64_t id, typename LT::T* out, int64_t ld_out, uint32_t flags,
+                      int64_t* pivots_out, int64_t* rank_out) {
+    using T = typename LT::T;
+    Item<LT>* it = find_item<LT>(b, id);
+    if (!it) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld for %d-bit storage", (long long)id,
+                         (int)(8 * sizeof(T)));
+    if (!it->finished) return fail(LINAC_ERR_INVALID, "matrix %lld is not finished", (long long)id);
+    if (out && ld_out < it->d.M) return fail(LINAC_ERR_INVALID, "ld_out < cols");
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    hipStream_t s = b->ctx->stream;
+    LINAC_TRY(engine_of(b, LT{})->drain());  // the item's last trailing-update work has finished
+    ElimWorkspace ws{it->d.st, it->d.piv_rows, it->d.piv_cols, it->d.used, std::max<int64_t>(1, std::min(it->d.N, it->d.M))};
+    int64_t rank = 0;
+    std::vector<int64_t> pc;
+    LINAC_TRY(gf_finalize(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, true, &rank, &pc));
+    if (out) {
+        LINAC_HIP(hipMemcpy2DAsync(out, ld_out * sizeof(T), it->d.A, it->d.ld * sizeof(T), it->d.M * sizeof(T), it->d.N,
+                                   (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipStreamSynchronize(s));
+    }
+    if (pivots_out)
+        for (int64_t k = 0; k < rank; ++k) pivots_out[k] = pc[k];
+    if (rank_out) *rank_out = rank;
+    batch_release_item(b, it);
+    return LINAC_OK;
+}
+
+template <class LT>
+static int batch_take_many(linac_batch* b, const int64_t* ids, int64_t count, typename LT::T* const* outs,
+                           const int64_t* ld_outs, uint32_t flags, int64_t* const* pivots_out, int64_t* ranks_out) {
+    using T = typename LT::T;
+    using BM = BMat<LT>;
+    if (count < 0 || (count > 0 && !ids)) return fail(LINAC_ERR_INVALID, "bad id list");
+    if (count == 0) return LINAC_OK;
+    std::vector<Item<LT>*> its((size_t)count);
+    for (int64_t i = 0; i < count; ++i) {
+        its[i] = find_item<LT>(b, ids[i]);
+        if (!its[i]) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld for %d-bit storage",
+                                 (long long)ids[i], (int)(8 * sizeof(T)));
+        if (!its[i]->finished) return fail(LINAC_ERR_INVALID, "matrix %lld is not finished", (long long)ids[i]);
+        for (int64_t j = 0; j < i; ++j)
+            if (its[j] == its[i]) return fail(LINAC_ERR_INVALID, "id %lld listed twice", (long long)ids[i]);
+        if (outs && outs[i] && (!ld_outs || ld_outs[i] < its[i]->d.M))
+            return fail(LINAC_ERR_INVALID, "ld_out < cols for id %lld", (long long)ids[i]);
+    }
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    hipStream_t s = b->ctx->stream;
+    LINAC_TRY(engine_of(b, LT{})->drain());
+    // descriptors, ranks and pivot lists in one scratch block: two round trips in all
+    int64_t cap_total = 0;
+    for (auto* it : its) cap_total += std::max<int64_t>(1, std::min(it->d.N, it->d.M));
+    const size_t desc_b = (size_t)round_up(count * (int64_t)sizeof(BM), 256);
+    const size_t offs_b = (size_t)round_up((count + 1) * 8, 256);
+    void* scratch = nullptr;
+    LINAC_TRY(b->ctx->get(SLOT_AUX, desc_b + offs_b + 2 * (size_t)round_up(cap_total * 8, 256), &scratch));
+    BM* d_desc = (BM*)scratch;
+    int64_t* d_offs = (int64_t*)((char*)scratch + desc_b);
+    int64_t* d_rows = (int64_t*)((char*)scratch + desc_b + offs_b);
+    int64_t* d_cols = d_rows + round_up(cap_total * 8, 256) / 8;
+    std::vector<BM> hd((size_t)count);
+    for (int64_t i = 0; i < count; ++i) hd[i] = its[i]->d;
+    LINAC_HIP(hipMemcpyAsync(d_desc, hd.data(), count * sizeof(BM), hipMemcpyHostToDevice, s));
+    k_collect_rank<LT><<<(unsigned)count, 64, 0, s>>>(d_desc, d_offs);
+    std::vector<int64_t> rk((size_t)count), offs((size_t)count + 1, 0);
+    LINAC_HIP(hipMemcpyAsync(rk.data(), d_offs, count * 8, hipMemcpyDeviceToHost, s));
+    LINAC_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < count; ++i) offs[i + 1] = offs[i] + rk[i];
+    std::vector<int64_t> prow((size_t)std::max<int64_t>(1, offs[count])), pcol(prow.size());
+    if (offs[count] > 0) {
+        LINAC_HIP(hipMemcpyAsync(d_offs, offs.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
+        k_collect_pivots<LT><<<(unsigned)count, 256, 0, s>>>(d_desc, d_offs, d_rows, d_cols);
+        LINAC_HIP(hipMemcpyAsync(prow.data(), d_rows, offs[count] * 8, hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipMemcpyAsync(pcol.data(), d_cols, offs[count] * 8, hipMemcpyDeviceToHost, s));
+        LINAC_HIP(hipStreamSynchronize(s));
+    }
+    for (int64_t i = 0; i < count; ++i) {
+        Item<LT>* it = its[i];
+        std::vector<int64_t> r(prow.begin() + offs[i], prow.begin() + offs[i + 1]);
+        std::vector<int64_t> c(pcol.begin() + offs[i], pcol.begin() + offs[i + 1]);
+        const bool resorted = sort_pivots_by_column(r, c);
+        ElimWorkspace ws{it->d.st, it->d.piv_rows, it->d.piv_cols, it->d.used,
+                         std::max<int64_t>(1, std::min(it->d.N, it->d.M))};
+        LINAC_TRY(gf_finalize_apply(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, r, c, resorted));
+        if (outs && outs[i])
+            LINAC_HIP(hipMemcpy2DAsync(outs[i], ld_outs[i] * sizeof(T), it->d.A, it->d.ld * sizeof(T),
+                                       it->d.M * sizeof(T), it->d.N,
+                                       (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                       s));
+        if (pivots_out && pivots_out[i])
+            for (int64_t k = 0; k < (int64_t)c.size(); ++k) pivots_out[i][k] = c[k];
+        if (ranks_out) ranks_out[i] = rk[i];
+    }
+    LINAC_HIP(hipStreamSynchronize(s));
+    for (auto* it : its) batch_release_item(b, it);
+    return LINAC_OK;
+}
+
+static int batch_check_add(linac_batch* b, const void* a, int64_t rows, int64_t cols, int64_t ld, int64_t* id_out) {
+    if (!b || !id_out) return fail(LINAC_ERR_INVALID, "batch or id_out is NULL");
+    if (!batch_live(b)) return fail(LINAC_ERR_INVALID, "batch handle was released (linac_shutdown)");
+    if (rows <= 0 || cols <= 0) return fail(LINAC_ERR_INVALID, "empty matrix");
+    if (ld < cols) return fail(LINAC_ERR_INVALID, "ld < cols");
+    if (!a) return fail(LINAC_ERR_INVALID, "matrix pointer is NULL");
+    return LINAC_OK;
+}
+
 }  // namespace linac
 
 using namespace linac;
@@ -3144,7 +3344,13 @@ extern "C" {
 
 int linac_batch_footprint_u32(int64_t rows, int64_t cols, uint64_t* bytes_out) {
     if (rows < 0 || cols < 0 || !bytes_out) return fail(LINAC_ERR_INVALID, "bad arguments");
-    *bytes_out = (rows == 0 || cols == 0) ? 0 : (uint64_t)item_bytes(rows, cols);
+    *bytes_out = (rows == 0 || cols == 0) ? 0 : (uint64_t)item_bytes<LimbsU32>(rows, cols);
+    return LINAC_OK;
+}
+
+int linac_batch_footprint_u64(int64_t rows, int64_t cols, uint64_t* bytes_out) {
+    if (rows < 0 || cols < 0 || !bytes_out) return fail(LINAC_ERR_INVALID, "bad arguments");
+    *bytes_out = (rows == 0 || cols == 0) ? 0 : (uint64_t)item_bytes<LimbsU64>(rows, cols);
     return LINAC_OK;
 }
 
@@ -3187,54 +3393,20 @@ int linac_batch_memory(linac_batch* b, uint64_t* in_use_out, uint64_t* cap_out) 
 
 int linac_batch_add_u32(linac_batch* b, const uint32_t* a, int64_t rows, int64_t cols, int64_t ld, uint32_t p,
                         uint32_t flags, int64_t* id_out) {
-    if (!b || !id_out) return fail(LINAC_ERR_INVALID, "batch or id_out is NULL");
-    if (!batch_live(b)) return fail(LINAC_ERR_INVALID, "batch handle was released (linac_shutdown)");
-    if (rows <= 0 || cols <= 0) return fail(LINAC_ERR_INVALID, "empty matrix");
-    if (ld < cols) return fail(LINAC_ERR_INVALID, "ld < cols");
-    if (!a) return fail(LINAC_ERR_INVALID, "matrix pointer is NULL");
+    LINAC_TRY(batch_check_add(b, a, rows, cols, ld, id_out));
     if (!is_prime_u64(p) || p == 2) return fail(LINAC_ERR_INVALID, "field characteristic %u is not an odd prime", p);
     if (p > LIMB_P_MAX) return fail(LINAC_ERR_INVALID, "p = %u exceeds the batched path's limit %llu", p,
                                     (unsigned long long)LIMB_P_MAX);
-    const bool in_place = (flags & LINAC_IN_PLACE) != 0;
-    if (in_place && !(flags & LINAC_DEVICE_PTR)) return fail(LINAC_ERR_INVALID, "LINAC_IN_PLACE needs LINAC_DEVICE_PTR");
-    const size_t need = item_bytes(rows, cols) - (in_place ? matrix_bytes(rows, cols) : 0);
-    if (b->cap && b->in_use + need > b->cap)
-        return fail(LINAC_ERR_NOMEM, "admission: %zu bytes needed, %llu of %llu in use", need,
-                    (unsigned long long)b->in_use, (unsigned long long)b->cap);
-    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
-    hipStream_t s = b->ctx->stream;
-    auto* it = new Item32();
-    const int ra = b->arena.alloc(need, &it->arena);
-    if (ra != LINAC_OK) {
-        delete it;
-        return ra;
-    }
-    it->bytes = need;
-    char* base = (char*)it->arena;
-    if (in_place) {
-        describe(it->d, const_cast<uint32_t*>(a), ld, rows, cols, make_field_u32(p));
-        carve(it->d, base, workspace_layout<LimbsU32>(rows, cols, true), rows, cols, true);
-        carve_compact(it->dc, base, workspace_layout<LimbsU32>(rows, cols, true));
-    } else {
-        const int64_t ldp = round_up(cols, 64);
-        uint32_t* A = (uint32_t*)base;
-        hipError_t ce = hipMemcpy2DAsync(A, ldp * 4, a, ld * 4, cols * 4, rows,
-                                         (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
-        if (ce != hipSuccess) {
-            b->arena.release(it->arena, it->bytes);
-            delete it;
-            return fail(LINAC_ERR_DEVICE, "copy into the batch failed: %s", hipGetErrorString(ce));
-        }
-        describe(it->d, A, ldp, rows, cols, make_field_u32(p));
-        carve(it->d, base + matrix_bytes(rows, cols), workspace_layout<LimbsU32>(rows, cols, true), rows, cols, true);
-        carve_compact(it->dc, base + matrix_bytes(rows, cols), workspace_layout<LimbsU32>(rows, cols, true));
-    }
-    it->id = b->next_id++;
-    b->in_use += need;
-    b->items.push_back(it);
-    b->fresh.push_back(it);
-    *id_out = it->id;
-    return LINAC_OK;
+    return batch_add<LimbsU32>(b, a, rows, cols, ld, make_field_u32(p), flags, id_out);
+}
+
+int linac_batch_add_u64(linac_batch* b, const uint64_t* a, int64_t rows, int64_t cols, int64_t ld, uint64_t p,
+                        uint32_t flags, int64_t* id_out) {
+    LINAC_TRY(batch_check_add(b, a, rows, cols, ld, id_out));
+    if (p < LIMB64_P_MIN || p >= LIMB64_P_MAX || !is_prime_u64(p))
+        return fail(LINAC_ERR_INVALID, "p = %llu is not a prime in [2^32, 2^63) (smaller primes: linac_batch_add_u32)",
+                    (unsigned long long)p);
+    return batch_add<LimbsU64>(b, a, rows, cols, ld, make_field_u64(p), flags, id_out);
 }
 
 int linac_batch_step(linac_batch* b, int64_t* unfinished_out) {
@@ -3242,13 +3414,12 @@ int linac_batch_step(linac_batch* b, int64_t* unfinished_out) {
     if (!batch_live(b)) return fail(LINAC_ERR_INVALID, "batch handle was released (linac_shutdown)");
     std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
     hipStream_t s = b->ctx->stream;
-    LINAC_TRY(b->eng->init_items(b->fresh, s));
-    b->fresh.clear();
-    Profiler* prof = nullptr;
-    LINAC_TRY(b->eng->step(b->items, s, prof));
+    LINAC_TRY(batch_step<LimbsU32>(b, s));
+    LINAC_TRY(batch_step<LimbsU64>(b, s));
     LINAC_HIP(hipStreamSynchronize(s));
     int64_t n = 0;
     for (auto* it : b->items) n += it->finished ? 0 : 1;
+    for (auto* it : b->items64) n += it->finished ? 0 : 1;
     if (unfinished_out) *unfinished_out = n;
     return LINAC_OK;
 }
@@ -3261,110 +3432,35 @@ int linac_batch_run(linac_batch* b) {
 
 int linac_batch_status(linac_batch* b, int64_t id) {
     if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
-    Item32* it = find_item(b, id);
-    if (!it) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)id);
-    return it->finished ? 1 : 0;
+    if (Item<LimbsU32>* it = find_item<LimbsU32>(b, id)) return it->finished ? 1 : 0;
+    if (Item<LimbsU64>* it = find_item<LimbsU64>(b, id)) return it->finished ? 1 : 0;
+    return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)id);
 }
 
 int linac_batch_take_u32(linac_batch* b, int64_t id, uint32_t* out, int64_t ld_out, uint32_t flags,
                          int64_t* pivots_out, int64_t* rank_out) {
     if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
-    Item32* it = find_item(b, id);
-    if (!it) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)id);
-    if (!it->finished) return fail(LINAC_ERR_INVALID, "matrix %lld is not finished", (long long)id);
-    if (out && ld_out < it->d.M) return fail(LINAC_ERR_INVALID, "ld_out < cols");
-    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
-    hipStream_t s = b->ctx->stream;
-    LINAC_TRY(b->eng->drain());  // the item's last trailing-update work has finished
-    ElimWorkspace ws{it->d.st, it->d.piv_rows, it->d.piv_cols, it->d.used, std::max<int64_t>(1, std::min(it->d.N, it->d.M))};
-    int64_t rank = 0;
-    std::vector<int64_t> pc;
-    LINAC_TRY(gf_finalize(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, true, &rank, &pc));
-    if (out) {
-        LINAC_HIP(hipMemcpy2DAsync(out, ld_out * 4, it->d.A, it->d.ld * 4, it->d.M * 4, it->d.N,
-                                   (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-        LINAC_HIP(hipStreamSynchronize(s));
-    }
-    if (pivots_out)
-        for (int64_t k = 0; k < rank; ++k) pivots_out[k] = pc[k];
-    if (rank_out) *rank_out = rank;
-    b->in_use -= it->bytes;
-    b->arena.release(it->arena, it->bytes);
-    b->items.erase(std::find(b->items.begin(), b->items.end(), it));
-    delete it;
-    return LINAC_OK;
+    return batch_take<LimbsU32>(b, id, out, ld_out, flags, pivots_out, rank_out);
+}
+
+int linac_batch_take_u64(linac_batch* b, int64_t id, uint64_t* out, int64_t ld_out, uint32_t flags,
+                         int64_t* pivots_out, int64_t* rank_out) {
+    if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
+    return batch_take<LimbsU64>(b, id, out, ld_out, flags, pivots_out, rank_out);
 }
 
 int linac_batch_take_many_u32(linac_batch* b, const int64_t* ids, int64_t count, uint32_t* const* outs,
                               const int64_t* ld_outs, uint32_t flags, int64_t* const* pivots_out,
                               int64_t* ranks_out) {
     if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
-    if (count < 0 || (count > 0 && !ids)) return fail(LINAC_ERR_INVALID, "bad id list");
-    if (count == 0) return LINAC_OK;
-    std::vector<Item32*> its((size_t)count);
-    for (int64_t i = 0; i < count; ++i) {
-        its[i] = find_item(b, ids[i]);
-        if (!its[i]) return fail(LINAC_ERR_INVALID, "unknown or already taken id %lld", (long long)ids[i]);
-        if (!its[i]->finished) return fail(LINAC_ERR_INVALID, "matrix %lld is not finished", (long long)ids[i]);
-        for (int64_t j = 0; j < i; ++j)
-            if (its[j] == its[i]) return fail(LINAC_ERR_INVALID, "id %lld listed twice", (long long)ids[i]);
-        if (outs && outs[i] && (!ld_outs || ld_outs[i] < its[i]->d.M))
-            return fail(LINAC_ERR_INVALID, "ld_out < cols for id %lld", (long long)ids[i]);
-    }
-    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
-    hipStream_t s = b->ctx->stream;
-    LINAC_TRY(b->eng->drain());
-    // descriptors, ranks and pivot lists in one scratch block: two round trips in all
-    int64_t cap_total = 0;
-    for (auto* it : its) cap_total += std::max<int64_t>(1, std::min(it->d.N, it->d.M));
-    const size_t desc_b = (size_t)round_up(count * (int64_t)sizeof(BMat32), 256);
-    const size_t offs_b = (size_t)round_up((count + 1) * 8, 256);
-    void* scratch = nullptr;
-    LINAC_TRY(b->ctx->get(SLOT_AUX, desc_b + offs_b + 2 * (size_t)round_up(cap_total * 8, 256), &scratch));
-    BMat32* d_desc = (BMat32*)scratch;
-    int64_t* d_offs = (int64_t*)((char*)scratch + desc_b);
-    int64_t* d_rows = (int64_t*)((char*)scratch + desc_b + offs_b);
-    int64_t* d_cols = d_rows + round_up(cap_total * 8, 256) / 8;
-    std::vector<BMat32> hd((size_t)count);
-    for (int64_t i = 0; i < count; ++i) hd[i] = its[i]->d;
-    LINAC_HIP(hipMemcpyAsync(d_desc, hd.data(), count * sizeof(BMat32), hipMemcpyHostToDevice, s));
-    k_collect_rank<LimbsU32><<<(unsigned)count, 64, 0, s>>>(d_desc, d_offs);
-    std::vector<int64_t> rk((size_t)count), offs((size_t)count + 1, 0);
-    LINAC_HIP(hipMemcpyAsync(rk.data(), d_offs, count * 8, hipMemcpyDeviceToHost, s));
-    LINAC_HIP(hipStreamSynchronize(s));
-    for (int64_t i = 0; i < count; ++i) offs[i + 1] = offs[i] + rk[i];
-    std::vector<int64_t> prow((size_t)std::max<int64_t>(1, offs[count])), pcol(prow.size());
-    if (offs[count] > 0) {
-        LINAC_HIP(hipMemcpyAsync(d_offs, offs.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
-        k_collect_pivots<LimbsU32><<<(unsigned)count, 256, 0, s>>>(d_desc, d_offs, d_rows, d_cols);
-        LINAC_HIP(hipMemcpyAsync(prow.data(), d_rows, offs[count] * 8, hipMemcpyDeviceToHost, s));
-        LINAC_HIP(hipMemcpyAsync(pcol.data(), d_cols, offs[count] * 8, hipMemcpyDeviceToHost, s));
-        LINAC_HIP(hipStreamSynchronize(s));
-    }
-    for (int64_t i = 0; i < count; ++i) {
-        Item32* it = its[i];
-        std::vector<int64_t> r(prow.begin() + offs[i], prow.begin() + offs[i + 1]);
-        std::vector<int64_t> c(pcol.begin() + offs[i], pcol.begin() + offs[i + 1]);
-        const bool resorted = sort_pivots_by_column(r, c);
-        ElimWorkspace ws{it->d.st, it->d.piv_rows, it->d.piv_cols, it->d.used,
-                         std::max<int64_t>(1, std::min(it->d.N, it->d.M))};
-        LINAC_TRY(gf_finalize_apply(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, r, c, resorted));
-        if (outs && outs[i])
-            LINAC_HIP(hipMemcpy2DAsync(outs[i], ld_outs[i] * 4, it->d.A, it->d.ld * 4, it->d.M * 4, it->d.N,
-                                       (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                                       s));
-        if (pivots_out && pivots_out[i])
-            for (int64_t k = 0; k < (int64_t)c.size(); ++k) pivots_out[i][k] = c[k];
-        if (ranks_out) ranks_out[i] = rk[i];
-    }
-    LINAC_HIP(hipStreamSynchronize(s));
-    for (auto* it : its) {
-        b->in_use -= it->bytes;
-        b->arena.release(it->arena, it->bytes);
-        b->items.erase(std::find(b->items.begin(), b->items.end(), it));
-        delete it;
-    }
-    return LINAC_OK;
+    return batch_take_many<LimbsU32>(b, ids, count, outs, ld_outs, flags, pivots_out, ranks_out);
+}
+
+int linac_batch_take_many_u64(linac_batch* b, const int64_t* ids, int64_t count, uint64_t* const* outs,
+                              const int64_t* ld_outs, uint32_t flags, int64_t* const* pivots_out,
+                              int64_t* ranks_out) {
+    if (!b) return fail(LINAC_ERR_INVALID, "batch is NULL");
+    return batch_take_many<LimbsU64>(b, ids, count, outs, ld_outs, flags, pivots_out, ranks_out);
 }
 
 int linac_gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda, int64_t a_ncols, const int64_t* a_cols,

@@ -354,6 +354,11 @@ template int gf_finalize_apply<FieldU32>(uint32_t*, int64_t, int64_t, int64_t, c
                                          const std::vector<int64_t>&, bool);
 template int gf_finalize<FieldU32>(uint32_t*, int64_t, int64_t, int64_t, const FieldU32&, hipStream_t,
                                    const ElimWorkspace&, DeviceContext*, bool, bool, int64_t*, std::vector<int64_t>*);
+template int gf_finalize_apply<FieldU64>(uint64_t*, int64_t, int64_t, int64_t, const FieldU64&, hipStream_t,
+                                         const ElimWorkspace&, DeviceContext*, bool, const std::vector<int64_t>&,
+                                         const std::vector<int64_t>&, bool);
+template int gf_finalize<FieldU64>(uint64_t*, int64_t, int64_t, int64_t, const FieldU64&, hipStream_t,
+                                   const ElimWorkspace&, DeviceContext*, bool, bool, int64_t*, std::vector<int64_t>*);
 
 // Shared entry: validation, host<->device staging, method dispatch.
 template <class F>

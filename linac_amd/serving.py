@@ -36,6 +36,10 @@ from .hip_row_reduce import rref_device
 # Eight 31-bit primes: numpy.random.RandomState(2024), x = randint(2**30, 2**31),
 # next prime after x (all below 2**31).
 SERVING_PRIMES = [1104465403, 1261898177, 1451761303, 1482257969, 1688125709, 1881848357, 1928907403, 2099153543]
+# Primes that need 64-bit storage (2^32 <= p < 2^63): the Mersenne prime
+# 2^61 - 1 and the largest primes below 2^62 and 2^63.
+SERVING_PRIMES_64 = [2 ** 61 - 1, 2 ** 62 - 57, 2 ** 63 - 25]
+WIDE_P_MIN = 2 ** 32
 
 
 @dataclass
@@ -43,9 +47,9 @@ class Request:
     rid: int
     rows: int
     cols: int
-    p: int                 # prime, or 0 for float64
+    p: int                 # prime, or 0 for float64 (p >= 2^32: uint64 storage)
     arrival: float = 0.0   # seconds after the start of the run
-    tensor: object = None  # device tensor (int32 bits of uint32 for GF, float64), set when materialised
+    tensor: object = None  # device tensor (int32 bits of uint32 / int64 for GF, float64), set when materialised
 
 
 @dataclass
@@ -105,8 +109,9 @@ class Server:
     def _bytes(self, req):
         if req.p == 0:
             return req.rows * req.cols * 8 + float_footprint(req.rows, req.cols)
-        return req.rows * req.cols * 4 + footprint(req.rows, req.cols) - \
-            (req.rows * ((req.cols + 63) // 64 * 64) * 4 + 255) // 256 * 256
+        w = 8 if req.p >= WIDE_P_MIN else 4
+        return req.rows * req.cols * w + footprint(req.rows, req.cols, req.p) - \
+            (req.rows * ((req.cols + 63) // 64 * 64) * w + 255) // 256 * 256
 
     def _admit(self):
         while self.waiting:
@@ -185,16 +190,17 @@ def make_requests(n, rate, seed=0, n_min=200, n_max=8000, float_share=0.1, prime
 
 
 def materialise(req, seed=0):
-    """The request's matrix in HBM: uniform [0, p) (GF) or [0, 1) (float64),
-    generated on the device from (seed, rid)."""
+    """The request's matrix in HBM: uniform [0, p) (GF; int32 bits of uint32,
+    or int64 for p >= 2^32) or [0, 1) (float64), generated on the device from
+    (seed, rid)."""
     import torch
     g = torch.Generator(device="cuda")
     g.manual_seed(seed * 1000003 + req.rid)
     if req.p == 0:
         req.tensor = torch.rand((req.rows, req.cols), generator=g, device="cuda", dtype=torch.float64)
     else:
-        req.tensor = torch.randint(0, req.p, (req.rows, req.cols), generator=g, device="cuda",
-                                   dtype=torch.int64).to(torch.int32)
+        t = torch.randint(0, req.p, (req.rows, req.cols), generator=g, device="cuda", dtype=torch.int64)
+        req.tensor = t if req.p >= WIDE_P_MIN else t.to(torch.int32)
     return req
 
 
@@ -253,26 +259,29 @@ def sequential_plain(requests, seed=0, method="plain"):
         if req.p == 0:
             rref_device(req.tensor.data_ptr(), req.rows, req.cols, req.cols, 0, numpy.float64, method=method)
         else:
-            _rref_u32_unaligned(req, method)
+            _rref_gf_unaligned(req, method)
         torch.cuda.synchronize()
         out.append(time.perf_counter() - t)
         req.tensor = None
     return out
 
 
-def _rref_u32_unaligned(req, method):
+def _rref_gf_unaligned(req, method):
     """Single-call GF(p) solve of a device matrix whose row stride need not be
     16-byte aligned: copied into an aligned buffer (the device entry point's
     requirement), solved there, copied back."""
     import torch
-    ld = (req.cols + 3) // 4 * 4
+    wide = req.p >= WIDE_P_MIN
+    per = 2 if wide else 4  # elements per 16 bytes
+    dt = numpy.uint64 if wide else numpy.uint32
+    ld = (req.cols + per - 1) // per * per
     if ld == req.cols:
-        rref_device(req.tensor.data_ptr(), req.rows, req.cols, req.cols, req.p, numpy.uint32, method=method)
+        rref_device(req.tensor.data_ptr(), req.rows, req.cols, req.cols, req.p, dt, method=method)
         return
-    buf = torch.zeros((req.rows, ld), dtype=torch.int32, device="cuda")
+    buf = torch.zeros((req.rows, ld), dtype=req.tensor.dtype, device="cuda")
     buf[:, :req.cols].copy_(req.tensor)
     torch.cuda.synchronize()
-    rref_device(buf.data_ptr(), req.rows, req.cols, ld, req.p, numpy.uint32, method=method)
+    rref_device(buf.data_ptr(), req.rows, req.cols, ld, req.p, dt, method=method)
     req.tensor.copy_(buf[:, :req.cols])
 
 

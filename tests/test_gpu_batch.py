@@ -148,3 +148,82 @@ def test_take_many_order_subsets_and_errors(gpu):
         with pytest.raises(ValueError):
             b.take_many([small, big])  # big is not finished
         assert b.take_many([small])[0][2] == 30
+
+
+PRIMES_64 = [2 ** 32 + 15, 2 ** 61 - 1, 2 ** 62 - 57, 2 ** 63 - 25]
+
+
+def _mat64(rows, cols, rank, p, seed):
+    rs = numpy.random.RandomState(seed)
+    L = numpy.array([[int(x) for x in rs.randint(0, 2 ** 62, size=rank, dtype=numpy.int64)] for _ in range(rows)],
+                    dtype=object) % p
+    U = numpy.array([[int(x) for x in rs.randint(0, 2 ** 62, size=cols, dtype=numpy.int64)] for _ in range(rank)],
+                    dtype=object) % p
+    return numpy.array((L @ U) % p, dtype=numpy.uint64)
+
+
+def test_mixed_32_and_64_bit_primes(gpu):
+    """One batch holding uint32 matrices (p < 2^32) and uint64 matrices
+    (2^32 <= p < 2^63), joined at different steps, host and in-place device
+    inputs, taken together: every result equals the oracle bit for bit."""
+    import torch
+    rs = numpy.random.RandomState(21)
+    cases = []
+    for i in range(10):
+        rows, cols = int(rs.randint(1, 420)), int(rs.randint(1, 420))
+        rank = max(1, int(rs.randint(0, min(rows, cols) + 1)))
+        if i % 2:
+            p = PRIMES_64[(i // 2) % len(PRIMES_64)]
+            A = _mat64(rows, cols, rank, p, 100 + i)
+        else:
+            p = PRIMES[(i // 2) % len(PRIMES)]
+            A = _mat(rows, cols, rank, p, 100 + i)
+        if i % 3 == 0 and cols > 5:
+            A[:, 1] = 0
+        cases.append((A, p))
+    dev = {}
+    with gpu.Batch() as b:
+        ids = []
+        for i, (A, p) in enumerate(cases):
+            if i == 6:
+                b.step()  # the rest join a running batch
+            if i % 4 == 1:  # uint64 (int64 bits) in HBM, reduced in place
+                t = torch.from_numpy(A.view(numpy.int64).copy()).cuda() if A.dtype == numpy.uint64 else \
+                    torch.from_numpy(A.view(numpy.int32).copy()).cuda()
+                torch.cuda.synchronize()
+                ids.append(b.add_device(t.data_ptr(), A.shape[0], A.shape[1], A.shape[1], p, in_place=True))
+                dev[ids[-1]] = t
+            else:
+                ids.append(b.add(A, p))
+        b.run()
+        results = b.take_many(ids)
+    torch.cuda.synchronize()
+    for rid, (A, p), (R, piv, rank) in zip(ids, cases, results):
+        Ro, po = crref.rref_modp(A, p)
+        if R is None:
+            R = dev[rid].cpu().numpy().view(numpy.uint64 if p >= 2 ** 32 else numpy.uint32)
+        assert piv == po and rank == len(po), (A.shape, p)
+        assert numpy.array_equal(R.astype(numpy.uint64), Ro.astype(numpy.uint64)), (A.shape, p)
+
+
+def test_batch_u64_rejects_small_primes_and_wrong_width(gpu):
+    import ctypes
+    from linac_amd import _lib
+    lib = _lib.load()
+    A = numpy.ones((4, 4), dtype=numpy.uint64)
+    with gpu.Batch() as b:
+        rid = ctypes.c_int64(-1)
+        with pytest.raises(ValueError, match="2\\^32"):
+            _lib.check(lib.linac_batch_add_u64(b._h, A.ctypes.data_as(ctypes.c_void_p), 4, 4, 4, 2 ** 31 - 1, 0,
+                                               ctypes.byref(rid)))
+        with pytest.raises(ValueError):
+            _lib.check(lib.linac_batch_add_u64(b._h, A.ctypes.data_as(ctypes.c_void_p), 4, 4, 4, 2 ** 61 + 1, 0,
+                                               ctypes.byref(rid)))  # not prime
+        i64 = b.add(A, 2 ** 61 - 1)
+        b.run()
+        # a 64-bit item is not taken through the 32-bit call
+        out = numpy.zeros((4, 4), dtype=numpy.uint32)
+        with pytest.raises(ValueError, match="32-bit"):
+            _lib.check(lib.linac_batch_take_u32(b._h, i64, out.ctypes.data_as(ctypes.c_void_p), 4, 0, None, None))
+        R, piv, rank = b.take(i64)
+        assert rank == 1 and piv == [0] and R[0].tolist() == [1, 1, 1, 1]

@@ -104,3 +104,44 @@ def test_growing_loop_matches_reference_loop(gpu):
     assert piv == po and numpy.array_equal(R.astype(numpy.uint64), Ro[:len(po)].astype(numpy.uint64))
     with pytest.raises(Exception, match="exceeded dimension 60"):
         pivots_of_growing_matrix(sample_rows, cols, p, iteration_start=20, iteration_step=20, max_iteration=2)
+
+
+P_LIMB_MAX = 4278124237  # the largest prime the int8-limb GEMM accepts (<= 4278124287)
+
+
+def _from_digits(d, p):
+    """Canonical value in [0, p) whose centred representative has the
+    balanced base-256 digits d (low digit first)."""
+    c = sum(v * 256 ** i for i, v in enumerate(d))
+    assert -(p // 2) <= c <= p // 2
+    return c % p
+
+
+@pytest.mark.parametrize("k", [512, 1100])
+@pytest.mark.parametrize("pattern", ["same_sign", "mixed"])
+def test_gemm_sub_extreme_digits(gpu, k, pattern):
+    """Operands whose every balanced digit sits at the end of its range, so
+    that every int32 diagonal sum S_s reaches its largest magnitude: +2^14 per
+    digit product (both digits -128) or -16256 (digits -128 and 127), at the
+    single-chunk depth 512 and the chunked depth 1100; against Python
+    integers."""
+    from linac_amd.incremental import gemm_sub
+    p = P_LIMB_MAX
+    lo = _from_digits([-128, -128, -128, -126], p)
+    hi = _from_digits([127, 127, 127, 126], p)
+    m, n = 64, 72
+    rs = numpy.random.RandomState(k)
+    if pattern == "same_sign":
+        A = numpy.full((m, k), lo, dtype=numpy.int64)
+        B = numpy.full((k, n), lo, dtype=numpy.int64)
+        B[:, n // 2:] = hi  # half the columns at the negative extreme
+    else:
+        A = numpy.where(rs.randint(0, 2, size=(m, k)) == 1, lo, hi).astype(numpy.int64)
+        B = numpy.where(rs.randint(0, 2, size=(k, n)) == 1, lo, hi).astype(numpy.int64)
+        A[: m // 4] = lo  # rows and columns at one extreme throughout as well
+        B[:, : n // 4] = lo
+    C = numpy.where(rs.randint(0, 2, size=(m, n)) == 1, 0, p - 1).astype(numpy.int64)
+    dC = _dev(C)
+    gemm_sub(dC, _dev(A), _dev(B), p, k=k)
+    want = (C.astype(object) - A.astype(object) @ B.astype(object)) % p
+    assert numpy.array_equal(_host(dC).astype(object), want)

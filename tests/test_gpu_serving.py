@@ -85,3 +85,21 @@ def test_served_results_equal_the_oracle(gpu):
             Ro, po = crref.rref_f64(A)
             assert float(numpy.max(numpy.abs(got - Ro))) <= 1e-9, req
         assert res.pivots == po and res.rank == len(po), req
+
+
+def test_serving_with_64_bit_primes(gpu):
+    """Requests over primes that need 64-bit storage are served in the same
+    batch as 32-bit ones; results equal the oracle bit for bit."""
+    from linac_amd import serving
+    from oracle import crref
+    primes = serving.SERVING_PRIMES[:3] + serving.SERVING_PRIMES_64
+    reqs = serving.make_requests(16, rate=0, seed=13, n_min=30, n_max=260, float_share=0.0, primes=primes)
+    assert any(r.p >= 2 ** 32 for r in reqs) and any(r.p < 2 ** 32 for r in reqs)
+    results, wall, stats = serving.serve(reqs, seed=13, keep=True)
+    for req, res in zip(sorted(reqs, key=lambda r: r.rid), results):
+        A = serving.materialise(serving.Request(req.rid, req.rows, req.cols, req.p), 13).tensor.cpu().numpy()
+        wide = req.p >= 2 ** 32
+        Ro, po = crref.rref_modp(A.view(numpy.uint64 if wide else numpy.uint32), req.p)
+        got = res.tensor.cpu().numpy().view(numpy.uint64 if wide else numpy.uint32)
+        assert numpy.array_equal(got, Ro), req
+        assert res.pivots == po and res.rank == len(po), req
