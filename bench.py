@@ -46,7 +46,40 @@ def parse():
                     help="instead of the headline line: one JSON line measuring every GF(p) and float configuration")
     ap.add_argument("--with-100k", action="store_true", help="with --configs: include the 100000^2 u32 run (~40 GB)")
     ap.add_argument("--only", default="", help="with --configs: comma list of cases, e.g. u32-20000-blocked,f64-8000")
+    ap.add_argument("--gemm", type=int, default=0, metavar="N",
+                    help="instead of the headline line: the trailing-update kernel alone, C (N x N) -= A (N x 512) "
+                         "B (512 x N) mod 2^31-1 (linac_gemm_sub_u32), --steps timed calls")
     return ap.parse_args()
+
+
+def gemm_main(args):
+    """The modular GEMM of the trailing update in isolation: one launch of
+    the int8-limb kernel over N x N outputs at depth 512 (the depth of a
+    two-panel update), after the operands' limb planes are formed."""
+    import torch
+    from linac_amd.incremental import gemm_sub
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import certificates
+    p, n, k = 2 ** 31 - 1, args.gemm, 512
+    C = certificates.uniform_u32(n, n, p, 1)
+    A = certificates.uniform_u32(n, k, p, 2)
+    B = certificates.uniform_u32(k, n, p, 3)
+    for _ in range(args.warmup):
+        gemm_sub(C, A, B, p, k=k)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        gemm_sub(C, A, B, p, k=k)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    ops = 2.0 * n * n * k * 16  # int8 operations: 16 limb products per modular MAC
+    best = min(times)
+    print(json.dumps({"gemm": {"n": n, "k": k, "p": p, "calls": args.steps, "best_ms": round(best * 1e3, 3),
+                               "median_ms": round(sorted(times)[len(times) // 2] * 1e3, 3),
+                               "int8_TOPs_incl_planes": round(ops / best / 1e12, 1),
+                               "note": "host clock around the whole call (limb planes + kernel); the kernel's own "
+                                       "time comes from rocprofv3"}}), flush=True)
 
 
 def _cpu_model():
@@ -481,6 +514,12 @@ def configs_main(args):
 
 def main():
     args = parse()
+    if args.gemm:
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            sys.exit(2)
+        gemm_main(args)
+        return
     if args.configs:
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)

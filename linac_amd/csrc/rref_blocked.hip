@@ -2794,6 +2794,7 @@ static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, cons
     LINAC_HIP(hipStreamSynchronize(s));
     if (getenv("LINAC_DEBUG_TIMING")) {
         print_timers(it.d, ctx);
+        print_timers(it.dc, ctx);  // inner steps on the candidate blocks of full-rank panels
         unsigned long long tp[8];
         int rate_khz = 0;
         if (hipMemcpy(tp, it.dc.bs->tphase, sizeof(tp), hipMemcpyDeviceToHost) == hipSuccess && tp[6] &&
