@@ -44,6 +44,7 @@ extern "C" {
 #define LINAC_METHOD_AUTO     0x00u  /* library picks (blocked when it applies)     */
 #define LINAC_METHOD_PLAIN    0x10u  /* unblocked: one fused step per column        */
 #define LINAC_METHOD_BLOCKED  0x20u  /* panel + exact mod-p MFMA trailing update    */
+#define LINAC_METHOD_MIXED    0x30u  /* float64 only: FP32 trailing updates + refinement */
 #define LINAC_NO_FINALIZE     0x100u /* leave rows unpermuted (pivots_out says where)*/
 #define LINAC_PROFILE         0x200u /* time the call with HIP events (linac_profile_last) */
 #define LINAC_IN_PLACE        0x400u /* batch: work on the caller's device matrix, no copy */
@@ -213,10 +214,25 @@ int linac_rref_c128(double* a, int64_t rows, int64_t cols, int64_t ld, double to
  * linac_refine_last reports this thread's last refined call:
  *   out[0] passes run, out[1] max|correction| / max|X| of the first pass (an
  *   estimate of the unrefined result's relative error), out[2] the same for
- *   the last pass, out[3] the number of non-pivot columns.
- * Writes min(n, 4) values and returns how many were written (zeros after a
+ *   the last pass, out[3] the number of non-pivot columns, out[4] 1 when a
+ *   LINAC_METHOD_MIXED call was redone with FP64 updates (below).
+ * Writes min(n, 5) values and returns how many were written (zeros after a
  * call without LINAC_REFINE). */
 int linac_refine_last(double* out, int n);
+/* LINAC_METHOD_MIXED (linac_rref_f64 only; opt-in): the blocked sweep with its
+ * trailing updates on FP32 matrix cores -- operands rounded to FP32, products
+ * summed in FP32 and added to the FP64 matrix -- while pivot search, row
+ * scaling and the panel factors stay FP64; then refinement passes as with
+ * LINAC_REFINE until a pass corrects less than 2^-51 relative (at most 10,
+ * and no longer once a pass corrects more than half as much as the one
+ * before).  If the correction is still above 2^-51 the call starts again from
+ * the original matrix with FP64 updates and two refinement passes (as
+ * LINAC_METHOD_BLOCKED | LINAC_REFINE), so the result has FP64 accuracy
+ * either way; linac_refine_last reports the passes and whether this
+ * happened.  The FP32 updates perturb the matrix by about 2^-24 relative,
+ * which the pivot test sees: intended for full-rank systems with cond(A) well
+ * below 2^24, where it needs no rerun.  No counterpart in the reference (its
+ * float path is FP64 throughout, linac/row_reduce.cu:138-196). */
 
 /* ---- resumable float RREF (blocked method, device matrix) ----
  * The same elimination as linac_rref_f64 / linac_rref_c128 with

@@ -2,7 +2,7 @@
 
 Drop-in names (``linac/__init__.py:1-24``): ``cuda_row_reduce``,
 ``cuda_load_matrix``, ``load_matrices``, ``CUDA_FOUND``, ``GALOIS_FOUND``,
-``timeit``.  The compute runs in ``liblinac_amd.so`` (HIP C ABI,
+``timeit``, ``iterative_gaussian_solver``.  The compute runs in ``liblinac_amd.so`` (HIP C ABI,
 ``include/linac_amd.h``) through ctypes.
 """
 from ._optional import CUDA_FOUND, GALOIS_FOUND, GPU_FOUND
@@ -11,6 +11,7 @@ from .hip_matrix_loader import cuda_load_matrix, load_matrices, load_matrices_st
 from .batch import Batch, rref_batch
 from .timeit_decorator import timeit
 from .linear_algebra_tools import pivot_columns_from_row_reduced_echelon_form, drop_bottom_zero_rows
+from .linear_system_solver import iterative_gaussian_solver
 
 __version__ = "0.1.0"
 
@@ -30,4 +31,5 @@ __all__ = [
     "timeit",
     "pivot_columns_from_row_reduced_echelon_form",
     "drop_bottom_zero_rows",
+    "iterative_gaussian_solver",
 ]

@@ -29,7 +29,8 @@ NO_FINALIZE = 0x100
 PROFILE = 0x200
 REFINE = 0x800
 
-METHODS = {"auto": METHOD_AUTO, "plain": METHOD_PLAIN, "blocked": METHOD_BLOCKED}
+METHOD_MIXED = 0x30  # float64: FP32 trailing updates + refinement
+METHODS = {"auto": METHOD_AUTO, "plain": METHOD_PLAIN, "blocked": METHOD_BLOCKED, "mixed": METHOD_MIXED}
 
 _lock = threading.Lock()
 _lib = None
@@ -182,9 +183,9 @@ def profile_last():
 
 def refine_last():
     """Statistics of this thread's last LINAC_REFINE call (see linac_refine_last)."""
-    buf = (ctypes.c_double * 4)()
-    n = load().linac_refine_last(buf, 4)
-    keys = ["passes", "first_correction", "last_correction", "solution_columns"]
+    buf = (ctypes.c_double * 5)()
+    n = load().linac_refine_last(buf, 5)
+    keys = ["passes", "first_correction", "last_correction", "solution_columns", "fallback"]
     return dict(zip(keys[:n], list(buf)[:n]))
 
 

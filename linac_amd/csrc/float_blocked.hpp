@@ -56,6 +56,7 @@ struct FloatSweep {
     int64_t npanels = 0, next = 0;
     DeviceContext* ctx = nullptr;
     FloatFactors* keep = nullptr;
+    bool f32_update = false;      // the trailing updates on FP32 matrix cores (the mixed method)
     // Returns 1 when the cooperative panel grid cannot be resident on this
     // device (the caller then takes the plain method).  Workspace: `ws` (at
     // least workspace_bytes(N, M)) when given, else the context's `slot`.

@@ -40,6 +40,7 @@ struct RefineStats {
     double first_correction = 0;  // max |correction| / max |X| of the first pass
     double last_correction = 0;   // ... of the last pass
     double solution_columns = 0;
+    double fallback = 0;          // 1: a mixed call's FP32 run did not refine to FP64 accuracy and was redone in FP64
 };
 RefineStats& thread_refine_stats();
 
@@ -47,10 +48,13 @@ RefineStats& thread_refine_stats();
 // final gather: pivot rows in place, normalised).  Aorig: the original
 // matrix (unscaled), same shape, row stride ldo.  piv_rows / piv_cols: the
 // device pivot lists (rank entries, in column order); pc_host: the pivot
-// columns on the host.  Runs `passes` passes on stream s.
+// columns on the host.  Runs up to `passes` passes on stream s; with
+// stop_below > 0 it stops after a pass whose max |correction| / max |X| is at
+// most stop_below, or that corrected more than half as much as the pass before
+// (the iteration is not converging).
 template <class T>
 int float_refine(T* A, int64_t ld, const T* Aorig, int64_t ldo, int64_t N, int64_t M, const int64_t* piv_rows,
                  const int64_t* piv_cols, const std::vector<int64_t>& pc_host, const FloatFactors& fac, int passes,
-                 hipStream_t s, DeviceContext* ctx);
+                 hipStream_t s, DeviceContext* ctx, double stop_below = 0.0);
 
 }  // namespace linac

@@ -27,10 +27,14 @@ namespace linac {
 template <class T>
 int float_blocked_run(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStream_t s, int64_t* rank,
                       uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, Profiler* prof,
-                      FloatFactors* keep);
+                      FloatFactors* keep, bool f32_update);
 
-// Refinement passes run with LINAC_REFINE.
+// Refinement passes run with LINAC_REFINE; the mixed method refines until a
+// pass corrects less than MIXED_STOP (relative to max |X|), at most
+// MIXED_PASSES times.
 constexpr int REFINE_PASSES = 2;
+constexpr int MIXED_PASSES = 10;
+constexpr double MIXED_STOP = 0x1p-51;
 
 struct FloatState {
     int64_t prow;
@@ -184,9 +188,13 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
     int64_t* piv_cols = (int64_t*)((char*)p + 256 + round_up(cap * 8, 256));
     uint8_t* used = (uint8_t*)((char*)p + 256 + 2 * round_up(cap * 8, 256));
     const uint32_t method = flags & LINAC_METHOD_MASK;
-    if (method != LINAC_METHOD_PLAIN && method != LINAC_METHOD_AUTO && method != LINAC_METHOD_BLOCKED)
+    if (method != LINAC_METHOD_PLAIN && method != LINAC_METHOD_AUTO && method != LINAC_METHOD_BLOCKED &&
+        method != LINAC_METHOD_MIXED)
         return fail(LINAC_ERR_INVALID, "unknown method flag 0x%x", method);
-    const bool refine = (flags & LINAC_REFINE) != 0;
+    const bool mixed = method == LINAC_METHOD_MIXED;
+    if (mixed && !std::is_same<T, double>::value)
+        return fail(LINAC_ERR_INVALID, "the mixed method is for float64 matrices");
+    const bool refine = (flags & LINAC_REFINE) != 0 || mixed;
     if (refine && method == LINAC_METHOD_PLAIN)
         return fail(LINAC_ERR_INVALID, "LINAC_REFINE needs the blocked float method (its panel factors)");
     T* Aorig = nullptr;
@@ -208,7 +216,7 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
     FloatFactors fac;
     if (blocked) {
         const int rc = float_blocked_run<T>(A, lda, rows, cols, tol, s, &st->rank, used, piv_rows, piv_cols, ctx, prof,
-                                            refine ? &fac : nullptr);
+                                            refine ? &fac : nullptr, mixed);
         if (rc < 0) return rc;
         if (rc == 1) {
             if (method == LINAC_METHOD_BLOCKED || refine)
@@ -240,7 +248,28 @@ static int rref_float_entry(T* a, int64_t rows, int64_t cols, int64_t ld, double
         LINAC_HIP(hipStreamSynchronize(s));
         std::vector<int64_t> pcs(rk);
         if (rk) LINAC_HIP(hipMemcpy(pcs.data(), piv_cols, rk * 8, hipMemcpyDeviceToHost));
-        LINAC_TRY(float_refine<T>(A, lda, Aorig, lda, rows, cols, piv_rows, piv_cols, pcs, fac, REFINE_PASSES, s, ctx));
+        LINAC_TRY(float_refine<T>(A, lda, Aorig, lda, rows, cols, piv_rows, piv_cols, pcs, fac,
+                                  mixed ? MIXED_PASSES : REFINE_PASSES, s, ctx, mixed ? MIXED_STOP : 0.0));
+        if (mixed && thread_refine_stats().last_correction > MIXED_STOP) {
+            // the FP32 updates left the refinement short of FP64 accuracy
+            // (cond(A) too close to 2^24): the elimination again from the
+            // original matrix with FP64 updates, then the passes of LINAC_REFINE
+            LINAC_HIP(hipMemcpy2DAsync(A, lda * sizeof(T), Aorig, lda * sizeof(T), cols * sizeof(T), rows,
+                                       hipMemcpyDeviceToDevice, s));
+            k_row_scale<T><<<(unsigned)rows, 256, 0, s>>>(A, lda, rows, cols);
+            k_float_init<<<(unsigned)std::min<int64_t>(1024, ceil_div(rows, 256)), 256, 0, s>>>(st, used, rows);
+            fac = FloatFactors{};
+            const int rc = float_blocked_run<T>(A, lda, rows, cols, tol, s, &st->rank, used, piv_rows, piv_cols, ctx,
+                                                nullptr, &fac, false);
+            if (rc != LINAC_OK) return rc < 0 ? rc : fail(LINAC_ERR_DEVICE, "FP64 rerun of a mixed call failed");
+            LINAC_HIP(hipMemcpyAsync(&rk, &st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            LINAC_HIP(hipStreamSynchronize(s));
+            pcs.assign((size_t)rk, 0);
+            if (rk) LINAC_HIP(hipMemcpy(pcs.data(), piv_cols, rk * 8, hipMemcpyDeviceToHost));
+            LINAC_TRY(float_refine<T>(A, lda, Aorig, lda, rows, cols, piv_rows, piv_cols, pcs, fac, REFINE_PASSES, s,
+                                      ctx));
+            thread_refine_stats().fallback = 1;
+        }
     } else {
         thread_refine_stats() = RefineStats{};
     }

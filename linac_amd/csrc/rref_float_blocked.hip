@@ -31,6 +31,7 @@
 //
 // No host synchronisation between panels: the pivot count of a panel stays on
 // the device and sizes the GEMM's depth there.
+#include <type_traits>
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -379,6 +380,7 @@ __global__ __launch_bounds__(256) void k_fgather(const T* __restrict__ A, int64_
 
 constexpr int FG_M = 64, FG_N = 64, FG_KC = 16, FG_T = 256;
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 struct FGemmArgs {
     const double* L;   // [N][ldl]
@@ -398,9 +400,14 @@ struct FGemmArgs {
     int64_t tiles_main;
 };
 
+// F32: the operands are rounded to FP32 in LDS, the products accumulated by
+// FP32 matrix cores and the sums added to the FP64 matrix (the mixed method).
+template <bool F32>
 __global__ __launch_bounds__(FG_T, 2) void k_fgemm(FGemmArgs a) {
-    __shared__ double sA[2][FG_M][FG_KC + 1];
-    __shared__ double sB[2][FG_KC][FG_N + 1];
+    using E = typename std::conditional<F32, float, double>::type;
+    using V = typename std::conditional<F32, v4f, v4d>::type;
+    __shared__ E sA[2][FG_M][FG_KC + 1];
+    __shared__ E sB[2][FG_KC][FG_N + 1];
     const int k = *a.kpanel;
     if (k == 0) return;
     const int K = a.kreal * k;
@@ -452,15 +459,15 @@ __global__ __launch_bounds__(FG_T, 2) void k_fgemm(FGemmArgs a) {
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) sA[buf][ar][ak + e] = ra[e];
+        for (int e = 0; e < 4; ++e) sA[buf][ar][ak + e] = (E)ra[e];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) sB[buf][bk][bn + e] = rb[e];
+        for (int e = 0; e < 4; ++e) sB[buf][bk][bn + e] = (E)rb[e];
     };
-    v4d acc[2][2];
+    V acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 2; ++j) acc[i][j] = V{0, 0, 0, 0};
     load(0);
     store(0);
     __syncthreads();
@@ -470,7 +477,7 @@ __global__ __launch_bounds__(FG_T, 2) void k_fgemm(FGemmArgs a) {
         if (ch + 1 < nchunks) load(ch + 1);
 #pragma unroll
         for (int ks = 0; ks < FG_KC / 4; ++ks) {
-            double av[2], bv[2];
+            E av[2], bv[2];
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi) av[mi] = sA[buf][wm * 32 + mi * 16 + r16][ks * 4 + kg];
 #pragma unroll
@@ -478,8 +485,12 @@ __global__ __launch_bounds__(FG_T, 2) void k_fgemm(FGemmArgs a) {
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-                for (int ni = 0; ni < 2; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+                for (int ni = 0; ni < 2; ++ni) {
+                    if constexpr (F32)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+                    else
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+                }
         }
         if (ch + 1 < nchunks) {
             store(buf ^ 1);
@@ -491,8 +502,9 @@ __global__ __launch_bounds__(FG_T, 2) void k_fgemm(FGemmArgs a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             // v_mfma_f64_16x16x4f64 output: lane holds D[lane / 16 + 4 v][lane % 16]
-            // (verified on gfx950; the int8 16x16x64 form uses 4 (lane / 16) + v)
-            const int64_t row = m0 + wm * 32 + mi * 16 + kg + 4 * v;
+            // (verified on gfx950); the FP32 16x16x4 form, like the int8
+            // 16x16x64 one, D[4 (lane / 16) + v][lane % 16]
+            const int64_t row = m0 + wm * 32 + mi * 16 + (F32 ? 4 * kg + v : kg + 4 * v);
             if (row >= Mrows) continue;
             double* crow;
             if (qtile) {
@@ -505,8 +517,8 @@ __global__ __launch_bounds__(FG_T, 2) void k_fgemm(FGemmArgs a) {
             for (int ni = 0; ni < 2; ++ni) {
                 const int64_t col = n0 + wn * 32 + ni * 16 + r16;
                 if (col >= a.nTr) continue;
-                if (qtile) crow[col] = acc[mi][ni][v];
-                else crow[col] += acc[mi][ni][v];
+                if (qtile) crow[col] = (double)acc[mi][ni][v];
+                else crow[col] += (double)acc[mi][ni][v];
             }
         }
 }
@@ -666,7 +678,7 @@ int FloatSweep<T>::run(hipStream_t s, int64_t max_panels, Profiler* prof) {
         // 1. Y = Z[Q] A[Q, T] into the pivot rows (one tile row: k <= 64)
         ga.tiles_q = ga.tiles_n;
         ga.tiles_main = 0;
-        k_fgemm<<<(unsigned)ga.tiles_q, FG_T, 0, s>>>(ga);
+        k_fgemm<false><<<(unsigned)ga.tiles_q, FG_T, 0, s>>>(ga);
         // 2. the right operand becomes Y; every other row: A[r, T] -= P[r, pc] Y
         k_fgather<T><<<ggrid, 256, 0, s>>>(A, ld, T0, nT, P.qrows, P.kpanel, BQ);
         ga.tiles_q = 0;
@@ -674,7 +686,8 @@ int FloatSweep<T>::run(hipStream_t s, int64_t max_panels, Profiler* prof) {
         // algorithmic work of the launch: N x nTr x (kReal * k) real MACs; k is
         // known after the sweep, so the sample holds N x nTr until then
         if (prof) LINAC_TRY(prof->sample_begin(s, pi, 16.0 * (double)N * (double)nTr, (double)N * (double)nTr));
-        k_fgemm<<<(unsigned)round_up(ga.tiles_main, 8), FG_T, 0, s>>>(ga);
+        if (f32_update) k_fgemm<true><<<(unsigned)round_up(ga.tiles_main, 8), FG_T, 0, s>>>(ga);
+        else k_fgemm<false><<<(unsigned)round_up(ga.tiles_main, 8), FG_T, 0, s>>>(ga);
         if (prof) LINAC_TRY(prof->sample_end(s));
         LINAC_HIP(hipGetLastError());
     }
@@ -713,10 +726,11 @@ int FloatSweep<T>::finish(hipStream_t s, Profiler* prof) {
 template <class T>
 int float_blocked_run(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStream_t s, int64_t* rank,
                       uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, Profiler* prof,
-                      FloatFactors* keep) {
+                      FloatFactors* keep, bool f32_update) {
     FloatSweep<T> sw;
     const int rc = sw.setup(A, ld, N, M, tol, s, rank, used, piv_rows, piv_cols, ctx, SLOT_PANEL_A, keep);
     if (rc != LINAC_OK) return rc;
+    sw.f32_update = f32_update;
     LINAC_TRY(sw.run(s, -1, prof));
     return sw.finish(s, prof);
 }
@@ -724,8 +738,8 @@ int float_blocked_run(T* A, int64_t ld, int64_t N, int64_t M, double tol, hipStr
 template struct FloatSweep<double>;
 template struct FloatSweep<CplxF>;
 template int float_blocked_run<double>(double*, int64_t, int64_t, int64_t, double, hipStream_t, int64_t*, uint8_t*,
-                                       int64_t*, int64_t*, DeviceContext*, Profiler*, FloatFactors*);
+                                       int64_t*, int64_t*, DeviceContext*, Profiler*, FloatFactors*, bool);
 template int float_blocked_run<CplxF>(CplxF*, int64_t, int64_t, int64_t, double, hipStream_t, int64_t*, uint8_t*,
-                                      int64_t*, int64_t*, DeviceContext*, Profiler*, FloatFactors*);
+                                      int64_t*, int64_t*, DeviceContext*, Profiler*, FloatFactors*, bool);
 
 }  // namespace linac

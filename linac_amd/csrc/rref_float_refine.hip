@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void k_refine_apply(T* __restrict__ A, int64_t
 template <class T>
 int float_refine(T* A, int64_t ld, const T* Aorig, int64_t ldo, int64_t N, int64_t M, const int64_t* piv_rows,
                  const int64_t* piv_cols, const std::vector<int64_t>& pc_host, const FloatFactors& fac, int passes,
-                 hipStream_t s, DeviceContext* ctx) {
+                 hipStream_t s, DeviceContext* ctx, double stop_below) {
     RefineStats& rs = thread_refine_stats();
     rs = RefineStats{};
     const int64_t rank = (int64_t)pc_host.size();
@@ -353,19 +353,21 @@ int float_refine(T* A, int64_t ld, const T* Aorig, int64_t ldo, int64_t N, int64
         __builtin_memcpy(&md, &st[0], 8);
         __builtin_memcpy(&mx, &st[1], 8);
         const double ratio = mx > 0.0 ? md / mx : 0.0;
+        const double prev = rs.last_correction;
         if (pass == 0) rs.first_correction = ratio;
         rs.last_correction = ratio;
         rs.iterations = pass + 1;
+        if (stop_below > 0.0 && (ratio <= stop_below || (pass > 0 && ratio > 0.5 * prev))) break;
     }
     return LINAC_OK;
 }
 
 template int float_refine<double>(double*, int64_t, const double*, int64_t, int64_t, int64_t, const int64_t*,
                                   const int64_t*, const std::vector<int64_t>&, const FloatFactors&, int, hipStream_t,
-                                  DeviceContext*);
+                                  DeviceContext*, double);
 template int float_refine<CplxF>(CplxF*, int64_t, const CplxF*, int64_t, int64_t, int64_t, const int64_t*,
                                  const int64_t*, const std::vector<int64_t>&, const FloatFactors&, int, hipStream_t,
-                                 DeviceContext*);
+                                 DeviceContext*, double);
 
 }  // namespace linac
 
@@ -376,8 +378,8 @@ extern "C" {
 int linac_refine_last(double* out, int n) {
     if (!out || n <= 0) return 0;
     const RefineStats& rs = thread_refine_stats();
-    const double v[4] = {rs.iterations, rs.first_correction, rs.last_correction, rs.solution_columns};
-    const int m = n < 4 ? n : 4;
+    const double v[5] = {rs.iterations, rs.first_correction, rs.last_correction, rs.solution_columns, rs.fallback};
+    const int m = n < 5 ? n : 5;
     for (int i = 0; i < m; ++i) out[i] = v[i];
     return m;
 }

@@ -375,6 +375,9 @@ static int rref_gf_entry(typename F::T* a, int64_t rows, int64_t cols, int64_t l
     if (on_device && ((ld * (int64_t)sizeof(T)) % 16 != 0 || ((uintptr_t)a & 15) != 0))
         return fail(LINAC_ERR_INVALID, "device matrix rows must be 16-byte aligned (ld*%d %% 16 == 0)", (int)sizeof(T));
     const uint32_t method = flags & LINAC_METHOD_MASK;
+    if (method == LINAC_METHOD_MIXED) return fail(LINAC_ERR_INVALID, "the mixed method is for float64 matrices");
+    if (method != LINAC_METHOD_PLAIN && method != LINAC_METHOD_AUTO && method != LINAC_METHOD_BLOCKED)
+        return fail(LINAC_ERR_INVALID, "unknown method flags 0x%x", method);
     Profiler* prof = (flags & LINAC_PROFILE) ? &thread_profiler() : nullptr;
 
     DeviceContext* ctx = nullptr;

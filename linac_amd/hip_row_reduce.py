@@ -97,8 +97,9 @@ def cuda_row_reduce(matrix, field_characteristic=0, verbose=False, _real=None, _
             The characteristic p of the field, or 0 for real and complex matrices.
         verbose (bool, optional, default: False):
             If True, prints additional information.
-        method ("auto" | "plain" | "blocked"):
+        method ("auto" | "plain" | "blocked" | "mixed"):
             Elimination method on the GPU (extension; default picks).
+            "mixed" (float64 only): FP32 trailing updates plus refinement.
         refine (bool):
             Real and complex matrices only: residual-based refinement of the
             non-pivot columns (extension; blocked method).

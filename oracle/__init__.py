@@ -17,4 +17,5 @@ loader    restatement of ``linac/pycuda_matrix_loader.py:127-230`` and of the
           ``LoadMatrix`` kernel's arithmetic (``linac/matrix_loader.cu:88-112``).
 pivots    restatement of ``linac/linear_algebra_tools.py:10-36,48-78`` (pivots, canonical kernels).
 coo       restatement of ``linac/sparse_matrix_tools.py:20-42`` (COO dictionaries).
+solver    restatement of ``linac/linear_system_solver.py:24-79`` (iterative_gaussian_solver).
 """

@@ -184,3 +184,47 @@ def test_float_jobs_with_own_workspace_interleave(gpu):
         assert job.finish() == ref
         torch.cuda.synchronize()
         assert torch.equal(m, one)
+
+
+@pytest.mark.parametrize("n, cols", [(1000, 1001), (2500, 2503), (777, 800)])
+def test_mixed_method_against_lapack(gpu, n, cols):
+    """FP32 trailing updates plus refinement: the pivots of the FP64 path, the
+    identity block exact, and the solution columns within 1e-11 of LAPACK;
+    these sizes converge without the FP64 rerun."""
+    from linac_amd import _lib
+    rs = numpy.random.RandomState(n)
+    A = rs.random_sample((n, cols))
+    R, piv, rk = gpu.rref_with_pivots(A, 0, method="mixed")
+    st = _lib.refine_last()
+    assert piv == list(range(n)) and rk == n and 1 <= st["passes"] <= 10
+    assert st["fallback"] == 0 and st["last_correction"] <= 2.0 ** -51, st  # converged without an FP64 rerun
+    assert float(numpy.max(numpy.abs(R[:, :n] - numpy.eye(n)))) <= 1e-12
+    x = numpy.linalg.solve(A[:, :n], A[:, n:])
+    rel = float(numpy.max(numpy.abs(R[:, n:] - x)) / numpy.max(numpy.abs(x)))
+    assert rel < 1e-11, (rel, st)
+
+
+def test_mixed_method_rejected_outside_float64(gpu):
+    with pytest.raises(ValueError, match="float64"):
+        gpu.rref_with_pivots(numpy.ones((4, 5)) + 1j, 0, method="mixed")
+    with pytest.raises(ValueError, match="float64"):
+        gpu.rref_with_pivots(numpy.ones((4, 5), dtype=numpy.uint32), 7, method="mixed")
+
+
+def test_mixed_method_reruns_in_fp64_when_refinement_stalls(gpu):
+    """cond(A) = 1e9 (singular values log-spaced 1 .. 1e-9): FP32 updates are
+    far too coarse, the refinement stalls, and the call redoes the
+    elimination with FP64 updates -- the result equals the refined FP64 one."""
+    from linac_amd import _lib
+    rs = numpy.random.RandomState(31)
+    n = 600
+    q1, _ = numpy.linalg.qr(rs.standard_normal((n, n)))
+    q2, _ = numpy.linalg.qr(rs.standard_normal((n, n)))
+    A = (q1 * numpy.logspace(0, -9, n)) @ q2.T
+    Ab = numpy.concatenate([A, rs.standard_normal((n, 1))], axis=1)
+    Rm, pm, rm = gpu.rref_with_pivots(Ab, 0, method="mixed")
+    st = _lib.refine_last()
+    Rr, pr, rr = gpu.rref_with_pivots(Ab, 0, method="blocked", refine=True)
+    assert st["fallback"] == 1 and st["passes"] == 2
+    assert pm == pr and rm == rr
+    assert numpy.array_equal(Rm, Rr)

@@ -12,7 +12,10 @@ bits, so its oracle runs in the test.  The FP64 oracle is itself cond(A) * 2^-53
 * unrefined results (blocked and plain) must agree with the FP64 oracle to
   within 16 cond(A) 2^-53, the size of the error both carry;
 * refined results (LINAC_REFINE) must be within 1e-10 of the exact solution
-  and, for the well-conditioned systems, within 1e-10 of the FP64 oracle too.
+  and, for the well-conditioned systems, within 1e-10 of the FP64 oracle too;
+* the mixed method (FP32 trailing updates + refinement, redone with FP64
+  updates when the refinement does not converge) on f64 8000 within 1e-10 of
+  the exact solution.
 Errors are max |x - x_ref| / max |x_ref| over the solution column."""
 import hashlib
 import os
@@ -89,6 +92,19 @@ def test_float_config_against_fp64_oracle(gpu, fixture, kind, n):
     assert report["refined"][1] <= 1e-10, report
     if 16 * cond * U <= 1e-10:
         assert report["refined"][0] <= 1e-10, report
+    if kind == "f64":
+        # the opt-in mixed method: FP32 trailing updates, then refinement;
+        # cond of this system (4.8e8) is beyond what FP32 updates can refine
+        # from, so the call redoes the elimination with FP64 updates
+        R, piv, rank = _solve(Ab, kind, "mixed", False)
+        ms = _lib.refine_last()
+        report["mixed"] = (_rel(R[:, n], x_or), _rel(R[:, n], x_star))
+        print(kind, n, "mixed", tuple(f"{e:.3g}" for e in report["mixed"]), "passes", ms["passes"],
+              "correction", f"{ms['first_correction']:.3g}", f"{ms['last_correction']:.3g}",
+              "fp64 rerun", ms["fallback"])
+        assert rank == n and piv == list(range(n)) and 1 <= ms["passes"] <= 10
+        assert float(numpy.max(numpy.abs(R[:, :n] - numpy.eye(n)))) <= 1e-12
+        assert report["mixed"][1] <= 1e-10, report
 
 
 @pytest.mark.parametrize("complex_", [False, True])
