@@ -75,14 +75,16 @@ def float_footprint(rows, cols):
 
 class Server:
     """GF(p) continuous batch plus interleaved float64 solves under one HBM
-    cap.  ``float_budget_s``: wall time per round given to the float job while
-    GF(p) requests are resident (at least one panel per round); with none
-    resident the float job runs to its end."""
+    cap.  ``float_budget_s``: wall time per round given to the float jobs (at
+    least one panel per round).  ``float_age_s``: once a float request has
+    waited this long its jobs get four times the budget (off by default: at
+    200 requests/s it moved the overall p99 from 0.11 to 0.20-0.27 s, the GF(p)
+    requests paying more than the float ones gained)."""
 
-    def __init__(self, mem_cap_bytes=200 * 10 ** 9, float_budget_s=0.002, float_ratio=None):
+    def __init__(self, mem_cap_bytes=200 * 10 ** 9, float_budget_s=0.002, float_age_s=None):
         self.cap = int(mem_cap_bytes)
         self.float_budget_s = float(float_budget_s)
-        self.float_ratio = float_ratio       # float time per round at most this times the last batch step's
+        self.float_age_s = float_age_s       # a float request older than this gets 4 x the budget per round
         self.last_step_s = 0.0
         self.fjobs = []                      # [request, bytes, FloatJob] of the open float solves
         self.batch = Batch(self.cap)
@@ -138,8 +140,10 @@ class Server:
         self._admit()
         if self.fjobs:
             budget = self.float_budget_s
-            if self.active and self.float_ratio is not None:
-                budget = min(budget, self.float_ratio * self.last_step_s)
+            if self.float_age_s is not None:
+                now = clock()
+                if any(now - r.arrival > self.float_age_s for r, _, _ in self.fjobs):
+                    budget *= 4
             t_end = time.perf_counter() + budget
             while self.fjobs:
                 ent = min(self.fjobs, key=_remaining_work)
@@ -204,14 +208,14 @@ def materialise(req, seed=0):
     return req
 
 
-def serve(requests, mem_cap_bytes=200 * 10 ** 9, seed=0, keep=False, float_budget_s=0.002, float_ratio=None):
+def serve(requests, mem_cap_bytes=200 * 10 ** 9, seed=0, keep=False, float_budget_s=0.002, float_age_s=None):
     """Run timed requests through a Server against the wall clock.
 
     Each request is materialised in HBM when it arrives (not before), then
     submitted.  Returns (results sorted by rid, wall seconds, server stats)."""
     import torch
     reqs = sorted(requests, key=lambda r: r.arrival)
-    srv = Server(mem_cap_bytes, float_budget_s, float_ratio)
+    srv = Server(mem_cap_bytes, float_budget_s, float_age_s)
     results = []
     t0 = time.perf_counter()
 
