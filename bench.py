@@ -46,10 +46,33 @@ def parse():
                     help="instead of the headline line: one JSON line measuring every GF(p) and float configuration")
     ap.add_argument("--with-100k", action="store_true", help="with --configs: include the 100000^2 u32 run (~40 GB)")
     ap.add_argument("--only", default="", help="with --configs: comma list of cases, e.g. u32-20000-blocked,f64-8000")
+    ap.add_argument("--cpu-once", type=int, default=0, metavar="N",
+                    help="instead of the headline line: one timed run of the CPU baseline (numpy restatement of "
+                         "linac.row_reduce, 1 core) on the uniform N x N matrix, with progress lines")
     ap.add_argument("--gemm", type=int, default=0, metavar="N",
                     help="instead of the headline line: the trailing-update kernel alone, C (N x N) -= A (N x 512) "
                          "B (512 x N) mod 2^31-1 (linac_gemm_sub_u32), --steps timed calls")
     return ap.parse_args()
+
+
+def cpu_once_main(args):
+    """One run of the CPU baseline at a size the default protocol skips
+    (minutes per run): RandomState(N).randint(2^31-1) as in the protocol, no
+    warm-up, a progress line every 512 columns."""
+    import numpy
+    from oracle import rref
+    p, n = 2 ** 31 - 1, args.cpu_once
+    A = numpy.random.RandomState(n).randint(p, size=(n, n)).astype(numpy.int64)
+    t0 = time.perf_counter()
+
+    def progress(j):
+        print(f"cpu baseline {n}x{n}: column {j} at {time.perf_counter() - t0:.1f} s", flush=True)
+    R = rref.rref_modp(A, p, progress=progress)
+    dt = time.perf_counter() - t0
+    ok = bool(numpy.array_equal(numpy.diagonal(R), numpy.ones(n, dtype=numpy.int64)))
+    print(json.dumps({"cpu_once": {"n": n, "seconds": round(dt, 2), "runs": 1, "cores": 1, "identity": ok,
+                                   "kind": "port", "host_cpu": _cpu_model(), "os_cpu_count": os.cpu_count(),
+                                   "numpy": numpy.__version__}}), flush=True)
 
 
 def gemm_main(args):
@@ -514,6 +537,9 @@ def configs_main(args):
 
 def main():
     args = parse()
+    if args.cpu_once:
+        cpu_once_main(args)
+        return
     if args.gemm:
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)

@@ -46,8 +46,10 @@ def extended_euclid(a, b):
     return old_s, old_t, old_r
 
 
-def row_reduce(matrix, pivoting=1, scaling=True, reduced_echelon=True, threshold=10 ** -9, prime=None):
-    """Row reduction to (reduced) echelon form; see module docstring."""
+def row_reduce(matrix, pivoting=1, scaling=True, reduced_echelon=True, threshold=10 ** -9, prime=None, progress=None):
+    """Row reduction to (reduced) echelon form; see module docstring.
+    ``progress`` (not in the reference): called with the column index every
+    512 columns, for long timed runs."""
     if prime == 0:
         prime = None
     if pivoting not in [0, 1, 2, 3]:
@@ -101,11 +103,13 @@ def row_reduce(matrix, pivoting=1, scaling=True, reduced_echelon=True, threshold
 
         if prime is not None:
             matrix[:, j:] = matrix[:, j:] % prime
+        if progress is not None and j % 512 == 0:
+            progress(j)
 
     return matrix, variable_order
 
 
-def rref_modp(matrix, prime):
+def rref_modp(matrix, prime, progress=None):
     """The prime path the reference's own callers use.
 
     ``row_reduce(A, pivoting=1, scaling=False, threshold=0, prime=p)`` as at
@@ -119,7 +123,7 @@ def rref_modp(matrix, prime):
     matrix = numpy.asarray(matrix)
     if matrix.dtype != object:
         matrix = numpy.mod(matrix.astype(numpy.int64) if matrix.dtype != numpy.uint64 else matrix, prime).astype(numpy.int64)
-    return row_reduce(matrix, pivoting=1, scaling=False, threshold=0, prime=prime)[0]
+    return row_reduce(matrix, pivoting=1, scaling=False, threshold=0, prime=prime, progress=progress)[0]
 
 
 def rref_float(matrix, threshold=10 ** -9):
