@@ -740,6 +740,9 @@ __device__ __forceinline__ bool batch_tile(const TileSched& ts, int* m_out, int6
     const int64_t end = ts.t1 < 0 ? pre[B] : ts.t1;
     const int64_t per_xcd = (end - ts.t0 + NUM_XCD - 1) / NUM_XCD;
     const int64_t bid = blockIdx.x;
+    // (a grid larger than the schedule -- the fast path's, sized before its
+    // gate drops matrices -- must not reach into the next XCD's range)
+    if (bid / NUM_XCD >= per_xcd) return false;
     const int64_t t = ts.t0 + (bid % NUM_XCD) * per_xcd + bid / NUM_XCD;
     if (t >= end) return false;
     int lo = 0, hi = B;
@@ -1878,6 +1881,26 @@ __global__ __launch_bounds__(64 * FB_WAVES, 4) void k_block_factor(const BMat<Li
     }
 }
 
+// The fast path's kernels are queued before the host knows which candidate
+// blocks reached full rank: this gate gives every other matrix nT = 0 (the
+// fast kernels then skip it) and no tiles in the Z GEMM's schedule
+// (prefix sums pre[0..B] rewritten in place).  One thread.
+template <class LT>
+__global__ __launch_bounds__(64) void k_fast_gate(BMat<LT>* __restrict__ Dw, const BMat<LT>* __restrict__ Dc,
+                                                  int64_t* __restrict__ pre, int B) {
+    if (threadIdx.x != 0) return;
+    int64_t kept = 0, prev = pre[0];
+    for (int m = 0; m < B; ++m) {
+        const int64_t tiles = pre[m + 1] - prev;
+        prev = pre[m + 1];
+        const int bcur = Dw[m].bcur;
+        const bool full = Dw[m].nT > 0 && Dc[m].st->rank == bcur && Dc[m].bs->q0 == bcur;
+        if (!full) Dw[m].nT = 0;
+        kept += full ? tiles : 0;
+        pre[m + 1] = kept;
+    }
+}
+
 // Operands of Z = -P B for the matrices on the fast path (Dw[m].nT > 0, depth
 // Kq = bcur rounded up to 64): workgroups [0, nl) write the limb planes of
 // the panel P = X[:, 0:bcur] (every row, zero beyond bcur) with row stride Kq
@@ -2490,25 +2513,16 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
             else k_gemm_inner<<<igrid, G_T, 0, s>>>(cs.desc, cs.inner, epoch, cs.inner_split);
             ++epoch;
         }
+    // The Z GEMM and the commit are queued for every selected matrix before
+    // the host reads whether its block reached full rank (k_fast_gate drops
+    // the others on the device): the status read-back overlaps these kernels
+    // instead of holding the GPU idle for a host round trip.
+    Staged ws;
+    LINAC_TRY(upload(act, s, 4, 5, &ws));
+    k_fast_gate<LT><<<1, 64, 0, s>>>(ws.desc, cs.desc, const_cast<int64_t*>(ws.main.pre), (int)B);
     k_gather_status<LT><<<B, 1, 0, s>>>(cs.desc, cs.status);
     LINAC_HIP(hipMemcpyAsync(cs.status_h, cs.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
     LINAC_HIP(hipEventRecord(ev_status, s));
-    for (;;) {
-        const hipError_t q = hipEventQuery(ev_status);
-        if (q == hipSuccess) break;
-        if (q != hipErrorNotReady) return fail(LINAC_ERR_DEVICE, "hipEventQuery: %s", hipGetErrorString(q));
-        __builtin_ia32_pause();
-    }
-    int nfast = 0;
-    for (unsigned m = 0; m < B; ++m) {
-        const BM& d = act[m]->d;
-        fast[m] = sel[m] && cs.status_h[m].q0 == d.bcur && cs.status_h[m].rank == d.bcur ? 1 : 0;
-        nfast += fast[m];
-    }
-    if (nfast == 0) return LINAC_OK;
-    sel = fast;
-    Staged ws;
-    LINAC_TRY(upload(act, s, 4, 5, &ws));
     const int nl = (int)gs_grid(maxN * (BO / 4), 16384);
     const int nb = (int)ceil_div((int64_t)BO * BO, 256);
     k_fast_operands<LT><<<dim3(nl + nb, B), 256, 0, s>>>(ws.desc, cs.desc, nl);
@@ -2519,6 +2533,16 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     k_fast_commit<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc, ws.desc, act[0]->d.e0);
     k_fast_finish<LT><<<B, 1024, 0, s>>>(in.desc, cs.desc, ws.desc);
     LINAC_HIP(hipGetLastError());
+    for (;;) {
+        const hipError_t q = hipEventQuery(ev_status);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) return fail(LINAC_ERR_DEVICE, "hipEventQuery: %s", hipGetErrorString(q));
+        __builtin_ia32_pause();
+    }
+    for (unsigned m = 0; m < B; ++m) {
+        const BM& d = act[m]->d;
+        fast[m] = sel[m] && cs.status_h[m].q0 == d.bcur && cs.status_h[m].rank == d.bcur ? 1 : 0;
+    }
     return LINAC_OK;
 }
 
