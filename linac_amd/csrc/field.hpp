@@ -52,6 +52,39 @@ __host__ __device__ __forceinline__ int clz64(uint64_t x) {
 #endif
 }
 
+// a^{-1} mod p (p an odd prime, 0 < a < p) by the binary extended Euclidean
+// algorithm (Knuth, TAOCP vol. 2, 4.5.2, binary gcd with cofactors), under the
+// invariants x1 a = u and x2 a = v (mod p): halve the even one of u, v (and its
+// cofactor, (x + p) / 2 for odd x), subtract the smaller from the larger, until
+// one reaches 1.  At most about 2 log2(p) halvings, each with one subtraction
+// at most.  The control flow depends on the data, so it is meant for a value
+// held alike in every lane (the loop then runs on the scalar unit), where it
+// is several times shorter than the Fermat chain of `inverse`.  a = 0 gives 0.
+template <class T>
+__host__ __device__ inline T binary_inverse(T a, T p) {
+    if (a == 0) return 0;
+    T u = a, v = p, x1 = 1, x2 = 0;
+    const T half = (p >> 1) + 1;  // (x + p) / 2 = (x >> 1) + half for odd x
+    while (u != 1 && v != 1) {
+        while ((u & 1) == 0) {
+            u >>= 1;
+            x1 = (x1 & 1) ? (x1 >> 1) + half : x1 >> 1;
+        }
+        while ((v & 1) == 0) {
+            v >>= 1;
+            x2 = (x2 & 1) ? (x2 >> 1) + half : x2 >> 1;
+        }
+        if (u >= v) {
+            u -= v;
+            x1 = x1 >= x2 ? x1 - x2 : x1 + (p - x2);
+        } else {
+            v -= u;
+            x2 = x2 >= x1 ? x2 - x1 : x2 + (p - x1);
+        }
+    }
+    return u == 1 ? x1 : x2;
+}
+
 struct FieldU32 {
     using T = uint32_t;
     uint32_t p;

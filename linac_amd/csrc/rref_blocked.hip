@@ -228,8 +228,10 @@ __device__ static void block_state_init(const BMat<LT>& d, int reset_timers) {
 static_assert(BI + CI == 64, "one augmented column per lane");
 static_assert((CI & (CI - 1)) == 0 && CI <= 32, "pivot slots index with a mask; the inner GEMM holds k <= 32 in its first two lane groups");
 
-// Candidate rows live in LDS as [panel columns | identity]; wave w owns rows
-// w and w + 16.  (Each column step is a chain of dependent LDS reads and
+// Candidate rows live in LDS as [panel columns | identity] (32-bit storage;
+// 64-bit storage uses a compact form of half the width, described at its
+// elimination loop); wave w owns rows w and w + 16 (w, w + 8, w + 16, w + 24
+// with the eight waves of 64-bit storage).  (Each column step is a chain of dependent LDS reads and
 // products: four waves with eight rows each took twice as long as sixteen
 // with two, which hide each other's latency.)  Per column every wave ballots
 // the column itself (all waves see the same data, so they agree on the pivot
@@ -238,6 +240,15 @@ static_assert((CI & (CI - 1)) == 0 && CI <= 32, "pivot slots index with a mask; 
 // reads of a row precede its writes in program order, so one barrier per
 // column suffices.  No rows are swapped: pivots are recorded by their LDS
 // row, and the identity columns index LDS rows directly.
+
+// A value held alike in every lane, moved to scalar registers (so that code
+// depending on it alone runs on the scalar unit).
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 
 template <class LT>
 __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t N,
@@ -250,8 +261,11 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
     constexpr int IF_T = LT::IF_T;
     constexpr int IF_RPW = CI / (IF_T / 64);  // candidate rows per wave
     static_assert(IF_RPW * (IF_T / 64) == CI && IF_T >= 256, "waves split the candidate rows evenly; SCAN rows per probe");
-    constexpr int W2 = BI + CI;
-    __shared__ T aug[CI][W2 + 1];
+    // the compact form (32 slots a row) for 64-bit storage, [panel | identity]
+    // (64 columns a row) for 32-bit storage; one padding column
+    constexpr bool CMP = sizeof(T) == 8;
+    constexpr int AW = (CMP ? BI : BI + CI) + 1;
+    __shared__ T aug[CI][AW];
     __shared__ int64_t cand[CI + 1];
     __shared__ int prow_of[CI];   // LDS row of the t-th pivot
     __shared__ int pcol[CI];      // its column within the inner panel
@@ -326,7 +340,9 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
         return;
     }
 
-    // 2. [X[S, c0:c0+w] | I] in LDS; rows >= nC are zero and never pivot
+    // 2. the candidate rows in LDS: [X[S, c0:c0+w] | I] in the 32-bit form,
+    //    X[S, c0:c0+w] alone in the compact 64-bit form (below); rows >= nC
+    //    and columns >= w are zero and never pivot
     {
         const int j = lane;
 #pragma unroll
@@ -335,77 +351,151 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
             T v = 0;
             if (ii < nC) {
                 if (j < w) v = X[cand[ii] * XLD + c0 + j];
-                else if (j >= BI) v = (j - BI == ii) ? 1 : 0;
+                else if (!CMP && j >= BI) v = (j - BI == ii) ? 1 : 0;
             }
-            aug[ii][j] = v;
+            if (j < AW - 1) aug[ii][j] = v;
         }
     }
     __syncthreads();
 
     const unsigned long long t2 = wall_clock64();
     // 3. fraction-free Gauss-Jordan: row_i <- (h * row_i - a_i * row_pv) R^{-1}
-    //    (rows stay non-zero multiples of the exact Gauss-Jordan rows)
+    //    (rows stay non-zero multiples of the exact Gauss-Jordan rows).  The
+    //    pivot heads h_t are kept: the pivot of the t-th pivot row ends as
+    //    d_t = h_t prod_{u > t} h_u R^{-1}, which step 4 inverts.
     unsigned long long pmask = 0ull;  // LDS rows already used as pivots (same in every wave)
     int t = 0;
     constexpr int NW = IF_T / 64;  // wave w owns rows w, w + NW, w + 2 NW, ...
-    for (int c = 0; c < w && t < nC; ++c) {
-        // the wave's own rows are read beside the pivot search (all reads of
-        // them come before any write)
-        T av[IF_RPW], xv[IF_RPW];
+    __shared__ T s_h[CI];
+    if constexpr (!CMP) {
+        for (int c = 0; c < w && t < nC; ++c) {
+            // the wave's own rows are read beside the pivot search (all reads
+            // of them come before any write)
+            T av[IF_RPW], xv[IF_RPW];
 #pragma unroll
-        for (int q = 0; q < IF_RPW; ++q) {
-            const int r = wid + NW * q;
-            av[q] = aug[r][c];
-            xv[q] = aug[r][lane];
-        }
-        const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
-        const unsigned long long m = __ballot(nz);
-        if (m == 0ull) continue;  // no pivot in this column (every wave sees the same)
-        const int pv = __ffsll((long long)m) - 1;
-        const T h = aug[pv][c];
-        const T y = aug[pv][lane];
+            for (int q = 0; q < IF_RPW; ++q) {
+                const int r = wid + NW * q;
+                av[q] = aug[r][c];
+                xv[q] = aug[r][lane];
+            }
+            const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
+            const unsigned long long m = __ballot(nz);
+            if (m == 0ull) continue;  // no pivot in this column (every wave sees the same)
+            const int pv = __ffsll((long long)m) - 1;
+            const T h = aug[pv][c];
+            const T y = aug[pv][lane];
 #pragma unroll
-        for (int q = 0; q < IF_RPW; ++q) {
-            const int r = wid + NW * q;
-            if (r < nC && r != pv) aug[r][lane] = f.ff_update(h, xv[q], av[q], y);
+            for (int q = 0; q < IF_RPW; ++q) {
+                const int r = wid + NW * q;
+                if (r < nC && r != pv) aug[r][lane] = f.ff_update(h, xv[q], av[q], y);
+            }
+            if (tid == 0) {
+                prow_of[t] = pv;
+                pcol[t] = c;
+                s_h[t] = h;
+            }
+            pmask |= 1ull << pv;
+            ++t;
+            __syncthreads();
         }
-        if (tid == 0) {
-            prow_of[t] = pv;
-            pcol[t] = c;
+    } else {
+        // Compact form: 32 slots per row, two rows per wave instruction (lanes
+        // 32 hf + j, slot j).  Slot j holds panel column j until column j is
+        // eliminated; from then on it holds the identity-part column of the
+        // row that pivoted there, scaled by a per-column constant.  Before step
+        // c every row that is not yet a pivot carries D_c = prod_{u < t} h_u
+        // R^{-1} on its own identity column and nothing on the others', so
+        // the identity part has at most t live columns besides that one
+        // diagonal, and t <= c slots are free to hold them.  Step c with pivot
+        // row pv writes the pivot's identity column, (0 h - a_i D_c) R^{-1} in
+        // row i and D_c in row pv, as -a_i and R (mod p) with the scale
+        // s_c = D_c R^{-1}, into slot c, where the panel column has just become
+        // zero (h in row pv).  Half the lanes of the 64-wide form, whose
+        // 64-bit products run at a quarter of the VALU rate.  The write into
+        // row pv waits until after the barrier (other waves read h there in
+        // this step); its owner lane makes it before it reads the row again.
+        const T one_m = f.to_mont((T)1);
+        const int hf = lane >> 5, j = lane & 31;
+        constexpr int RP = IF_RPW / 2;  // row pairs per wave: rows wid + NW (2 q + hf)
+        int pend = -1;                  // LDS row whose slot pend_c takes R (mod p)
+        int pend_c = 0;
+        for (int c = 0; c < w && t < nC; ++c) {
+            if (pend >= 0 && wid == pend % NW && lane == 32 * ((pend / NW) & 1) + pend_c) aug[pend][pend_c] = one_m;
+            pend = -1;
+            T av[RP], xv[RP];
+#pragma unroll
+            for (int q = 0; q < RP; ++q) {
+                const int r = wid + NW * (2 * q + hf);
+                av[q] = aug[r][c];
+                xv[q] = aug[r][j];
+            }
+            const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
+            const unsigned long long m = __ballot(nz);
+            if (m == 0ull) continue;
+            const int pv = __ffsll((long long)m) - 1;
+            const T h = aug[pv][c];
+            const T y = aug[pv][j];
+#pragma unroll
+            for (int q = 0; q < RP; ++q) {
+                const int r = wid + NW * (2 * q + hf);
+                if (r < nC && r != pv)
+                    aug[r][j] = j == c ? (av[q] != (T)0 ? f.p - av[q] : (T)0) : f.ff_update(h, xv[q], av[q], y);
+            }
+            if (tid == 0) {
+                prow_of[t] = pv;
+                pcol[t] = c;
+                s_h[t] = h;
+            }
+            pmask |= 1ull << pv;
+            pend = pv;
+            pend_c = c;
+            ++t;
+            __syncthreads();
         }
-        pmask |= 1ull << pv;
-        ++t;
-        __syncthreads();
+        if (pend >= 0 && wid == pend % NW && lane == 32 * ((pend / NW) & 1) + pend_c) aug[pend][pend_c] = one_m;
     }
     const int k = t;
     __syncthreads();
     const unsigned long long t3 = wall_clock64();
 
-    // 4. normalise the pivot rows.  The k inverses run side by side in the
-    //    lanes of wave 0 (one Fermat chain in total instead of one per wave,
-    //    which would serialise on the SIMDs), then each wave scales its rows.
-    __shared__ T s_invm[CI];
-    __shared__ int s_tof[CI];
-    if (wid == 0 && lane < k) {
-        const int r = prow_of[lane];
-        s_invm[lane] = f.to_mont(f.inverse(aug[r][pcol[lane]]));
-        s_tof[r] = lane;
-    }
-    __syncthreads();
+    // 4. the pivots' inverses from one inversion: with P_t = prod_{u < t} h_u
+    //    R^{-(t-1)} (P_0 = R), an exclusive product scan of the heads over the
+    //    lanes of wave 0, d_t^{-1} = P_t R^{-1} / P_k.  P_k is the same in
+    //    every lane, so its inverse runs once, on the scalar unit.
+    //    s_inv[t] = d_t^{-1} R, s_pre[t] = P_t.
+    __shared__ T s_inv[CI], s_pre[CI];
+    if (wid == 0 && k > 0) {
+        const T one_m = f.to_mont((T)1);
+        T v = lane < k ? s_h[lane] : one_m;
 #pragma unroll
-    for (int q = 0; q < IF_RPW; ++q) {
-        const int r = wid + NW * q;
-        if ((pmask >> r) & 1ull) aug[r][lane] = f.mont(s_invm[s_tof[r]], aug[r][lane]);
+        for (int off = 1; off < CI; off <<= 1) {
+            const T o = __shfl_up(v, off, 64);
+            if (lane >= off) v = f.mont(o, v);
+        }
+        T ex = __shfl_up(v, 1, 64);
+        if (lane == 0) ex = one_m;
+        const T tot = uniform(__shfl(v, k - 1, 64));
+        const T x = f.to_mont(binary_inverse<T>(tot, f.p));
+        if (lane < k) {
+            s_pre[lane] = ex;
+            s_inv[lane] = f.mont(ex, x);
+        }
     }
     __syncthreads();
 
     const unsigned long long t4 = wall_clock64();
-    // 5. outputs: Apiv^{-1} (Montgomery form), pivot rows/columns, bookkeeping
+    // 5. outputs: Apiv^{-1} (Montgomery form), pivot rows/columns, bookkeeping.
+    //    Entry (a, b) is the identity-part entry of pivot row a in column b
+    //    over d_a (times R for the Montgomery form); the compact form also
+    //    applies the column's scale s_b = P_b R^{-2}.
     const int64_t rank0 = st->rank;
     const int q0 = bs->q0;
     for (int x = tid; x < k * k; x += IF_T) {
         const int a = x / k, b = x - a * k;
-        bs->apinv[a * CI + b] = f.to_mont(aug[prow_of[a]][BI + prow_of[b]]);
+        T out;
+        if constexpr (CMP) out = f.mont(f.mont(aug[prow_of[a]][pcol[b]], s_pre[b]), s_inv[a]);
+        else out = f.to_mont(f.mont(s_inv[a], aug[prow_of[a]][BI + prow_of[b]]));
+        bs->apinv[a * CI + b] = out;
     }
     if (tid < k) {
         const int64_t r = cand[prow_of[tid]];
@@ -2810,13 +2900,18 @@ static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, cons
     std::vector<Item<LT>*> items{&it};
     // state and row flags were initialised by the caller; epochs here
     k_epoch_init<<<gs_grid(N), 256, 0, s>>>(it.d.epoch_of, N);
+    const bool timing = getenv("LINAC_DEBUG_TIMING") != nullptr;
+    if (timing) {  // the phase counters live in reused scratch
+        LINAC_HIP(hipMemsetAsync(it.d.bs->tphase, 0, sizeof(it.d.bs->tphase), s));
+        LINAC_HIP(hipMemsetAsync(it.dc.bs->tphase, 0, sizeof(it.dc.bs->tphase), s));
+    }
     int rc = LINAC_OK;
     while (!it.finished && rc == LINAC_OK) rc = eng.step(items, s, prof);
     const int rd = eng.drain();
     LINAC_TRY(rc);
     LINAC_TRY(rd);
     LINAC_HIP(hipStreamSynchronize(s));
-    if (getenv("LINAC_DEBUG_TIMING")) {
+    if (timing) {
         print_timers(it.d, ctx);
         print_timers(it.dc, ctx);  // inner steps on the candidate blocks of full-rank panels
         unsigned long long tp[8];

@@ -8,6 +8,7 @@
 //   U64 p h x a y          FieldU64::ff_update   (h x - a y) 2^-64 mod p
 //   M32 p a b / M64 p a b  mul
 //   I32 p h / I64 p h      inverse
+//   B32 p h / B64 p h      binary_inverse
 //   D32 c / D64 c          balanced base-256 digits of a centred value
 #include <cstdio>
 #include <cstring>
@@ -40,6 +41,11 @@ int main() {
             in >> p >> c;
             for (auto& v : s) in >> v;
             std::cout << fold_sub64(c, s, p, 1.0 / (double)p) << "\n";
+        } else if (op == "B32" || op == "B64") {
+            uint64_t p, a;
+            in >> p >> a;
+            if (op == "B32") std::cout << binary_inverse<uint32_t>((uint32_t)a, (uint32_t)p) << "\n";
+            else std::cout << binary_inverse<uint64_t>(a, p) << "\n";
         } else if (op == "U32" || op == "M32" || op == "I32") {
             uint64_t p, a, b, c, d;
             in >> p >> a;

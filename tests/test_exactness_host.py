@@ -9,7 +9,8 @@ field.hpp, compiled for the host) against Python integers.
 * fold_sub_k32 (inner GEMM, K <= 32) likewise at |S_s| <= 2^21.
 * fold_sub64 (8 digits, 15 diagonals, K = 512, |S_s| <= 2^26) for primes
   from just above 2^32 to just below 2^63, on random sign patterns.
-* ff_update, mul, inverse (Montgomery, R = 2^32 / 2^64) on boundary operands.
+* ff_update, mul, inverse (Montgomery, R = 2^32 / 2^64) and binary_inverse
+  (binary extended Euclid) on boundary operands.
 * balanced digits of centred values at the ends of their ranges.
 
 These replace the reference's arbitrary-precision modular arithmetic
@@ -144,6 +145,14 @@ def test_montgomery_update_mul_inverse(harness, bits):
         for h in ops:
             if h:
                 lines.append(f"I{bits} {p} {h}")
+                want.append(pow(h, -1, p))
+                lines.append(f"B{bits} {p} {h}")
+                want.append(pow(h, -1, p))
+        # the binary inverse's control flow depends on the value: powers of
+        # two (long halving runs), values next to p and to its half
+        for h in [2 ** e for e in range(1, bits) if 2 ** e < p] + [p - 3, (p - 1) // 2 - 1, (p + 1) // 2 + 1] + [rng.randrange(1, p) for _ in range(40)]:
+            if 0 < h < p:
+                lines.append(f"B{bits} {p} {h}")
                 want.append(pow(h, -1, p))
     got = harness(lines)
     bad = [(ln, g, w) for ln, g, w in zip(lines, got, want) if int(g) != w]
