@@ -96,6 +96,7 @@ struct BlockState {
     int32_t c0_cur;      // first column of the last inner step
     int32_t bcur;        // width of the current outer panel
     int32_t pad_;
+    int64_t lo0;         // first unused row when the outer panel was loaded
     int64_t R[CI];       // pivot rows of the last inner step (pivot order)
     int32_t Cp[CI];      // their columns relative to the inner panel start
     T apinv[CI * CI];    // Montgomery form of (X[R, c0 + Cp])^{-1}
@@ -146,6 +147,8 @@ struct BMat {
     int32_t full;       // every row is a pivot row: update every trailing column now, then stop
     int64_t rank_start; // index of its first pivot in piv_rows (the deferred panel's kA precede it)
     int64_t T0, nT;     // trailing columns [T0, T0 + nT)
+    int64_t r0;         // first row of the trailing update, the panel write-back and the left operand
+                        // (a multiple of 64; every row above it is a pivot row of an earlier panel)
     int64_t* cand;      // [BO] rows of the compact candidate block (full-rank panel fast path)
 };
 
@@ -171,7 +174,10 @@ __global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__
     constexpr int PER = 16 / (int)sizeof(T);
     constexpr int VPR = XLD / PER;  // vectors per panel-buffer row
     const BMat<LT>& d = D[blockIdx.y];
-    if (blockIdx.x == 0 && threadIdx.x == 0) block_state_init<LT>(d, reset_timers);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        block_state_init<LT>(d, reset_timers);
+        d.bs->lo0 = d.st->first_unused;  // no pivot of this panel lies above it
+    }
     const int64_t total = d.N * VPR;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = x / VPR;
@@ -201,7 +207,8 @@ __global__ __launch_bounds__(256) void k_panel_store(const BMat<LT>* __restrict_
     const BMat<LT>& d = D[blockIdx.y];
     const int j = threadIdx.x;
     if (j >= d.bcur) return;
-    for (int64_t r = blockIdx.x; r < d.N; r += gridDim.x) d.A[r * d.ld + d.J + j] = d.X[r * XLD + j];
+    // (rows above r0 keep the values they had: the trailing update skips them too)
+    for (int64_t r = d.r0 + blockIdx.x; r < d.N; r += gridDim.x) d.A[r * d.ld + d.J + j] = d.X[r * XLD + j];
 }
 
 __global__ __launch_bounds__(256) void k_epoch_init(int32_t* epoch, int64_t N) {
@@ -668,7 +675,7 @@ __device__ static void make_lhs_body(const BMat<LT>& d, int src, int e1, int64_t
     const int t = (threadIdx.x & (BO / 4 - 1)) * 4;
     if (t >= kw) return;
     constexpr int RPB = 256 / (BO / 4);  // rows per block and pass
-    for (int64_t m = bx * RPB + threadIdx.x / (BO / 4); m < d.N; m += nbx * RPB) {
+    for (int64_t m = d.r0 + bx * RPB + threadIdx.x / (BO / 4); m < d.N; m += nbx * RPB) {
         int8_t dg[NL][4] = {};
         if (t < width) {
             T w[4];
@@ -880,7 +887,7 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
     const int64_t planeB = JOB == 0 ? d.strideB : STRIDE_W;
     const int Kpad = JOB == 0 ? d.Kq : d.Kq - d.KqA;
     const double pd = d.pd, pinv = d.pinv;
-    const int64_t m_base = tm * G_M, n_base = tn * G_N;
+    const int64_t m_base = d.r0 + tm * G_M, n_base = tn * G_N;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
@@ -1036,7 +1043,7 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer_lds(const BMat<LimbsU32>*
     const int64_t planeB = JOB == 0 ? d.strideB : STRIDE_W;
     const int Kpad = JOB == 0 ? d.Kq : d.Kq - d.KqA;
     const double pd = d.pd, pinv = d.pinv;
-    const int64_t m_base = tm * G_M, n_base = tn * G_N;
+    const int64_t m_base = d.r0 + tm * G_M, n_base = tn * G_N;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
@@ -1357,7 +1364,7 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
     const int Kpad = JOB == 0 ? d.Kq : d.Kq - d.KqA;
     const uint64_t p = d.f.p;
     const double pinv = d.pinv;
-    const int64_t m_base = tm * G_M, n_base = tn * TN64;
+    const int64_t m_base = d.r0 + tm * G_M, n_base = tn * TN64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
@@ -1556,6 +1563,7 @@ __global__ __launch_bounds__(256) void k_item_init(const BMat<LT>* __restrict__ 
 struct ItemStatus {
     int32_t c0_next, bcur, q0, pad;
     int64_t rank;
+    int64_t lo;  // first unused row when the outer panel was loaded
 };
 
 // Ranks, then pivot lists, of many finished matrices in one launch each
@@ -1576,8 +1584,10 @@ __global__ __launch_bounds__(256) void k_collect_pivots(const BMat<LT>* __restri
     }
 }
 
+// (Dl: the descriptors whose panel-load snapshot gives `lo`; null: D's own)
 template <class LT>
-__global__ void k_gather_status(const BMat<LT>* __restrict__ D, ItemStatus* out) {
+__global__ void k_gather_status(const BMat<LT>* __restrict__ D, ItemStatus* out,
+                                const BMat<LT>* __restrict__ Dl = nullptr) {
     const BMat<LT>& d = D[blockIdx.x];
     ItemStatus s;
     s.c0_next = d.bs->c0_next;
@@ -1585,6 +1595,7 @@ __global__ void k_gather_status(const BMat<LT>* __restrict__ D, ItemStatus* out)
     s.q0 = d.bs->q0;
     s.pad = 0;
     s.rank = d.st->rank;
+    s.lo = (Dl ? Dl[blockIdx.x] : d).bs->lo0;
     out[blockIdx.x] = s;
 }
 
@@ -1993,7 +2004,7 @@ __global__ __launch_bounds__(64) void k_fast_gate(BMat<LT>* __restrict__ Dw, con
 
 // Operands of Z = -P B for the matrices on the fast path (Dw[m].nT > 0, depth
 // Kq = bcur rounded up to 64): workgroups [0, nl) write the limb planes of
-// the panel P = X[:, 0:bcur] (every row, zero beyond bcur) with row stride Kq
+// the panel P = X[:, 0:bcur] (every row from r0 on, zero beyond bcur) with row stride Kq
 // into planesL1; the rest those of B^T, B[c, s] = M^-1[slot of column c, s]
 // taken from the factorised compact block, into planesB1 (row s, stride Kq).
 template <class LT>
@@ -2005,10 +2016,10 @@ __global__ __launch_bounds__(256) void k_fast_operands(const BMat<LT>* __restric
     const int Kq = d.Kq, bcur = d.bcur;
     const int64_t bx = blockIdx.x;
     if (bx < nl) {
-        const int64_t total = d.N * (Kq / 4);
+        const int64_t total = (d.N - d.r0) * (Kq / 4);
         for (int64_t x = bx * 256 + threadIdx.x; x < total; x += (int64_t)nl * 256) {
-            const int64_t r = x / (Kq / 4);
-            const int c = (int)(x - r * (Kq / 4)) * 4;
+            const int64_t i = x / (Kq / 4), r = d.r0 + i;
+            const int c = (int)(x - i * (Kq / 4)) * 4;
             int8_t dg[NL][4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -2038,7 +2049,7 @@ __global__ __launch_bounds__(256) void k_fast_operands(const BMat<LT>* __restric
     }
 }
 
-// Panel columns of every row of a fast-path matrix set to zero (the pivot
+// Panel columns of every row from r0 on of a fast-path matrix set to zero (the pivot
 // rows get theirs back from the block in k_fast_commit).
 template <class LT>
 __global__ __launch_bounds__(256) void k_fast_zero(const BMat<LT>* __restrict__ Dw) {
@@ -2048,9 +2059,9 @@ __global__ __launch_bounds__(256) void k_fast_zero(const BMat<LT>* __restrict__ 
     constexpr int PER = 16 / (int)sizeof(T);
     const int vpr = (int)ceil_div(d.bcur, PER);
     const v4i z = {0, 0, 0, 0};
-    for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < d.N * vpr; x += (int64_t)gridDim.x * 256) {
+    for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < (d.N - d.r0) * vpr; x += (int64_t)gridDim.x * 256) {
         const int64_t r = x / vpr;
-        gst<v4i>(d.X + r * XLD + (x - r * vpr) * PER, z);
+        gst<v4i>(d.X + (d.r0 + r) * XLD + (x - r * vpr) * PER, z);
     }
 }
 
@@ -2226,6 +2237,12 @@ struct Item {
     bool finished = false;
     bool taken = false;
     int64_t id = -1;
+    // Forward elimination: each panel's updates start at row r0 (a panel that
+    // composes with a deferred one keeps that one's r0); the panels that
+    // skipped rows go to `echelon` for the back substitution.
+    bool forward = false;
+    int64_t r0 = 0;
+    std::vector<EchelonPanel> echelon;
 };
 
 template <class LT>
@@ -2295,6 +2312,7 @@ class Engine {
     int upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out, int nparts = 1);
     // parts 4 (compact block views) and 5 (fast-path GEMM Z = -P B): which items take part
     std::vector<char> sel;
+    std::vector<int64_t> lo_fast;  // first unused row at the panel load, read with the fast path's status
     int fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, int64_t ninner, int64_t maxN,
                    std::vector<char>& fast);
     void fill(const std::vector<It*>& act, int part, char* h, char* dv, size_t desc_b, size_t sched_b, Staged* out);
@@ -2452,6 +2470,7 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
             c.J = d.J;
             c.bcur = d.bcur;
             c.e0 = d.e0;
+            c.r0 = 0;
             d = c;
         } else if (part == 5) {  // Z = -P B into the panel buffer's Z columns (nT = 0: not fast)
             d.A = d.X + BO;
@@ -2489,8 +2508,8 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
             }
             const int64_t K = d.kA + d.q0;
             if (K > 0 && d.nT > 0) {
-                out->macs += (double)(d.N - K) * K * d.nT;
-                out->bytes += 2.0 * sizeof(typename LT::T) * d.N * d.nT;
+                out->macs += (double)std::max<int64_t>(0, d.N - d.r0 - K) * K * d.nT;
+                out->bytes += 2.0 * sizeof(typename LT::T) * (d.N - d.r0) * d.nT;
                 out->cols = std::max(out->cols, d.nT);
             }
         }
@@ -2518,12 +2537,14 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
         ma[m] = ceil_div(d.N, G_M);
         // inner GEMM: split workgroups per 64-row stripe
         pa[m + 1] = pa[m] + ((part == 0 || part == 4) ? ma[m] * split : 0);
-        const bool live = part != 0 && part != 4 && d.Kq > 0 && d.nT > 0;
-        const bool wlive = part == 1 && d.kA > 0 && d.q0 > 0 && d.nT > 0;
-        mb[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
-        pb[m + 1] = pb[m] + (wlive ? ceil_div(d.N, G_M) * ceil_div(d.kA, TN) : 0);
-        mc[m] = std::max<int64_t>(1, ceil_div(d.N, G_M));
-        pc[m + 1] = pc[m] + (live ? ceil_div(d.N, G_M) * ceil_div(d.nT, TN) : 0);
+        // (the trailing and composite GEMMs cover rows [r0, N))
+        const int64_t rt = ceil_div(d.N - d.r0, G_M);
+        const bool live = part != 0 && part != 4 && d.Kq > 0 && d.nT > 0 && rt > 0;
+        const bool wlive = part == 1 && d.kA > 0 && d.q0 > 0 && d.nT > 0 && rt > 0;
+        mb[m] = std::max<int64_t>(1, rt);
+        pb[m + 1] = pb[m] + (wlive ? rt * ceil_div(d.kA, TN) : 0);
+        mc[m] = std::max<int64_t>(1, rt);
+        pc[m + 1] = pc[m] + (live ? rt * ceil_div(d.nT, TN) : 0);
     }
     // direct-to-LDS staging: 16-byte aligned C rows, operand offsets below 2^31
     bool offs = true, aligned = true;
@@ -2610,7 +2631,7 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     Staged ws;
     LINAC_TRY(upload(act, s, 4, 5, &ws));
     k_fast_gate<LT><<<1, 64, 0, s>>>(ws.desc, cs.desc, const_cast<int64_t*>(ws.main.pre), (int)B);
-    k_gather_status<LT><<<B, 1, 0, s>>>(cs.desc, cs.status);
+    k_gather_status<LT><<<B, 1, 0, s>>>(cs.desc, cs.status, in.desc);
     LINAC_HIP(hipMemcpyAsync(cs.status_h, cs.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
     LINAC_HIP(hipEventRecord(ev_status, s));
     const int nl = (int)gs_grid(maxN * (BO / 4), 16384);
@@ -2629,9 +2650,11 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
         if (q != hipErrorNotReady) return fail(LINAC_ERR_DEVICE, "hipEventQuery: %s", hipGetErrorString(q));
         __builtin_ia32_pause();
     }
+    lo_fast.assign(B, 0);
     for (unsigned m = 0; m < B; ++m) {
         const BM& d = act[m]->d;
         fast[m] = sel[m] && cs.status_h[m].q0 == d.bcur && cs.status_h[m].rank == d.bcur ? 1 : 0;
+        lo_fast[m] = cs.status_h[m].lo;
     }
     return LINAC_OK;
 }
@@ -2738,7 +2761,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         // and the trailing update waits for the inner phase by an event
         for (unsigned m = 0; m < B; ++m) {
             const BM& d = act[m]->d;
-            st[m] = ItemStatus{d.bcur, d.bcur, d.bcur, 0, act[m]->rank + d.bcur};
+            st[m] = ItemStatus{d.bcur, d.bcur, d.bcur, 0, act[m]->rank + d.bcur, lo_fast[m]};
         }
         LINAC_HIP(hipEventRecord(ev_inner, s));
         LINAC_HIP(hipStreamWaitEvent(so, ev_inner, 0));
@@ -2753,9 +2776,20 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     // previous panel's deferred part (kA > 0: one composite GEMM) or deferred
     // to the next step (save_w).
     for (unsigned m = 0; m < B; ++m) {
-        BM& d = act[m]->d;
+        It* it = act[m];
+        BM& d = it->d;
         d.q0 = st[m].q0;
         d.rank_start = st[m].rank - st[m].q0;
+        // rows above the first unused row at this panel's load are pivot rows
+        // of earlier panels; with forward elimination they leave the updates
+        // (in whole 64-row tiles).  A composite step keeps the deferred
+        // panel's r0: both panels' updates then cover the same rows.
+        if (d.kA == 0) it->r0 = it->forward ? std::min<int64_t>(st[m].lo, d.N) / G_M * G_M : 0;
+        d.r0 = it->r0;
+        if (d.r0 > 0 && d.q0 > 0) it->echelon.push_back(EchelonPanel{d.J, d.bcur, d.r0});
+        if (getenv("LINAC_DEBUG_STEPS"))
+            fprintf(stderr, "step J=%lld lo=%lld r0=%lld kA=%d q0=%d rank=%lld fast=%d\n", (long long)d.J,
+                    (long long)st[m].lo, (long long)d.r0, d.kA, d.q0, (long long)st[m].rank, (int)fast[m]);
         d.T0 = d.J + d.bcur;
         d.nT = d.M - d.T0;
         d.KqA = (int32_t)round_up(d.kA, 64);
@@ -2883,7 +2917,8 @@ static Engine<LT>* cached_engine(DeviceContext* ctx) {
 // context's cached scratch, elimination state from `ws`.
 template <class LT>
 static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, const typename LT::F& f, hipStream_t s,
-                       const ElimWorkspace& ws, DeviceContext* ctx, Profiler* prof) {
+                       const ElimWorkspace& ws, DeviceContext* ctx, Profiler* prof,
+                       std::vector<EchelonPanel>* echelon) {
     const WorkspaceLayout w = workspace_layout<LT>(N, M, false);
     void* p = nullptr;
     LINAC_TRY(ctx->get(SLOT_PANEL_A, w.total, &p));
@@ -2895,6 +2930,7 @@ static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, cons
     it.d.piv_rows = ws.piv_rows;
     it.d.piv_cols = ws.piv_cols;
     it.d.used = ws.used;
+    it.forward = echelon != nullptr && tuning().echelon != 0;
     Engine<LT>& eng = *cached_engine<LT>(ctx);
     eng.reset_for_solve();
     std::vector<Item<LT>*> items{&it};
@@ -2910,6 +2946,7 @@ static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, cons
     const int rd = eng.drain();
     LINAC_TRY(rc);
     LINAC_TRY(rd);
+    if (echelon) *echelon = std::move(it.echelon);
     LINAC_HIP(hipStreamSynchronize(s));
     if (timing) {
         print_timers(it.d, ctx);
@@ -2930,27 +2967,32 @@ static int blocked_run(typename LT::T* A, int64_t ld, int64_t N, int64_t M, cons
 
 template <class F>
 int blocked_dispatch(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
-                     const ElimWorkspace& ws, DeviceContext* ctx, bool forced, Profiler* prof) {
+                     const ElimWorkspace& ws, DeviceContext* ctx, bool forced, Profiler* prof,
+                     std::vector<EchelonPanel>* echelon) {
     if constexpr (std::is_same<F, FieldU32>::value) {
         if (f.p > LIMB_P_MAX) return NOT_APPLICABLE;
         if (!forced && (N < tuning().gf_blocked_min_rows || M < tuning().gf_blocked_min_cols)) return NOT_APPLICABLE;
-        return blocked_run<LimbsU32>(A, ld, N, M, f, s, ws, ctx, prof);
+        return blocked_run<LimbsU32>(A, ld, N, M, f, s, ws, ctx, prof, echelon);
     } else if constexpr (std::is_same<F, FieldU64>::value) {
         if (f.p < LIMB64_P_MIN || f.p >= LIMB64_P_MAX) return NOT_APPLICABLE;
         if (!forced && (N < tuning().gf_blocked_min_rows || M < tuning().gf_blocked_min_cols)) return NOT_APPLICABLE;
-        return blocked_run<LimbsU64>(A, ld, N, M, f, s, ws, ctx, prof);
+        return blocked_run<LimbsU64>(A, ld, N, M, f, s, ws, ctx, prof, echelon);
     } else {
         (void)A; (void)ld; (void)N; (void)M; (void)f; (void)s; (void)ws; (void)ctx; (void)forced; (void)prof;
+        (void)echelon;
         return NOT_APPLICABLE;
     }
 }
 
 template int blocked_dispatch<FieldU32>(uint32_t*, int64_t, int64_t, int64_t, const FieldU32&, hipStream_t,
-                                        const ElimWorkspace&, DeviceContext*, bool, Profiler*);
+                                        const ElimWorkspace&, DeviceContext*, bool, Profiler*,
+                                        std::vector<EchelonPanel>*);
 template int blocked_dispatch<FieldU64>(uint64_t*, int64_t, int64_t, int64_t, const FieldU64&, hipStream_t,
-                                        const ElimWorkspace&, DeviceContext*, bool, Profiler*);
+                                        const ElimWorkspace&, DeviceContext*, bool, Profiler*,
+                                        std::vector<EchelonPanel>*);
 template int blocked_dispatch<FieldGF2>(uint32_t*, int64_t, int64_t, int64_t, const FieldGF2&, hipStream_t,
-                                        const ElimWorkspace&, DeviceContext*, bool, Profiler*);
+                                        const ElimWorkspace&, DeviceContext*, bool, Profiler*,
+                                        std::vector<EchelonPanel>*);
 
 // ---------------------------------------------------------------------------
 // Exact modular GEMM with index lists (linac_gemm_sub_u32):
@@ -2963,40 +3005,51 @@ template int blocked_dispatch<FieldGF2>(uint32_t*, int64_t, int64_t, int64_t, co
 constexpr int GS_KC = 2 * BO;
 using BMat32 = BMat<LimbsU32>;
 
-// planes[l][i][t] = digit l of A[i, cols[t0 + t]] for t < kc (cols NULL:
-// column t0 + t), zero up to Kpad; rows i < m.  (The GEMM kernel computes
-// C - L B, so L holds A itself here, where the trailing update holds -W.)
-__global__ __launch_bounds__(256) void k_planes_cols(const uint32_t* __restrict__ A, int64_t lda,
+// planes[l][i][t] = digit l of A[row(i), cols[t0 + t]] for t < kc (cols
+// NULL: column t0 + t; rows NULL: row(i) = i, else rows[i]), zero up to Kpad;
+// i < m.  (The GEMM kernel computes C - L B, so L holds A itself here, where
+// the trailing update holds -W.)
+template <class LT>
+__global__ __launch_bounds__(256) void k_planes_cols(const typename LT::T* __restrict__ A, int64_t lda,
+                                                     const int64_t* __restrict__ rows,
                                                      const int64_t* __restrict__ cols, int64_t t0, int kc, int Kpad,
-                                                     int64_t m, uint32_t p, int8_t* __restrict__ planes, int64_t stride) {
+                                                     int64_t m, typename LT::T p, int8_t* __restrict__ planes,
+                                                     int64_t stride) {
+    constexpr int NL = LT::NL;
     const int64_t total = m * Kpad;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = x / Kpad;
         const int t = (int)(x - i * Kpad);
-        int8_t dg[4] = {0, 0, 0, 0};
+        int8_t dg[NL] = {};
         if (t < kc) {
             const int64_t c = cols ? cols[t0 + t] : t0 + t;
-            balanced_digits(centred(A[i * lda + c], p), dg);
+            const int64_t r = rows ? rows[i] : i;
+            balanced_digits(centred(A[r * lda + c], p), dg);
         }
 #pragma unroll
-        for (int l = 0; l < 4; ++l) planes[l * stride + i * Kpad + t] = dg[l];
+        for (int l = 0; l < NL; ++l) planes[l * stride + i * Kpad + t] = dg[l];
     }
 }
 
 // planes[l][j][t] = digit l of B[rows[t0 + t], j] for t < kc (rows NULL: row
 // t0 + t), zero up to Kpad; columns j < n.  64 x 64 tiles through LDS
-// (blockIdx.x strides over column tiles, blockIdx.y = 64-deep tile).
-__global__ __launch_bounds__(256) void k_planes_rows(const uint32_t* __restrict__ B, int64_t ldb,
+// (blockIdx.x strides over column tiles, blockIdx.y = 64-deep tile); the 16
+// digits of a limb are packed into four words with shifts.
+template <class LT>
+__global__ __launch_bounds__(256) void k_planes_rows(const typename LT::T* __restrict__ B, int64_t ldb,
                                                      const int64_t* __restrict__ rows, int64_t t0, int kc, int Kpad,
-                                                     int64_t n, uint32_t p, int8_t* __restrict__ planes, int64_t stride) {
-    __shared__ int32_t tile[64][65];
+                                                     int64_t n, typename LT::T p, int8_t* __restrict__ planes,
+                                                     int64_t stride) {
+    using S = typename LT::S;
+    constexpr int NL = LT::NL;
+    __shared__ S tile[64][65];
     const int tb = blockIdx.y * 64;
     for (int64_t nb = (int64_t)blockIdx.x * 64; nb < n; nb += (int64_t)gridDim.x * 64) {
         for (int x = threadIdx.x; x < 64 * 64; x += 256) {
             const int tt = x >> 6, nn = x & 63;
             const int t = tb + tt;
             const int64_t j = nb + nn;
-            int32_t v = 0;
+            S v = 0;
             if (t < kc && j < n) v = centred(B[(rows ? rows[t0 + t] : t0 + t) * ldb + j], p);
             tile[tt][nn] = v;
         }
@@ -3004,18 +3057,21 @@ __global__ __launch_bounds__(256) void k_planes_rows(const uint32_t* __restrict_
         const int nn = threadIdx.x >> 2, tq = (threadIdx.x & 3) * 16;
         const int64_t j = nb + nn;
         if (j < n) {
-            int8_t buf[4][16];
+            uint32_t wd[NL][4];
+#pragma unroll
+            for (int l = 0; l < NL; ++l)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) wd[l][q] = 0u;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                int8_t dg[4];
+                int8_t dg[NL];
                 balanced_digits(tile[tq + i][nn], dg);
 #pragma unroll
-                for (int l = 0; l < 4; ++l) buf[l][i] = dg[l];
+                for (int l = 0; l < NL; ++l) wd[l][i >> 2] |= (uint32_t)(uint8_t)dg[l] << (8 * (i & 3));
             }
 #pragma unroll
-            for (int l = 0; l < 4; ++l) {
-                v4i w;
-                __builtin_memcpy(&w, buf[l], 16);
+            for (int l = 0; l < NL; ++l) {
+                const v4i w = {(int)wd[l][0], (int)wd[l][1], (int)wd[l][2], (int)wd[l][3]};
                 *reinterpret_cast<v4i*>(planes + l * stride + j * Kpad + tb + tq) = w;
             }
         }
@@ -3096,9 +3152,9 @@ static int gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda
     for (int64_t c = 0; c < nch; ++c) {
         const int64_t t0 = c * GS_KC;
         const int kc = (int)std::min<int64_t>(GS_KC, k - t0);
-        k_planes_cols<<<gs_grid(m * kp[c], 16384), 256, 0, s>>>(A, lda, a_cols ? d_acols : nullptr, t0, kc, (int)kp[c],
-                                                                 m, p, planesL + offL[c], sL[c]);
-        k_planes_rows<<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(n, 64)), (unsigned)(kp[c] / 64)), 256, 0, s>>>(
+        k_planes_cols<LimbsU32><<<gs_grid(m * kp[c], 16384), 256, 0, s>>>(
+            A, lda, nullptr, a_cols ? d_acols : nullptr, t0, kc, (int)kp[c], m, p, planesL + offL[c], sL[c]);
+        k_planes_rows<LimbsU32><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(n, 64)), (unsigned)(kp[c] / 64)), 256, 0, s>>>(
             B, ldb, b_rows ? d_brows : nullptr, t0, kc, (int)kp[c], n, p, planesB + offB[c], sB[c]);
     }
     const TileSched ts{d_sched, d_sched + 2, 1, (int)tuning().gemm_group, 0, -1};
@@ -3110,6 +3166,210 @@ static int gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda
     LINAC_HIP(hipStreamSynchronize(s));
     return LINAC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Echelon completion.  With forward elimination, a pivot row stops taking the
+// updates of later panels once it lies above their r0, so it still holds
+// entries in their pivot columns.  Sorted by column, the pivots give the
+// result rows: row k is pivot k, and panel l owns rows [s_l, s_l + q_l).  Any
+// vector v of the row space is v = sum_k v[c_k] R[k] (R: the RREF, c_k: the
+// pivot columns), so a pivot row v of panel j satisfies
+//     R[k] = v - sum_{l > j} v[P_l] R[rows of panel l],
+// and the pivot columns of R are the identity.  Only the non-pivot columns F
+// need arithmetic.  Processing the panels from the last, the rows of panel l
+// are final when its correction is applied to every earlier row:
+//     G[0:s_l, F] -= A[piv_rows[0:s_l], P_l] G[s_l:s_l+q_l, F]
+// with G the pivot rows' entries in F (result order): one exact limb GEMM of
+// s_l x q_l x |F| per panel on the trailing-update kernel.  Rows that took
+// panel l's update hold zeros in P_l, so the correction leaves them alone.
+// Columns of F left of the panel are zero in its rows and are skipped.
+// ---------------------------------------------------------------------------
+
+// G[k, j] = A[rows[k], fcols[j]] for k < rank, j < nF.
+template <class T>
+__global__ __launch_bounds__(256) void k_ech_gather(const T* __restrict__ A, int64_t ld,
+                                                    const int64_t* __restrict__ rows,
+                                                    const int32_t* __restrict__ fcols, int64_t rank, int64_t nF,
+                                                    T* __restrict__ G, int64_t ldg) {
+    for (int64_t k = blockIdx.y; k < rank; k += gridDim.y) {
+        const T* src = A + rows[k] * ld;
+        for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < nF; j += (int64_t)gridDim.x * 256)
+            G[k * ldg + j] = src[fcols[j]];
+    }
+}
+
+// A rewritten as the RREF: row k < rank holds 1 in its pivot column, 0 in
+// every other pivot column (cmap[c] = -1 - pivot index) and G[k, cmap[c]] in a
+// non-pivot column c; rows from rank on are zero.  vec: rows are 16-byte
+// aligned, so whole vectors within a row are stored as such.
+template <class T>
+__global__ __launch_bounds__(256) void k_ech_write(T* __restrict__ A, int64_t ld, int64_t N, int64_t M,
+                                                   const T* __restrict__ G, int64_t ldg,
+                                                   const int32_t* __restrict__ cmap, int64_t rank, int vec) {
+    constexpr int V = 16 / (int)sizeof(T);
+    const int64_t nv = (M + V - 1) / V;
+    for (int64_t k = blockIdx.y; k < N; k += gridDim.y) {
+        T* row = A + k * ld;
+        const bool live = k < rank;
+        auto value = [&](int64_t c) -> T {
+            if (!live) return (T)0;
+            const int32_t m = cmap[c];
+            return m < 0 ? (T)((int64_t)(-1 - m) == k ? 1 : 0) : G[k * ldg + m];
+        };
+        for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nv; v += (int64_t)gridDim.x * 256) {
+            const int64_t c0 = v * V;
+            if (vec && c0 + V <= M) {
+                Vec16<T> x;
+#pragma unroll
+                for (int e = 0; e < V; ++e) x.v[e] = value(c0 + e);
+                v4i out;
+                __builtin_memcpy(&out, &x, 16);
+                gst<v4i>(row + c0, out);
+            } else {
+                for (int e = 0; e < V && c0 + e < M; ++e) row[c0 + e] = value(c0 + e);
+            }
+        }
+    }
+}
+
+template <class F>
+int echelon_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
+                     const ElimWorkspace& ws, DeviceContext* ctx, const std::vector<int64_t>& rows,
+                     const std::vector<int64_t>& cols, bool resorted, const std::vector<EchelonPanel>& panels) {
+    using T = typename F::T;
+    constexpr bool u32 = std::is_same<F, FieldU32>::value, u64 = std::is_same<F, FieldU64>::value;
+    if constexpr (!u32 && !u64) {
+        return gf_finalize_apply(A, ld, N, M, f, s, ws, ctx, false, rows, cols, resorted);
+    } else {
+        using LT = typename std::conditional<u32, LimbsU32, LimbsU64>::type;
+        using BM = BMat<LT>;
+        constexpr int NL = LT::NL;
+        constexpr int64_t TN = u32 ? G_N : TN64;
+        if (panels.empty()) return gf_finalize_apply(A, ld, N, M, f, s, ws, ctx, false, rows, cols, resorted);
+        const int64_t rank = (int64_t)rows.size();
+        if (resorted && rank > 0) {
+            LINAC_HIP(hipMemcpyAsync(ws.piv_rows, rows.data(), rank * 8, hipMemcpyHostToDevice, s));
+            LINAC_HIP(hipMemcpyAsync(ws.piv_cols, cols.data(), rank * 8, hipMemcpyHostToDevice, s));
+        }
+        std::vector<int32_t> cmap((size_t)M, 0), fcols;
+        for (int64_t k = 0; k < rank; ++k) cmap[cols[k]] = (int32_t)(-1 - k);
+        std::vector<char> piv((size_t)M, 0);
+        for (int64_t k = 0; k < rank; ++k) piv[cols[k]] = 1;
+        fcols.reserve((size_t)(M - rank));
+        for (int64_t c = 0; c < M; ++c)
+            if (!piv[c]) {
+                cmap[c] = (int32_t)fcols.size();
+                fcols.push_back((int32_t)c);
+            }
+        const int64_t nF = (int64_t)fcols.size();
+        const int64_t ldg = round_up(std::max<int64_t>(nF, 1), 64);
+        // the GEMMs, last panel first
+        struct Job {
+            int64_t s, q, f0, Kpad;
+        };
+        std::vector<Job> jobs;
+        int64_t maxL = 0, maxB = 0;
+        if (nF > 0 && rank > 0)
+            for (auto it = panels.rbegin(); it != panels.rend(); ++it) {
+                const int64_t sl = std::lower_bound(cols.begin(), cols.end(), it->J) - cols.begin();
+                const int64_t el = std::lower_bound(cols.begin(), cols.end(), it->J + it->width) - cols.begin();
+                const int64_t f0 = std::lower_bound(fcols.begin(), fcols.end(), (int32_t)it->J) - fcols.begin();
+                if (el == sl || sl == 0 || f0 >= nF || it->r0 == 0) continue;
+                const int64_t Kpad = round_up(el - sl, 64);
+                jobs.push_back(Job{sl, el - sl, f0, Kpad});
+                maxL = std::max(maxL, round_up(sl * Kpad, 256));
+                maxB = std::max(maxB, round_up((nF - f0) * Kpad, 256));
+            }
+        const size_t cmap_b = (size_t)round_up(M * 4, 256), fcol_b = (size_t)round_up(std::max<int64_t>(nF, 1) * 4, 256);
+        const size_t desc_b = (size_t)round_up((int64_t)(jobs.size() * sizeof(BM)), 256);
+        const size_t sched_b = (size_t)round_up((int64_t)jobs.size() * 3 * 8, 256);
+        void* pc = nullptr;
+        LINAC_TRY(ctx->get(SLOT_PANEL_C, cmap_b + fcol_b + desc_b + sched_b, &pc));
+        int32_t* d_cmap = (int32_t*)pc;
+        int32_t* d_fcols = (int32_t*)((char*)pc + cmap_b);
+        BM* d_desc = (BM*)((char*)pc + cmap_b + fcol_b);
+        int64_t* d_sched = (int64_t*)((char*)pc + cmap_b + fcol_b + desc_b);
+        LINAC_HIP(hipMemcpyAsync(d_cmap, cmap.data(), M * 4, hipMemcpyHostToDevice, s));
+        T* G = nullptr;
+        if (nF > 0 && rank > 0) {
+            void* pg = nullptr;
+            LINAC_TRY(ctx->get(SLOT_OUT, (size_t)(rank * ldg) * sizeof(T), &pg));
+            G = (T*)pg;
+            LINAC_HIP(hipMemcpyAsync(d_fcols, fcols.data(), nF * 4, hipMemcpyHostToDevice, s));
+            k_ech_gather<T><<<dim3((unsigned)ceil_div(nF, 256), (unsigned)std::min<int64_t>(rank, 65535)), 256, 0, s>>>(
+                A, ld, ws.piv_rows, d_fcols, rank, nF, G, ldg);
+        }
+        if (!jobs.empty()) {
+            void* pp = nullptr;
+            LINAC_TRY(ctx->get(SLOT_PANEL_B, (size_t)NL * (size_t)(maxL + maxB), &pp));
+            int8_t* planesL = (int8_t*)pp;
+            int8_t* planesB = planesL + NL * maxL;
+            std::vector<BM> hd(jobs.size());
+            std::vector<int64_t> hs(jobs.size() * 3);
+            for (size_t i = 0; i < jobs.size(); ++i) {
+                const Job& j = jobs[i];
+                BM d{};
+                d.A = G + j.f0;
+                d.ld = ldg;
+                d.N = j.s;
+                d.M = nF - j.f0;
+                d.f = f;
+                d.pd = (double)f.p;
+                d.pinv = 1.0 / (double)f.p;
+                d.planesL = planesL;
+                d.strideL = maxL;
+                d.planesB = planesB;
+                d.strideB = maxB;
+                d.T0 = 0;
+                d.nT = nF - j.f0;
+                d.Kq = (int32_t)j.Kpad;
+                hd[i] = d;
+                hs[3 * i] = 0;
+                hs[3 * i + 1] = ceil_div(j.s, G_M) * ceil_div(nF - j.f0, TN);
+                hs[3 * i + 2] = ceil_div(j.s, G_M);
+            }
+            LINAC_HIP(hipMemcpyAsync(d_desc, hd.data(), jobs.size() * sizeof(BM), hipMemcpyHostToDevice, s));
+            LINAC_HIP(hipMemcpyAsync(d_sched, hs.data(), hs.size() * 8, hipMemcpyHostToDevice, s));
+            for (size_t i = 0; i < jobs.size(); ++i) {
+                const Job& j = jobs[i];
+                const int64_t n = nF - j.f0;
+                // L = A[piv_rows[0:s], P_l] (the pivots s .. s + q - 1 give the columns)
+                k_planes_cols<LT><<<gs_grid(j.s * j.Kpad, 16384), 256, 0, s>>>(
+                    A, ld, ws.piv_rows, ws.piv_cols, j.s, (int)j.q, (int)j.Kpad, j.s, f.p, planesL, maxL);
+                // B = G[s:s+q, f0:], rows contiguous
+                k_planes_rows<LT><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(n, 64)), (unsigned)(j.Kpad / 64)),
+                                    256, 0, s>>>(G + j.s * ldg + j.f0, ldg, nullptr, 0, (int)j.q, (int)j.Kpad, n, f.p,
+                                                 planesB, maxB);
+                const TileSched ts{d_sched + 3 * i, d_sched + 3 * i + 2, 1, (int)tuning().gemm_group, 0, -1};
+                const unsigned grid = (unsigned)(ceil_div(hs[3 * i + 1], NUM_XCD) * NUM_XCD);
+                if constexpr (u32) {
+                    const bool lds_ok = ((((uintptr_t)(G + j.f0)) | (uintptr_t)(ldg * 4)) & 15) == 0 &&
+                                        j.s * j.Kpad < (1ll << 31) && n * j.Kpad < (1ll << 31);
+                    launch_outer32<0>(grid, s, d_desc + i, ts, lds_ok);
+                } else {
+                    k_gemm_outer64<0><<<grid, G_T, 0, s>>>(d_desc + i, ts);
+                }
+            }
+        }
+        constexpr int V = 16 / (int)sizeof(T);
+        const int vec = ((ld * (int64_t)sizeof(T)) % 16 == 0 && ((uintptr_t)A & 15) == 0) ? 1 : 0;
+        k_ech_write<T><<<dim3((unsigned)std::min<int64_t>(64, ceil_div(ceil_div(M, V), 256)),
+                              (unsigned)std::min<int64_t>(N, 65535)), 256, 0, s>>>(A, ld, N, M, G, ldg, d_cmap, rank,
+                                                                                    vec);
+        LINAC_HIP(hipGetLastError());
+        return LINAC_OK;
+    }
+}
+
+template int echelon_finalize<FieldU32>(uint32_t*, int64_t, int64_t, int64_t, const FieldU32&, hipStream_t,
+                                        const ElimWorkspace&, DeviceContext*, const std::vector<int64_t>&,
+                                        const std::vector<int64_t>&, bool, const std::vector<EchelonPanel>&);
+template int echelon_finalize<FieldU64>(uint64_t*, int64_t, int64_t, int64_t, const FieldU64&, hipStream_t,
+                                        const ElimWorkspace&, DeviceContext*, const std::vector<int64_t>&,
+                                        const std::vector<int64_t>&, bool, const std::vector<EchelonPanel>&);
+template int echelon_finalize<FieldGF2>(uint32_t*, int64_t, int64_t, int64_t, const FieldGF2&, hipStream_t,
+                                        const ElimWorkspace&, DeviceContext*, const std::vector<int64_t>&,
+                                        const std::vector<int64_t>&, bool, const std::vector<EchelonPanel>&);
 
 // ---------------------------------------------------------------------------
 // batch objects: matrices join and leave between steps (continuous batching)
@@ -3322,6 +3582,7 @@ static int batch_add(linac_batch* b, const typename LT::T* a, int64_t rows, int6
         carve_compact(it->dc, base + matrix_bytes<LT>(rows, cols), wl);
     }
     it->id = b->next_id++;
+    it->forward = tuning().echelon != 0;
     b->in_use += need;
     items_of(b, LT{}).push_back(it);
     fresh_of(b, LT{}).push_back(it);
@@ -3364,7 +3625,8 @@ static int batch_take(linac_batch* b, int64_t id, typename LT::T* out, int64_t l
     ElimWorkspace ws{it->d.st, it->d.piv_rows, it->d.piv_cols, it->d.used, std::max<int64_t>(1, std::min(it->d.N, it->d.M))};
     int64_t rank = 0;
     std::vector<int64_t> pc;
-    LINAC_TRY(gf_finalize(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, true, &rank, &pc));
+    LINAC_TRY(gf_finalize(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, true, &rank, &pc,
+                          &it->echelon));
     if (out) {
         LINAC_HIP(hipMemcpy2DAsync(out, ld_out * sizeof(T), it->d.A, it->d.ld * sizeof(T), it->d.M * sizeof(T), it->d.N,
                                    (flags & LINAC_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
@@ -3432,7 +3694,12 @@ static int batch_take_many(linac_batch* b, const int64_t* ids, int64_t count, ty
         const bool resorted = sort_pivots_by_column(r, c);
         ElimWorkspace ws{it->d.st, it->d.piv_rows, it->d.piv_cols, it->d.used,
                          std::max<int64_t>(1, std::min(it->d.N, it->d.M))};
-        LINAC_TRY(gf_finalize_apply(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, r, c, resorted));
+        if (it->echelon.empty())
+            LINAC_TRY(gf_finalize_apply(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, false, r, c,
+                                        resorted));
+        else
+            LINAC_TRY(echelon_finalize(it->d.A, it->d.ld, it->d.N, it->d.M, it->d.f, s, ws, b->ctx, r, c, resorted,
+                                       it->echelon));
         if (outs && outs[i])
             LINAC_HIP(hipMemcpy2DAsync(outs[i], ld_outs[i] * sizeof(T), it->d.A, it->d.ld * sizeof(T),
                                        it->d.M * sizeof(T), it->d.N,

@@ -331,7 +331,7 @@ bool sort_pivots_by_column(std::vector<int64_t>& rows, std::vector<int64_t>& col
 template <class F>
 int gf_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
                 const ElimWorkspace& ws, DeviceContext* ctx, bool normalize, bool sort_by_column,
-                int64_t* rank_host, std::vector<int64_t>* piv_cols_host) {
+                int64_t* rank_host, std::vector<int64_t>* piv_cols_host, const std::vector<EchelonPanel>* echelon) {
     LINAC_HIP(hipMemcpyAsync(ctx->pinned, &ws.st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     LINAC_HIP(hipStreamSynchronize(s));
     const int64_t rank = ctx->pinned[0];
@@ -342,7 +342,10 @@ int gf_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, 
         LINAC_HIP(hipStreamSynchronize(s));
     }
     const bool resorted = sort_by_column && sort_pivots_by_column(rows, cols);
-    LINAC_TRY(gf_finalize_apply(A, ld, N, M, f, s, ws, ctx, normalize, rows, cols, resorted));
+    if (echelon && !echelon->empty())
+        LINAC_TRY(echelon_finalize(A, ld, N, M, f, s, ws, ctx, rows, cols, resorted, *echelon));
+    else
+        LINAC_TRY(gf_finalize_apply(A, ld, N, M, f, s, ws, ctx, normalize, rows, cols, resorted));
     LINAC_HIP(hipStreamSynchronize(s));
     *rank_host = rank;
     *piv_cols_host = cols;
@@ -353,12 +356,14 @@ template int gf_finalize_apply<FieldU32>(uint32_t*, int64_t, int64_t, int64_t, c
                                          const ElimWorkspace&, DeviceContext*, bool, const std::vector<int64_t>&,
                                          const std::vector<int64_t>&, bool);
 template int gf_finalize<FieldU32>(uint32_t*, int64_t, int64_t, int64_t, const FieldU32&, hipStream_t,
-                                   const ElimWorkspace&, DeviceContext*, bool, bool, int64_t*, std::vector<int64_t>*);
+                                   const ElimWorkspace&, DeviceContext*, bool, bool, int64_t*, std::vector<int64_t>*,
+                                   const std::vector<EchelonPanel>*);
 template int gf_finalize_apply<FieldU64>(uint64_t*, int64_t, int64_t, int64_t, const FieldU64&, hipStream_t,
                                          const ElimWorkspace&, DeviceContext*, bool, const std::vector<int64_t>&,
                                          const std::vector<int64_t>&, bool);
 template int gf_finalize<FieldU64>(uint64_t*, int64_t, int64_t, int64_t, const FieldU64&, hipStream_t,
-                                   const ElimWorkspace&, DeviceContext*, bool, bool, int64_t*, std::vector<int64_t>*);
+                                   const ElimWorkspace&, DeviceContext*, bool, bool, int64_t*, std::vector<int64_t>*,
+                                   const std::vector<EchelonPanel>*);
 
 // Shared entry: validation, host<->device staging, method dispatch.
 template <class F>
@@ -402,8 +407,11 @@ static int rref_gf_entry(typename F::T* a, int64_t rows, int64_t cols, int64_t l
 
     bool normalize = true, sorted = false;
     int rc = NOT_APPLICABLE;
+    // (left unfinalised, the rows must be final: Gauss-Jordan form)
+    std::vector<EchelonPanel> echelon;
     if (method == LINAC_METHOD_BLOCKED || method == LINAC_METHOD_AUTO)
-        rc = blocked_dispatch(A, lda, rows, cols, f, s, ws, ctx, method == LINAC_METHOD_BLOCKED, prof);
+        rc = blocked_dispatch(A, lda, rows, cols, f, s, ws, ctx, method == LINAC_METHOD_BLOCKED, prof,
+                              (flags & LINAC_NO_FINALIZE) ? nullptr : &echelon);
     if (rc < 0) return rc;
     if (rc == LINAC_OK) {
         normalize = false;
@@ -421,7 +429,7 @@ static int rref_gf_entry(typename F::T* a, int64_t rows, int64_t cols, int64_t l
     int64_t rank = 0;
     std::vector<int64_t> pc;
     if (!(flags & LINAC_NO_FINALIZE)) {
-        LINAC_TRY(gf_finalize(A, lda, rows, cols, f, s, ws, ctx, normalize, sorted, &rank, &pc));
+        LINAC_TRY(gf_finalize(A, lda, rows, cols, f, s, ws, ctx, normalize, sorted, &rank, &pc, &echelon));
     } else {
         LINAC_HIP(hipMemcpyAsync(ctx->pinned, &ws.st->rank, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         LINAC_HIP(hipStreamSynchronize(s));

@@ -133,6 +133,7 @@ struct Tuning {
     int64_t timeslice = 0;
     int64_t gemm_lds = 1;
     int64_t fast_panel = 2;
+    int64_t echelon = 1;  // blocked GF(p): forward elimination plus back substitution (0: Gauss-Jordan)
 };
 Tuning& tuning();
 
