@@ -178,10 +178,14 @@ __global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__
         block_state_init<LT>(d, reset_timers);
         d.bs->lo0 = d.st->first_unused;  // no pivot of this panel lies above it
     }
-    const int64_t total = d.N * VPR;
+    // rows from r0 on (the previous step's: rows above it are pivot rows
+    // that no later update touches, so their panel-buffer rows are never read
+    // back; the candidates, the fast path's GEMM rows and everything written
+    // back lie at or beyond it)
+    const int64_t total = (d.N - d.r0) * VPR;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = x / VPR;
-        const int j = (int)(x - r * VPR) * PER;
+        const int64_t i = x / VPR, r = d.r0 + i;
+        const int j = (int)(x - i * VPR) * PER;
         Vec16<T> v = {};
         if (j + PER <= d.bcur) {
             const v4i raw = gld<v4i>(d.A + r * d.ld + d.J + j);

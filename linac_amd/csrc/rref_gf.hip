@@ -31,15 +31,32 @@ namespace linac {
 // kernels
 // ---------------------------------------------------------------------------
 
+// Every entry reduced into [0, p).  Rows along blockIdx.y, 16-byte vectors
+// along x where rows are 16-byte aligned (vec); a vector is written back only
+// when one of its entries changed, so a canonical matrix costs one read.
 template <class F>
 __global__ __launch_bounds__(256) void k_reduce_mod(typename F::T* __restrict__ A, int64_t ld, int64_t rows,
-                                                    int64_t cols, F f) {
-    const int64_t total = rows * cols;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total;
-         x += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = x / cols, c = x - r * cols;
-        auto& v = A[r * ld + c];
-        v = f.reduce(v);
+                                                    int64_t cols, F f, int vec) {
+    using T = typename F::T;
+    constexpr int V = 16 / (int)sizeof(T);
+    const int64_t nv = vec ? cols / V : 0;
+    for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
+        T* row = A + r * ld;
+        for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nv; v += (int64_t)gridDim.x * 256) {
+            Vec16<T> x = reinterpret_cast<const Vec16<T>*>(row)[v];
+            bool changed = false;
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                const T y = f.reduce(x.v[e]);
+                changed = changed || y != x.v[e];
+                x.v[e] = y;
+            }
+            if (changed) reinterpret_cast<Vec16<T>*>(row)[v] = x;
+        }
+        for (int64_t c = nv * V + (int64_t)blockIdx.x * 256 + threadIdx.x; c < cols; c += (int64_t)gridDim.x * 256) {
+            const T y = f.reduce(row[c]);
+            if (y != row[c]) row[c] = y;
+        }
     }
 }
 
@@ -222,9 +239,12 @@ __global__ __launch_bounds__(256) void k_copy_rows_gated(T* __restrict__ dst, in
 template <class F>
 int plain_launch_init(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F& f, hipStream_t s,
                       const ElimWorkspace& ws) {
-    const int64_t total = N * M;
-    const int blocks = (int)std::min<int64_t>(4096, std::max<int64_t>(1, ceil_div(total, 256)));
-    k_reduce_mod<F><<<blocks, 256, 0, s>>>(A, ld, N, M, f);
+    using T = typename F::T;
+    constexpr int V = 16 / (int)sizeof(T);
+    const int vec = ((ld * (int64_t)sizeof(T)) % 16 == 0 && ((uintptr_t)A & 15) == 0) ? 1 : 0;
+    const int64_t bx = std::min<int64_t>(16, std::max<int64_t>(1, ceil_div(vec ? ceil_div(M, V) : M, 256)));
+    const int64_t by = std::min<int64_t>(std::max<int64_t>(N, 1), std::max<int64_t>(1, 32768 / bx));
+    k_reduce_mod<F><<<dim3((unsigned)bx, (unsigned)by), 256, 0, s>>>(A, ld, N, M, f, vec);
     k_init_state<<<(int)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(N, 256))), 256, 0, s>>>(ws.st, ws.used, N);
     LINAC_HIP(hipGetLastError());
     return LINAC_OK;
