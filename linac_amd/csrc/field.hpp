@@ -141,15 +141,29 @@ struct FieldU32 {
     __host__ __device__ __forceinline__ uint32_t reduce(uint32_t a) const { return a >= p ? a % p : a; }
 };
 
+// p = 2^61 - 1: with 2^61 == 1 (mod p), R = 2^64 == 8 and R^{-1} == 2^58, so
+// a Montgomery product t R^{-1} is the residue of t (folded at bit 61) rotated
+// left by 58 within 61 bits -- no multiplications beyond the product itself.
+// m61_redc takes t = hi 2^64 + lo with hi < 2^59 (t < 2^123).
+constexpr uint64_t M61 = (1ull << 61) - 1;
+__host__ __device__ __forceinline__ uint64_t m61_redc(uint64_t lo, uint64_t hi) {
+    uint64_t x = (lo & M61) + (lo >> 61) + (hi << 3);  // < 2^61 + 8 + 2^62
+    x = (x & M61) + (x >> 61);                         // < 2^61 + 4 < 2 p
+    x = x >= M61 ? x - M61 : x;                        // t mod p
+    return ((x & 7ull) << 58) | (x >> 3);              // x 2^58 mod p (x < p: never p itself)
+}
+
 struct FieldU64 {
     using T = uint64_t;
     uint64_t p;
     uint64_t pneg;  // -p^{-1} mod 2^64
     uint64_t r2;    // 2^128 mod p
+    uint64_t m61;   // 1 when p = 2^61 - 1 (Mersenne reduction), else 0
 
     __host__ __device__ __forceinline__ uint64_t mont(uint64_t a, uint64_t b) const {
         const uint64_t t_lo = a * b;
         const uint64_t t_hi = mulhi64(a, b);
+        if (m61) return m61_redc(t_lo, t_hi);
         const uint64_t m = t_lo * pneg;
         const uint64_t u = t_hi + mulhi64(m, p) + (t_lo != 0ull);
         return u >= p ? u - p : u;
@@ -169,6 +183,7 @@ struct FieldU64 {
         const uint64_t l1 = h * x, l2 = b * y;
         const uint64_t t_lo = l1 + l2;
         const uint64_t t_hi = mulhi64(h, x) + mulhi64(b, y) + (t_lo < l1 ? 1ull : 0ull);
+        if (m61) return m61_redc(t_lo, t_hi);
         const uint64_t m = t_lo * pneg;
         const uint64_t u = t_hi + mulhi64(m, p) + (t_lo != 0ull);
         return u >= p ? u - p : u;
@@ -222,6 +237,7 @@ inline FieldU64 make_field_u64(uint64_t p) {
     // 2^128 mod p = ((2^64 mod p)^2) mod p
     const unsigned __int128 r = ((unsigned __int128)1 << 64) % p;
     f.r2 = (uint64_t)((r * r) % p);
+    f.m61 = p == M61 ? 1ull : 0ull;
     return f;
 }
 

@@ -44,6 +44,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <chrono>
 #include <deque>
 #include <iterator>
 #include <map>
@@ -154,6 +155,13 @@ struct BMat {
 
 constexpr int64_t STRIDE_W = (int64_t)BO * BO;
 
+// LINAC_DEBUG_STEPS: one line per outer panel on stderr (row offsets, rank,
+// host time of the step so far and its wait for the status read)
+static bool debug_steps() {
+    static const bool on = getenv("LINAC_DEBUG_STEPS") != nullptr;
+    return on;
+}
+
 // ---------------------------------------------------------------------------
 // panel buffer management
 // ---------------------------------------------------------------------------
@@ -261,7 +269,7 @@ __device__ __forceinline__ uint64_t uniform(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-template <class LT>
+template <class LT, bool CMP>
 __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t N,
                                          uint8_t* __restrict__ used, int32_t* __restrict__ slot_of,
                                          int32_t* __restrict__ epoch_of, int epoch,
@@ -272,9 +280,9 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
     constexpr int IF_T = LT::IF_T;
     constexpr int IF_RPW = CI / (IF_T / 64);  // candidate rows per wave
     static_assert(IF_RPW * (IF_T / 64) == CI && IF_T >= 256, "waves split the candidate rows evenly; SCAN rows per probe");
-    // the compact form (32 slots a row) for 64-bit storage, [panel | identity]
-    // (64 columns a row) for 32-bit storage; one padding column
-    constexpr bool CMP = sizeof(T) == 8;
+    // CMP: the compact form (32 slots a row), else [panel | identity] (64
+    // columns a row); one padding column
+    static_assert(!CMP || IF_RPW % 2 == 0, "the compact form takes rows in pairs");
     constexpr int AW = (CMP ? BI : BI + CI) + 1;
     __shared__ T aug[CI][AW];
     __shared__ int64_t cand[CI + 1];
@@ -551,10 +559,21 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
 }
 
 
-template <class LT>
+template <class LT, bool CMP>
 __global__ __launch_bounds__(LT::IF_T, LT::IF_WAVES) void k_inner_factor(const BMat<LT>* __restrict__ D, int epoch) {
     const BMat<LT>& d = D[blockIdx.x];
-    inner_factor_body<LT>(d.X, d.N, d.used, d.slot_of, d.epoch_of, epoch, d.J, d.f, d.st, d.piv_rows, d.piv_cols, d.bs);
+    inner_factor_body<LT, CMP>(d.X, d.N, d.used, d.slot_of, d.epoch_of, epoch, d.J, d.f, d.st, d.piv_rows, d.piv_cols,
+                               d.bs);
+}
+
+// The inner factorisation of storage width LT: the compact form always for
+// 64-bit storage, by tuning knob "inner_compact" for 32-bit storage.
+template <class LT>
+static void launch_inner_factor(unsigned B, hipStream_t s, const BMat<LT>* desc, int epoch) {
+    if (std::is_same<LT, LimbsU64>::value || tuning().inner_compact != 0)
+        k_inner_factor<LT, true><<<B, LT::IF_T, 0, s>>>(desc, epoch);
+    else
+        k_inner_factor<LT, false><<<B, LT::IF_T, 0, s>>>(desc, epoch);
 }
 
 // U = Apiv^{-1} X[R, :] (k x XLD, Montgomery inverse times plain rows gives plain
@@ -2317,6 +2336,7 @@ class Engine {
     // parts 4 (compact block views) and 5 (fast-path GEMM Z = -P B): which items take part
     std::vector<char> sel;
     std::vector<int64_t> lo_fast;  // first unused row at the panel load, read with the fast path's status
+    double spin_us = 0;            // host wait for the last status read (debug output)
     int fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, int64_t ninner, int64_t maxN,
                    std::vector<char>& fast);
     void fill(const std::vector<It*>& act, int part, char* h, char* dv, size_t desc_b, size_t sched_b, Staged* out);
@@ -2622,7 +2642,7 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     }
     if (wide || tuning().fast_panel == 2)  // the inner-step kernels on the block
         for (int64_t i = 0; i < ninner; ++i) {
-            k_inner_factor<LT><<<B, LT::IF_T, 0, s>>>(cs.desc, epoch);
+            launch_inner_factor<LT>(B, s, cs.desc, epoch);
             k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(cs.desc);
             if constexpr (wide) k_gemm_inner64<<<igrid, G_T, 0, s>>>(cs.desc, cs.inner, epoch, cs.inner_split);
             else k_gemm_inner<<<igrid, G_T, 0, s>>>(cs.desc, cs.inner, epoch, cs.inner_split);
@@ -2648,12 +2668,14 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     k_fast_commit<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc, ws.desc, act[0]->d.e0);
     k_fast_finish<LT><<<B, 1024, 0, s>>>(in.desc, cs.desc, ws.desc);
     LINAC_HIP(hipGetLastError());
+    const auto spin0 = std::chrono::steady_clock::now();
     for (;;) {
         const hipError_t q = hipEventQuery(ev_status);
         if (q == hipSuccess) break;
         if (q != hipErrorNotReady) return fail(LINAC_ERR_DEVICE, "hipEventQuery: %s", hipGetErrorString(q));
         __builtin_ia32_pause();
     }
+    spin_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - spin0).count();
     lo_fast.assign(B, 0);
     for (unsigned m = 0; m < B; ++m) {
         const BM& d = act[m]->d;
@@ -2666,6 +2688,8 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
 template <class LT>
 int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* prof) {
     constexpr bool wide = std::is_same<LT, LimbsU64>::value;
+    const auto step0 = std::chrono::steady_clock::now();
+    spin_us = 0;
     std::vector<It*> act;
     for (auto* it : items)
         if (!it->finished) act.push_back(it);
@@ -2716,7 +2740,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
             }
             LINAC_HIP(hipEventRecord(ev_c[i], so));
         }
-        k_inner_factor<LT><<<B, LT::IF_T, 0, s>>>(in.desc, epoch);
+        launch_inner_factor<LT>(B, s, in.desc, epoch);
         k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(in.desc);
         if (sl) LINAC_HIP(hipStreamWaitEvent(s, ev_c[i], 0));
         const unsigned grid = (unsigned)(ceil_div(in.inner_tiles, NUM_XCD) * NUM_XCD);
@@ -2791,9 +2815,12 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         if (d.kA == 0) it->r0 = it->forward ? std::min<int64_t>(st[m].lo, d.N) / G_M * G_M : 0;
         d.r0 = it->r0;
         if (d.r0 > 0 && d.q0 > 0) it->echelon.push_back(EchelonPanel{d.J, d.bcur, d.r0});
-        if (getenv("LINAC_DEBUG_STEPS"))
-            fprintf(stderr, "step J=%lld lo=%lld r0=%lld kA=%d q0=%d rank=%lld fast=%d\n", (long long)d.J,
-                    (long long)st[m].lo, (long long)d.r0, d.kA, d.q0, (long long)st[m].rank, (int)fast[m]);
+        if (debug_steps())
+            fprintf(stderr, "step J=%lld lo=%lld r0=%lld kA=%d q0=%d rank=%lld fast=%d host %.1f us spin %.1f us\n",
+                    (long long)d.J, (long long)st[m].lo, (long long)d.r0, d.kA, d.q0, (long long)st[m].rank,
+                    (int)fast[m],
+                    std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - step0).count(),
+                    spin_us);
         d.T0 = d.J + d.bcur;
         d.nT = d.M - d.T0;
         d.KqA = (int32_t)round_up(d.kA, 64);

@@ -388,7 +388,8 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"timeslice", &t.timeslice, 0, 1},
                  {"gemm_lds", &t.gemm_lds, 0, 1},
                  {"fast_panel", &t.fast_panel, 0, 2},
-                 {"echelon", &t.echelon, 0, 1}};
+                 {"echelon", &t.echelon, 0, 1},
+                 {"inner_compact", &t.inner_compact, 0, 1}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             if (value < k.lo || value > k.hi)
@@ -410,7 +411,8 @@ int linac_tune_get(const char* key, int64_t* value) {
                  {"float_blocked_min_rows", t.float_blocked_min_rows},
                  {"float_blocked_min_cols", t.float_blocked_min_cols}, {"gemm_group", t.gemm_group},
                  {"lookahead", t.lookahead}, {"timeslice", t.timeslice}, {"gemm_lds", t.gemm_lds},
-                 {"fast_panel", t.fast_panel}, {"echelon", t.echelon}};
+                 {"fast_panel", t.fast_panel}, {"echelon", t.echelon},
+                 {"inner_compact", t.inner_compact}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             *value = k.v;

@@ -134,6 +134,7 @@ struct Tuning {
     int64_t gemm_lds = 1;
     int64_t fast_panel = 2;
     int64_t echelon = 1;  // blocked GF(p): forward elimination plus back substitution (0: Gauss-Jordan)
+    int64_t inner_compact = 0;  // 32-bit inner factorisation in the compact form (32 slots a row)
 };
 Tuning& tuning();
 
