@@ -2009,9 +2009,23 @@ __global__ __launch_bounds__(64 * FB_WAVES, 4) void k_block_factor(const BMat<Li
 // blocks reached full rank: this gate gives every other matrix nT = 0 (the
 // fast kernels then skip it) and no tiles in the Z GEMM's schedule
 // (prefix sums pre[0..B] rewritten in place).  One thread.
+// (and, in the same launch, the compact blocks' status with the main
+// descriptors' panel-load snapshot of the first unused row, as k_gather_status)
 template <class LT>
-__global__ __launch_bounds__(64) void k_fast_gate(BMat<LT>* __restrict__ Dw, const BMat<LT>* __restrict__ Dc,
-                                                  int64_t* __restrict__ pre, int B) {
+__global__ __launch_bounds__(256) void k_fast_gate(BMat<LT>* __restrict__ Dw, const BMat<LT>* __restrict__ Dc,
+                                                   int64_t* __restrict__ pre, int B, ItemStatus* __restrict__ out,
+                                                   const BMat<LT>* __restrict__ Dl) {
+    for (int m = threadIdx.x; m < B; m += blockDim.x) {
+        const BMat<LT>& c = Dc[m];
+        ItemStatus st;
+        st.c0_next = c.bs->c0_next;
+        st.bcur = c.bs->bcur;
+        st.q0 = c.bs->q0;
+        st.pad = 0;
+        st.rank = c.st->rank;
+        st.lo = Dl[m].bs->lo0;
+        out[m] = st;
+    }
     if (threadIdx.x != 0) return;
     int64_t kept = 0, prev = pre[0];
     for (int m = 0; m < B; ++m) {
@@ -2028,8 +2042,9 @@ __global__ __launch_bounds__(64) void k_fast_gate(BMat<LT>* __restrict__ Dw, con
 // Operands of Z = -P B for the matrices on the fast path (Dw[m].nT > 0, depth
 // Kq = bcur rounded up to 64): workgroups [0, nl) write the limb planes of
 // the panel P = X[:, 0:bcur] (every row from r0 on, zero beyond bcur) with row stride Kq
-// into planesL1; the rest those of B^T, B[c, s] = M^-1[slot of column c, s]
-// taken from the factorised compact block, into planesB1 (row s, stride Kq).
+// into planesL1 and set those panel columns to zero; the rest write those of
+// B^T, B[c, s] = M^-1[slot of column c, s] taken from the factorised compact
+// block, into planesB1 (row s, stride Kq).
 template <class LT>
 __global__ __launch_bounds__(256) void k_fast_operands(const BMat<LT>* __restrict__ Dw, const BMat<LT>* __restrict__ Dc,
                                                        int nl) {
@@ -2051,6 +2066,10 @@ __global__ __launch_bounds__(256) void k_fast_operands(const BMat<LT>* __restric
 #pragma unroll
                 for (int l = 0; l < NL; ++l) dg[l][e] = g[l];
             }
+            // the panel columns of every such row end as zero (the pivot rows
+            // get theirs back from the block in k_fast_commit); c < Kq <= BO
+#pragma unroll
+            for (int e = 0; e < 4; ++e) d.X[r * XLD + c + e] = (typename LT::T)0;
 #pragma unroll
             for (int l = 0; l < NL; ++l) {
                 int32_t pk;
@@ -2069,22 +2088,6 @@ __global__ __launch_bounds__(256) void k_fast_operands(const BMat<LT>* __restric
         balanced_digits(centred(c.X[c.piv_rows[t] * XLD + BO + sidx], d.f.p), g);
 #pragma unroll
         for (int l = 0; l < NL; ++l) d.planesB1[l * STRIDE_W + (int64_t)sidx * Kq + col] = g[l];
-    }
-}
-
-// Panel columns of every row from r0 on of a fast-path matrix set to zero (the pivot
-// rows get theirs back from the block in k_fast_commit).
-template <class LT>
-__global__ __launch_bounds__(256) void k_fast_zero(const BMat<LT>* __restrict__ Dw) {
-    using T = typename LT::T;
-    const BMat<LT>& d = Dw[blockIdx.y];
-    if (d.nT == 0) return;
-    constexpr int PER = 16 / (int)sizeof(T);
-    const int vpr = (int)ceil_div(d.bcur, PER);
-    const v4i z = {0, 0, 0, 0};
-    for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < (d.N - d.r0) * vpr; x += (int64_t)gridDim.x * 256) {
-        const int64_t r = x / vpr;
-        gst<v4i>(d.X + (d.r0 + r) * XLD + (x - r * vpr) * PER, z);
     }
 }
 
@@ -2316,7 +2319,7 @@ class Engine {
     // side two steps after its copy was queued (the inner phase of a step
     // waits for the previous step's first part, which follows the parts
     // before it on the same stream).
-    static constexpr int NSLOT = 5;  // 3: compact block views, 4: fast-path GEMM descriptors
+    static constexpr int NSLOT = 4;  // 3: compact block views and the fast path's GEMM descriptors
     struct Staged {
         BM* desc;
         TileSched inner, wq, main;  // inner GEMM, composite W GEMM, trailing update
@@ -2627,8 +2630,10 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
         any = any || sel[m];
     }
     if (!any) return LINAC_OK;
-    Staged cs;
-    LINAC_TRY(upload(act, s, 3, 4, &cs));
+    // the compact views (part 4) and the Z GEMM's descriptors (part 5) in one copy
+    Staged cw[2];
+    LINAC_TRY(upload(act, s, 3, 4, cw, 2));
+    const Staged &cs = cw[0], &ws = cw[1];
     k_cand_gather<LT><<<B, 1024, 0, s>>>(in.desc, cs.desc);
     k_cand_copy<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc);
     const unsigned mu_grid =
@@ -2652,10 +2657,8 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     // the host reads whether its block reached full rank (k_fast_gate drops
     // the others on the device): the status read-back overlaps these kernels
     // instead of holding the GPU idle for a host round trip.
-    Staged ws;
-    LINAC_TRY(upload(act, s, 4, 5, &ws));
-    k_fast_gate<LT><<<1, 64, 0, s>>>(ws.desc, cs.desc, const_cast<int64_t*>(ws.main.pre), (int)B);
-    k_gather_status<LT><<<B, 1, 0, s>>>(cs.desc, cs.status, in.desc);
+    k_fast_gate<LT><<<1, 256, 0, s>>>(ws.desc, cs.desc, const_cast<int64_t*>(ws.main.pre), (int)B, cs.status,
+                                      in.desc);
     LINAC_HIP(hipMemcpyAsync(cs.status_h, cs.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
     LINAC_HIP(hipEventRecord(ev_status, s));
     const int nl = (int)gs_grid(maxN * (BO / 4), 16384);
@@ -2664,7 +2667,6 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     const unsigned grid = (unsigned)(ceil_div(ws.main_tiles, NUM_XCD) * NUM_XCD);
     if constexpr (wide) k_gemm_outer64<0><<<grid, G_T, 0, s>>>(ws.desc, ws.main);
     else launch_outer32<0>(grid, s, ws.desc, ws.main, ws.lds_main);
-    k_fast_zero<LT><<<dim3(gs_grid(maxN * (BO / 4), 8192), B), 256, 0, s>>>(ws.desc);
     k_fast_commit<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc, ws.desc, act[0]->d.e0);
     k_fast_finish<LT><<<B, 1024, 0, s>>>(in.desc, cs.desc, ws.desc);
     LINAC_HIP(hipGetLastError());
