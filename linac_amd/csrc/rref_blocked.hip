@@ -2341,7 +2341,7 @@ class Engine {
     std::vector<int64_t> lo_fast;  // first unused row at the panel load, read with the fast path's status
     double spin_us = 0;            // host wait for the last status read (debug output)
     int fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, int64_t ninner, int64_t maxN,
-                   std::vector<char>& fast);
+                   std::vector<char>& fast, Profiler* prof);
     void fill(const std::vector<It*>& act, int part, char* h, char* dv, size_t desc_b, size_t sched_b, Staged* out);
     int trailing(const Staged& t, Profiler* prof, int64_t tag, bool rhs_ready = false);
     // A trailing part beyond the next panel, queued to run beside a later
@@ -2535,8 +2535,11 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
             }
             const int64_t K = d.kA + d.q0;
             if (K > 0 && d.nT > 0) {
+                // modular MACs of the rows that need the update; bytes: C read
+                // and written plus both operands' limb planes once
                 out->macs += (double)std::max<int64_t>(0, d.N - d.r0 - K) * K * d.nT;
-                out->bytes += 2.0 * sizeof(typename LT::T) * (d.N - d.r0) * d.nT;
+                out->bytes += 2.0 * sizeof(typename LT::T) * (d.N - d.r0) * d.nT +
+                              (double)LT::NL * d.Kq * ((d.N - d.r0) + d.nT);
                 out->cols = std::max(out->cols, d.nT);
             }
         }
@@ -2618,7 +2621,7 @@ int Engine<LT>::init_items(const std::vector<It*>& items, hipStream_t s) {
 // phase.  Runs on s after the panel load.
 template <class LT>
 int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, int64_t ninner,
-                           int64_t maxN, std::vector<char>& fast) {
+                           int64_t maxN, std::vector<char>& fast, Profiler* prof) {
     constexpr bool wide = std::is_same<LT, LimbsU64>::value;
     const unsigned B = (unsigned)act.size();
     fast.assign(B, 0);
@@ -2665,8 +2668,13 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     const int nb = (int)ceil_div((int64_t)BO * BO, 256);
     k_fast_operands<LT><<<dim3(nl + nb, B), 256, 0, s>>>(ws.desc, cs.desc, nl);
     const unsigned grid = (unsigned)(ceil_div(ws.main_tiles, NUM_XCD) * NUM_XCD);
+    // (profiled as a launch of the trailing-update kernel; its work is known
+    // once the status says which blocks were of full rank)
+    const size_t zsample = prof ? prof->samples.size() : 0;
+    if (prof) LINAC_TRY(prof->sample_begin(s, act[0]->d.J, 0.0, 0.0));
     if constexpr (wide) k_gemm_outer64<0><<<grid, G_T, 0, s>>>(ws.desc, ws.main);
     else launch_outer32<0>(grid, s, ws.desc, ws.main, ws.lds_main);
+    if (prof) LINAC_TRY(prof->sample_end(s));
     k_fast_commit<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc, ws.desc, act[0]->d.e0);
     k_fast_finish<LT><<<B, 1024, 0, s>>>(in.desc, cs.desc, ws.desc);
     LINAC_HIP(hipGetLastError());
@@ -2683,6 +2691,14 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
         const BM& d = act[m]->d;
         fast[m] = sel[m] && cs.status_h[m].q0 == d.bcur && cs.status_h[m].rank == d.bcur ? 1 : 0;
         lo_fast[m] = cs.status_h[m].lo;
+        if (prof && fast[m]) {
+            // Z = -P B for the rows from r0 on outside the block (the block's
+            // rows get theirs from it): MACs as for a trailing update of depth
+            // and width bcur; bytes: Z read and written, both planes once
+            const double rows = (double)(d.N - d.r0), kq = (double)round_up(d.bcur, 64);
+            prof->samples[zsample].macs += std::max(0.0, rows - d.bcur) * d.bcur * d.bcur;
+            prof->samples[zsample].bytes += 2.0 * sizeof(typename LT::T) * rows * d.bcur + LT::NL * kq * (rows + d.bcur);
+        }
     }
     return LINAC_OK;
 }
@@ -2757,7 +2773,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     std::vector<char> fast(B, 0);
     int nfast = 0;
     if (tuning().fast_panel != 0 && !slicing) {
-        LINAC_TRY(fast_panel(act, s, in, ninner, maxN, fast));
+        LINAC_TRY(fast_panel(act, s, in, ninner, maxN, fast, prof));
         for (char f : fast) nfast += f;
     }
     std::vector<ItemStatus> st(B);

@@ -40,18 +40,27 @@ __global__ __launch_bounds__(256) void k_reduce_mod(typename F::T* __restrict__ 
     using T = typename F::T;
     constexpr int V = 16 / (int)sizeof(T);
     const int64_t nv = vec ? cols / V : 0;
+    constexpr int U = 4;  // vectors in flight per thread
+    const int64_t stride = (int64_t)gridDim.x * 256;
     for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
         T* row = A + r * ld;
-        for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nv; v += (int64_t)gridDim.x * 256) {
-            Vec16<T> x = reinterpret_cast<const Vec16<T>*>(row)[v];
-            bool changed = false;
+        for (int64_t v0 = (int64_t)blockIdx.x * 256 + threadIdx.x; v0 < nv; v0 += U * stride) {
+            Vec16<T> x[U];
 #pragma unroll
-            for (int e = 0; e < V; ++e) {
-                const T y = f.reduce(x.v[e]);
-                changed = changed || y != x.v[e];
-                x.v[e] = y;
+            for (int u = 0; u < U; ++u)
+                if (v0 + u * stride < nv) x[u] = reinterpret_cast<const Vec16<T>*>(row)[v0 + u * stride];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (v0 + u * stride >= nv) break;
+                bool changed = false;
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    const T y = f.reduce(x[u].v[e]);
+                    changed = changed || y != x[u].v[e];
+                    x[u].v[e] = y;
+                }
+                if (changed) reinterpret_cast<Vec16<T>*>(row)[v0 + u * stride] = x[u];
             }
-            if (changed) reinterpret_cast<Vec16<T>*>(row)[v] = x;
         }
         for (int64_t c = nv * V + (int64_t)blockIdx.x * 256 + threadIdx.x; c < cols; c += (int64_t)gridDim.x * 256) {
             const T y = f.reduce(row[c]);
@@ -242,8 +251,8 @@ int plain_launch_init(typename F::T* A, int64_t ld, int64_t N, int64_t M, const 
     using T = typename F::T;
     constexpr int V = 16 / (int)sizeof(T);
     const int vec = ((ld * (int64_t)sizeof(T)) % 16 == 0 && ((uintptr_t)A & 15) == 0) ? 1 : 0;
-    const int64_t bx = std::min<int64_t>(16, std::max<int64_t>(1, ceil_div(vec ? ceil_div(M, V) : M, 256)));
-    const int64_t by = std::min<int64_t>(std::max<int64_t>(N, 1), std::max<int64_t>(1, 32768 / bx));
+    const int64_t bx = std::min<int64_t>(16, std::max<int64_t>(1, ceil_div(vec ? ceil_div(M, V) : M, 1024)));
+    const int64_t by = std::min<int64_t>(std::max<int64_t>(N, 1), std::max<int64_t>(1, 16384 / bx));
     k_reduce_mod<F><<<dim3((unsigned)bx, (unsigned)by), 256, 0, s>>>(A, ld, N, M, f, vec);
     k_init_state<<<(int)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(N, 256))), 256, 0, s>>>(ws.st, ws.used, N);
     LINAC_HIP(hipGetLastError());
