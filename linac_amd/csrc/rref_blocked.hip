@@ -175,13 +175,23 @@ static bool debug_steps() {
 template <class LT>
 __device__ static void block_state_init(const BMat<LT>& d, int reset_timers);
 
-// (workgroup 0 of each matrix also resets its inner-step state)
 template <class LT>
-__global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__ D, int reset_timers) {
+__device__ static void cand_gather_body(const BMat<LT>& d, const BMat<LT>& c);
+
+// (workgroup 0 of each matrix also resets its inner-step state; with Dc, the
+// last workgroup gathers the full-rank fast path's candidate rows)
+template <class LT>
+__global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__ D, int reset_timers,
+                                                    const BMat<LT>* __restrict__ Dc = nullptr) {
     using T = typename LT::T;
     constexpr int PER = 16 / (int)sizeof(T);
     constexpr int VPR = XLD / PER;  // vectors per panel-buffer row
     const BMat<LT>& d = D[blockIdx.y];
+    const unsigned nload = gridDim.x - (Dc ? 1u : 0u);
+    if (Dc && blockIdx.x == nload) {
+        cand_gather_body<LT>(d, Dc[blockIdx.y]);
+        return;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         block_state_init<LT>(d, reset_timers);
         d.bs->lo0 = d.st->first_unused;  // no pivot of this panel lies above it
@@ -191,7 +201,7 @@ __global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__
     // back; the candidates, the fast path's GEMM rows and everything written
     // back lie at or beyond it)
     const int64_t total = (d.N - d.r0) * VPR;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)nload * blockDim.x) {
         const int64_t i = x / VPR, r = d.r0 + i;
         const int j = (int)(x - i * VPR) * PER;
         Vec16<T> v = {};
@@ -1637,15 +1647,15 @@ __global__ void k_gather_status(const BMat<LT>* __restrict__ D, ItemStatus* out,
 // ---------------------------------------------------------------------------
 
 // Candidate block of each matrix with Dc[m].N > 0: the first Dc[m].N unused
-// rows (row order) into Dc[m].cand, their panel-buffer rows into the compact
-// buffer; the compact view's state reset (a matrix with Dc[m].N == 0 gets a
-// finished block state, so that the compact inner steps skip it).
+// rows (row order) into Dc[m].cand; the compact view's state reset (a matrix
+// with Dc[m].N == 0 gets a finished block state, so that the compact inner
+// steps skip it).  One workgroup of any whole number of waves (<= 16) per
+// matrix, probing blockDim.x rows at a time; it runs as the last workgroup of
+// the panel load (it reads only row flags and the first unused row, which the
+// load does not touch).
 template <class LT>
-__global__ __launch_bounds__(1024) void k_cand_gather(const BMat<LT>* __restrict__ D, const BMat<LT>* __restrict__ Dc) {
-    using T = typename LT::T;
-    const BMat<LT>& d = D[blockIdx.x];
-    const BMat<LT>& c = Dc[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+__device__ static void cand_gather_body(const BMat<LT>& d, const BMat<LT>& c) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
     if (tid == 0) {
         c.st->prow = -1;
         c.st->rank = 0;
@@ -1660,7 +1670,7 @@ __global__ __launch_bounds__(1024) void k_cand_gather(const BMat<LT>* __restrict
     __shared__ int s_wcnt[16];
     if (tid == 0) s_total = 0;
     __syncthreads();
-    for (int64_t base = d.st->first_unused; base < d.N; base += 1024) {
+    for (int64_t base = d.st->first_unused; base < d.N; base += blockDim.x) {
         const int64_t r = base + tid;
         const bool fl = r < d.N && !d.used[r];
         const unsigned long long m = __ballot(fl);
@@ -1673,7 +1683,7 @@ __global__ __launch_bounds__(1024) void k_cand_gather(const BMat<LT>* __restrict
         __syncthreads();
         if (tid == 0) {
             int sum = 0;
-            for (int q = 0; q < 16; ++q) sum += s_wcnt[q];
+            for (int q = 0; q < nw; ++q) sum += s_wcnt[q];
             s_total += sum;
         }
         __syncthreads();
@@ -2313,13 +2323,15 @@ class Engine {
     }
 
   private:
-    // Staging slots (pinned host + device): 0 = inner phase; 1, 2 = the three
-    // parts of the trailing update of even and odd steps.  A slot's device
+    // Staging slots (pinned host + device): 0 = inner phase (with the fast
+    // path's compact views and Z-GEMM descriptors); 1, 2 = the three parts of
+    // the trailing update of even and odd steps.  Slot 0 is refilled on `s`
+    // after every kernel of the previous step that read it.  A slot's device
     // copy is refilled on `so` after every kernel that read it, and its host
     // side two steps after its copy was queued (the inner phase of a step
     // waits for the previous step's first part, which follows the parts
     // before it on the same stream).
-    static constexpr int NSLOT = 4;  // 3: compact block views and the fast path's GEMM descriptors
+    static constexpr int NSLOT = 3;
     struct Staged {
         BM* desc;
         TileSched inner, wq, main;  // inner GEMM, composite W GEMM, trailing update
@@ -2336,12 +2348,14 @@ class Engine {
     // of the columns beyond it (composite updates only)
     // parts part .. part + nparts - 1 into one staging slot, one copy
     int upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out, int nparts = 1);
+    // the listed parts (at most 8) into one staging slot, one copy
+    int upload_list(const std::vector<It*>& act, hipStream_t s, int slot, const int* parts, int nparts, Staged* out);
     // parts 4 (compact block views) and 5 (fast-path GEMM Z = -P B): which items take part
     std::vector<char> sel;
     std::vector<int64_t> lo_fast;  // first unused row at the panel load, read with the fast path's status
     double spin_us = 0;            // host wait for the last status read (debug output)
-    int fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, int64_t ninner, int64_t maxN,
-                   std::vector<char>& fast, Profiler* prof);
+    int fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, const Staged& cs, const Staged& ws,
+                   int64_t ninner, int64_t maxN, std::vector<char>& fast, Profiler* prof);
     void fill(const std::vector<It*>& act, int part, char* h, char* dv, size_t desc_b, size_t sched_b, Staged* out);
     int trailing(const Staged& t, Profiler* prof, int64_t tag, bool rhs_ready = false);
     // A trailing part beyond the next panel, queued to run beside a later
@@ -2462,6 +2476,14 @@ int Engine<LT>::stage(int slot, size_t bytes) {
 // (pb, mb) composite W of a deferred panel, (pc, mc) trailing update.
 template <class LT>
 int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int part, Staged* out, int nparts) {
+    int parts[8];
+    for (int i = 0; i < nparts; ++i) parts[i] = part + i;
+    return upload_list(act, s, slot, parts, nparts, out);
+}
+
+template <class LT>
+int Engine<LT>::upload_list(const std::vector<It*>& act, hipStream_t s, int slot, const int* parts, int nparts,
+                            Staged* out) {
     const size_t B = act.size();
     const size_t desc_b = (size_t)round_up((int64_t)(B * sizeof(BM)), 256);
     const size_t sched_b = (size_t)round_up((int64_t)((6 * B + 3) * 8), 256);
@@ -2470,7 +2492,7 @@ int Engine<LT>::upload(const std::vector<It*>& act, hipStream_t s, int slot, int
     LINAC_TRY(stage(slot, one * nparts));
     char* h = (char*)h_buf[slot];
     char* dv = (char*)d_buf[slot];
-    for (int i = 0; i < nparts; ++i) fill(act, part + i, h + i * one, dv + i * one, desc_b, sched_b, out + i);
+    for (int i = 0; i < nparts; ++i) fill(act, parts[i], h + i * one, dv + i * one, desc_b, sched_b, out + i);
     LINAC_HIP(hipMemcpyAsync(dv, h, one * (nparts - 1) + desc_b + sched_b, hipMemcpyHostToDevice, s));
     return LINAC_OK;
 }
@@ -2620,24 +2642,12 @@ int Engine<LT>::init_items(const std::vector<It*>& items, hipStream_t s) {
 // the commit.  `fast[m]` tells which items are done with this panel's inner
 // phase.  Runs on s after the panel load.
 template <class LT>
-int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, int64_t ninner,
-                           int64_t maxN, std::vector<char>& fast, Profiler* prof) {
+int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Staged& in, const Staged& cs,
+                           const Staged& ws, int64_t ninner, int64_t maxN, std::vector<char>& fast, Profiler* prof) {
     constexpr bool wide = std::is_same<LT, LimbsU64>::value;
     const unsigned B = (unsigned)act.size();
     fast.assign(B, 0);
-    sel.assign(B, 0);
-    bool any = false;
-    for (unsigned m = 0; m < B; ++m) {
-        const BM& d = act[m]->d;
-        sel[m] = d.N - act[m]->rank >= d.bcur ? 1 : 0;
-        any = any || sel[m];
-    }
-    if (!any) return LINAC_OK;
-    // the compact views (part 4) and the Z GEMM's descriptors (part 5) in one copy
-    Staged cw[2];
-    LINAC_TRY(upload(act, s, 3, 4, cw, 2));
-    const Staged &cs = cw[0], &ws = cw[1];
-    k_cand_gather<LT><<<B, 1024, 0, s>>>(in.desc, cs.desc);
+    // (the candidate rows were gathered by the panel load's last workgroup)
     k_cand_copy<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc);
     const unsigned mu_grid =
         (unsigned)(MU_BLOCKS + ((wide || cs.inner_split > 1) ? std::min<int64_t>(4096, ceil_div(BO, MU_ROWS)) : 0));
@@ -2726,15 +2736,6 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         maxNX = std::max(maxNX, d.N * XLD);
         ninner = std::max<int64_t>(ninner, ceil_div(d.bcur, BI));
     }
-    // --- inner phase on s, after the previous step's first trailing part
-    Staged in;
-    LINAC_TRY(upload(act, s, 0, 0, &in));  // (its copy need not wait for the event)
-    if (ev_pending) LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
-    k_panel_load<LT><<<dim3(gs_grid(maxNX / 4, 8192), B), 256, 0, s>>>(in.desc, first_step ? 1 : 0);
-    first_step = false;
-    // (the 32-bit inner GEMM captures its left operand itself)
-    const bool planes_a = wide || in.inner_split > 1;
-    const unsigned mu_grid = (unsigned)(MU_BLOCKS + (planes_a ? std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)) : 0));
     // Trailing parts due now run in slices, one beside each inner step's
     // factorisation and U; the step's inner GEMM, which needs the whole GPU,
     // waits for its slice.
@@ -2744,6 +2745,32 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         if (b.due <= step_no) due_tiles += b.t.main_tiles - b.next;
     const bool slicing = due_tiles > 0 && tuning().lookahead != 0 && tuning().timeslice != 0;
     const int64_t per_slice = slicing ? ceil_div(due_tiles, std::min<int64_t>(ninner, NEV)) : 0;
+    // which items may take the full-rank fast path (at least bcur unused rows)
+    sel.assign(B, 0);
+    bool any_fast = false;
+    if (tuning().fast_panel != 0 && !slicing)
+        for (unsigned m = 0; m < B; ++m) {
+            const BM& d = act[m]->d;
+            sel[m] = d.N - act[m]->rank >= d.bcur ? 1 : 0;
+            any_fast = any_fast || sel[m];
+        }
+    // --- inner phase on s, after the previous step's first trailing part; its
+    // descriptors, with the fast path's compact views (part 4) and Z-GEMM
+    // descriptors (part 5), in one copy (which need not wait for the event)
+    Staged st3[3];
+    const int parts[3] = {0, 4, 5};
+    LINAC_TRY(upload_list(act, s, 0, parts, any_fast ? 3 : 1, st3));
+    const Staged& in = st3[0];
+    if (ev_pending) LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
+    const unsigned lgrid = gs_grid(maxNX / 4, 8192);
+    if (any_fast)  // the last workgroup gathers the candidate rows
+        k_panel_load<LT><<<dim3(lgrid + 1, B), 256, 0, s>>>(in.desc, first_step ? 1 : 0, st3[1].desc);
+    else
+        k_panel_load<LT><<<dim3(lgrid, B), 256, 0, s>>>(in.desc, first_step ? 1 : 0);
+    first_step = false;
+    // (the 32-bit inner GEMM captures its left operand itself)
+    const bool planes_a = wide || in.inner_split > 1;
+    const unsigned mu_grid = (unsigned)(MU_BLOCKS + (planes_a ? std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)) : 0));
     auto group = [&](int64_t i) -> int {
         const bool sl = slicing && i < NEV;
         if (sl) {
@@ -2772,8 +2799,8 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     // over every row for the rest
     std::vector<char> fast(B, 0);
     int nfast = 0;
-    if (tuning().fast_panel != 0 && !slicing) {
-        LINAC_TRY(fast_panel(act, s, in, ninner, maxN, fast, prof));
+    if (any_fast) {
+        LINAC_TRY(fast_panel(act, s, in, st3[1], st3[2], ninner, maxN, fast, prof));
         for (char f : fast) nfast += f;
     }
     std::vector<ItemStatus> st(B);
