@@ -138,11 +138,13 @@ def load():
         # the HIP runtime are still up (the library's own exit hook would run
         # after the interpreter is gone)
         atexit.register(_shutdown_at_exit)
-    # a saved tuning for this device (linac_amd.autotune) applies from the start
+    # a saved tuning for this device (linac_amd.autotune) applies from the
+    # start, except the float crossovers (they can change float results in the
+    # last bits; autotune.load() applies them on request)
     try:
         from . import autotune
         if lib.linac_available():
-            autotune.load()
+            autotune.load(float_crossovers=False)
     except (OSError, ValueError, RuntimeError):
         pass
     return lib

@@ -54,20 +54,30 @@ def tuning_file():
     return os.path.join(os.path.expanduser("~"), ".cache", "linac_amd", f"tuning-{tag}.json")
 
 
-def apply(values):
+# Knobs that move float matrices between the plain and blocked methods, whose
+# results agree only within rounding; every other knob leaves every result
+# unchanged (GF(p) results are exact and unique).
+FLOAT_CROSSOVERS = ("float_blocked_min_rows", "float_blocked_min_cols")
+
+
+def apply(values, float_crossovers=True):
     for k, v in values.items():
-        if k in KNOBS:
+        if k in KNOBS and (float_crossovers or k not in FLOAT_CROSSOVERS):
             set_knob(k, v)
 
 
-def load(path=None):
-    """Apply a saved tuning (returns it), or None when there is none."""
+def load(path=None, float_crossovers=True):
+    """Apply a saved tuning (returns it), or None when there is none.
+    ``float_crossovers=False`` applies only the knobs that cannot change a
+    result (the package does this by itself when it loads the library, so
+    that float results do not depend on a file outside the repository unless
+    the caller asks for it)."""
     path = path or tuning_file()
     if not os.path.exists(path):
         return None
     with open(path) as f:
         data = json.load(f)
-    apply(data.get("knobs", {}))
+    apply(data.get("knobs", {}), float_crossovers)
     return data
 
 

@@ -169,3 +169,21 @@ def test_tuning_file_round_trip(tmp_path):
         assert autotune.load(str(tmp_path / "missing.json")) is None
     finally:
         autotune.apply(before)
+
+
+def test_tuning_file_without_float_crossovers(tmp_path):
+    """The tuning the package applies by itself leaves the float crossovers
+    (which can change float results in the last bits) alone."""
+    from linac_amd import autotune
+    before = autotune.current()
+    path = tmp_path / "t.json"
+    try:
+        autotune.save({"knobs": dict(before, gf_blocked_min_rows=555,
+                                     float_blocked_min_rows=before["float_blocked_min_rows"] + 7)}, str(path))
+        autotune.load(str(path), float_crossovers=False)
+        assert autotune.get("gf_blocked_min_rows") == 555
+        assert autotune.get("float_blocked_min_rows") == before["float_blocked_min_rows"]
+        autotune.load(str(path))
+        assert autotune.get("float_blocked_min_rows") == before["float_blocked_min_rows"] + 7
+    finally:
+        autotune.apply(before)
