@@ -594,11 +594,14 @@ static void launch_inner_factor(unsigned B, hipStream_t s, const BMat<LT>* desc,
 // several of its workgroups share a row stripe (always for 64-bit storage) its
 // left operand is captured here, before any of them runs; a 32-bit workgroup
 // that owns its stripe captures its own.  The pivot rows X[R, :] get U here.
-// Workgroups [0, MU_BLOCKS) build U (8 columns each, 32 threads per column),
-// the rest planesA (MU_ROWS rows each, grid-strided).
-constexpr int MU_COLS = 8;
+// Workgroups [0, MU_BLOCKS) build U (2 columns each; each entry's dot product
+// of length k split over MU_SPLIT adjacent lanes and summed by shuffles, so
+// that its chain of dependent products is k / MU_SPLIT long), the rest
+// planesA (MU_ROWS rows each, grid-strided).
+constexpr int MU_COLS = 2;
+constexpr int MU_SPLIT = 4;
 constexpr int MU_BLOCKS = XLD / MU_COLS;
-static_assert(MU_COLS * CI == 256 && 2 * CI == KPI, "one thread per (column, pivot slot); slots pad to KPI");
+static_assert(MU_COLS * CI * MU_SPLIT == 256 && 2 * CI == KPI, "one thread per (column, pivot slot, part); slots pad to KPI");
 constexpr int MU_ROWS = 16;
 
 template <class LT>
@@ -646,12 +649,20 @@ __device__ static void make_u_body(typename LT::T* __restrict__ X, int64_t N, ty
     // the block's columns of the k pivot rows, fetched at once (a load per
     // term of the dot product would expose the memory latency k times)
     __shared__ T s_x[CI][MU_COLS];
-    const int ts = fl % CI, y = fl / CI, n = blockIdx.x * MU_COLS + y;  // pivot slot, column
-    if (ts < k) s_x[ts][y] = X[s_R[ts] * XLD + n];
+    for (int x = fl; x < k * MU_COLS; x += 256) {
+        const int sl = x / MU_COLS, yy = x - sl * MU_COLS;
+        s_x[sl][yy] = X[s_R[sl] * XLD + blockIdx.x * MU_COLS + yy];
+    }
     __syncthreads();
+    // lane layout: part = fl % MU_SPLIT (adjacent lanes), pivot slot, column
+    const int part = fl % MU_SPLIT, ts = (fl / MU_SPLIT) % CI, y = fl / (MU_SPLIT * CI);
+    const int64_t n = (int64_t)blockIdx.x * MU_COLS + y;
     T u = 0;
-    if (ts < k) {
-        for (int s = 0; s < k; ++s) u = f.add(u, f.mont(s_inv[ts * CI + s], s_x[s][y]));
+    if (ts < k)
+        for (int s = part; s < k; s += MU_SPLIT) u = f.add(u, f.mont(s_inv[ts * CI + s], s_x[s][y]));
+#pragma unroll
+    for (int off = 1; off < MU_SPLIT; off <<= 1) u = f.add(u, (T)__shfl_xor(u, off, 64));
+    if (ts < k && part == 0) {
         U[ts * XLD + n] = u;
         // the pivot rows' new values go straight into the panel (the inner
         // GEMM leaves those rows alone); every read of X by this workgroup is
@@ -661,10 +672,11 @@ __device__ static void make_u_body(typename LT::T* __restrict__ X, int64_t N, ty
     }
     int8_t d[NL] = {};
     if (ts < k) balanced_digits(centred(u, f.p), d);
+    // part 0 writes slot ts, part 1 slot ts + CI (the zero padding up to KPI)
+    if (part < 2) {
+        const int slot = ts + part * CI;
 #pragma unroll
-    for (int l = 0; l < NL; ++l) {  // slots ts and ts + CI (the zero padding up to KPI)
-        planesU[l * strideU + (int64_t)n * KPI + ts] = d[l];
-        planesU[l * strideU + (int64_t)n * KPI + CI + ts] = 0;
+        for (int l = 0; l < NL; ++l) planesU[l * strideU + n * KPI + slot] = part == 0 ? d[l] : (int8_t)0;
     }
 }
 
