@@ -181,3 +181,23 @@ def test_a_planted_fold_error_is_caught(harness):
     s = [0, 0, 0, 0, 0, 0, 1]
     got = int(harness([f"F32 {p} 5 " + " ".join(map(str, s))])[0])
     assert got == _fold_expected(p, 5, s) and got != (5 - 0) % p
+
+
+def test_mersenne61_products_random(harness):
+    """p = 2^61 - 1 takes the Mersenne path (four 32-bit products, a fold at
+    bit 61, a rotation for R^-1): many random and near-2^32 operands."""
+    p = 2 ** 61 - 1
+    rng = random.Random(61)
+    near = [2 ** 32 - 1, 2 ** 32, 2 ** 32 + 1, 2 ** 61 - 2, 2 ** 60, 2 ** 29, (2 ** 61 - 1) // 3]
+    ops = near + [rng.randrange(p) for _ in range(60)]
+    lines, want = [], []
+    rinv = pow(2 ** 64, -1, p)
+    for _ in range(3000):
+        h, x, a, y = (rng.choice(ops) for _ in range(4))
+        lines.append(f"U64 {p} {h} {x} {a} {y}")
+        want.append((h * x - a * y) * rinv % p)
+        lines.append(f"M64 {p} {h} {x}")
+        want.append(h * x % p)
+    got = harness(lines)
+    bad = [(ln, g, w) for ln, g, w in zip(lines, got, want) if int(g) != w]
+    assert not bad, bad[:3]
