@@ -2686,7 +2686,9 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
                                       in.desc);
     LINAC_HIP(hipMemcpyAsync(cs.status_h, cs.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
     LINAC_HIP(hipEventRecord(ev_status, s));
-    const int nl = (int)gs_grid(maxN * (BO / 4), 16384);
+    int64_t maxR = 1;  // rows from the row offset on (the part-5 descriptors carry the items' current r0)
+    for (auto* it : act) maxR = std::max<int64_t>(maxR, it->d.N - it->d.r0);
+    const int nl = (int)gs_grid(maxR * (BO / 4), 16384);
     const int nb = (int)ceil_div((int64_t)BO * BO, 256);
     k_fast_operands<LT><<<dim3(nl + nb, B), 256, 0, s>>>(ws.desc, cs.desc, nl);
     const unsigned grid = (unsigned)(ceil_div(ws.main_tiles, NUM_XCD) * NUM_XCD);
@@ -2736,7 +2738,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     if (act.empty()) return LINAC_OK;
     LINAC_TRY(ensure_stream());
     const unsigned B = (unsigned)act.size();
-    int64_t maxN = 1, maxNX = 1, ninner = 1;
+    int64_t maxN = 1, ninner = 1;
     for (auto* it : act) {
         BM& d = it->d;
         d.bcur = (int32_t)std::min<int64_t>(BO, d.M - d.J);
@@ -2745,7 +2747,6 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         d.Kq = 0;
         d.nT = 0;
         maxN = std::max(maxN, d.N);
-        maxNX = std::max(maxNX, d.N * XLD);
         ninner = std::max<int64_t>(ninner, ceil_div(d.bcur, BI));
     }
     // Trailing parts due now run in slices, one beside each inner step's
@@ -2774,7 +2775,9 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     LINAC_TRY(upload_list(act, s, 0, parts, any_fast ? 3 : 1, st3));
     const Staged& in = st3[0];
     if (ev_pending) LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
-    const unsigned lgrid = gs_grid(maxNX / 4, 8192);
+    int64_t maxRX = 1;  // panel-buffer entries from the row offset on
+    for (auto* it : act) maxRX = std::max<int64_t>(maxRX, (it->d.N - it->d.r0) * XLD);
+    const unsigned lgrid = gs_grid(maxRX / 4, 8192);
     if (any_fast)  // the last workgroup gathers the candidate rows
         k_panel_load<LT><<<dim3(lgrid + 1, B), 256, 0, s>>>(in.desc, first_step ? 1 : 0, st3[1].desc);
     else
@@ -2896,7 +2899,9 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         // -W of the current panel (kept in W when deferred) beside, for
         // composite items, the planes of W[Q']; then W <- W + W' W[Q']; then -W
         // of the composite beside the right operand of the next panel's update
-        const int nl = (int)gs_grid(maxN * (BO / 4), 16384);
+        int64_t maxR = 1;  // rows from this step's row offset on
+        for (auto* it : act) maxR = std::max<int64_t>(maxR, it->d.N - it->d.r0);
+        const int nl = (int)gs_grid(maxR * (BO / 4), 16384);
         const bool comp = t1.wq_tiles > 0, main1 = t1.main_tiles > 0;
         constexpr int RW = BO / 64;  // right-operand tiles of W[Q'] per dimension
         const int rx = (int)std::min<int64_t>(2048, ceil_div(t1.cols, 64));
@@ -2927,8 +2932,8 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     // (after it on that stream) overwrites.
     LINAC_HIP(hipEventRecord(ev_ready, so));
     LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
-    int64_t maxpanel = 1;
-    for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, it->d.N * it->d.bcur);
+    int64_t maxpanel = 1;  // (one row per workgroup pass, rows from the row offset on)
+    for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, (it->d.N - it->d.r0) * it->d.bcur);
     k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, s>>>(t1.desc);
     LINAC_HIP(hipGetLastError());
     // The next inner phase waits only for the above.  The composite update's
