@@ -150,6 +150,9 @@ struct BMat {
     int64_t T0, nT;     // trailing columns [T0, T0 + nT)
     int64_t r0;         // first row of the trailing update, the panel write-back and the left operand
                         // (a multiple of 64; every row above it is a pivot row of an earlier panel)
+    int64_t Jst, bst;   // panel [Jst, Jst + bst) whose write-back the next panel load does (bst = 0: none)
+    int32_t store_now;  // this step writes its panel back itself (the item's last step)
+    int32_t pad1;
     int64_t* cand;      // [BO] rows of the compact candidate block (full-rank panel fast path)
 };
 
@@ -200,10 +203,21 @@ __global__ __launch_bounds__(256) void k_panel_load(const BMat<LT>* __restrict__
     // that no later update touches, so their panel-buffer rows are never read
     // back; the candidates, the fast path's GEMM rows and everything written
     // back lie at or beyond it)
+    // The previous panel's write-back (bst > 0) goes first, element by element
+    // from the same vector of the buffer: the previous step's rows from r0 on,
+    // which are the rows loaded here.
     const int64_t total = (d.N - d.r0) * VPR;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (int64_t)nload * blockDim.x) {
         const int64_t i = x / VPR, r = d.r0 + i;
         const int j = (int)(x - i * VPR) * PER;
+        if (j < d.bst) {
+            const v4i raw = gld<v4i>(d.X + r * XLD + j);
+            Vec16<T> o;
+            __builtin_memcpy(&o, &raw, 16);
+#pragma unroll
+            for (int e = 0; e < PER; ++e)
+                if (j + e < d.bst) d.A[r * d.ld + d.Jst + j + e] = o.v[e];
+        }
         Vec16<T> v = {};
         if (j + PER <= d.bcur) {
             const v4i raw = gld<v4i>(d.A + r * d.ld + d.J + j);
@@ -228,7 +242,7 @@ __global__ __launch_bounds__(256) void k_panel_store(const BMat<LT>* __restrict_
     static_assert(BO == 256, "one thread per panel column");
     const BMat<LT>& d = D[blockIdx.y];
     const int j = threadIdx.x;
-    if (j >= d.bcur) return;
+    if (j >= d.bcur || !d.store_now) return;  // (otherwise the next panel load writes it back)
     // (rows above r0 keep the values they had: the trailing update skips them too)
     for (int64_t r = d.r0 + blockIdx.x; r < d.N; r += gridDim.x) d.A[r * d.ld + d.J + j] = d.X[r * XLD + j];
 }
@@ -2852,11 +2866,13 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
             st[m] = ItemStatus{d.bcur, d.bcur, d.bcur, 0, act[m]->rank + d.bcur, lo_fast[m]};
         }
         LINAC_HIP(hipEventRecord(ev_inner, s));
-        LINAC_HIP(hipStreamWaitEvent(so, ev_inner, 0));
     }
     for (unsigned m = 0; m < B; ++m) act[m]->rank = st[m].rank;
     // whatever of the parts due by now is left (rank-deficient panels add
-    // groups; slices round), before this step's own trailing update
+    // groups; slices round), before this step's own trailing update and
+    // before the copy below refills the staging slot of the parts two steps
+    // back (never the case on the fast path, which runs without slices: its
+    // parts due now were launched at the end of the previous step)
     LINAC_TRY(flush_due(step_no, prof, tag));
     // --- trailing update on so: first the next panel's columns (then the
     // panel goes back to A and the next inner phase may start), then the rest.
@@ -2886,12 +2902,17 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
         d.KqA = (int32_t)round_up(d.kA, 64);
         d.Kq = d.KqA + (int32_t)round_up(d.q0, 64);
         d.full = st[m].rank >= d.N ? 1 : 0;  // no row left to pivot: the remaining columns are final after this step
+        d.store_now = (d.J + d.bcur >= d.M || d.full) ? 1 : 0;  // the item's last step
+        d.bst = 0;  // (the panel load of this step has written the previous panel back)
         d.save_w = (!d.full && d.kA == 0 && d.q0 > 0 && d.nT > BO) ? 1 : 0;
     }
     const int p1 = 1 + parity;
     parity ^= 1;
     Staged tp[3];
+    // (the descriptor copy goes ahead of the wait for the fast path's tail
+    // kernels, which do not read it)
     LINAC_TRY(upload(act, so, p1, 1, tp, 3));
+    if (nfast == (int)B) LINAC_HIP(hipStreamWaitEvent(so, ev_inner, 0));
     const Staged &t1 = tp[0], &t2 = tp[1], &t3 = tp[2];
     bool operands = t1.main_tiles > 0;
     for (auto* it : act) operands = operands || it->d.save_w;
@@ -2932,9 +2953,12 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     // (after it on that stream) overwrites.
     LINAC_HIP(hipEventRecord(ev_ready, so));
     LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
-    int64_t maxpanel = 1;  // (one row per workgroup pass, rows from the row offset on)
-    for (auto* it : act) maxpanel = std::max<int64_t>(maxpanel, (it->d.N - it->d.r0) * it->d.bcur);
-    k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, s>>>(t1.desc);
+    // An item's last panel is written back here; the others' by the next
+    // panel load, from the same buffer rows it then refills.
+    int64_t maxpanel = 0;  // (one row per workgroup pass, rows from the row offset on)
+    for (auto* it : act)
+        if (it->d.store_now) maxpanel = std::max<int64_t>(maxpanel, (it->d.N - it->d.r0) * it->d.bcur);
+    if (maxpanel > 0) k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, s>>>(t1.desc);
     LINAC_HIP(hipGetLastError());
     // The next inner phase waits only for the above.  The composite update's
     // first half is due beside the next step's inner phase, its second half
@@ -2952,6 +2976,10 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     ev_pending = !lookahead;  // with look-ahead the inner stream has waited already
     for (auto* it : act) {
         it->d.kA = it->d.save_w ? it->d.q0 : 0;  // deferred to the next step
+        if (!it->d.store_now) {  // the next panel load writes this panel back
+            it->d.Jst = it->d.J;
+            it->d.bst = it->d.bcur;
+        }
         it->d.J += it->d.bcur;
         if (it->d.J >= it->d.M || it->d.full) it->finished = true;
     }
