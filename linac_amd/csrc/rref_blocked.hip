@@ -2952,13 +2952,16 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     // changes, and reads the panel buffer, which only the next inner phase
     // (after it on that stream) overwrites.
     LINAC_HIP(hipEventRecord(ev_ready, so));
-    LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
     // An item's last panel is written back here; the others' by the next
-    // panel load, from the same buffer rows it then refills.
+    // panel load, from the same buffer rows it then refills.  (The next step
+    // waits for ev_ready after its descriptor copy, which needs no wait.)
     int64_t maxpanel = 0;  // (one row per workgroup pass, rows from the row offset on)
     for (auto* it : act)
         if (it->d.store_now) maxpanel = std::max<int64_t>(maxpanel, (it->d.N - it->d.r0) * it->d.bcur);
-    if (maxpanel > 0) k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, s>>>(t1.desc);
+    if (maxpanel > 0) {
+        LINAC_HIP(hipStreamWaitEvent(s, ev_ready, 0));
+        k_panel_store<LT><<<dim3(gs_grid(maxpanel, 8192), B), 256, 0, s>>>(t1.desc);
+    }
     LINAC_HIP(hipGetLastError());
     // The next inner phase waits only for the above.  The composite update's
     // first half is due beside the next step's inner phase, its second half
@@ -2973,7 +2976,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     else if (tuning().timeslice == 0) LINAC_TRY(flush_due(step_no + 1, prof, tag));
     if (!lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));  // the next inner phase waits for all of it
     ++step_no;
-    ev_pending = !lookahead;  // with look-ahead the inner stream has waited already
+    ev_pending = true;  // the next panel load waits for ev_ready (the next panel's columns, or all parts)
     for (auto* it : act) {
         it->d.kA = it->d.save_w ? it->d.q0 : 0;  // deferred to the next step
         if (!it->d.store_now) {  // the next panel load writes this panel back
