@@ -151,6 +151,7 @@ struct BMat {
     int64_t r0;         // first row of the trailing update, the panel write-back and the left operand
                         // (a multiple of 64; every row above it is a pivot row of an earlier panel)
     int64_t Jst, bst;   // panel [Jst, Jst + bst) whose write-back the next panel load does (bst = 0: none)
+    int64_t rank0;      // rank when the step began
     int32_t store_now;  // this step writes its panel back itself (the item's last step)
     int32_t pad1;
     int64_t* cand;      // [BO] rows of the compact candidate block (full-rank panel fast path)
@@ -2129,7 +2130,7 @@ __global__ __launch_bounds__(256) void k_fast_operands(const BMat<LT>* __restric
 
 // The block's pivot rows back into the panel buffer (normalised panel row
 // and M^-1 row) and their bookkeeping -- row flags, epochs, slots, pivot lists
-// -- two pivots per workgroup (blockIdx.x); k_fast_finish then advances the
+// -- two pivots per workgroup (blockIdx.x); the last workgroup advances the
 // first unused row and sets rank and block state.
 template <class LT>
 __global__ __launch_bounds__(256) void k_fast_commit(const BMat<LT>* __restrict__ D, const BMat<LT>* __restrict__ Dc,
@@ -2139,6 +2140,34 @@ __global__ __launch_bounds__(256) void k_fast_commit(const BMat<LT>* __restrict_
     const BMat<LT>& c = Dc[blockIdx.y];
     if (Dw[blockIdx.y].nT == 0) return;
     const int q = (int)c.st->rank;  // == bcur on the fast path
+    if (blockIdx.x == gridDim.x - 1) {
+        // the bookkeeping: rank, block state and the first unused row, which
+        // lies beyond the block's last row (the block is the first q unused
+        // rows, so every row up to its last one is used once it is committed)
+        // -- found from rows no other workgroup of this launch writes
+        const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+        __shared__ unsigned long long s_first;
+        if (tid == 0) s_first = ~0ull;
+        __syncthreads();
+        for (int64_t base = c.cand[q - 1] + 1; base < d.N; base += blockDim.x) {
+            const int64_t r = base + tid;
+            const unsigned long long m = __ballot(r < d.N && !d.used[r]);
+            if (lane == 0 && m) atomicMin(&s_first, (unsigned long long)(base + wid * 64 + __ffsll((long long)m) - 1));
+            __syncthreads();
+            const bool found = s_first != ~0ull;
+            __syncthreads();
+            if (found) break;
+        }
+        if (tid == 0) {
+            d.st->rank = d.rank0 + q;
+            d.st->first_unused = s_first != ~0ull ? (int64_t)s_first : d.N;
+            d.bs->k = 0;
+            d.bs->q0 = q;
+            d.bs->c0_cur = 0;
+            d.bs->c0_next = d.bcur;
+        }
+        return;
+    }
     const int t = (int)blockIdx.x * 2 + (int)(threadIdx.x / 128);
     if (t >= q) return;
     constexpr int PER = 16 / (int)sizeof(T), VPR = XLD / PER;
@@ -2147,42 +2176,12 @@ __global__ __launch_bounds__(256) void k_fast_commit(const BMat<LT>* __restrict_
     for (int v = threadIdx.x % 128; v < VPR; v += 128)
         gst<v4i>(d.X + gr * XLD + v * PER, gld<v4i>(c.X + cr * XLD + v * PER));
     if (threadIdx.x % 128 == 0) {
-        const int64_t rank0 = d.st->rank;  // updated only by k_fast_finish, after this kernel
+        const int64_t rank0 = d.rank0;
         d.used[gr] = 1;
         d.epoch_of[gr] = epoch;
         d.slot_of[gr] = t;
         d.piv_rows[rank0 + t] = gr;
         d.piv_cols[rank0 + t] = c.piv_cols[t];
-    }
-}
-
-template <class LT>
-__global__ __launch_bounds__(1024) void k_fast_finish(const BMat<LT>* __restrict__ D, const BMat<LT>* __restrict__ Dc,
-                                                      const BMat<LT>* __restrict__ Dw) {
-    const BMat<LT>& d = D[blockIdx.x];
-    const BMat<LT>& c = Dc[blockIdx.x];
-    if (Dw[blockIdx.x].nT == 0) return;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int q = (int)c.st->rank;
-    __shared__ unsigned long long s_first;
-    if (tid == 0) s_first = ~0ull;
-    __syncthreads();
-    for (int64_t base = d.st->first_unused; base < d.N; base += 1024) {
-        const int64_t r = base + tid;
-        const unsigned long long m = __ballot(r < d.N && !d.used[r]);
-        if (lane == 0 && m) atomicMin(&s_first, (unsigned long long)(base + wid * 64 + __ffsll((long long)m) - 1));
-        __syncthreads();
-        const bool found = s_first != ~0ull;
-        __syncthreads();
-        if (found) break;
-    }
-    if (tid == 0) {
-        d.st->rank += q;
-        d.st->first_unused = s_first != ~0ull ? (int64_t)s_first : d.N;
-        d.bs->k = 0;
-        d.bs->q0 = q;
-        d.bs->c0_cur = 0;
-        d.bs->c0_next = d.bcur;
     }
 }
 
@@ -2698,13 +2697,15 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     // instead of holding the GPU idle for a host round trip.
     k_fast_gate<LT><<<1, 256, 0, s>>>(ws.desc, cs.desc, const_cast<int64_t*>(ws.main.pre), (int)B, cs.status,
                                       in.desc);
-    LINAC_HIP(hipMemcpyAsync(cs.status_h, cs.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
-    LINAC_HIP(hipEventRecord(ev_status, s));
     int64_t maxR = 1;  // rows from the row offset on (the part-5 descriptors carry the items' current r0)
     for (auto* it : act) maxR = std::max<int64_t>(maxR, it->d.N - it->d.r0);
     const int nl = (int)gs_grid(maxR * (BO / 4), 16384);
     const int nb = (int)ceil_div((int64_t)BO * BO, 256);
     k_fast_operands<LT><<<dim3(nl + nb, B), 256, 0, s>>>(ws.desc, cs.desc, nl);
+    // (the status copy after the operands: off the chain to the Z GEMM; the
+    // host still has it well before the tail ends)
+    LINAC_HIP(hipMemcpyAsync(cs.status_h, cs.status, B * sizeof(ItemStatus), hipMemcpyDeviceToHost, s));
+    LINAC_HIP(hipEventRecord(ev_status, s));
     const unsigned grid = (unsigned)(ceil_div(ws.main_tiles, NUM_XCD) * NUM_XCD);
     // (profiled as a launch of the trailing-update kernel; its work is known
     // once the status says which blocks were of full rank)
@@ -2713,8 +2714,7 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     if constexpr (wide) k_gemm_outer64<0><<<grid, G_T, 0, s>>>(ws.desc, ws.main);
     else launch_outer32<0>(grid, s, ws.desc, ws.main, ws.lds_main);
     if (prof) LINAC_TRY(prof->sample_end(s));
-    k_fast_commit<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc, ws.desc, act[0]->d.e0);
-    k_fast_finish<LT><<<B, 1024, 0, s>>>(in.desc, cs.desc, ws.desc);
+    k_fast_commit<LT><<<dim3(BO / 2 + 1, B), 256, 0, s>>>(in.desc, cs.desc, ws.desc, act[0]->d.e0);
     LINAC_HIP(hipGetLastError());
     const auto spin0 = std::chrono::steady_clock::now();
     for (;;) {
@@ -2756,6 +2756,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     for (auto* it : act) {
         BM& d = it->d;
         d.bcur = (int32_t)std::min<int64_t>(BO, d.M - d.J);
+        d.rank0 = it->rank;
         d.e0 = epoch;
         d.q0 = 0;
         d.Kq = 0;
