@@ -2598,9 +2598,15 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
     int64_t stripes = 0;
     for (size_t m = 0; m < B; ++m) stripes += ceil_div(hd[m].N, G_M);
     // (64-bit tiles are half as wide and their fold is heavier: up to eight per stripe)
+    // (knob inner_split: the cap, 0 = 8 for 64-bit and 4 for 32-bit storage)
+    const int64_t cap_knob = tuning().inner_split;
     const int split = std::is_same<LT, LimbsU64>::value
-                          ? (int)std::max<int64_t>(1, std::min<int64_t>(8, 512 / std::max<int64_t>(1, stripes)))
-                          : (int)std::max<int64_t>(1, std::min<int64_t>(4, 256 / std::max<int64_t>(1, stripes)));
+                          ? (int)std::max<int64_t>(1, std::min<int64_t>(cap_knob ? cap_knob : 8,
+                                                                        (cap_knob ? 64 * cap_knob : 512) /
+                                                                            std::max<int64_t>(1, stripes)))
+                          : (int)std::max<int64_t>(1, std::min<int64_t>(cap_knob ? cap_knob : 4,
+                                                                        (cap_knob ? 64 * cap_knob : 256) /
+                                                                            std::max<int64_t>(1, stripes)));
     out->inner_split = split;
     int64_t* pa = hs;
     int64_t* ma = pa + (B + 1);

@@ -389,7 +389,8 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"gemm_lds", &t.gemm_lds, 0, 1},
                  {"fast_panel", &t.fast_panel, 0, 2},
                  {"echelon", &t.echelon, 0, 1},
-                 {"inner_compact", &t.inner_compact, 0, 1}};
+                 {"inner_compact", &t.inner_compact, 0, 1},
+                 {"inner_split", &t.inner_split, 0, 32}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             if (value < k.lo || value > k.hi)
@@ -412,7 +413,7 @@ int linac_tune_get(const char* key, int64_t* value) {
                  {"float_blocked_min_cols", t.float_blocked_min_cols}, {"gemm_group", t.gemm_group},
                  {"lookahead", t.lookahead}, {"timeslice", t.timeslice}, {"gemm_lds", t.gemm_lds},
                  {"fast_panel", t.fast_panel}, {"echelon", t.echelon},
-                 {"inner_compact", t.inner_compact}};
+                 {"inner_compact", t.inner_compact}, {"inner_split", t.inner_split}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             *value = k.v;

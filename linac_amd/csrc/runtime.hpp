@@ -135,6 +135,7 @@ struct Tuning {
     int64_t fast_panel = 2;
     int64_t echelon = 1;  // blocked GF(p): forward elimination plus back substitution (0: Gauss-Jordan)
     int64_t inner_compact = 0;  // 32-bit inner factorisation in the compact form (32 slots a row)
+    int64_t inner_split = 16;   // workgroups per row stripe of the inner GEMM at most (0: 8 for 64-bit, 4 for 32-bit)
 };
 Tuning& tuning();
 
