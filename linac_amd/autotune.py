@@ -3,12 +3,16 @@ reference, whose kernels are specialised per shape by nvcc JIT,
 ``linac/pycuda_tools.py:17-49``).
 
 Knobs (``include/linac_amd.h``, ``linac_tune_set``): the sizes from which
-``method="auto"`` switches to the blocked GF(p) and float methods, the number
-of tile rows the GEMMs walk together (``gemm_group``: the tile order's L2
-reuse), and whether a panel's trailing update overlaps the next panel's inner
-steps (``lookahead``).  ``tune()`` times the candidates on this device with
-matrices already in HBM, sets the winners and saves them to a per-device JSON
-file; the package applies that file when it loads the library.  GF(p) results
+``method="auto"`` switches to the blocked GF(p) and float methods, whether a
+panel's trailing update overlaps the next panel's inner steps
+(``lookahead``), and, per size class of the matrix (largest side below 3072,
+below 12288, larger: keys ``<knob>.s`` / ``.m`` / ``.l``), the GEMM shapes: the
+number of tile rows the GEMMs walk together (``gemm_group``: the tile order's
+L2 reuse), the inner GEMM's workgroups per 64-row stripe (``inner_split``) and
+the 64-bit trailing GEMM's tile, 64 x 32 or 64 x 64 (``tile64``).  ``tune()``
+times the candidates on this device with matrices already in HBM, sets the
+winners and saves them to a per-device JSON file; the package applies that
+file when it loads the library.  GF(p) results
 do not depend on the knobs (the RREF is unique); float results may change in
 the last bits when a crossover moves a size to the other method.
 """
@@ -22,8 +26,11 @@ import numpy
 
 from . import _lib
 
+SIZED = ("gemm_group", "inner_split", "tile64")  # knobs that also take per-size-class values
+CLASSES = ("s", "m", "l")                        # largest side < 3072, < 12288, larger
 KNOBS = ("gf_blocked_min_rows", "gf_blocked_min_cols", "float_blocked_min_rows", "float_blocked_min_cols",
-         "gemm_group", "lookahead", "timeslice", "gemm_lds")
+         "gemm_group", "lookahead", "timeslice", "gemm_lds", "inner_split", "tile64") + \
+    tuple(f"{k}.{c}" for k in SIZED for c in CLASSES)
 
 
 def get(key):
@@ -101,19 +108,22 @@ def _timed(fn, repeat):
     return best
 
 
-def _gf_case(n, method, seed=0):
-    """Closure timing one in-place GF(2^31-1) RREF of an n x n matrix in HBM."""
+def _gf_case(n, method, seed=0, wide=False):
+    """Closure timing one in-place RREF of an n x n matrix in HBM over
+    GF(2^31-1), or GF(2^61-1) in 64-bit storage (wide)."""
     import torch
     from .hip_row_reduce import rref_device
-    p = 2 ** 31 - 1
+    p = 2 ** 61 - 1 if wide else 2 ** 31 - 1
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
-    src = torch.randint(0, p, (n, n), generator=g, device="cuda", dtype=torch.int64).to(torch.int32)
+    src = torch.randint(0, p, (n, n), generator=g, device="cuda", dtype=torch.int64)
+    if not wide:
+        src = src.to(torch.int32)
     work = torch.empty_like(src)
 
     def run():
         work.copy_(src)
-        rref_device(work.data_ptr(), n, n, n, p, numpy.uint32, method=method)
+        rref_device(work.data_ptr(), n, n, n, p, numpy.uint64 if wide else numpy.uint32, method=method)
     return run
 
 
@@ -167,12 +177,47 @@ def tune(quick=False, save_result=True, path=None):
                 best = (t, grp, la)
     set_knob("gemm_group", best[1])
     set_knob("lookahead", best[2])
+    # per size class: the GEMM tile order, the inner GEMM's workgroups per row
+    # stripe and the 64-bit trailing tile width, one knob after the other
+    # (coordinate descent) on a matrix of the class
+    sizes = {"s": 1024, "m": 4096} if quick else {"s": 2048, "m": 8000, "l": 16000}
+    sizes64 = {"s": 1024, "m": 4096} if quick else {"s": 2048, "m": 8000, "l": 12000}
+    meas["by_class"] = {}
+    for c in CLASSES:
+        set_knob(f"gemm_group.{c}", 0)
+        set_knob(f"inner_split.{c}", 0)
+        set_knob(f"tile64.{c}", 0)
+    by_class = {}
+    for c, n in sizes.items():
+        rows = []
+        run = _gf_case(n, "blocked")
+        run()  # (the first call at a size grows the workspace)
+        for key, cands in (("gemm_group", (4, 8, 16)), ("inner_split", (4, 8, 16))):
+            times = {}
+            for v in cands:
+                set_knob(f"{key}.{c}", v)
+                times[v] = _timed(run, reps)
+                rows.append({"knob": f"{key}.{c}", "value": v, "n": n, "seconds": times[v]})
+            best_v = min(times, key=times.get)
+            set_knob(f"{key}.{c}", best_v)
+            by_class[f"{key}.{c}"] = best_v
+        run = _gf_case(sizes64[c], "blocked", wide=True)
+        run()
+        times = {}
+        for v in (1, 2):
+            set_knob(f"tile64.{c}", v)
+            times[v] = _timed(run, reps)
+            rows.append({"knob": f"tile64.{c}", "value": v, "n": sizes64[c], "seconds": times[v], "p": "2^61-1"})
+        best_v = min(times, key=times.get)
+        set_knob(f"tile64.{c}", best_v)
+        by_class[f"tile64.{c}"] = best_v
+        meas["by_class"][c] = rows
     gf_sizes = (256, 512, 1024) if quick else (64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536)
     gf_n, meas["gf_crossover"] = _crossover(_gf_case, gf_sizes, reps)
     fl_sizes = (128, 256) if quick else (32, 48, 64, 96, 128, 192, 256, 384, 512)
     fl_n, meas["float_crossover"] = _crossover(_float_case, fl_sizes, reps)
     knobs = {"gemm_group": best[1], "lookahead": best[2], "gf_blocked_min_rows": gf_n, "gf_blocked_min_cols": gf_n,
-             "float_blocked_min_rows": fl_n, "float_blocked_min_cols": fl_n}
+             "float_blocked_min_rows": fl_n, "float_blocked_min_cols": fl_n, **by_class}
     apply(knobs)
     data = {"device": device_name(), "knobs": knobs, "measurements": meas}
     if save_result:

@@ -1372,15 +1372,17 @@ constexpr int TN64 = 32 * NI64;     // tile width
 constexpr int BQ64 = TN64 / 8;      // B-stage 16-byte chunks per thread (TN64 rows x 4 x 8 limbs / 256)
 constexpr int GW64 = NI64 == 1 ? 2 : 1;  // workgroups per CU (accumulators: 120 NI64 registers)
 
-// acc[l1 + l2] += A_l1 B_l2 over one staged K slice (all 64 limb pairs).
-__device__ __forceinline__ void mfma_stage64(const int8_t (*sA)[G_M * G_ROW], const int8_t (*sB)[TN64 * G_ROW],
-                                             int wm, int wn, int r16, int g, v4i (&acc)[ND64][2][NI64]) {
-    v4i bf[NL64][NI64];
+// acc[l1 + l2] += A_l1 B_l2 over one staged K slice (all 64 limb pairs); NI
+// 16-column blocks per wave (tile width 32 NI).
+template <int NI>
+__device__ __forceinline__ void mfma_stage64(const int8_t (*sA)[G_M * G_ROW], const int8_t (*sB)[32 * NI * G_ROW],
+                                             int wm, int wn, int r16, int g, v4i (&acc)[ND64][2][NI]) {
+    v4i bf[NL64][NI];
 #pragma unroll
     for (int l = 0; l < NL64; ++l)
 #pragma unroll
-        for (int ni = 0; ni < NI64; ++ni)
-            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][lds_at(wn * 16 * NI64 + ni * 16 + r16, g)]);
+        for (int ni = 0; ni < NI; ++ni)
+            bf[l][ni] = *reinterpret_cast<const v4i*>(&sB[l][lds_at(wn * 16 * NI + ni * 16 + r16, g)]);
 #pragma unroll
     for (int l1 = 0; l1 < NL64; ++l1) {
         v4i af[2];
@@ -1392,7 +1394,7 @@ __device__ __forceinline__ void mfma_stage64(const int8_t (*sA)[G_M * G_ROW], co
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-                for (int ni = 0; ni < NI64; ++ni)
+                for (int ni = 0; ni < NI; ++ni)
                     acc[l1 + l2][mi][ni] =
                         __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[l2][ni], acc[l1 + l2][mi][ni], 0, 0, 0);
     }
@@ -1403,11 +1405,25 @@ __device__ __forceinline__ void mfma_stage64(const int8_t (*sA)[G_M * G_ROW], co
 constexpr int BLS64 = G_T / (TN64 * 4);
 static_assert(G_T % (TN64 * 4) == 0, "whole limbs per B-stage pass");
 
-// Trailing-update GEMMs for 64-bit storage (JOB as in k_gemm_outer).
-template <int JOB>
-__global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>* __restrict__ D, TileSched ts) {
+// Trailing-update GEMMs for 64-bit storage (JOB as in k_gemm_outer), tiles of
+// 64 x 32 NI: NI = 1 (120 accumulator registers, two workgroups per CU) or
+// NI = 2 (240, one workgroup per CU, half the right-operand stage loads per
+// MAC); the tuning knob tile64 picks one for the trailing parts.
+template <int NI>
+struct Tile64 {
+    static constexpr int TN = 32 * NI;           // tile width
+    static constexpr int BQ = TN / 8;            // B-stage 16-byte chunks per thread (TN rows x 4 x 8 limbs / 256)
+    static constexpr int BLS = G_T / (TN * 4);   // limb stride of a thread's B-stage chunks
+    static constexpr int GW = NI == 1 ? 2 : 1;   // workgroups per CU
+    static_assert(G_T % (TN * 4) == 0, "whole limbs per B-stage pass");
+};
+
+template <int JOB, int NI = 1>
+__global__ __launch_bounds__(G_T, Tile64<NI>::GW) void k_gemm_outer64(const BMat<LimbsU64>* __restrict__ D,
+                                                                     TileSched ts) {
+    constexpr int TN = Tile64<NI>::TN, BQ = Tile64<NI>::BQ, BLS = Tile64<NI>::BLS;
     __shared__ __attribute__((aligned(16))) int8_t sA[NL64][G_M * G_ROW];
-    __shared__ __attribute__((aligned(16))) int8_t sB[NL64][TN64 * G_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sB[NL64][TN * G_ROW];
     int mi_;
     int64_t tm, tn;
     if (!batch_tile(ts, &mi_, &tm, &tn)) return;
@@ -1424,34 +1440,34 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
     const int Kpad = JOB == 0 ? d.Kq : d.Kq - d.KqA;
     const uint64_t p = d.f.p;
     const double pinv = d.pinv;
-    const int64_t m_base = d.r0 + tm * G_M, n_base = tn * TN64;
+    const int64_t m_base = d.r0 + tm * G_M, n_base = tn * TN;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = wid >> 1, wn = wid & 1;
     const int r16 = lane & 15, g = lane >> 4;
     const v4i zero = {0, 0, 0, 0};
 
-    // A stage: limb q of row tid/4, quarter tid%4; B stage: BQ64 chunks
-    v4i ra[NL64], rb[BQ64];
+    // A stage: limb q of row tid/4, quarter tid%4; B stage: BQ chunks
+    v4i ra[NL64], rb[BQ];
     const int srow = tid >> 2, squ = tid & 3;
     const int8_t* pa = L + min(m_base + srow, Mrows - 1) * ldL + squ * 16;
     const int soff = lds_at(srow, squ);
-    const int bl0 = tid / (TN64 * 4), brow = (tid % (TN64 * 4)) >> 2, bqu = tid & 3;
+    const int bl0 = tid / (TN * 4), brow = (tid % (TN * 4)) >> 2, bqu = tid & 3;
     const int8_t* pb = B + bl0 * planeB + min(n_base + brow, Ncols - 1) * Kpad + bqu * 16;
     const int boff = lds_at(brow, bqu);
     auto gload = [&](int kc) {
 #pragma unroll
         for (int q = 0; q < NL64; ++q) ra[q] = gld<v4i>(pa + q * planeL + kc);
 #pragma unroll
-        for (int q = 0; q < BQ64; ++q) rb[q] = gld<v4i>(pb + q * BLS64 * planeB + kc);
+        for (int q = 0; q < BQ; ++q) rb[q] = gld<v4i>(pb + q * BLS * planeB + kc);
     };
 
-    v4i acc[ND64][2][NI64];
+    v4i acc[ND64][2][NI];
 #pragma unroll
     for (int s = 0; s < ND64; ++s)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < NI64; ++j) acc[s][i][j] = zero;
+            for (int j = 0; j < NI; ++j) acc[s][i][j] = zero;
 
     const int nk = Kpad / G_K;
     gload(0);
@@ -1459,35 +1475,35 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
 #pragma unroll
         for (int q = 0; q < NL64; ++q) *reinterpret_cast<v4i*>(&sA[q][soff]) = ra[q];
 #pragma unroll
-        for (int q = 0; q < BQ64; ++q) *reinterpret_cast<v4i*>(&sB[bl0 + q * BLS64][boff]) = rb[q];
+        for (int q = 0; q < BQ; ++q) *reinterpret_cast<v4i*>(&sB[bl0 + q * BLS][boff]) = rb[q];
         __syncthreads();
         if (kk + 1 < nk) gload((kk + 1) * G_K);
-        mfma_stage64(sA, sB, wm, wn, r16, g, acc);
+        mfma_stage64<NI>(sA, sB, wm, wn, r16, g, acc);
         __syncthreads();  // the next stage overwrites the buffer
     }
     // C values: issued together after the last stage (prefetching them into
     // the stage registers would keep 16 more addresses live through the loop,
     // beyond the register budget of two workgroups per CU); the other
     // workgroup's MFMAs cover the latency
-    uint64_t cv[2][NI64][4];
+    uint64_t cv[2][NI][4];
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < NI64; ++ni)
+        for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {  // unconditional loads, as in k_gemm_outer
                 const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, Mrows - 1);
-                const int64_t col = min(n_base + wn * 16 * NI64 + ni * 16 + r16, Ncols - 1);
+                const int64_t col = min(n_base + wn * 16 * NI + ni * 16 + r16, Ncols - 1);
                 cv[mi][ni][i] = gld<uint64_t>(C + row * ldc + col);
             }
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < NI64; ++ni)
+        for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-                const int64_t col = n_base + wn * 16 * NI64 + ni * 16 + r16;
+                const int64_t col = n_base + wn * 16 * NI + ni * 16 + r16;
                 if (row < Mrows && col < Ncols) {
                     int32_t sv[ND64];
 #pragma unroll
@@ -1495,6 +1511,13 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_outer64(const BMat<LimbsU64>
                     C[row * ldc + col] = fold_sub64(cv[mi][ni][i], sv, p, pinv);
                 }
             }
+}
+
+// The 64-bit trailing GEMM with the tile width NI picked for the launch.
+template <int JOB>
+static void launch_outer64(unsigned grid, hipStream_t st, const BMat<LimbsU64>* desc, const TileSched& ts, int ni) {
+    if (ni == 2) k_gemm_outer64<JOB, 2><<<grid, G_T, 0, st>>>(desc, ts);
+    else k_gemm_outer64<JOB, 1><<<grid, G_T, 0, st>>>(desc, ts);
 }
 
 // Inner-step GEMM for 64-bit storage: as k_gemm_inner, a 64-row stripe per
@@ -1573,7 +1596,7 @@ __global__ __launch_bounds__(G_T, GW64) void k_gemm_inner64(const BMat<LimbsU64>
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < NI64; ++j) acc[s][i][j] = zero;
-        mfma_stage64(sA, sB, wm, wn, r16, g, acc);
+        mfma_stage64<NI64>(sA, sB, wm, wn, r16, g, acc);
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -2367,6 +2390,7 @@ class Engine {
         double macs, bytes;         // algorithmic work of the trailing update (profiler)
         int64_t cols;               // widest column range of the part
         bool lds_main, lds_wq;      // 32-bit trailing / composite GEMMs may stage through direct-to-LDS loads
+        int ni64;                   // 64-bit trailing / composite GEMMs: tile width 32 ni64
     };
     int stage(int slot, size_t bytes);
     // part 0: inner phase; 1: the next panel's columns; 2, 3: the two halves
@@ -2446,7 +2470,7 @@ int Engine<LT>::launch_slice(Bulk& b, int64_t ntiles, Profiler* prof, int64_t ta
     const double frac = (double)n / (double)t.main_tiles;
     if (prof) LINAC_TRY(prof->sample_begin(so, tag, t.bytes * frac, t.macs * frac));
     const unsigned grid = (unsigned)(ceil_div(n, NUM_XCD) * NUM_XCD);
-    if constexpr (std::is_same<LT, LimbsU64>::value) k_gemm_outer64<0><<<grid, G_T, 0, so>>>(t.desc, ts);
+    if constexpr (std::is_same<LT, LimbsU64>::value) launch_outer64<0>(grid, so, t.desc, ts, t.ni64);
     else launch_outer32<0>(grid, so, t.desc, ts, t.lds_main);
     if (prof) LINAC_TRY(prof->sample_end(so));
     LINAC_HIP(hipGetLastError());
@@ -2476,7 +2500,7 @@ int Engine<LT>::trailing(const Staged& t, Profiler* prof, int64_t tag, bool rhs_
             t.desc, 0);
     if (prof) LINAC_TRY(prof->sample_begin(so, tag, t.bytes, t.macs));
     const unsigned grid = (unsigned)(ceil_div(t.main_tiles, NUM_XCD) * NUM_XCD);
-    if constexpr (std::is_same<LT, LimbsU64>::value) k_gemm_outer64<0><<<grid, G_T, 0, so>>>(t.desc, t.main);
+    if constexpr (std::is_same<LT, LimbsU64>::value) launch_outer64<0>(grid, so, t.desc, t.main, t.ni64);
     else launch_outer32<0>(grid, so, t.desc, t.main, t.lds_main);
     if (prof) LINAC_TRY(prof->sample_end(so));
     LINAC_HIP(hipGetLastError());
@@ -2592,14 +2616,20 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
         }
         hd[m] = d;
     }
-    constexpr int64_t TN = std::is_same<LT, LimbsU64>::value ? TN64 : G_N;  // GEMM tile width
+    // GEMM tile width: 64-bit trailing parts take the tuned width (knob
+    // tile64), the fast path's Z GEMM 32
+    int64_t nmax = 1;  // the step's size class: its largest matrix (not the view of this part)
+    for (size_t m = 0; m < B; ++m) nmax = std::max(nmax, std::max(act[m]->d.N, act[m]->d.M));
+    const int ni64 = (part >= 1 && part <= 3 && tuning().pick(Tuning::TILE64, nmax) == 2) ? 2 : 1;
+    const int64_t TN = std::is_same<LT, LimbsU64>::value ? 32 * ni64 : G_N;
+    out->ni64 = ni64;
     // 32-bit inner GEMM: one workgroup per 64-row stripe, or (few stripes)
     // up to four that share a stripe's column tiles
     int64_t stripes = 0;
     for (size_t m = 0; m < B; ++m) stripes += ceil_div(hd[m].N, G_M);
     // (64-bit tiles are half as wide and their fold is heavier: up to eight per stripe)
     // (knob inner_split: the cap, 0 = 8 for 64-bit and 4 for 32-bit storage)
-    const int64_t cap_knob = tuning().inner_split;
+    const int64_t cap_knob = tuning().pick(Tuning::INNER_SPLIT, nmax);
     const int split = std::is_same<LT, LimbsU64>::value
                           ? (int)std::max<int64_t>(1, std::min<int64_t>(cap_knob ? cap_knob : 8,
                                                                         (cap_knob ? 64 * cap_knob : 512) /
@@ -2639,7 +2669,7 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
     out->lds_main = offs && aligned;
     out->lds_wq = offs;
     out->desc = (BM*)dv;
-    const int grp = (int)tuning().gemm_group;
+    const int grp = (int)tuning().pick(Tuning::GEMM_GROUP, nmax);
     out->inner = TileSched{ds, ds + (B + 1), (int)B, grp, 0, -1};
     out->wq = TileSched{ds + (B + 1) + B, ds + 2 * (B + 1) + B, (int)B, grp, 0, -1};
     out->main = TileSched{ds + 2 * (B + 1) + 2 * B, ds + 3 * (B + 1) + 2 * B, (int)B, grp, 0, -1};
@@ -2946,7 +2976,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
                                                                               RW);
             if (comp) {
                 const unsigned grid = (unsigned)(ceil_div(t1.wq_tiles, NUM_XCD) * NUM_XCD);
-                if constexpr (wide) k_gemm_outer64<1><<<grid, G_T, 0, so>>>(t1.desc, t1.wq);
+                if constexpr (wide) launch_outer64<1>(grid, so, t1.desc, t1.wq, t1.ni64);
                 else launch_outer32<1>(grid, so, t1.desc, t1.wq, t1.lds_wq);
             }
             k_make_ops<LT><<<dim3(nl + nr, B), 256, 0, so>>>(t1.desc, 1, epoch, nl, main1 ? 0 : -1, rx);
@@ -3288,7 +3318,7 @@ static int gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda
         k_planes_rows<LimbsU32><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(n, 64)), (unsigned)(kp[c] / 64)), 256, 0, s>>>(
             B, ldb, b_rows ? d_brows : nullptr, t0, kc, (int)kp[c], n, p, planesB + offB[c], sB[c]);
     }
-    const TileSched ts{d_sched, d_sched + 2, 1, (int)tuning().gemm_group, 0, -1};
+    const TileSched ts{d_sched, d_sched + 2, 1, (int)tuning().pick(Tuning::GEMM_GROUP, std::max(m, n)), 0, -1};
     bool lds_ok = (((uintptr_t)C | (uintptr_t)(ldc * 4)) & 15) == 0;
     for (int64_t c = 0; c < nch; ++c) lds_ok = lds_ok && m * kp[c] < (1ll << 31) && n * kp[c] < (1ll << 31);
     for (int64_t c = 0; c < nch; ++c)
@@ -3375,7 +3405,7 @@ int echelon_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F
         using LT = typename std::conditional<u32, LimbsU32, LimbsU64>::type;
         using BM = BMat<LT>;
         constexpr int NL = LT::NL;
-        constexpr int64_t TN = u32 ? G_N : TN64;
+        constexpr int64_t TN = u32 ? G_N : Tile64<1>::TN;
         if (panels.empty()) return gf_finalize_apply(A, ld, N, M, f, s, ws, ctx, false, rows, cols, resorted);
         const int64_t rank = (int64_t)rows.size();
         if (resorted && rank > 0) {
@@ -3471,7 +3501,8 @@ int echelon_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F
                 k_planes_rows<LT><<<dim3((unsigned)std::min<int64_t>(2048, ceil_div(n, 64)), (unsigned)(j.Kpad / 64)),
                                     256, 0, s>>>(G + j.s * ldg + j.f0, ldg, nullptr, 0, (int)j.q, (int)j.Kpad, n, f.p,
                                                  planesB, maxB);
-                const TileSched ts{d_sched + 3 * i, d_sched + 3 * i + 2, 1, (int)tuning().gemm_group, 0, -1};
+                const TileSched ts{d_sched + 3 * i, d_sched + 3 * i + 2, 1,
+                                   (int)tuning().pick(Tuning::GEMM_GROUP, std::max(N, M)), 0, -1};
                 const unsigned grid = (unsigned)(ceil_div(hs[3 * i + 1], NUM_XCD) * NUM_XCD);
                 if constexpr (u32) {
                     const bool lds_ok = ((((uintptr_t)(G + j.f0)) | (uintptr_t)(ldg * 4)) & 15) == 0 &&

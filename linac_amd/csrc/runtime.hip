@@ -372,6 +372,29 @@ int linac_shutdown(void) {
 
 int linac_is_prime(uint64_t n) { return is_prime_u64(n) ? 1 : 0; }
 
+// The per-size-class slot named by key ("gemm_group.m" ...), with its range.
+static bool sized_slot(Tuning& t, const char* key, int64_t** slot, int64_t* lo, int64_t* hi) {
+    static const struct {
+        const char* name;
+        Tuning::Sized k;
+        int64_t lo, hi;
+    } sized[] = {{"gemm_group", Tuning::GEMM_GROUP, 1, 64},
+                 {"inner_split", Tuning::INNER_SPLIT, 1, 32},
+                 {"tile64", Tuning::TILE64, 1, 2}};
+    const char* dot = strchr(key, '.');
+    if (!dot || strlen(dot) != 2) return false;
+    const int c = dot[1] == 's' ? 0 : dot[1] == 'm' ? 1 : dot[1] == 'l' ? 2 : -1;
+    if (c < 0) return false;
+    for (auto& x : sized)
+        if (strlen(x.name) == (size_t)(dot - key) && !strncmp(key, x.name, dot - key)) {
+            *slot = &t.by_class[x.k][c];
+            *lo = x.lo;
+            *hi = x.hi;
+            return true;
+        }
+    return false;
+}
+
 int linac_tune_set(const char* key, int64_t value) {
     if (!key) return fail(LINAC_ERR_INVALID, "tuning key is NULL");
     Tuning& t = tuning();
@@ -390,7 +413,8 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"fast_panel", &t.fast_panel, 0, 2},
                  {"echelon", &t.echelon, 0, 1},
                  {"inner_compact", &t.inner_compact, 0, 1},
-                 {"inner_split", &t.inner_split, 0, 32}};
+                 {"inner_split", &t.inner_split, 0, 32},
+                 {"tile64", &t.tile64, 1, 2}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             if (value < k.lo || value > k.hi)
@@ -399,6 +423,16 @@ int linac_tune_set(const char* key, int64_t value) {
             *k.slot = value;
             return LINAC_OK;
         }
+    // per size class: "<knob>.s" / ".m" / ".l", 0 = the global value
+    int64_t* slot = nullptr;
+    int64_t lo = 0, hi = 0;
+    if (sized_slot(t, key, &slot, &lo, &hi)) {
+        if (value != 0 && (value < lo || value > hi))
+            return fail(LINAC_ERR_INVALID, "%s = %lld outside {0} and [%lld, %lld]", key, (long long)value,
+                        (long long)lo, (long long)hi);
+        *slot = value;
+        return LINAC_OK;
+    }
     return fail(LINAC_ERR_INVALID, "unknown tuning key '%s'", key);
 }
 
@@ -413,12 +447,19 @@ int linac_tune_get(const char* key, int64_t* value) {
                  {"float_blocked_min_cols", t.float_blocked_min_cols}, {"gemm_group", t.gemm_group},
                  {"lookahead", t.lookahead}, {"timeslice", t.timeslice}, {"gemm_lds", t.gemm_lds},
                  {"fast_panel", t.fast_panel}, {"echelon", t.echelon},
-                 {"inner_compact", t.inner_compact}, {"inner_split", t.inner_split}};
+                 {"inner_compact", t.inner_compact}, {"inner_split", t.inner_split},
+                 {"tile64", t.tile64}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             *value = k.v;
             return LINAC_OK;
         }
+    int64_t* slot = nullptr;
+    int64_t lo = 0, hi = 0;
+    if (sized_slot(tuning(), key, &slot, &lo, &hi)) {
+        *value = *slot;
+        return LINAC_OK;
+    }
     return fail(LINAC_ERR_INVALID, "unknown tuning key '%s'", key);
 }
 

@@ -19,6 +19,11 @@ def test_quick_tune_sets_and_saves(gpu, tmp_path):
         assert saved["knobs"] == data["knobs"] == {k: autotune.get(k) for k in data["knobs"]}
         assert data["measurements"]["gf_crossover"] and data["measurements"]["float_crossover"]
         assert "gfx" in data["device"]
+        for c in ("s", "m"):  # the GEMM shapes were chosen per size class
+            assert data["knobs"][f"gemm_group.{c}"] in (4, 8, 16)
+            assert data["knobs"][f"inner_split.{c}"] in (4, 8, 16)
+            assert data["knobs"][f"tile64.{c}"] in (1, 2)
+            assert data["measurements"]["by_class"][c]
     finally:
         autotune.apply(before)
 
@@ -99,3 +104,41 @@ def test_direct_to_lds_trailing_gemm_matches_register_staged(gpu, n, rank, p):
         R, piv, r, G = out[v]
         assert piv == po and r == len(po) and numpy.array_equal(R, Ro), v
         assert numpy.array_equal(G.astype(object), want), v
+
+
+def test_size_class_knobs_keep_results_exact(gpu):
+    """Per size class GEMM shapes (tile order, inner-GEMM split, the 64-bit
+    64 x 64 tile) change no result: small matrices against the oracle, a
+    medium-class 64-bit one against the same solve with the global knobs."""
+    import torch
+    from linac_amd import autotune
+    before = autotune.current()
+    p32, p64 = 2 ** 31 - 1, 2 ** 61 - 1
+    A = modp.low_rank(1300, 1400, 1200, p32, 5)
+    rs = numpy.random.RandomState(6)
+    B = rs.randint(0, 2 ** 62, size=(1100, 1200), dtype=numpy.int64).astype(numpy.uint64) % numpy.uint64(p64)
+    B[700:] = (B[:400] + B[300:700]) % numpy.uint64(p64)
+    n = 3200
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    src = torch.randint(0, p64, (n, n + 64), generator=g, device="cuda", dtype=torch.int64)
+    src[:, n:] = src[:, :64]  # non-pivot columns
+    try:
+        ref = src.clone()
+        piv_ref, rank_ref = gpu.rref_device(ref.data_ptr(), n, n + 64, n + 64, p64, numpy.uint64, method="blocked")
+        for c, v in (("s", 1), ("s", 2), ("m", 2)):
+            autotune.set_knob(f"gemm_group.{c}", 1 if v == 1 else 16)
+            autotune.set_knob(f"inner_split.{c}", 2 if v == 1 else 8)
+            autotune.set_knob(f"tile64.{c}", v)
+        Ra, pa, ra = gpu.rref_with_pivots(A, p32, method="blocked")
+        Rb, pb, rb = gpu.rref_with_pivots(B, p64, method="blocked")
+        work = src.clone()
+        piv, rank = gpu.rref_device(work.data_ptr(), n, n + 64, n + 64, p64, numpy.uint64, method="blocked")
+    finally:
+        autotune.apply(before)
+    Ro, po = crref.rref_modp(A, p32)
+    assert pa == po and numpy.array_equal(Ra, Ro)
+    Ro, po = crref.rref_modp(B, p64)
+    assert pb == po and numpy.array_equal(Rb.astype(numpy.uint64), Ro.astype(numpy.uint64))
+    assert piv == piv_ref and rank == rank_ref == n
+    assert torch.equal(work, ref)
