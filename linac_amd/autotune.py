@@ -9,7 +9,9 @@ panel's trailing update overlaps the next panel's inner steps
 below 12288, larger: keys ``<knob>.s`` / ``.m`` / ``.l``), the GEMM shapes: the
 number of tile rows the GEMMs walk together (``gemm_group``: the tile order's
 L2 reuse), the inner GEMM's workgroups per 64-row stripe (``inner_split``) and
-the 64-bit trailing GEMM's tile, 64 x 32 or 64 x 64 (``tile64``).  ``tune()``
+the 64-bit trailing GEMM's tile, 64 x 32 or 64 x 64 (``tile64``), and the
+float64 panel kernel's rows per wave, 16 or 32 (``fpanel_rows``: half the
+workgroups in the per-column candidate exchange).  ``tune()``
 times the candidates on this device with matrices already in HBM, sets the
 winners and saves them to a per-device JSON file; the package applies that
 file when it loads the library.  GF(p) results
@@ -26,10 +28,10 @@ import numpy
 
 from . import _lib
 
-SIZED = ("gemm_group", "inner_split", "tile64")  # knobs that also take per-size-class values
+SIZED = ("gemm_group", "inner_split", "tile64", "fpanel_rows")  # knobs that also take per-size-class values
 CLASSES = ("s", "m", "l")                        # largest side < 3072, < 12288, larger
 KNOBS = ("gf_blocked_min_rows", "gf_blocked_min_cols", "float_blocked_min_rows", "float_blocked_min_cols",
-         "gemm_group", "lookahead", "timeslice", "gemm_lds", "inner_split", "tile64") + \
+         "gemm_group", "lookahead", "timeslice", "gemm_lds", "inner_split", "tile64", "fpanel_rows") + \
     tuple(f"{k}.{c}" for k in SIZED for c in CLASSES)
 
 
@@ -183,10 +185,10 @@ def tune(quick=False, save_result=True, path=None):
     sizes = {"s": 1024, "m": 4096} if quick else {"s": 2048, "m": 8000, "l": 16000}
     sizes64 = {"s": 1024, "m": 4096} if quick else {"s": 2048, "m": 8000, "l": 12000}
     meas["by_class"] = {}
+    sizes_f = {"s": 1024, "m": 4096} if quick else {"s": 2048, "m": 8000}
     for c in CLASSES:
-        set_knob(f"gemm_group.{c}", 0)
-        set_knob(f"inner_split.{c}", 0)
-        set_knob(f"tile64.{c}", 0)
+        for k in SIZED:
+            set_knob(f"{k}.{c}", 0)
     by_class = {}
     for c, n in sizes.items():
         rows = []
@@ -211,6 +213,18 @@ def tune(quick=False, save_result=True, path=None):
         best_v = min(times, key=times.get)
         set_knob(f"tile64.{c}", best_v)
         by_class[f"tile64.{c}"] = best_v
+        if c in sizes_f:  # float64 panel kernel: rows per wave
+            run = _float_case(sizes_f[c], "blocked")
+            run()
+            times = {}
+            for v in (16, 32):
+                set_knob(f"fpanel_rows.{c}", v)
+                times[v] = _timed(run, reps)
+                rows.append({"knob": f"fpanel_rows.{c}", "value": v, "n": sizes_f[c], "seconds": times[v],
+                             "dtype": "float64"})
+            best_v = min(times, key=times.get)
+            set_knob(f"fpanel_rows.{c}", best_v)
+            by_class[f"fpanel_rows.{c}"] = best_v
         meas["by_class"][c] = rows
     gf_sizes = (256, 512, 1024) if quick else (64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536)
     gf_n, meas["gf_crossover"] = _crossover(_gf_case, gf_sizes, reps)

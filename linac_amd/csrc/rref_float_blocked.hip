@@ -573,8 +573,15 @@ int FloatSweep<T>::setup(T* A, int64_t ld, int64_t N, int64_t M, double tol, hip
                          uint8_t* used, int64_t* piv_rows, int64_t* piv_cols, DeviceContext* ctx, ScratchSlot slot,
                          FloatFactors* keep_, void* ws) {
     bool fits = false;
-    LINAC_TRY((cooperative_fits<T, 16>(ceil_div(N, FWAVES * 16), ctx, &fits, &lds)));
+    // rows per wave: 16 where that grid is resident, else 32.  For float64
+    // the knob fpanel_rows (per size class) can ask for 32: half the
+    // workgroups exchange candidates per column, each updating twice the
+    // rows (faster from about N = 6000, slower below 3000; complex128, with
+    // twice the registers per row, keeps 16)
     int rw = 16;
+    const bool want32 =
+        std::is_same<T, double>::value && tuning().pick(Tuning::FPANEL_ROWS, std::max(N, M)) == 32;
+    if (!want32) LINAC_TRY((cooperative_fits<T, 16>(ceil_div(N, FWAVES * 16), ctx, &fits, &lds)));
     if (!fits) {
         LINAC_TRY((cooperative_fits<T, 32>(ceil_div(N, FWAVES * 32), ctx, &fits, &lds)));
         rw = 32;

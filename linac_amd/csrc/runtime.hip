@@ -380,7 +380,8 @@ static bool sized_slot(Tuning& t, const char* key, int64_t** slot, int64_t* lo, 
         int64_t lo, hi;
     } sized[] = {{"gemm_group", Tuning::GEMM_GROUP, 1, 64},
                  {"inner_split", Tuning::INNER_SPLIT, 1, 32},
-                 {"tile64", Tuning::TILE64, 1, 2}};
+                 {"tile64", Tuning::TILE64, 1, 2},
+                 {"fpanel_rows", Tuning::FPANEL_ROWS, 16, 32}};
     const char* dot = strchr(key, '.');
     if (!dot || strlen(dot) != 2) return false;
     const int c = dot[1] == 's' ? 0 : dot[1] == 'm' ? 1 : dot[1] == 'l' ? 2 : -1;
@@ -414,7 +415,8 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"echelon", &t.echelon, 0, 1},
                  {"inner_compact", &t.inner_compact, 0, 1},
                  {"inner_split", &t.inner_split, 0, 32},
-                 {"tile64", &t.tile64, 1, 2}};
+                 {"tile64", &t.tile64, 1, 2},
+                 {"fpanel_rows", &t.fpanel_rows, 16, 32}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             if (value < k.lo || value > k.hi)
@@ -448,7 +450,7 @@ int linac_tune_get(const char* key, int64_t* value) {
                  {"lookahead", t.lookahead}, {"timeslice", t.timeslice}, {"gemm_lds", t.gemm_lds},
                  {"fast_panel", t.fast_panel}, {"echelon", t.echelon},
                  {"inner_compact", t.inner_compact}, {"inner_split", t.inner_split},
-                 {"tile64", t.tile64}};
+                 {"tile64", t.tile64}, {"fpanel_rows", t.fpanel_rows}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             *value = k.v;

@@ -142,3 +142,25 @@ def test_size_class_knobs_keep_results_exact(gpu):
     assert pb == po and numpy.array_equal(Rb.astype(numpy.uint64), Ro.astype(numpy.uint64))
     assert piv == piv_ref and rank == rank_ref == n
     assert torch.equal(work, ref)
+
+
+def test_float_panel_rows_per_wave_same_result(gpu):
+    """The float64 panel kernel with 16 or 32 rows per wave (knob
+    fpanel_rows, per size class) computes the same RREF."""
+    from linac_amd import autotune
+    before = autotune.current()
+    rs = numpy.random.RandomState(8)
+    A = rs.rand(900, 901)
+    out = []
+    try:
+        for v in (16, 32):
+            autotune.set_knob("fpanel_rows.s", v)
+            R, piv, rank = gpu.rref_with_pivots(A, method="blocked")
+            out.append((R, piv, rank))
+    finally:
+        autotune.apply(before)
+    (R1, p1, r1), (R2, p2, r2) = out
+    assert p1 == p2 and r1 == r2 == 900
+    assert numpy.allclose(R1, R2, rtol=0, atol=1e-12)
+    x = numpy.linalg.solve(A[:, :900], A[:, 900])
+    assert numpy.allclose(R2[:, 900], x, rtol=1e-9, atol=1e-9)
