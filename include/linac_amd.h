@@ -100,7 +100,21 @@ int linac_profile_last(double* out, int n);
  *       waiting for its slice);
  *   "gemm_lds" 1: the 32-bit trailing GEMM brings its operand stages and its
  *       C tile into LDS with direct-to-LDS loads where every matrix's rows are
- *       16-byte aligned (0: register-staged copies everywhere).
+ *       16-byte aligned (0: register-staged copies everywhere);
+ *   "fast_panel" 2: full-rank panels through their candidate block with the
+ *       inner-step kernels (1: one workgroup per block; 0: off);
+ *   "echelon" 1: blocked GF(p) by forward elimination to echelon form and a
+ *       block back substitution (0: Gauss-Jordan form throughout); results
+ *       are the same (the RREF is unique);
+ *   "inner_compact" 0: the 32-bit inner factorisation in the compact form;
+ *   "inner_split" 16: at most this many workgroups per 64-row stripe of the
+ *       inner GEMM (0: 8 for 64-bit, 4 for 32-bit storage);
+ *   "tile64" 1: the 64-bit trailing GEMM's tile, 64 x 32 (1) or 64 x 64 (2);
+ *   "fpanel_rows" 16: rows per wave of the float64 panel kernel, 16 or 32.
+ * "gemm_group", "inner_split", "tile64" and "fpanel_rows" also take values per
+ * size class of a matrix (largest side < 3072, < 12288, larger) under
+ * "<key>.s", "<key>.m", "<key>.l" (0: the global value; "fpanel_rows.m" and
+ * ".l" default to 32); a launch uses the class of its largest matrix.
  * Unknown keys and out-of-range values return LINAC_ERR_INVALID. */
 int linac_tune_set(const char* key, int64_t value);
 int linac_tune_get(const char* key, int64_t* value);
