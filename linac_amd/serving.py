@@ -81,7 +81,7 @@ class Server:
     200 requests/s it moved the overall p99 from 0.11 to 0.20-0.27 s, the GF(p)
     requests paying more than the float ones gained)."""
 
-    def __init__(self, mem_cap_bytes=200 * 10 ** 9, float_budget_s=0.002, float_age_s=None):
+    def __init__(self, mem_cap_bytes=200 * 10 ** 9, float_budget_s=0.0015, float_age_s=None):
         self.cap = int(mem_cap_bytes)
         self.float_budget_s = float(float_budget_s)
         self.float_age_s = float_age_s       # a float request older than this gets 4 x the budget per round
@@ -208,7 +208,7 @@ def materialise(req, seed=0):
     return req
 
 
-def serve(requests, mem_cap_bytes=200 * 10 ** 9, seed=0, keep=False, float_budget_s=0.002, float_age_s=None):
+def serve(requests, mem_cap_bytes=200 * 10 ** 9, seed=0, keep=False, float_budget_s=0.0015, float_age_s=None):
     """Run timed requests through a Server against the wall clock.
 
     Each request is materialised in HBM when it arrives (not before), then
