@@ -294,7 +294,7 @@ __device__ __forceinline__ uint64_t uniform(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-template <class LT, bool CMP>
+template <class LT, bool CMP, int IF_T>
 __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t N,
                                          uint8_t* __restrict__ used, int32_t* __restrict__ slot_of,
                                          int32_t* __restrict__ epoch_of, int epoch,
@@ -302,7 +302,6 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
                                          int64_t* __restrict__ piv_rows, int64_t* __restrict__ piv_cols,
                                          BlockState<typename LT::T>* bs) {
     using T = typename LT::T;
-    constexpr int IF_T = LT::IF_T;
     constexpr int IF_RPW = CI / (IF_T / 64);  // candidate rows per wave
     static_assert(IF_RPW * (IF_T / 64) == CI && IF_T >= 256, "waves split the candidate rows evenly; SCAN rows per probe");
     // CMP: the compact form (32 slots a row), else [panel | identity] (64
@@ -408,6 +407,7 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
     //    pivot heads h_t are kept: the pivot of the t-th pivot row ends as
     //    d_t = h_t prod_{u > t} h_u R^{-1}, which step 4 inverts.
     unsigned long long pmask = 0ull;  // LDS rows already used as pivots (same in every wave)
+    const unsigned long long live = (1ull << nC) - 1ull;  // the candidate rows (nC <= CI < 64)
     int t = 0;
     constexpr int NW = IF_T / 64;  // wave w owns rows w, w + NW, w + 2 NW, ...
     __shared__ T s_h[CI];
@@ -422,12 +422,22 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
                 av[q] = aug[r][c];
                 xv[q] = aug[r][lane];
             }
-            const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
-            const unsigned long long m = __ballot(nz);
-            if (m == 0ull) continue;  // no pivot in this column (every wave sees the same)
-            const int pv = __ffsll((long long)m) - 1;
-            const T h = aug[pv][c];
-            const T y = aug[pv][lane];
+            // the pivot is the first row not yet a pivot with a non-zero entry:
+            // the first row not yet a pivot (ps) whenever its entry is non-zero,
+            // which is read beside the rows above; the search over all rows
+            // (a second round trip to LDS) only when it is zero
+            const int ps = __ffsll((long long)(~pmask & live)) - 1;
+            T h = uniform(aug[ps][c]);
+            T y = aug[ps][lane];
+            int pv = ps;
+            if (h == (T)0) {
+                const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
+                const unsigned long long m = __ballot(nz);
+                if (m == 0ull) continue;  // no pivot in this column (every wave sees the same)
+                pv = __ffsll((long long)m) - 1;
+                h = aug[pv][c];
+                y = aug[pv][lane];
+            }
 #pragma unroll
             for (int q = 0; q < IF_RPW; ++q) {
                 const int r = wid + NW * q;
@@ -473,12 +483,19 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
                 av[q] = aug[r][c];
                 xv[q] = aug[r][j];
             }
-            const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
-            const unsigned long long m = __ballot(nz);
-            if (m == 0ull) continue;
-            const int pv = __ffsll((long long)m) - 1;
-            const T h = aug[pv][c];
-            const T y = aug[pv][j];
+            // (the first row not yet a pivot, unless its entry is zero: as above)
+            const int ps = __ffsll((long long)(~pmask & live)) - 1;
+            T h = uniform(aug[ps][c]);
+            T y = aug[ps][j];
+            int pv = ps;
+            if (h == (T)0) {
+                const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
+                const unsigned long long m = __ballot(nz);
+                if (m == 0ull) continue;
+                pv = __ffsll((long long)m) - 1;
+                h = aug[pv][c];
+                y = aug[pv][j];
+            }
 #pragma unroll
             for (int q = 0; q < RP; ++q) {
                 const int r = wid + NW * (2 * q + hf);
@@ -584,21 +601,39 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
 }
 
 
-template <class LT, bool CMP>
-__global__ __launch_bounds__(LT::IF_T, LT::IF_WAVES) void k_inner_factor(const BMat<LT>* __restrict__ D, int epoch) {
+// IF_T threads (LT::IF_T and its register cap, or fewer waves, each with more
+// candidate rows, capped only by what one workgroup needs)
+template <class LT, bool CMP, int IF_T>
+__global__ __launch_bounds__(IF_T, IF_T == LT::IF_T ? LT::IF_WAVES : IF_T / 256) void k_inner_factor(
+    const BMat<LT>* __restrict__ D, int epoch) {
     const BMat<LT>& d = D[blockIdx.x];
-    inner_factor_body<LT, CMP>(d.X, d.N, d.used, d.slot_of, d.epoch_of, epoch, d.J, d.f, d.st, d.piv_rows, d.piv_cols,
-                               d.bs);
+    inner_factor_body<LT, CMP, IF_T>(d.X, d.N, d.used, d.slot_of, d.epoch_of, epoch, d.J, d.f, d.st, d.piv_rows,
+                                     d.piv_cols, d.bs);
 }
 
 // The inner factorisation of storage width LT: the compact form always for
-// 64-bit storage, by tuning knob "inner_compact" for 32-bit storage.
+// 64-bit storage, by tuning knob "inner_compact" for 32-bit storage; 32-bit
+// storage takes its workgroup size from tuning knob "inner_threads".
 template <class LT>
 static void launch_inner_factor(unsigned B, hipStream_t s, const BMat<LT>* desc, int epoch) {
-    if (std::is_same<LT, LimbsU64>::value || tuning().inner_compact != 0)
-        k_inner_factor<LT, true><<<B, LT::IF_T, 0, s>>>(desc, epoch);
-    else
-        k_inner_factor<LT, false><<<B, LT::IF_T, 0, s>>>(desc, epoch);
+    if constexpr (std::is_same<LT, LimbsU64>::value) {
+        k_inner_factor<LT, true, LT::IF_T><<<B, LT::IF_T, 0, s>>>(desc, epoch);
+    } else {
+        const bool cmp = tuning().inner_compact != 0;
+        switch (tuning().inner_threads) {
+            case 256:
+                if (cmp) k_inner_factor<LT, true, 256><<<B, 256, 0, s>>>(desc, epoch);
+                else k_inner_factor<LT, false, 256><<<B, 256, 0, s>>>(desc, epoch);
+                break;
+            case 512:
+                if (cmp) k_inner_factor<LT, true, 512><<<B, 512, 0, s>>>(desc, epoch);
+                else k_inner_factor<LT, false, 512><<<B, 512, 0, s>>>(desc, epoch);
+                break;
+            default:
+                if (cmp) k_inner_factor<LT, true, LT::IF_T><<<B, LT::IF_T, 0, s>>>(desc, epoch);
+                else k_inner_factor<LT, false, LT::IF_T><<<B, LT::IF_T, 0, s>>>(desc, epoch);
+        }
+    }
 }
 
 // U = Apiv^{-1} X[R, :] (k x XLD, Montgomery inverse times plain rows gives plain
@@ -794,21 +829,31 @@ __device__ static void make_rhs_body(const BMat<LT>& d, int job, int64_t bx, int
     int8_t* planes = job == 0 ? d.planesB : d.planesW;
     const int64_t pstride = job == 0 ? d.strideB : STRIDE_W;
     for (int64_t nb = bx * 64; nb < ncols; nb += nbx * 64) {
-        for (int x = threadIdx.x; x < 64 * 64; x += 256) {
+        // all sixteen loads of a thread in flight together (one after the
+        // other they expose the memory latency sixteen times)
+        typename LT::T raw[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int x = threadIdx.x + 256 * u;
             const int tt = x >> 6, nn = x & 63;
             const int t = tb + tt;
             const int64_t n = nb + nn;
-            S v = 0;
+            const typename LT::T* src = nullptr;
             if (n < ncols) {
                 if (job == 1) {
-                    if (t < d.q0) v = centred(d.W[Q[t] * BO + n], d.f.p);
+                    if (t < d.q0) src = d.W + Q[t] * BO + n;
                 } else if (t < d.KqA) {
-                    if (t < d.kA) v = centred(d.A[QA[t] * d.ld + d.T0 + n], d.f.p);
+                    if (t < d.kA) src = d.A + QA[t] * d.ld + d.T0 + n;
                 } else if (t - d.KqA < d.q0) {
-                    v = centred(d.A[Q[t - d.KqA] * d.ld + d.T0 + n], d.f.p);
+                    src = d.A + Q[t - d.KqA] * d.ld + d.T0 + n;
                 }
             }
-            tile[tt][nn] = v;
+            raw[u] = src ? *src : (typename LT::T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int x = threadIdx.x + 256 * u;
+            tile[x >> 6][x & 63] = centred(raw[u], d.f.p);
         }
         __syncthreads();
         const int nn = threadIdx.x >> 2, tq = (threadIdx.x & 3) * 16;
@@ -2949,6 +2994,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     // (the descriptor copy goes ahead of the wait for the fast path's tail
     // kernels, which do not read it)
     LINAC_TRY(upload(act, so, p1, 1, tp, 3));
+    const auto t_up = std::chrono::steady_clock::now();
     if (nfast == (int)B) LINAC_HIP(hipStreamWaitEvent(so, ev_inner, 0));
     const Staged &t1 = tp[0], &t2 = tp[1], &t3 = tp[2];
     bool operands = t1.main_tiles > 0;
@@ -3012,6 +3058,10 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     if (!lookahead) LINAC_TRY(flush_due(INT64_MAX, prof, tag));
     else if (tuning().timeslice == 0) LINAC_TRY(flush_due(step_no + 1, prof, tag));
     if (!lookahead) LINAC_HIP(hipEventRecord(ev_ready, so));  // the next inner phase waits for all of it
+    if (debug_steps())
+        fprintf(stderr, "  upload at %.1f us, step end at %.1f us\n",
+                std::chrono::duration<double, std::micro>(t_up - step0).count(),
+                std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - step0).count());
     ++step_no;
     ev_pending = true;  // the next panel load waits for ev_ready (the next panel's columns, or all parts)
     for (auto* it : act) {

@@ -414,6 +414,7 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"fast_panel", &t.fast_panel, 0, 2},
                  {"echelon", &t.echelon, 0, 1},
                  {"inner_compact", &t.inner_compact, 0, 1},
+                 {"inner_threads", &t.inner_threads, 256, 1024},
                  {"inner_split", &t.inner_split, 0, 32},
                  {"tile64", &t.tile64, 1, 2},
                  {"fpanel_rows", &t.fpanel_rows, 16, 32}};
@@ -422,6 +423,8 @@ int linac_tune_set(const char* key, int64_t value) {
             if (value < k.lo || value > k.hi)
                 return fail(LINAC_ERR_INVALID, "%s = %lld outside [%lld, %lld]", key, (long long)value,
                             (long long)k.lo, (long long)k.hi);
+            if (!strcmp(key, "inner_threads") && value != 256 && value != 512 && value != 1024)
+                return fail(LINAC_ERR_INVALID, "inner_threads = %lld: 256, 512 or 1024", (long long)value);
             *k.slot = value;
             return LINAC_OK;
         }
@@ -449,7 +452,8 @@ int linac_tune_get(const char* key, int64_t* value) {
                  {"float_blocked_min_cols", t.float_blocked_min_cols}, {"gemm_group", t.gemm_group},
                  {"lookahead", t.lookahead}, {"timeslice", t.timeslice}, {"gemm_lds", t.gemm_lds},
                  {"fast_panel", t.fast_panel}, {"echelon", t.echelon},
-                 {"inner_compact", t.inner_compact}, {"inner_split", t.inner_split},
+                 {"inner_compact", t.inner_compact}, {"inner_threads", t.inner_threads},
+                 {"inner_split", t.inner_split},
                  {"tile64", t.tile64}, {"fpanel_rows", t.fpanel_rows}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {

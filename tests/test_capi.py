@@ -149,12 +149,19 @@ def test_tuning_knobs_set_get_and_validation():
         assert autotune.get("gemm_lds") == 0
         with pytest.raises(ValueError):
             autotune.set_knob("gemm_lds", 2)
+        for v in (256, 512, 1024):
+            autotune.set_knob("inner_threads", v)
+            assert autotune.get("inner_threads") == v
+        for v in (128, 300, 2048):
+            with pytest.raises(ValueError):
+                autotune.set_knob("inner_threads", v)
         with pytest.raises(ValueError):
             autotune.set_knob("no_such_knob", 1)
         with pytest.raises(ValueError):
             autotune.get("no_such_knob")
     finally:
         autotune.apply(before)
+        autotune.set_knob("inner_threads", 1024)
     assert autotune.current() == before
 
 

@@ -144,6 +144,31 @@ def test_size_class_knobs_keep_results_exact(gpu):
     assert torch.equal(work, ref)
 
 
+@pytest.mark.parametrize("threads,compact", [(256, 0), (512, 0), (512, 1), (256, 1)])
+def test_inner_factor_workgroup_sizes_exact(gpu, threads, compact):
+    """The 32-bit inner factorisation with 4 or 8 waves (knob inner_threads),
+    in either form, gives the oracle's RREF: full-rank panels, rank-deficient
+    ones (whose first free candidate row is often zero in the pivot column)
+    and a matrix with zero columns."""
+    from linac_amd import autotune
+    p = 2 ** 31 - 1
+    A = modp.low_rank(1300, 1400, 1200, p, 11)
+    rs = numpy.random.RandomState(12)
+    B = rs.randint(0, p, size=(1100, 1200)).astype(numpy.uint32)
+    B[:, 100:140] = 0
+    B[600:] = (B[:500].astype(numpy.uint64) * 3 % p).astype(numpy.uint32)
+    try:
+        autotune.set_knob("inner_threads", threads)
+        autotune.set_knob("inner_compact", compact)
+        out = [gpu.rref_with_pivots(M, p, method="blocked") for M in (A, B)]
+    finally:
+        autotune.set_knob("inner_threads", 1024)
+        autotune.set_knob("inner_compact", 0)
+    for M, (R, piv, rank) in zip((A, B), out):
+        Ro, po = crref.rref_modp(M, p)
+        assert piv == po and rank == len(po) and numpy.array_equal(R, Ro)
+
+
 def test_float_panel_rows_per_wave_same_result(gpu):
     """The float64 panel kernel with 16 or 32 rows per wave (knob
     fpanel_rows, per size class) computes the same RREF."""
