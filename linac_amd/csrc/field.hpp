@@ -127,14 +127,29 @@ struct FieldU32 {
     // update with a constant extra factor R^{-1}.  For p < 2^31 the two
     // products share one reduction: h x + (p - a) y < 2 p^2 < p 2^32.
     __host__ __device__ __forceinline__ uint32_t ff_update(uint32_t h, uint32_t x, uint32_t a, uint32_t y) const {
-        if (p < 0x80000000u) {
-            const uint64_t t = (uint64_t)h * x + (uint64_t)(p - a) * y;
-            const uint32_t t_lo = (uint32_t)t, t_hi = (uint32_t)(t >> 32);
-            const uint32_t m = t_lo * pneg;
-            const uint32_t u = t_hi + mulhi32(m, p) + (t_lo != 0u);
-            return u >= p ? u - p : u;
-        }
+        if (p < 0x80000000u) return ff_update_small(h, x, a, y);
         return sub(mont(h, x), mont(a, y));
+    }
+    __host__ __device__ __forceinline__ uint32_t ff_update_small(uint32_t h, uint32_t x, uint32_t a,
+                                                                 uint32_t y) const {
+        const uint64_t t = (uint64_t)h * x + (uint64_t)(p - a) * y;
+        const uint32_t t_lo = (uint32_t)t, t_hi = (uint32_t)(t >> 32);
+        const uint32_t m = t_lo * pneg;
+        const uint32_t u = t_hi + mulhi32(m, p) + (t_lo != 0u);
+        return u >= p ? u - p : u;
+    }
+    // N updates with one pivot row value y and head h, the choice of reduction
+    // made once for all of them (so that their independent chains interleave)
+    template <int N>
+    __host__ __device__ __forceinline__ void ff_update_n(uint32_t h, const uint32_t* x, const uint32_t* a, uint32_t y,
+                                                         uint32_t* out) const {
+        if (p < 0x80000000u) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) out[i] = ff_update_small(h, x[i], a[i], y);
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) out[i] = sub(mont(h, x[i]), mont(a[i], y));
+        }
     }
     // a * b mod p for two normal-form operands.
     __host__ __device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t b) const { return mont(to_mont(a), b); }
@@ -178,15 +193,38 @@ struct FieldU64 {
     // (h x - a y) R^{-1} mod p, the fraction-free row update of FieldU32 with
     // one shared reduction: t = h x + (p - a) y < 2 p^2 < p 2^64 (p < 2^63) is
     // a valid REDC input, summed in 128 bits.
-    __host__ __device__ __forceinline__ uint64_t ff_update(uint64_t h, uint64_t x, uint64_t a, uint64_t y) const {
+    __host__ __device__ __forceinline__ void ff_sum(uint64_t h, uint64_t x, uint64_t a, uint64_t y, uint64_t& t_lo,
+                                                    uint64_t& t_hi) const {
         const uint64_t b = p - a;
         const uint64_t l1 = h * x, l2 = b * y;
-        const uint64_t t_lo = l1 + l2;
-        const uint64_t t_hi = mulhi64(h, x) + mulhi64(b, y) + (t_lo < l1 ? 1ull : 0ull);
-        if (m61) return m61_redc(t_lo, t_hi);
+        t_lo = l1 + l2;
+        t_hi = mulhi64(h, x) + mulhi64(b, y) + (t_lo < l1 ? 1ull : 0ull);
+    }
+    __host__ __device__ __forceinline__ uint64_t redc(uint64_t t_lo, uint64_t t_hi) const {
         const uint64_t m = t_lo * pneg;
         const uint64_t u = t_hi + mulhi64(m, p) + (t_lo != 0ull);
         return u >= p ? u - p : u;
+    }
+    __host__ __device__ __forceinline__ uint64_t ff_update(uint64_t h, uint64_t x, uint64_t a, uint64_t y) const {
+        uint64_t t_lo, t_hi;
+        ff_sum(h, x, a, y, t_lo, t_hi);
+        if (m61) return m61_redc(t_lo, t_hi);
+        return redc(t_lo, t_hi);
+    }
+    // N updates, the reduction chosen once for all of them
+    template <int N>
+    __host__ __device__ __forceinline__ void ff_update_n(uint64_t h, const uint64_t* x, const uint64_t* a, uint64_t y,
+                                                         uint64_t* out) const {
+        uint64_t lo[N], hi[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) ff_sum(h, x[i], a[i], y, lo[i], hi[i]);
+        if (m61) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) out[i] = m61_redc(lo[i], hi[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) out[i] = redc(lo[i], hi[i]);
+        }
     }
     __host__ __device__ __forceinline__ uint64_t inverse(uint64_t h) const {
         const uint64_t e = p - 2ull;

@@ -316,8 +316,8 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
     __shared__ int s_wcnt[IF_T / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const unsigned long long t0 = wall_clock64();
-    const int c0 = bs->c0_next;
-    const int bcur = bs->bcur;
+    const int c0 = (int)uniform((uint32_t)bs->c0_next);
+    const int bcur = (int)uniform((uint32_t)bs->bcur);
     if (c0 >= bcur) {  // every inner panel of this outer panel is complete
         if (tid == 0) bs->k = 0;
         return;
@@ -371,7 +371,9 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
         if (s_total > CI) break;
     }
     const unsigned long long t1 = wall_clock64();
-    const int total = s_total;
+    // (read from LDS, so uniform in fact but not to the compiler: in scalar
+    // registers the elimination loop below is scalar control flow)
+    const int total = (int)uniform((uint32_t)s_total);
     const int nC = total < CI ? total : CI;
     const int more = total > CI ? 1 : 0;
     if (nC == 0) {
@@ -438,10 +440,14 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
                 h = aug[pv][c];
                 y = aug[pv][lane];
             }
+            // every row's update first (independent chains, interleaved), then
+            // the stores of the rows that change
+            T nv[IF_RPW];
+            f.template ff_update_n<IF_RPW>(h, xv, av, y, nv);
 #pragma unroll
             for (int q = 0; q < IF_RPW; ++q) {
                 const int r = wid + NW * q;
-                if (r < nC && r != pv) aug[r][lane] = f.ff_update(h, xv[q], av[q], y);
+                if (r < nC && r != pv) aug[r][lane] = nv[q];
             }
             if (tid == 0) {
                 prow_of[t] = pv;
@@ -496,11 +502,15 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
                 h = aug[pv][c];
                 y = aug[pv][j];
             }
+            T nv[RP];
+            f.template ff_update_n<RP>(h, xv, av, y, nv);
+#pragma unroll
+            for (int q = 0; q < RP; ++q)
+                if (j == c) nv[q] = av[q] != (T)0 ? f.p - av[q] : (T)0;
 #pragma unroll
             for (int q = 0; q < RP; ++q) {
                 const int r = wid + NW * (2 * q + hf);
-                if (r < nC && r != pv)
-                    aug[r][j] = j == c ? (av[q] != (T)0 ? f.p - av[q] : (T)0) : f.ff_update(h, xv[q], av[q], y);
+                if (r < nC && r != pv) aug[r][j] = nv[q];
             }
             if (tid == 0) {
                 prow_of[t] = pv;
