@@ -1123,6 +1123,14 @@ __device__ __forceinline__ void dma16(const void* src, void* dst) {
         __builtin_amdgcn_readfirstlane((uint32_t)(size_t)(__attribute__((address_space(3))) int8_t*)dst);
     asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds) : "memory", "m0");
 }
+// The same with the address split the instruction's way: a wave-uniform
+// 64-bit base in scalar registers plus a 32-bit per-lane offset, and the LDS
+// destination given as a wave-uniform LDS byte address (no vector address
+// arithmetic per load: a stage's bases advance on the scalar unit).
+__device__ __forceinline__ void dma16_so(uint32_t voff, const void* sbase, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(sbase), "s"(lds)
+                 : "memory", "m0");
+}
 // every wave's direct-to-LDS loads have landed and every wave is past its
 // reads of the buffers they refill
 __device__ __forceinline__ void lds_dma_barrier() {
@@ -1164,19 +1172,27 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer_lds(const BMat<LimbsU32>*
     const int r16 = lane & 15, g = lane >> 4;
     const v4i zero = {0, 0, 0, 0};
     const int lrow = lane >> 2, lqu = (lane & 3) ^ ((lane >> 4) & 3);
-    const int8_t* la = L + wid * d.strideL;
-    const int8_t* lb = B + wid * planeB;
+    const int wu = __builtin_amdgcn_readfirstlane(wid);  // (the wave's limb, in scalar registers)
+    const int8_t* la = L + wu * d.strideL;
+    const int8_t* lb = B + wu * planeB;
     uint32_t oa[4], ob[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         oa[c] = (uint32_t)(min(m_base + 16 * c + lrow, Mrows - 1) * ldL + lqu * 16);
         ob[c] = (uint32_t)(min(n_base + 16 * c + lrow, Ncols - 1) * Kpad + lqu * 16);
     }
-    auto issue = [&](int kk, StageBuf& dA, StageBuf& dB) {
+    // LDS byte address of this wave's limb in a stage buffer
+    auto lds_of = [&](StageBuf& b) -> uint32_t {
+        return __builtin_amdgcn_readfirstlane((uint32_t)(size_t)(__attribute__((address_space(3))) int8_t*)&b[wu][0]);
+    };
+    const uint32_t lA0 = lds_of(sA0), lA1 = lds_of(sA1), lB0 = lds_of(sB0), lB1 = lds_of(sB1);
+    auto issue = [&](int kk, uint32_t dA, uint32_t dB) {
+        const int8_t* sa = la + kk * G_K;
+        const int8_t* sb = lb + kk * G_K;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            dma16(la + (oa[c] + (uint32_t)(kk * G_K)), &dA[wid][c * 1024]);
-            dma16(lb + (ob[c] + (uint32_t)(kk * G_K)), &dB[wid][c * 1024]);
+            dma16_so(oa[c], sa, dA + c * 1024);
+            dma16_so(ob[c], sb, dB + c * 1024);
         }
     };
     // wave w, chunk c: rows 16 w + 4 c .. + 3; lane i: row + i / 16, columns 4 (i % 16) .. + 3
@@ -1222,16 +1238,16 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer_lds(const BMat<LimbsU32>*
         }
     };
     const int nk = Kpad / G_K;
-    auto stage = [&](int kk, StageBuf& cA, StageBuf& cB, StageBuf& nA, StageBuf& nB) {
+    auto stage = [&](int kk, StageBuf& cA, StageBuf& cB, StageBuf& nA, uint32_t nlA, uint32_t nlB) {
         lds_dma_barrier();
-        if (kk + 1 < nk) issue(kk + 1, nA, nB);
+        if (kk + 1 < nk) issue(kk + 1, nlA, nlB);
         else issue_c(nA);
         mfmas(cA, cB);
     };
-    issue(0, sA0, sB0);
+    issue(0, lA0, lB0);
     for (int kk = 0; kk < nk; kk += 2) {
-        stage(kk, sA0, sB0, sA1, sB1);
-        if (kk + 1 < nk) stage(kk + 1, sA1, sB1, sA0, sB0);
+        stage(kk, sA0, sB0, sA1, lA1, lB1);
+        if (kk + 1 < nk) stage(kk + 1, sA1, sB1, sA0, lA0, lB0);
     }
     lds_dma_barrier();
     const uint32_t* sc = reinterpret_cast<const uint32_t*>(nk & 1 ? &sA1[0][0] : &sA0[0][0]);
