@@ -546,7 +546,10 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
         T ex = __shfl_up(v, 1, 64);
         if (lane == 0) ex = one_m;
         const T tot = uniform(__shfl(v, k - 1, 64));
-        const T x = f.to_mont(binary_inverse<T>(tot, f.p));
+        T inv;
+        if constexpr (sizeof(T) == 4) inv = euclid_inverse_u32(tot, f.p);
+        else inv = binary_inverse<T>(tot, f.p);
+        const T x = f.to_mont(inv);
         if (lane < k) {
             s_pre[lane] = ex;
             s_inv[lane] = f.mont(ex, x);

@@ -9,6 +9,7 @@
 //   M32 p a b / M64 p a b  mul
 //   I32 p h / I64 p h      inverse
 //   B32 p h / B64 p h      binary_inverse
+//   E32 p h                euclid_inverse_u32
 //   D32 c / D64 c          balanced base-256 digits of a centred value
 #include <cstdio>
 #include <cstring>
@@ -46,6 +47,10 @@ int main() {
             in >> p >> a;
             if (op == "B32") std::cout << binary_inverse<uint32_t>((uint32_t)a, (uint32_t)p) << "\n";
             else std::cout << binary_inverse<uint64_t>(a, p) << "\n";
+        } else if (op == "E32") {
+            uint64_t p, a;
+            in >> p >> a;
+            std::cout << euclid_inverse_u32((uint32_t)a, (uint32_t)p) << "\n";
         } else if (op == "U32" || op == "M32" || op == "I32") {
             uint64_t p, a, b, c, d;
             in >> p >> a;

@@ -9,8 +9,9 @@ field.hpp, compiled for the host) against Python integers.
 * fold_sub_k32 (inner GEMM, K <= 32) likewise at |S_s| <= 2^21.
 * fold_sub64 (8 digits, 15 diagonals, K = 512, |S_s| <= 2^26) for primes
   from just above 2^32 to just below 2^63, on random sign patterns.
-* ff_update, mul, inverse (Montgomery, R = 2^32 / 2^64) and binary_inverse
-  (binary extended Euclid) on boundary operands.
+* ff_update, mul, inverse (Montgomery, R = 2^32 / 2^64), binary_inverse
+  (binary extended Euclid) and euclid_inverse_u32 (extended Euclid in double
+  precision, the inner factorisation's one inversion) on boundary operands.
 * balanced digits of centred values at the ends of their ranges.
 
 These replace the reference's arbitrary-precision modular arithmetic
@@ -148,12 +149,25 @@ def test_montgomery_update_mul_inverse(harness, bits):
                 want.append(pow(h, -1, p))
                 lines.append(f"B{bits} {p} {h}")
                 want.append(pow(h, -1, p))
+                if bits == 32:
+                    lines.append(f"E32 {p} {h}")
+                    want.append(pow(h, -1, p))
         # the binary inverse's control flow depends on the value: powers of
         # two (long halving runs), values next to p and to its half
         for h in [2 ** e for e in range(1, bits) if 2 ** e < p] + [p - 3, (p - 1) // 2 - 1, (p + 1) // 2 + 1] + [rng.randrange(1, p) for _ in range(40)]:
             if 0 < h < p:
                 lines.append(f"B{bits} {p} {h}")
                 want.append(pow(h, -1, p))
+                if bits == 32:  # (long and short quotient chains alike)
+                    lines.append(f"E32 {p} {h}")
+                    want.append(pow(h, -1, p))
+        if bits == 32:
+            lines.append(f"E32 {p} 0")
+            want.append(0)
+            for h in {1, 2, 3, p - 1, p - 2} | {(p * k) // 13 for k in range(1, 13)}:
+                if 0 < h < p:
+                    lines.append(f"E32 {p} {h}")
+                    want.append(pow(h, -1, p))
     got = harness(lines)
     bad = [(ln, g, w) for ln, g, w in zip(lines, got, want) if int(g) != w]
     assert not bad, bad[:3]
