@@ -107,8 +107,9 @@ int linac_profile_last(double* out, int n);
  *       block back substitution (0: Gauss-Jordan form throughout); results
  *       are the same (the RREF is unique);
  *   "inner_compact" 0: the 32-bit inner factorisation in the compact form;
- *   "inner_threads" 1024: threads of the 32-bit inner factorisation's
- *       workgroup, 256, 512 or 1024 (fewer waves, each with more rows);
+ *   "inner_threads" 0: threads of the inner factorisation's workgroup, 256,
+ *       512 or 1024 (fewer waves, each with more rows; 0: 1024 for 32-bit,
+ *       512 for 64-bit storage);
  *   "inner_split" 16: at most this many workgroups per 64-row stripe of the
  *       inner GEMM (0: 8 for 64-bit, 4 for 32-bit storage);
  *   "tile64" 1: the 64-bit trailing GEMM's tile, 64 x 32 (1) or 64 x 64 (2);

@@ -625,12 +625,16 @@ __global__ __launch_bounds__(IF_T, IF_T == LT::IF_T ? LT::IF_WAVES : IF_T / 256)
 }
 
 // The inner factorisation of storage width LT: the compact form always for
-// 64-bit storage, by tuning knob "inner_compact" for 32-bit storage; 32-bit
-// storage takes its workgroup size from tuning knob "inner_threads".
+// 64-bit storage, by tuning knob "inner_compact" for 32-bit storage; the
+// workgroup size from tuning knob "inner_threads" (0: LT::IF_T).
 template <class LT>
 static void launch_inner_factor(unsigned B, hipStream_t s, const BMat<LT>* desc, int epoch) {
     if constexpr (std::is_same<LT, LimbsU64>::value) {
-        k_inner_factor<LT, true, LT::IF_T><<<B, LT::IF_T, 0, s>>>(desc, epoch);
+        switch (tuning().inner_threads) {
+            case 256: k_inner_factor<LT, true, 256><<<B, 256, 0, s>>>(desc, epoch); break;
+            case 1024: k_inner_factor<LT, true, 1024><<<B, 1024, 0, s>>>(desc, epoch); break;
+            default: k_inner_factor<LT, true, LT::IF_T><<<B, LT::IF_T, 0, s>>>(desc, epoch);
+        }
     } else {
         const bool cmp = tuning().inner_compact != 0;
         switch (tuning().inner_threads) {

@@ -414,7 +414,7 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"fast_panel", &t.fast_panel, 0, 2},
                  {"echelon", &t.echelon, 0, 1},
                  {"inner_compact", &t.inner_compact, 0, 1},
-                 {"inner_threads", &t.inner_threads, 256, 1024},
+                 {"inner_threads", &t.inner_threads, 0, 1024},
                  {"inner_split", &t.inner_split, 0, 32},
                  {"tile64", &t.tile64, 1, 2},
                  {"fpanel_rows", &t.fpanel_rows, 16, 32}};
@@ -423,8 +423,8 @@ int linac_tune_set(const char* key, int64_t value) {
             if (value < k.lo || value > k.hi)
                 return fail(LINAC_ERR_INVALID, "%s = %lld outside [%lld, %lld]", key, (long long)value,
                             (long long)k.lo, (long long)k.hi);
-            if (!strcmp(key, "inner_threads") && value != 256 && value != 512 && value != 1024)
-                return fail(LINAC_ERR_INVALID, "inner_threads = %lld: 256, 512 or 1024", (long long)value);
+            if (!strcmp(key, "inner_threads") && value != 0 && value != 256 && value != 512 && value != 1024)
+                return fail(LINAC_ERR_INVALID, "inner_threads = %lld: 0, 256, 512 or 1024", (long long)value);
             *k.slot = value;
             return LINAC_OK;
         }

@@ -135,7 +135,7 @@ struct Tuning {
     int64_t fast_panel = 2;
     int64_t echelon = 1;  // blocked GF(p): forward elimination plus back substitution (0: Gauss-Jordan)
     int64_t inner_compact = 0;  // 32-bit inner factorisation in the compact form (32 slots a row)
-    int64_t inner_threads = 1024;  // 32-bit inner factorisation: threads per workgroup (256, 512 or 1024)
+    int64_t inner_threads = 0;  // inner factorisation: threads per workgroup (256, 512, 1024; 0: 1024 for 32-bit, 512 for 64-bit)
     int64_t fpanel_rows = 16;   // float64 panel kernel: rows per wave (16, or 32 for half the workgroups)
     int64_t tile64 = 1;         // 64-bit trailing GEMM tile: 64 x 32 (1) or 64 x 64 (2)
     int64_t inner_split = 16;   // workgroups per row stripe of the inner GEMM at most (0: 8 for 64-bit, 4 for 32-bit)

@@ -149,7 +149,7 @@ def test_tuning_knobs_set_get_and_validation():
         assert autotune.get("gemm_lds") == 0
         with pytest.raises(ValueError):
             autotune.set_knob("gemm_lds", 2)
-        for v in (256, 512, 1024):
+        for v in (0, 256, 512, 1024):
             autotune.set_knob("inner_threads", v)
             assert autotune.get("inner_threads") == v
         for v in (128, 300, 2048):
@@ -161,7 +161,7 @@ def test_tuning_knobs_set_get_and_validation():
             autotune.get("no_such_knob")
     finally:
         autotune.apply(before)
-        autotune.set_knob("inner_threads", 1024)
+        autotune.set_knob("inner_threads", 0)
     assert autotune.current() == before
 
 

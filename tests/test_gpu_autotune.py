@@ -162,11 +162,29 @@ def test_inner_factor_workgroup_sizes_exact(gpu, threads, compact):
         autotune.set_knob("inner_compact", compact)
         out = [gpu.rref_with_pivots(M, p, method="blocked") for M in (A, B)]
     finally:
-        autotune.set_knob("inner_threads", 1024)
+        autotune.set_knob("inner_threads", 0)
         autotune.set_knob("inner_compact", 0)
     for M, (R, piv, rank) in zip((A, B), out):
         Ro, po = crref.rref_modp(M, p)
         assert piv == po and rank == len(po) and numpy.array_equal(R, Ro)
+
+
+@pytest.mark.parametrize("threads", [256, 1024])
+def test_inner_factor_workgroup_sizes_exact_u64(gpu, threads):
+    """The same for 64-bit storage (compact form) with 4 or 16 waves."""
+    from linac_amd import autotune
+    p = 2 ** 61 - 1
+    rs = numpy.random.RandomState(13)
+    B = rs.randint(0, 2 ** 62, size=(1100, 1200), dtype=numpy.int64).astype(numpy.uint64) % numpy.uint64(p)
+    B[:, 200:230] = 0
+    B[700:] = (B[:400] + B[300:700]) % numpy.uint64(p)
+    try:
+        autotune.set_knob("inner_threads", threads)
+        R, piv, rank = gpu.rref_with_pivots(B, p, method="blocked")
+    finally:
+        autotune.set_knob("inner_threads", 0)
+    Ro, po = crref.rref_modp(B, p)
+    assert piv == po and rank == len(po) and numpy.array_equal(R.astype(numpy.uint64), Ro.astype(numpy.uint64))
 
 
 def test_float_panel_rows_per_wave_same_result(gpu):
