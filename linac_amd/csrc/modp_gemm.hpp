@@ -79,12 +79,11 @@ __host__ __device__ __forceinline__ uint32_t fold_sub(uint32_t cval, const int32
     r = red_rint(fma(r, 65536.0, P2), p, pinv);
     r = red_rint(fma(r, 65536.0, P1), p, pinv);
     r = fma(r, 65536.0, P0);  // |r| < 2^48: reduced together with C below
-    double t = (double)cval - r;
-    const double q = floor(t * pinv);
-    t = fma(-q, p, t);
-    t = t < 0.0 ? t + p : t;
-    t = t >= p ? t - p : t;
-    return (uint32_t)t;
+    // |t| < 2^48 + p, so t / p < 2^18 rounds with an error far below 1/2 and
+    // the nearest-quotient remainder lies within [-p/2, p/2] up to ties, well
+    // inside (-p, p): one correction brings it to [0, p)
+    const double t = red_rint((double)cval - r, p, pinv);
+    return (uint32_t)(t < 0.0 ? t + p : t);
 }
 
 // fold_sub for sums of at most 32 limb products per diagonal entry (the
@@ -97,12 +96,8 @@ __host__ __device__ __forceinline__ uint32_t fold_sub_k32(uint32_t cval, const i
     r = red_rint(fma(r, 65536.0, (double)Q2), p, pinv);
     r = red_rint(fma(r, 65536.0, (double)Q1), p, pinv);
     r = fma(r, 65536.0, (double)Q0);
-    double t = (double)cval - r;
-    const double q = floor(t * pinv);
-    t = fma(-q, p, t);
-    t = t < 0.0 ? t + p : t;
-    t = t >= p ? t - p : t;
-    return (uint32_t)t;
+    const double t = red_rint((double)cval - r, p, pinv);  // within (-p, p), as in fold_sub
+    return (uint32_t)(t < 0.0 ? t + p : t);
 }
 
 // ---------------------------------------------------------------------------

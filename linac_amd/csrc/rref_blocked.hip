@@ -1258,21 +1258,26 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer_lds(const BMat<LimbsU32>*
     }
     lds_dma_barrier();
     const uint32_t* sc = reinterpret_cast<const uint32_t*>(nk & 1 ? &sA1[0][0] : &sA0[0][0]);
+    // one address and bound per (mi, ni); the lane's four rows i step by ldc
+    // (no 64-bit multiply per element)
+    const int rows_left = (int)min<int64_t>(Mrows - m_base, G_M);  // rows of the tile inside the matrix
+    const int cols_left = (int)min<int64_t>(Ncols - n_base, G_N);
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
+        for (int ni = 0; ni < 2; ++ni) {
+            const int rl0 = wm * 32 + mi * 16 + 4 * g, cl = wn * 32 + ni * 16 + r16;
+            uint32_t* cp = C + (m_base + rl0) * ldc + n_base + cl;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int rl = wm * 32 + mi * 16 + 4 * g + i, cl = wn * 32 + ni * 16 + r16;
-                const int64_t row = m_base + rl, col = n_base + cl;
-                if (row < Mrows && col < Ncols) {
+                if (rl0 + i < rows_left && cl < cols_left) {
                     int32_t sv[7];
 #pragma unroll
                     for (int q = 0; q < 7; ++q) sv[q] = acc[q][mi][ni][i];
-                    gst<uint32_t>(C + row * ldc + col, fold_sub(sc[rl * 64 + cl], sv, pd, pinv));
+                    gst<uint32_t>(cp + i * ldc, fold_sub(sc[(rl0 + i) * 64 + cl], sv, pd, pinv));
                 }
             }
+        }
 }
 
 // The 32-bit trailing GEMM launch: direct-to-LDS staging when allowed (tuning
