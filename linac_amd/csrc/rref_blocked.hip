@@ -1568,32 +1568,41 @@ __global__ __launch_bounds__(G_T, Tile64<NI>::GW) void k_gemm_outer64(const BMat
     // the stage registers would keep 16 more addresses live through the loop,
     // beyond the register budget of two workgroups per CU); the other
     // workgroup's MFMAs cover the latency
+    // (one address per 16 x 16 block, the lane's four rows ldc apart; rows
+    // past the matrix read its last row)
     uint64_t cv[2][NI][4];
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < NI; ++ni)
+        for (int ni = 0; ni < NI; ++ni) {  // unconditional loads, as in k_gemm_outer
+            const int64_t row0 = m_base + wm * 32 + mi * 16 + 4 * g;
+            const int64_t col = min(n_base + wn * 16 * NI + ni * 16 + r16, Ncols - 1);
+            const uint64_t* cp = C + min(row0, Mrows - 1) * ldc + col;
+            const uint64_t* last = C + (Mrows - 1) * ldc + col;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {  // unconditional loads, as in k_gemm_outer
-                const int64_t row = min(m_base + wm * 32 + mi * 16 + 4 * g + i, Mrows - 1);
-                const int64_t col = min(n_base + wn * 16 * NI + ni * 16 + r16, Ncols - 1);
-                cv[mi][ni][i] = gld<uint64_t>(C + row * ldc + col);
+            for (int i = 0; i < 4; ++i) {
+                const uint64_t* q = cp + i * ldc;
+                cv[mi][ni][i] = gld<uint64_t>(q < last ? q : last);
             }
+        }
+    const int rows_left = (int)min<int64_t>(Mrows - m_base, G_M);  // rows of the tile inside the matrix
+    const int cols_left = (int)min<int64_t>(Ncols - n_base, 32 * NI);
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < NI; ++ni)
+        for (int ni = 0; ni < NI; ++ni) {
+            const int rl0 = wm * 32 + mi * 16 + 4 * g, cl = wn * 16 * NI + ni * 16 + r16;
+            uint64_t* cp = C + (m_base + rl0) * ldc + n_base + cl;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int64_t row = m_base + wm * 32 + mi * 16 + 4 * g + i;
-                const int64_t col = n_base + wn * 16 * NI + ni * 16 + r16;
-                if (row < Mrows && col < Ncols) {
+                if (rl0 + i < rows_left && cl < cols_left) {
                     int32_t sv[ND64];
 #pragma unroll
                     for (int q = 0; q < ND64; ++q) sv[q] = acc[q][mi][ni][i];
-                    C[row * ldc + col] = fold_sub64(cv[mi][ni][i], sv, p, pinv);
+                    cp[i * ldc] = fold_sub64(cv[mi][ni][i], sv, p, pinv);
                 }
             }
+        }
 }
 
 // The 64-bit trailing GEMM with the tile width NI picked for the launch.
