@@ -159,6 +159,22 @@ struct BMat {
 
 constexpr int64_t STRIDE_W = (int64_t)BO * BO;
 
+// Limb planes of the trailing GEMMs' operands (planesL / planesW / planesB /
+// planesL1 / planesB1 and the planes of the echelon completion and of
+// linac_gemm_sub) are stored in tiles of 16 rows x 64 bytes of depth: byte kb
+// of row r of a plane with row length ld (a multiple of 64) sits at
+//   (r / 16) * 16 ld + (kb / 64) * 1024 + (r % 16) * 64 + kb % 64,
+// so the 16 rows x 64 bytes that one direct-to-LDS load instruction stages
+// are one contiguous kilobyte (eight whole 128-byte lines) instead of 16
+// pieces of 64 bytes ld apart, and a stage advances by 1024 bytes.  A depth
+// offset kb0 (a multiple of 64) is a pointer offset of 16 kb0 bytes; a row
+// offset r0 (a multiple of 16) one of r0 ld bytes.  Planes hold their row
+// count rounded up to 16 (prows).
+__host__ __device__ __forceinline__ int64_t ptile(int64_t r, int64_t kb, int64_t ld) {
+    return (r >> 4) * (ld << 4) + ((kb >> 6) << 10) + ((r & 15) << 6) + (kb & 63);
+}
+__host__ __device__ __forceinline__ int64_t prows(int64_t rows) { return (rows + 15) & ~(int64_t)15; }
+
 // LINAC_DEBUG_STEPS: one line per outer panel on stderr (row offsets, rank,
 // host time of the step so far and its wait for the status read)
 static bool debug_steps() {
@@ -818,7 +834,7 @@ __device__ static void make_lhs_body(const BMat<LT>& d, int src, int e1, int64_t
         for (int l = 0; l < NL; ++l) {
             int32_t pk;
             __builtin_memcpy(&pk, dg[l], 4);
-            *reinterpret_cast<int32_t*>(d.planesL + l * d.strideL + m * d.Kq + koff + t) = pk;
+            *reinterpret_cast<int32_t*>(d.planesL + l * d.strideL + ptile(m, koff + t, d.Kq)) = pk;
         }
     }
 }
@@ -893,7 +909,7 @@ __device__ static void make_rhs_body(const BMat<LT>& d, int job, int64_t bx, int
 #pragma unroll
             for (int l = 0; l < NL; ++l) {
                 const v4i w = {(int)wd[l][0], (int)wd[l][1], (int)wd[l][2], (int)wd[l][3]};
-                *reinterpret_cast<v4i*>(planes + l * pstride + n * depth + tb + tq) = w;
+                *reinterpret_cast<v4i*>(planes + l * pstride + ptile(n, tb + tq, depth)) = w;
             }
         }
         __syncthreads();
@@ -1003,7 +1019,7 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
     const int64_t Mrows = d.N;
     const int64_t Ncols = JOB == 0 ? d.nT : d.kA;
     const int64_t ldL = d.Kq;
-    const int8_t* L = d.planesL + (JOB == 0 ? 0 : d.KqA);
+    const int8_t* L = d.planesL + (JOB == 0 ? 0 : 16 * (int64_t)d.KqA);
     const int64_t planeL = d.strideL;
     const int8_t* B = JOB == 0 ? d.planesB : d.planesW;
     const int64_t planeB = JOB == 0 ? d.strideB : STRIDE_W;
@@ -1038,14 +1054,14 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer(const BMat<LimbsU32>* __r
     // each per thread); the row/limb part of each address is fixed per tile
     v4i ra[4], rb[4];
     const int srow = tid >> 2, squ = tid & 3;  // the same for the 4 limbs (q = limb)
-    const int8_t* pa = L + min(m_base + srow, Mrows - 1) * ldL + squ * 16;
-    const int8_t* pb = B + min(n_base + srow, Ncols - 1) * Kpad + squ * 16;
+    const int8_t* pa = L + ptile(min(m_base + srow, Mrows - 1), squ * 16, ldL);
+    const int8_t* pb = B + ptile(min(n_base + srow, Ncols - 1), squ * 16, Kpad);
     const int soff = lds_at(srow, squ);
-    auto gload = [&](int kc) {
+    auto gload = [&](int kc) {  // (depth kc: 16 kc bytes into the tiled planes)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            ra[q] = gld<v4i>(pa + q * planeL + kc);
-            rb[q] = gld<v4i>(pb + q * planeB + kc);
+            ra[q] = gld<v4i>(pa + q * planeL + 16 * kc);
+            rb[q] = gld<v4i>(pb + q * planeB + 16 * kc);
         }
     };
     auto lstore = [&](int buf) {
@@ -1168,7 +1184,7 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer_lds(const BMat<LimbsU32>*
     const int64_t Mrows = d.N;
     const int64_t Ncols = JOB == 0 ? d.nT : d.kA;
     const int64_t ldL = d.Kq;
-    const int8_t* L = d.planesL + (JOB == 0 ? 0 : d.KqA);
+    const int8_t* L = d.planesL + (JOB == 0 ? 0 : 16 * (int64_t)d.KqA);
     const int8_t* B = JOB == 0 ? d.planesB : d.planesW;
     const int64_t planeB = JOB == 0 ? d.strideB : STRIDE_W;
     const int Kpad = JOB == 0 ? d.Kq : d.Kq - d.KqA;
@@ -1185,17 +1201,17 @@ __global__ __launch_bounds__(G_T, 2) void k_gemm_outer_lds(const BMat<LimbsU32>*
     uint32_t oa[4], ob[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        oa[c] = (uint32_t)(min(m_base + 16 * c + lrow, Mrows - 1) * ldL + lqu * 16);
-        ob[c] = (uint32_t)(min(n_base + 16 * c + lrow, Ncols - 1) * Kpad + lqu * 16);
+        oa[c] = (uint32_t)ptile(min(m_base + 16 * c + lrow, Mrows - 1), lqu * 16, ldL);
+        ob[c] = (uint32_t)ptile(min(n_base + 16 * c + lrow, Ncols - 1), lqu * 16, Kpad);
     }
     // LDS byte address of this wave's limb in a stage buffer
     auto lds_of = [&](StageBuf& b) -> uint32_t {
         return __builtin_amdgcn_readfirstlane((uint32_t)(size_t)(__attribute__((address_space(3))) int8_t*)&b[wu][0]);
     };
     const uint32_t lA0 = lds_of(sA0), lA1 = lds_of(sA1), lB0 = lds_of(sB0), lB1 = lds_of(sB1);
-    auto issue = [&](int kk, uint32_t dA, uint32_t dB) {
-        const int8_t* sa = la + kk * G_K;
-        const int8_t* sb = lb + kk * G_K;
+    auto issue = [&](int kk, uint32_t dA, uint32_t dB) {  // (a stage is 1024 bytes of the tiled planes)
+        const int8_t* sa = la + kk * 1024;
+        const int8_t* sb = lb + kk * 1024;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             dma16_so(oa[c], sa, dA + c * 1024);
@@ -1516,7 +1532,7 @@ __global__ __launch_bounds__(G_T, Tile64<NI>::GW) void k_gemm_outer64(const BMat
     const int64_t Mrows = d.N;
     const int64_t Ncols = JOB == 0 ? d.nT : d.kA;
     const int64_t ldL = d.Kq;
-    const int8_t* L = d.planesL + (JOB == 0 ? 0 : d.KqA);
+    const int8_t* L = d.planesL + (JOB == 0 ? 0 : 16 * (int64_t)d.KqA);
     const int64_t planeL = d.strideL;
     const int8_t* B = JOB == 0 ? d.planesB : d.planesW;
     const int64_t planeB = JOB == 0 ? d.strideB : STRIDE_W;
@@ -1532,16 +1548,16 @@ __global__ __launch_bounds__(G_T, Tile64<NI>::GW) void k_gemm_outer64(const BMat
     // A stage: limb q of row tid/4, quarter tid%4; B stage: BQ chunks
     v4i ra[NL64], rb[BQ];
     const int srow = tid >> 2, squ = tid & 3;
-    const int8_t* pa = L + min(m_base + srow, Mrows - 1) * ldL + squ * 16;
+    const int8_t* pa = L + ptile(min(m_base + srow, Mrows - 1), squ * 16, ldL);
     const int soff = lds_at(srow, squ);
     const int bl0 = tid / (TN * 4), brow = (tid % (TN * 4)) >> 2, bqu = tid & 3;
-    const int8_t* pb = B + bl0 * planeB + min(n_base + brow, Ncols - 1) * Kpad + bqu * 16;
+    const int8_t* pb = B + bl0 * planeB + ptile(min(n_base + brow, Ncols - 1), bqu * 16, Kpad);
     const int boff = lds_at(brow, bqu);
-    auto gload = [&](int kc) {
+    auto gload = [&](int kc) {  // (depth kc: 16 kc bytes into the tiled planes)
 #pragma unroll
-        for (int q = 0; q < NL64; ++q) ra[q] = gld<v4i>(pa + q * planeL + kc);
+        for (int q = 0; q < NL64; ++q) ra[q] = gld<v4i>(pa + q * planeL + 16 * kc);
 #pragma unroll
-        for (int q = 0; q < BQ; ++q) rb[q] = gld<v4i>(pb + q * BLS * planeB + kc);
+        for (int q = 0; q < BQ; ++q) rb[q] = gld<v4i>(pb + q * BLS * planeB + 16 * kc);
     };
 
     v4i acc[ND64][2][NI];
@@ -2226,7 +2242,7 @@ __global__ __launch_bounds__(256) void k_fast_operands(const BMat<LT>* __restric
             for (int l = 0; l < NL; ++l) {
                 int32_t pk;
                 __builtin_memcpy(&pk, dg[l], 4);
-                *reinterpret_cast<int32_t*>(d.planesL1 + l * d.strideL1 + r * Kq + c) = pk;
+                *reinterpret_cast<int32_t*>(d.planesL1 + l * d.strideL1 + ptile(r, c, Kq)) = pk;
             }
         }
         return;
@@ -2239,7 +2255,7 @@ __global__ __launch_bounds__(256) void k_fast_operands(const BMat<LT>* __restric
         int8_t g[NL];
         balanced_digits(centred(c.X[c.piv_rows[t] * XLD + BO + sidx], d.f.p), g);
 #pragma unroll
-        for (int l = 0; l < NL; ++l) d.planesB1[l * STRIDE_W + (int64_t)sidx * Kq + col] = g[l];
+        for (int l = 0; l < NL; ++l) d.planesB1[l * STRIDE_W + ptile(sidx, col, Kq)] = g[l];
     }
 }
 
@@ -2320,10 +2336,10 @@ static WorkspaceLayout workspace_layout(int64_t N, int64_t M, bool with_elim_sta
     const int64_t cap = std::max<int64_t>(1, std::min(N, M));
     size_t off = 0;
     w.X = off; off += al((size_t)N * XLD * E);
-    w.L = off; off += al(NL * round_up(N * 2 * BO, 256));
+    w.L = off; off += al(NL * round_up(prows(N) * 2 * BO, 256));
     w.Wp = off; off += al(NL * STRIDE_W);
-    w.B = off; off += al(NL * round_up(std::max<int64_t>(M, XLD) * 2 * BO, 256));
-    w.L1 = off; off += al(NL * round_up(N * BO, 256));
+    w.B = off; off += al(NL * round_up(prows(std::max<int64_t>(M, XLD)) * 2 * BO, 256));
+    w.L1 = off; off += al(NL * round_up(prows(N) * BO, 256));
     w.B1 = off; off += al(NL * STRIDE_W);
     w.Ai = off; off += al(NL * round_up(N * KPI, 256));
     w.Ui = off; off += al(NL * XLD * KPI);
@@ -2357,12 +2373,12 @@ static void carve(BMat<LT>& d, char* base, const WorkspaceLayout& w, int64_t N, 
     using T = typename LT::T;
     d.X = (T*)(base + w.X);
     d.planesL = (int8_t*)(base + w.L);
-    d.strideL = round_up(N * 2 * BO, 256);
+    d.strideL = round_up(prows(N) * 2 * BO, 256);
     d.planesW = (int8_t*)(base + w.Wp);
     d.planesB = (int8_t*)(base + w.B);
-    d.strideB = round_up(std::max<int64_t>(M, XLD) * 2 * BO, 256);
+    d.strideB = round_up(prows(std::max<int64_t>(M, XLD)) * 2 * BO, 256);
     d.planesL1 = (int8_t*)(base + w.L1);
-    d.strideL1 = round_up(N * BO, 256);
+    d.strideL1 = round_up(prows(N) * BO, 256);
     d.planesB1 = (int8_t*)(base + w.B1);
     d.planesAi = (int8_t*)(base + w.Ai);
     d.strideAi = round_up(N * KPI, 256);
@@ -2755,7 +2771,7 @@ void Engine<LT>::fill(const std::vector<It*>& act, int part, char* h, char* dv, 
     bool offs = true, aligned = true;
     for (size_t m = 0; m < B; ++m) {
         const BM& d = hd[m];
-        offs = offs && d.N * (int64_t)d.Kq < (1ll << 31) && d.nT * (int64_t)d.Kq < (1ll << 31);
+        offs = offs && prows(d.N) * (int64_t)d.Kq < (1ll << 31) && prows(d.nT) * (int64_t)d.Kq < (1ll << 31);
         aligned = aligned && ((((uintptr_t)(d.A + d.T0)) | (uintptr_t)(d.ld * (int64_t)sizeof(typename LT::T))) & 15) == 0;
     }
     out->lds_main = offs && aligned;
@@ -3285,7 +3301,7 @@ __global__ __launch_bounds__(256) void k_planes_cols(const typename LT::T* __res
             balanced_digits(centred(A[r * lda + c], p), dg);
         }
 #pragma unroll
-        for (int l = 0; l < NL; ++l) planes[l * stride + i * Kpad + t] = dg[l];
+        for (int l = 0; l < NL; ++l) planes[l * stride + ptile(i, t, Kpad)] = dg[l];
     }
 }
 
@@ -3330,7 +3346,7 @@ __global__ __launch_bounds__(256) void k_planes_rows(const typename LT::T* __res
 #pragma unroll
             for (int l = 0; l < NL; ++l) {
                 const v4i w = {(int)wd[l][0], (int)wd[l][1], (int)wd[l][2], (int)wd[l][3]};
-                *reinterpret_cast<v4i*>(planes + l * stride + j * Kpad + tb + tq) = w;
+                *reinterpret_cast<v4i*>(planes + l * stride + ptile(j, tb + tq, Kpad)) = w;
             }
         }
         __syncthreads();
@@ -3363,8 +3379,8 @@ static int gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda
     int64_t totL = 0, totB = 0;
     for (int64_t c = 0; c < nch; ++c) {
         kp[c] = round_up(std::min<int64_t>(GS_KC, k - c * GS_KC), 64);
-        sL[c] = round_up(m * kp[c], 256);
-        sB[c] = round_up(n * kp[c], 256);
+        sL[c] = round_up(prows(m) * kp[c], 256);
+        sB[c] = round_up(prows(n) * kp[c], 256);
         offL[c] = totL;
         offB[c] = totB;
         totL += 4 * sL[c];
@@ -3417,7 +3433,7 @@ static int gemm_sub_u32(uint32_t* C, int64_t ldc, const uint32_t* A, int64_t lda
     }
     const TileSched ts{d_sched, d_sched + 2, 1, (int)tuning().pick(Tuning::GEMM_GROUP, std::max(m, n)), 0, -1};
     bool lds_ok = (((uintptr_t)C | (uintptr_t)(ldc * 4)) & 15) == 0;
-    for (int64_t c = 0; c < nch; ++c) lds_ok = lds_ok && m * kp[c] < (1ll << 31) && n * kp[c] < (1ll << 31);
+    for (int64_t c = 0; c < nch; ++c) lds_ok = lds_ok && prows(m) * kp[c] < (1ll << 31) && prows(n) * kp[c] < (1ll << 31);
     for (int64_t c = 0; c < nch; ++c)
         launch_outer32<0>((unsigned)(ceil_div(ntiles, NUM_XCD) * NUM_XCD), s, d_desc + c, ts, lds_ok);
     LINAC_HIP(hipGetLastError());
@@ -3535,8 +3551,8 @@ int echelon_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F
                 if (el == sl || sl == 0 || f0 >= nF || it->r0 == 0) continue;
                 const int64_t Kpad = round_up(el - sl, 64);
                 jobs.push_back(Job{sl, el - sl, f0, Kpad});
-                maxL = std::max(maxL, round_up(sl * Kpad, 256));
-                maxB = std::max(maxB, round_up((nF - f0) * Kpad, 256));
+                maxL = std::max(maxL, round_up(prows(sl) * Kpad, 256));
+                maxB = std::max(maxB, round_up(prows(nF - f0) * Kpad, 256));
             }
         const size_t cmap_b = (size_t)round_up(M * 4, 256), fcol_b = (size_t)round_up(std::max<int64_t>(nF, 1) * 4, 256);
         const size_t desc_b = (size_t)round_up((int64_t)(jobs.size() * sizeof(BM)), 256);
@@ -3603,7 +3619,7 @@ int echelon_finalize(typename F::T* A, int64_t ld, int64_t N, int64_t M, const F
                 const unsigned grid = (unsigned)(ceil_div(hs[3 * i + 1], NUM_XCD) * NUM_XCD);
                 if constexpr (u32) {
                     const bool lds_ok = ((((uintptr_t)(G + j.f0)) | (uintptr_t)(ldg * 4)) & 15) == 0 &&
-                                        j.s * j.Kpad < (1ll << 31) && n * j.Kpad < (1ll << 31);
+                                        prows(j.s) * j.Kpad < (1ll << 31) && prows(n) * j.Kpad < (1ll << 31);
                     launch_outer32<0>(grid, s, d_desc + i, ts, lds_ok);
                 } else {
                     k_gemm_outer64<0><<<grid, G_T, 0, s>>>(d_desc + i, ts);
