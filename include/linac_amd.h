@@ -110,6 +110,8 @@ int linac_profile_last(double* out, int n);
  *   "inner_threads" 0: threads of the inner factorisation's workgroup, 256,
  *       512 or 1024 (fewer waves, each with more rows; 0: 1024 for 32-bit,
  *       512 for 64-bit storage);
+ *   "u_passes" 0: column pairs per workgroup building U, 1, 2, 4 or 8 (0: 1
+ *       for fewer than 8 matrices in a step, 4 from 8 on);
  *   "inner_split" 16: at most this many workgroups per 64-row stripe of the
  *       inner GEMM (0: 8 for 64-bit, 4 for 32-bit storage);
  *   "tile64" 1: the 64-bit trailing GEMM's tile, 64 x 32 (1) or 64 x 64 (2);

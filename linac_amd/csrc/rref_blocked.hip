@@ -686,12 +686,19 @@ constexpr int MU_SPLIT = 4;
 constexpr int MU_BLOCKS = XLD / MU_COLS;
 static_assert(MU_COLS * CI * MU_SPLIT == 256 && 2 * CI == KPI, "one thread per (column, pivot slot, part); slots pad to KPI");
 constexpr int MU_ROWS = 16;
+// Column pairs per U workgroup: every U workgroup reads the whole k x k
+// inverse, so a batch of many matrices takes several pairs per workgroup
+// (tuning knob "u_passes"; 0: 1 below 8 matrices, 4 from 8 on).
+static inline int u_passes(unsigned B) {
+    const int64_t k = tuning().u_passes;
+    return k > 0 ? (int)k : (B >= 8 ? 4 : 1);
+}
 
 template <class LT>
 __device__ static void make_u_body(typename LT::T* __restrict__ X, int64_t N, typename LT::F f,
                                    const BlockState<typename LT::T>* __restrict__ bs, typename LT::T* __restrict__ U,
                                    int8_t* __restrict__ planesU, int64_t strideU,
-                                   int8_t* __restrict__ planesA, int64_t strideA) {
+                                   int8_t* __restrict__ planesA, int64_t strideA, int passes) {
     using T = typename LT::T;
     constexpr int NL = LT::NL;
     __shared__ T s_inv[CI * CI];
@@ -700,13 +707,14 @@ __device__ static void make_u_body(typename LT::T* __restrict__ X, int64_t N, ty
     const int k = bs->k;
     if (k == 0) return;  // the inner GEMM skips steps without pivots
     const int t = threadIdx.x;  // 0..63
-    if (blockIdx.x >= MU_BLOCKS) {
+    const int ublocks = MU_BLOCKS / passes;  // U workgroups, `passes` column pairs each
+    if ((int)blockIdx.x >= ublocks) {
         const int c0 = bs->c0_cur;
         if (threadIdx.y == 0 && t < k) s_cp[t] = bs->Cp[t];
         __syncthreads();
         const int64_t col = t < k ? c0 + s_cp[t] : 0;
-        for (int64_t r0 = (int64_t)(blockIdx.x - MU_BLOCKS) * MU_ROWS + threadIdx.y; r0 < N;
-             r0 += (int64_t)(gridDim.x - MU_BLOCKS) * MU_ROWS) {
+        for (int64_t r0 = (int64_t)(blockIdx.x - ublocks) * MU_ROWS + threadIdx.y; r0 < N;
+             r0 += (int64_t)(gridDim.x - ublocks) * MU_ROWS) {
             T v[MU_ROWS / 4];
 #pragma unroll
             for (int i = 0; i < MU_ROWS / 4; ++i) {
@@ -731,43 +739,50 @@ __device__ static void make_u_body(typename LT::T* __restrict__ X, int64_t N, ty
     __syncthreads();
     // the block's columns of the k pivot rows, fetched at once (a load per
     // term of the dot product would expose the memory latency k times)
-    __shared__ T s_x[CI][MU_COLS];
-    for (int x = fl; x < k * MU_COLS; x += 256) {
-        const int sl = x / MU_COLS, yy = x - sl * MU_COLS;
-        s_x[sl][yy] = X[s_R[sl] * XLD + blockIdx.x * MU_COLS + yy];
+    constexpr int MU_PASSES_MAX = 8;
+    __shared__ T s_x[CI][MU_COLS * MU_PASSES_MAX];
+    const int ncols = MU_COLS * passes;
+    for (int x = fl; x < k * ncols; x += 256) {
+        const int sl = x / ncols, yy = x - sl * ncols;
+        s_x[sl][yy] = X[s_R[sl] * XLD + (int64_t)blockIdx.x * ncols + yy];
     }
     __syncthreads();
     // lane layout: part = fl % MU_SPLIT (adjacent lanes), pivot slot, column
-    const int part = fl % MU_SPLIT, ts = (fl / MU_SPLIT) % CI, y = fl / (MU_SPLIT * CI);
-    const int64_t n = (int64_t)blockIdx.x * MU_COLS + y;
-    T u = 0;
-    if (ts < k)
-        for (int s = part; s < k; s += MU_SPLIT) u = f.add(u, f.mont(s_inv[ts * CI + s], s_x[s][y]));
+    const int part = fl % MU_SPLIT, ts = (fl / MU_SPLIT) % CI, y0 = fl / (MU_SPLIT * CI);
+    auto one_pass = [&](int y) {
+        const int64_t n = (int64_t)blockIdx.x * ncols + y;
+        T u = 0;
+        if (ts < k)
+            for (int s = part; s < k; s += MU_SPLIT) u = f.add(u, f.mont(s_inv[ts * CI + s], s_x[s][y]));
 #pragma unroll
-    for (int off = 1; off < MU_SPLIT; off <<= 1) u = f.add(u, (T)__shfl_xor(u, off, 64));
-    if (ts < k && part == 0) {
-        U[ts * XLD + n] = u;
-        // the pivot rows' new values go straight into the panel (the inner
-        // GEMM leaves those rows alone); every read of X by this workgroup is
-        // behind the barrier above, and the planes workgroups' reads of pivot
-        // rows feed only results that are never stored
-        X[s_R[ts] * XLD + n] = u;
-    }
-    int8_t d[NL] = {};
-    if (ts < k) balanced_digits(centred(u, f.p), d);
-    // part 0 writes slot ts, part 1 slot ts + CI (the zero padding up to KPI)
-    if (part < 2) {
-        const int slot = ts + part * CI;
+        for (int off = 1; off < MU_SPLIT; off <<= 1) u = f.add(u, (T)__shfl_xor(u, off, 64));
+        if (ts < k && part == 0) {
+            U[ts * XLD + n] = u;
+            // the pivot rows' new values go straight into the panel (the inner
+            // GEMM leaves those rows alone); every read of X by this workgroup is
+            // behind the barrier above, and the planes workgroups' reads of pivot
+            // rows feed only results that are never stored
+            X[s_R[ts] * XLD + n] = u;
+        }
+        int8_t d[NL] = {};
+        if (ts < k) balanced_digits(centred(u, f.p), d);
+        // part 0 writes slot ts, part 1 slot ts + CI (the zero padding up to KPI)
+        if (part < 2) {
+            const int slot = ts + part * CI;
 #pragma unroll
-        for (int l = 0; l < NL; ++l) planesU[l * strideU + n * KPI + slot] = part == 0 ? d[l] : (int8_t)0;
-    }
+            for (int l = 0; l < NL; ++l) planesU[l * strideU + n * KPI + slot] = part == 0 ? d[l] : (int8_t)0;
+        }
+    };
+    if (passes == 1) one_pass(y0);
+    else
+        for (int pass = 0; pass < passes; ++pass) one_pass(y0 + MU_COLS * pass);
 }
 
 
 template <class LT>
-__global__ __launch_bounds__(256) void k_make_u(const BMat<LT>* __restrict__ D) {
+__global__ __launch_bounds__(256) void k_make_u(const BMat<LT>* __restrict__ D, int passes) {
     const BMat<LT>& d = D[blockIdx.y];
-    make_u_body<LT>(d.X, d.N, d.f, d.bs, d.U, d.planesUi, (int64_t)XLD * KPI, d.planesAi, d.strideAi);
+    make_u_body<LT>(d.X, d.N, d.f, d.bs, d.U, d.planesUi, (int64_t)XLD * KPI, d.planesAi, d.strideAi, passes);
 }
 
 // ---------------------------------------------------------------------------
@@ -2194,16 +2209,58 @@ __global__ __launch_bounds__(256) void k_fast_gate(BMat<LT>* __restrict__ Dw, co
         st.lo = Dl[m].bs->lo0;
         out[m] = st;
     }
-    if (threadIdx.x != 0) return;
-    int64_t kept = 0, prev = pre[0];
-    for (int m = 0; m < B; ++m) {
-        const int64_t tiles = pre[m + 1] - prev;
-        prev = pre[m + 1];
-        const int bcur = Dw[m].bcur;
-        const bool full = Dw[m].nT > 0 && Dc[m].st->rank == bcur && Dc[m].bs->q0 == bcur;
-        if (!full) Dw[m].nT = 0;
-        kept += full ? tiles : 0;
-        pre[m + 1] = kept;
+    // drop the blocks that missed full rank and rewrite the Z GEMM's tile
+    // prefix sums in place over the kept ones: for a few matrices one thread
+    // in turn; for more, 256 matrices at a time, their reads all in flight
+    // together, an inclusive scan in LDS (the serial loop chains several
+    // dependent loads per matrix)
+    if (B <= 8) {
+        if (threadIdx.x != 0) return;
+        int64_t kept = 0, prev = pre[0];
+        for (int m = 0; m < B; ++m) {
+            const int64_t tiles = pre[m + 1] - prev;
+            prev = pre[m + 1];
+            const int bcur = Dw[m].bcur;
+            const bool full = Dw[m].nT > 0 && Dc[m].st->rank == bcur && Dc[m].bs->q0 == bcur;
+            if (!full) Dw[m].nT = 0;
+            kept += full ? tiles : 0;
+            pre[m + 1] = kept;
+        }
+        return;
+    }
+    __shared__ int64_t s_scan[256];
+    __shared__ int64_t s_run, s_edge;  // kept tiles before this chunk; the chunk's last original prefix
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_run = 0;
+        s_edge = pre[0];
+    }
+    __syncthreads();
+    for (int base = 0; base < B; base += 256) {
+        const int m = base + tid;
+        int64_t t = 0, hi = 0;
+        if (m < B) {
+            hi = pre[m + 1];
+            const int64_t lo = tid == 0 ? s_edge : pre[m];
+            const int bcur = Dw[m].bcur;
+            const bool full = Dw[m].nT > 0 && Dc[m].st->rank == bcur && Dc[m].bs->q0 == bcur;
+            t = full ? hi - lo : 0;
+            if (!full) Dw[m].nT = 0;
+        }
+        s_scan[tid] = t;
+        __syncthreads();  // (every original prefix of the chunk is read before any is rewritten)
+        if (tid == 255 || m == B - 1) s_edge = hi;
+#pragma unroll
+        for (int off = 1; off < 256; off <<= 1) {
+            const int64_t o = tid >= off ? s_scan[tid - off] : 0;
+            __syncthreads();
+            s_scan[tid] += o;
+            __syncthreads();
+        }
+        if (m < B) pre[m + 1] = s_run + s_scan[tid];
+        __syncthreads();
+        if (tid == 0) s_run += s_scan[255];
+        __syncthreads();
     }
 }
 
@@ -2818,8 +2875,9 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     fast.assign(B, 0);
     // (the candidate rows were gathered by the panel load's last workgroup)
     k_cand_copy<LT><<<dim3(BO / 2, B), 256, 0, s>>>(in.desc, cs.desc);
-    const unsigned mu_grid =
-        (unsigned)(MU_BLOCKS + ((wide || cs.inner_split > 1) ? std::min<int64_t>(4096, ceil_div(BO, MU_ROWS)) : 0));
+    const int mu_passes = u_passes(B);
+    const unsigned mu_grid = (unsigned)(MU_BLOCKS / mu_passes +
+                                        ((wide || cs.inner_split > 1) ? std::min<int64_t>(4096, ceil_div(BO, MU_ROWS)) : 0));
     const unsigned igrid = (unsigned)(ceil_div(cs.inner_tiles, NUM_XCD) * NUM_XCD);
     if constexpr (!wide) {
         if (tuning().fast_panel == 1) {  // the block's inner steps in one workgroup per matrix
@@ -2830,7 +2888,7 @@ int Engine<LT>::fast_panel(const std::vector<It*>& act, hipStream_t s, const Sta
     if (wide || tuning().fast_panel == 2)  // the inner-step kernels on the block
         for (int64_t i = 0; i < ninner; ++i) {
             launch_inner_factor<LT>(B, s, cs.desc, epoch);
-            k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(cs.desc);
+            k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(cs.desc, mu_passes);
             if constexpr (wide) k_gemm_inner64<<<igrid, G_T, 0, s>>>(cs.desc, cs.inner, epoch, cs.inner_split);
             else k_gemm_inner<<<igrid, G_T, 0, s>>>(cs.desc, cs.inner, epoch, cs.inner_split);
             ++epoch;
@@ -2944,7 +3002,9 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
     first_step = false;
     // (the 32-bit inner GEMM captures its left operand itself)
     const bool planes_a = wide || in.inner_split > 1;
-    const unsigned mu_grid = (unsigned)(MU_BLOCKS + (planes_a ? std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)) : 0));
+    const int mu_passes = u_passes(B);
+    const unsigned mu_grid =
+        (unsigned)(MU_BLOCKS / mu_passes + (planes_a ? std::min<int64_t>(4096, ceil_div(maxN, MU_ROWS)) : 0));
     auto group = [&](int64_t i) -> int {
         const bool sl = slicing && i < NEV;
         if (sl) {
@@ -2960,7 +3020,7 @@ int Engine<LT>::step(const std::vector<It*>& items, hipStream_t s, Profiler* pro
             LINAC_HIP(hipEventRecord(ev_c[i], so));
         }
         launch_inner_factor<LT>(B, s, in.desc, epoch);
-        k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(in.desc);
+        k_make_u<LT><<<dim3(mu_grid, B), dim3(64, 4), 0, s>>>(in.desc, mu_passes);
         if (sl) LINAC_HIP(hipStreamWaitEvent(s, ev_c[i], 0));
         const unsigned grid = (unsigned)(ceil_div(in.inner_tiles, NUM_XCD) * NUM_XCD);
         if constexpr (wide) k_gemm_inner64<<<grid, G_T, 0, s>>>(in.desc, in.inner, epoch, in.inner_split);

@@ -417,6 +417,7 @@ int linac_tune_set(const char* key, int64_t value) {
                  {"inner_threads", &t.inner_threads, 0, 1024},
                  {"inner_split", &t.inner_split, 0, 32},
                  {"tile64", &t.tile64, 1, 2},
+                 {"u_passes", &t.u_passes, 0, 8},
                  {"fpanel_rows", &t.fpanel_rows, 16, 32}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
@@ -425,6 +426,8 @@ int linac_tune_set(const char* key, int64_t value) {
                             (long long)k.lo, (long long)k.hi);
             if (!strcmp(key, "inner_threads") && value != 0 && value != 256 && value != 512 && value != 1024)
                 return fail(LINAC_ERR_INVALID, "inner_threads = %lld: 0, 256, 512 or 1024", (long long)value);
+            if (!strcmp(key, "u_passes") && (value & (value - 1)) != 0)
+                return fail(LINAC_ERR_INVALID, "u_passes = %lld: 0, 1, 2, 4 or 8", (long long)value);
             *k.slot = value;
             return LINAC_OK;
         }
@@ -454,7 +457,7 @@ int linac_tune_get(const char* key, int64_t* value) {
                  {"fast_panel", t.fast_panel}, {"echelon", t.echelon},
                  {"inner_compact", t.inner_compact}, {"inner_threads", t.inner_threads},
                  {"inner_split", t.inner_split},
-                 {"tile64", t.tile64}, {"fpanel_rows", t.fpanel_rows}};
+                 {"tile64", t.tile64}, {"u_passes", t.u_passes}, {"fpanel_rows", t.fpanel_rows}};
     for (auto& k : knobs)
         if (!strcmp(key, k.name)) {
             *value = k.v;

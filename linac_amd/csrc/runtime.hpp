@@ -138,6 +138,7 @@ struct Tuning {
     int64_t inner_threads = 0;  // inner factorisation: threads per workgroup (256, 512, 1024; 0: 1024 for 32-bit, 512 for 64-bit)
     int64_t fpanel_rows = 16;   // float64 panel kernel: rows per wave (16, or 32 for half the workgroups)
     int64_t tile64 = 1;         // 64-bit trailing GEMM tile: 64 x 32 (1) or 64 x 64 (2)
+    int64_t u_passes = 0;       // column pairs per U workgroup (1, 2, 4, 8; 0: 1 below 8 matrices, 4 from 8 on)
     int64_t inner_split = 16;   // workgroups per row stripe of the inner GEMM at most (0: 8 for 64-bit, 4 for 32-bit)
     // Per size class overrides of the knobs above that shape the GEMMs (0: the
     // global value applies).  The class of a step or call is that of its

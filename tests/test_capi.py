@@ -155,6 +155,12 @@ def test_tuning_knobs_set_get_and_validation():
         for v in (128, 300, 2048):
             with pytest.raises(ValueError):
                 autotune.set_knob("inner_threads", v)
+        for v in (0, 1, 2, 4, 8):
+            autotune.set_knob("u_passes", v)
+            assert autotune.get("u_passes") == v
+        for v in (3, 6, 16):
+            with pytest.raises(ValueError):
+                autotune.set_knob("u_passes", v)
         with pytest.raises(ValueError):
             autotune.set_knob("no_such_knob", 1)
         with pytest.raises(ValueError):
@@ -162,6 +168,7 @@ def test_tuning_knobs_set_get_and_validation():
     finally:
         autotune.apply(before)
         autotune.set_knob("inner_threads", 0)
+        autotune.set_knob("u_passes", 0)
     assert autotune.current() == before
 
 

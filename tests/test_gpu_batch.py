@@ -38,6 +38,24 @@ def test_mixed_shapes_and_primes(gpu):
         assert numpy.array_equal(R.astype(numpy.uint64), Ro.astype(numpy.uint64))
 
 
+def test_hundreds_of_items_in_one_batch(gpu):
+    """300 matrices in one batch: the full-rank gate and its tile prefix sums
+    run 256 matrices at a time, with rank-deficient blocks dropped on both
+    sides of the boundary; U is built several column pairs per workgroup."""
+    p = 2 ** 31 - 1
+    rs = numpy.random.RandomState(5)
+    mats = []
+    for i in range(300):
+        n = int(rs.randint(260, 330))
+        rank = n if i % 7 else int(rs.randint(100, n))  # every 7th rank-deficient
+        mats.append(_mat(n, n + int(rs.randint(0, 20)), rank, p, 1000 + i))
+    results = gpu.rref_batch(mats, [p] * len(mats))
+    for A, (R, piv, rank) in zip(mats, results):
+        Ro, po = crref.rref_modp(A, p)
+        assert piv == po and rank == len(po)
+        assert numpy.array_equal(R, Ro)
+
+
 def test_same_result_alone_and_in_the_mix(gpu):
     p = 2 ** 31 - 1
     rs = numpy.random.RandomState(1)
