@@ -1758,11 +1758,22 @@ __global__ __launch_bounds__(256) void k_item_init(const BMat<LT>* __restrict__ 
         d.used[r] = 0;
         d.epoch_of[r] = -1;
     }
-    const int64_t total = d.N * d.M;
-    for (int64_t x = g0; x < total; x += gs) {
-        const int64_t r = x / d.M, c = x - r * d.M;
-        auto& v = d.A[r * d.ld + c];
-        v = d.f.reduce(v);
+    // entries reduced mod p: a row per workgroup pass, four loads of a thread
+    // in flight, written back only where they change (no division per entry;
+    // loader output and canonical inputs are only read)
+    using T = typename LT::T;
+    for (int64_t r = blockIdx.x; r < d.N; r += gridDim.x) {
+        T* row = d.A + r * d.ld;
+        for (int64_t c0 = threadIdx.x; c0 < d.M; c0 += 4 * 256) {
+            T v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = c0 + 256 * u < d.M ? row[c0 + 256 * u] : (T)0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const T y = d.f.reduce(v[u]);
+                if (c0 + 256 * u < d.M && y != v[u]) row[c0 + 256 * u] = y;
+            }
+        }
     }
 }
 
