@@ -190,7 +190,7 @@ def sweep_bench(n_systems, p=2 ** 31 - 1):
     import linac_amd
     from linac_amd import sweep
     bases, lind, _ = sweep.sweep_problem(n_systems, 8, 6, 3000, 3000, p, 0)
-    sweep.reduce_sweep(bases[:2], lind, p)  # warm-up: grows the library's scratch
+    sweep.reduce_sweep(bases, lind, p)  # an untimed sweep of the same size: the batch arena grows to it
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     res = sweep.reduce_sweep(bases, lind, p)
@@ -220,7 +220,7 @@ def sweep_bench(n_systems, p=2 ** 31 - 1):
     macs = n_systems * 3000 * 3000 * 3001 / 2
     return {
         "workload": f"{n_systems} systems 3000x3000 u32 GF(2^31-1): loader (8 vars, 3000 monomials of degree <= 6, "
-                    "prefactor) into HBM + batched in-place RREF",
+                    "prefactor) into HBM + batched in-place RREF, after one untimed sweep of the same size",
         "correct": ok, "seconds": round(total, 4), "loader_ms": round(1e3 * load_s, 3),
         "reduce_s": round(reduce_s, 4),
         "solves_per_s": round(n_systems / total, 2),
