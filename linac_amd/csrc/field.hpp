@@ -202,6 +202,36 @@ __host__ __device__ __forceinline__ uint64_t m61_redc(uint64_t lo, uint64_t hi) 
     return ((x & 7ull) << 58) | (x >> 3);              // x 2^58 mod p (x < p: never p itself)
 }
 
+// a b mod 2^61 - 1 for a, b < 2^61 (the fold of m61_redc without its rotation)
+__host__ __device__ __forceinline__ uint64_t m61_mul(uint64_t a, uint64_t b) {
+    const uint64_t lo = a * b, hi = mulhi64(a, b);  // a b < 2^122: hi < 2^58
+    uint64_t x = (lo & M61) + (lo >> 61) + (hi << 3);
+    x = (x & M61) + (x >> 61);
+    return x >= M61 ? x - M61 : x;
+}
+// a^(p-2) mod p = a^{-1} for p = 2^61 - 1 (a = 0 gives 0): with x_k = a^(2^k - 1),
+// x_{j+k} = x_j^(2^k) x_k, build x_59 from x_1, x_2, x_3, x_6, x_12, x_24, x_48,
+// x_56 (with x_8 = x_6^4 x_2), x_58, x_59, then a^(2^61 - 3) = x_59^4 a: 62
+// squarings and 11 products, a fixed sequence for a value held alike in every
+// lane (a third of the steps of the binary extended GCD, none of its branches).
+__host__ __device__ inline uint64_t m61_inverse(uint64_t a) {
+    auto sq = [](uint64_t x, int k) {
+        for (int i = 0; i < k; ++i) x = m61_mul(x, x);
+        return x;
+    };
+    const uint64_t x1 = a;
+    const uint64_t x2 = m61_mul(sq(x1, 1), x1);
+    const uint64_t x3 = m61_mul(sq(x2, 1), x1);
+    const uint64_t x6 = m61_mul(sq(x3, 3), x3);
+    const uint64_t x12 = m61_mul(sq(x6, 6), x6);
+    const uint64_t x24 = m61_mul(sq(x12, 12), x12);
+    const uint64_t x48 = m61_mul(sq(x24, 24), x24);
+    const uint64_t x56 = m61_mul(sq(x48, 8), (m61_mul(sq(x6, 2), x2)));  // x_8 = x_6^4 x_2
+    const uint64_t x58 = m61_mul(sq(x56, 2), x2);
+    const uint64_t x59 = m61_mul(sq(x58, 1), x1);
+    return m61_mul(sq(x59, 2), x1);
+}
+
 struct FieldU64 {
     using T = uint64_t;
     uint64_t p;

@@ -564,7 +564,7 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
         const T tot = uniform(__shfl(v, k - 1, 64));
         T inv;
         if constexpr (sizeof(T) == 4) inv = euclid_inverse_u32(tot, f.p);
-        else inv = binary_inverse<T>(tot, f.p);
+        else inv = f.m61 ? m61_inverse(tot) : binary_inverse<T>(tot, f.p);
         const T x = f.to_mont(inv);
         if (lane < k) {
             s_pre[lane] = ex;

@@ -10,6 +10,7 @@
 //   I32 p h / I64 p h      inverse
 //   B32 p h / B64 p h      binary_inverse
 //   E32 p h                euclid_inverse_u32
+//   X61 h                  m61_inverse (p = 2^61 - 1)
 //   D32 c / D64 c          balanced base-256 digits of a centred value
 #include <cstdio>
 #include <cstring>
@@ -47,6 +48,10 @@ int main() {
             in >> p >> a;
             if (op == "B32") std::cout << binary_inverse<uint32_t>((uint32_t)a, (uint32_t)p) << "\n";
             else std::cout << binary_inverse<uint64_t>(a, p) << "\n";
+        } else if (op == "X61") {
+            uint64_t a;
+            in >> a;
+            std::cout << m61_inverse(a) << "\n";
         } else if (op == "E32") {
             uint64_t p, a;
             in >> p >> a;

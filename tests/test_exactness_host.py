@@ -173,6 +173,17 @@ def test_montgomery_update_mul_inverse(harness, bits):
     assert not bad, bad[:3]
 
 
+def test_mersenne61_inverse(harness):
+    """The addition-chain inverse for p = 2^61 - 1 (the 64-bit inner
+    factorisation's one inversion at that prime)."""
+    p = 2 ** 61 - 1
+    rng = random.Random(61)
+    hs = [1, 2, 3, p - 1, p - 2, 2 ** 60, 2 ** 58 + 1] + [2 ** e for e in range(1, 61)] + \
+        [rng.randrange(1, p) for _ in range(300)]
+    got = harness([f"X61 {h}" for h in hs] + ["X61 0"])
+    assert [int(g) for g in got] == [pow(h, -1, p) for h in hs] + [0]
+
+
 def test_balanced_digits(harness):
     rng = random.Random(3)
     c32 = [0, 1, -1, 127, 128, -128, -129, 2139062143, -2139062143] + [rng.randint(-2139062143, 2139062143)
