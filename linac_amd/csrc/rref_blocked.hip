@@ -304,6 +304,13 @@ static_assert((CI & (CI - 1)) == 0 && CI <= 32, "pivot slots index with a mask; 
 // A value held alike in every lane, moved to scalar registers (so that code
 // depending on it alone runs on the scalar unit).
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// v as held by lane l (l uniform), in scalar registers
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ uint64_t lane_value(uint64_t v, int l) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
 __device__ __forceinline__ uint64_t uniform(uint64_t v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
@@ -445,8 +452,8 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
             // which is read beside the rows above; the search over all rows
             // (a second round trip to LDS) only when it is zero
             const int ps = __ffsll((long long)(~pmask & live)) - 1;
-            T h = uniform(aug[ps][c]);
             T y = aug[ps][lane];
+            T h = lane_value(y, c);  // its entry in column c, from lane c (no second load)
             int pv = ps;
             if (h == (T)0) {
                 const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
@@ -507,8 +514,8 @@ __device__ static void inner_factor_body(typename LT::T* __restrict__ X, int64_t
             }
             // (the first row not yet a pivot, unless its entry is zero: as above)
             const int ps = __ffsll((long long)(~pmask & live)) - 1;
-            T h = uniform(aug[ps][c]);
             T y = aug[ps][j];
+            T h = lane_value(y, c);
             int pv = ps;
             if (h == (T)0) {
                 const bool nz = lane < nC && !((pmask >> lane) & 1ull) && aug[lane < CI ? lane : 0][c] != (T)0;
